@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the panel hot path on one MI355X.
+
+Workload (the headline): demean -> band-pass (cos² 5-10-70-90 Hz with a 50 Hz
+notch 4 Hz wide) -> Butterworth high cut 80 Hz 24 dB/oct -> AGC 250 ms on a
+2000 x 3001 float32 panel at 2 ms, live lengths uniform in [2400, 3001] per
+row, synthetic seeded data (Ricker wavelets on hyperbolic events + Gaussian
+noise).  A step is one Plan.apply src -> dst of one panel; inputs are resident
+in HBM and 8 distinct src/dst pairs rotate (384 MB, beyond the 256 MB
+Infinity Cache), so every step reads and writes HBM.
+
+Prints one JSON line: Msamples/s, the roofline of the fused kernel (HIP events
+on its stream), the accuracy against the float64 oracle on a sample of rows,
+and the CPU baseline (the oracle's ProcessPool clone on the host cores).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ROWS, COLS, DT = 2000, 3001, 2.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_flow():
+    from processseg_amd.workloads import big_flow
+    return big_flow()
+
+
+def cpu_baseline_leg(flow, x, live, gpu_out, seconds=12.0):
+    """The oracle (a C restatement of the reference) on the host: the
+    ProcessPool clone over the whole panel with every core, repeated for about
+    `seconds`, plus a one-core sample; and the accuracy of the GPU output
+    against the float64 oracle on 64 rows."""
+    import oracle as O
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    cores = min(cores, 16)
+    row_len = live.astype(np.int32)
+    pool = O.Pool(cores)
+    dst = np.empty_like(x)
+    pool.apply_f32(flow.ops, x, dst, row_len, DT)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        pool.apply_f32(flow.ops, x, dst, row_len, DT)
+        reps += 1
+    el = time.perf_counter() - t0
+    pool.close()
+    value = reps * x.size / el / 1e6
+    # accuracy vs the float64 oracle on 64 rows spread over the panel
+    idx = np.linspace(0, x.shape[0] - 1, 64).astype(int)
+    want = O.pipeline_apply(flow.ops, x[idx].astype(np.float64), row_len[idx], DT)
+    got = gpu_out[idx].astype(np.float64)
+    rel = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    mabs = float(np.max(np.abs(got - want)))
+    mirror = dst[idx].astype(np.float64)
+    rel_cpu = float(np.linalg.norm(mirror - want) / np.linalg.norm(want))
+    return ({"value": round(value, 2), "unit": "Msamples/s", "cores": cores, "kind": "port",
+             "sample": f"the whole {x.shape[0]}x{x.shape[1]} panel, {reps} applies in "
+                       f"{el:.1f} s on {cores} threads (ProcessPool clone, runs of 256 KiB)"},
+            {"vs": "float64 oracle, 64 rows", "rel_l2": rel, "max_abs": mabs,
+             "cpu_mirror_rel_l2": rel_cpu})
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--pairs", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the panel path on a single MI355X "
+              f"(rows shard across GPUs with no exchange; see DESIGN.md); --gpus {args.gpus} "
+              "is not run", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import processseg_amd as P
+
+    dev = torch.device("cuda:0")
+    from processseg_amd.workloads import big_panel
+    log("generating the 2000 x 3001 panel")
+    x, live = big_panel(ROWS, COLS, np.float32)
+    flow = make_flow()
+    plan = flow.compile(np.float32, DT, COLS)
+    row_len = torch.from_numpy(live.astype(np.int32)).to(dev)
+    srcs = [torch.from_numpy(x).to(dev) for _ in range(args.pairs)]
+    dsts = [torch.empty_like(srcs[0]) for _ in range(args.pairs)]
+    stream = torch.cuda.current_stream()
+
+    log(f"plan: {plan.steps}, lds {plan.lds_bytes} B")
+    for i in range(args.warmup):
+        plan.apply(srcs[i % args.pairs], dsts[i % args.pairs], row_len=row_len)
+    torch.cuda.synchronize()
+
+    # timed region: K steps, events around every launch on the plan's stream
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        k = i % args.pairs
+        starts[i].record(stream)
+        plan.apply(srcs[k], dsts[k], row_len=row_len, stream=stream)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = float(np.mean(per))
+    ms_per_step = wall * 1000 / args.steps
+    samples = ROWS * COLS
+    value = samples / (ms_per_step / 1000) / 1e6
+
+    algo_bytes = samples * 8  # 4 read + 4 written per sample
+    achieved = algo_bytes / (avg_ms / 1000) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_bandpass_agc_2000x3001_f32.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = dsts[(args.steps - 1) % args.pairs].cpu().numpy()
+    line = {
+        "metric": "Msamples/s band-pass+AGC 2000x3001 f32 (%HBM roofline); p50/p99 served latency",
+        "value": round(value, 1),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Ricker + hyperbolic events + Gaussian noise)",
+        "config": {"workload": "demean > band-pass 5-10-70-90 Hz + 50 Hz notch > Butterworth "
+                               "high cut 80 Hz 24 dB/oct > AGC 250 ms, 2000x3001 f32, dt 2 ms, "
+                               "live lengths U[2400, 3001]",
+                   "rows": ROWS, "cols": COLS, "dt_ms": DT,
+                   "nfft": [s["nfft"] for s in plan.steps if s["nfft"]],
+                   "buffer_pairs": args.pairs},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "pipeline_kernel<float,12>",
+                     "avg_launch_ms": round(avg_ms, 5),
+                     "algorithmic_bytes_per_launch": algo_bytes},
+    }
+    if not args.no_cpu:
+        log("CPU baseline leg")
+        cpu, acc = cpu_baseline_leg(flow, x, live, out)
+        line["cpu_baseline"] = cpu
+        line["accuracy"] = acc
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

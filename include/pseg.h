@@ -1,0 +1,233 @@
+/*
+ * pseg.h — C ABI of the MI355X (gfx950) panel hot path of processSEG.
+ *
+ * The library (processseg_amd/libpseg.so) compiles processSEG pipelines and
+ * correlation plans on the host (float64 tables, as the reference does) and
+ * applies them to [rows, cols] panels in HBM with hand-written HIP kernels.
+ * Every entry point replaces one reference interface; the citation is given
+ * beside each declaration (paths relative to the reference repository).
+ *
+ * Conventions.
+ *  - Every function returns a pseg_status; on failure a thread-local message
+ *    is kept for pseg_last_error.  Argument errors are found before any sample
+ *    changes (pipeline.nim:619-641), and data never raises: a NaN or Inf stays
+ *    in its own row.
+ *  - Panels are row-major, C-contiguous, float32 or float64, in device memory.
+ *    src and dst are the same panel (in place) or do not overlap.
+ *  - Rows are described by one int32 per row, `row_len`, in device memory:
+ *    the live length (ChunkMeta.liveSamples, core/types.nim:226-230) of a row
+ *    the op processes, or -1 for a row it leaves alone (dead, auxiliary or
+ *    failed: core/types.nim:221-224).  pseg_meta_flatten builds it on the host.
+ *  - `stream` is a hipStream_t (NULL: the default stream).  Apply calls are
+ *    asynchronous on it and allocate nothing.
+ *  - Plans are immutable after compile and may be shared across threads and
+ *    streams (pipeline.nim:78-82).
+ */
+#ifndef PSEG_H
+#define PSEG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pseg_status {
+  PSEG_OK = 0,
+  PSEG_E_INVALID = 1,      /* an argument the reference rejects with ValueError */
+  PSEG_E_UNSUPPORTED = 2,  /* an op outside the GPU path: keep the CPU plan */
+  PSEG_E_HIP = 3,          /* a HIP runtime error */
+  PSEG_E_WORKSPACE = 4,    /* workspace missing or smaller than the plan needs */
+  PSEG_E_BUSY = 5          /* a queue is full (serving) */
+} pseg_status;
+
+typedef enum pseg_dtype { PSEG_F32 = 0, PSEG_F64 = 1 } pseg_dtype;
+
+/* OpKind (pipeline.nim:92-108), same order. */
+typedef enum pseg_op_kind {
+  PSEG_OP_DEMEAN = 0, PSEG_OP_DETREND = 1, PSEG_OP_SCALE = 2, PSEG_OP_GAIN_DB = 3,
+  PSEG_OP_CLIP = 4, PSEG_OP_POLARITY = 5, PSEG_OP_MUTE_TOP = 6, PSEG_OP_MUTE_BOTTOM = 7,
+  PSEG_OP_FILTER = 8, PSEG_OP_IIR = 9, PSEG_OP_AGC = 10, PSEG_OP_NORMALIZE = 11,
+  PSEG_OP_GAIN_CURVE = 12, PSEG_OP_CALIBRATE = 13, PSEG_OP_UNCALIBRATE = 14
+} pseg_op_kind;
+
+#define PSEG_MAX_NOTCH 8
+#define PSEG_MAX_OPS 16
+
+/* FilterSpec (filter/design.nim:78-90): kind 0 = cos² trapezoid (corners
+ * f1 < f2 <= f3 < f4; f2 = 0: no low cut, f3 = f4 = Inf: no high cut),
+ * kind 1 = Butterworth magnitude (low_hz = 0: no low cut, high_hz = Inf: no
+ * high cut; slopes in dB/octave, effective). */
+typedef struct pseg_filter_spec {
+  int32_t kind;
+  int32_t zero_dc;
+  int32_t n_notch;
+  int32_t reserved;
+  double f1, f2, f3, f4;
+  double low_hz, low_slope, high_hz, high_slope;
+  double notch_hz[PSEG_MAX_NOTCH];
+  double notch_width[PSEG_MAX_NOTCH];
+} pseg_filter_spec;
+
+/* Op (pipeline.nim:158-174) as plain data; the builders of pipeline.nim:
+ * 258-316 fill it.  Fields a kind does not use are ignored. */
+typedef struct pseg_op {
+  int32_t kind;        /* pseg_op_kind */
+  int32_t trend;       /* filter: 0 trNone, 1 trMean, 2 trLinear (filter/edges.nim:8-14) */
+  int32_t mode;        /* agc: 0 agcRms, 1 agcMeanAbs (gain/gain.nim:92-95) */
+  int32_t stat;        /* normalize: 0 normMax, 1 normRms, 2 normPercentile */
+  int32_t scope;       /* normalize: 0 perTrace, 1 perPanel */
+  int32_t reserved;
+  double value;        /* scale: factor; gainDb: dB */
+  double lo, hi;       /* clip */
+  double window_ms;    /* agc */
+  double target;       /* agc, normalize */
+  double epsilon;      /* agc */
+  double percentile;   /* normalize */
+  pseg_filter_spec spec; /* filter */
+} pseg_op;
+
+typedef struct pseg_plan pseg_plan;
+typedef struct pseg_corr_plan pseg_corr_plan;
+
+/* What a compiled plan reports (pipeline.nim:562-602, filter/apply.nim:153-176). */
+typedef struct pseg_plan_info {
+  uint64_t generation;     /* process-unique, increasing (pipeline.nim:213,523) */
+  int32_t dtype;
+  int32_t n_steps;
+  int32_t parts;           /* partCount (pipeline.nim:643-647) */
+  int32_t samples;
+  int32_t record_samples;
+  int32_t margin;          /* requiredMargin: the sum of the steps' margins */
+  double dt_ms;
+  int32_t step_kind[PSEG_MAX_OPS];   /* the op kind of each step */
+  int32_t step_nfft[PSEG_MAX_OPS];   /* FFT length of a filter step, else 0 */
+  int32_t step_pad[PSEG_MAX_OPS];    /* pad of a filter step */
+  int32_t step_margin[PSEG_MAX_OPS]; /* impulse length L, or AGC half window h */
+  int32_t step_reflect[PSEG_MAX_OPS];
+  size_t lds_bytes;        /* LDS one row of the fused kernel holds */
+} pseg_plan_info;
+
+/* --------------------------------------------------------------- errors -- */
+
+/* The last error of this thread (the ValueError text of the reference,
+ * "op i (kind): ..." for an op: pipeline.nim:384-386).  Returns the length. */
+size_t pseg_last_error(char *buf, size_t n);
+
+/* --------------------------------------------------------------- device -- */
+
+/* Number of visible devices; 0 without a GPU.  Never fails. */
+int32_t pseg_device_count(void);
+
+/* ------------------------------------------------------------------ meta -- */
+
+/* Replaces ChunkMeta's isLive / liveSamples / isFailed (core/types.nim:
+ * 212-230) and correlation's row test (vibroseis/correlate.nim:442-443) with
+ * one host array: row_len[r] = clamp(num_samples[r] - first_sample, 0, cols)
+ * for a row the op uses, else -1.  status: 0 tsLive, 1 tsUnknown, 2 tsDead,
+ * 3 tsAuxiliary.  failed: ascending row indices (may be NULL when n_failed is
+ * 0).  for_correlation != 0: rows not dead and not failed (auxiliary rows
+ * included); else live or unknown rows that did not fail. */
+pseg_status pseg_meta_flatten(int32_t rows, int32_t cols, int64_t first_sample,
+                              const int64_t *num_samples, const uint8_t *status,
+                              const int64_t *failed, int32_t n_failed,
+                              int32_t for_correlation, int32_t *row_len);
+
+/* -------------------------------------------------------------- lengths -- */
+
+/* nextFastLen (fft/plan.nim:113-118): the smallest 2·3·5-smooth length
+ * >= max(n, 1), even unless even == 0. */
+int64_t pseg_next_fast_len(int64_t n, int32_t even);
+
+/* impulseLength (filter/design.nim:353-366), on the host in float64. */
+pseg_status pseg_impulse_length(const pseg_filter_spec *spec, double dt_ms, int64_t *length);
+
+/* response (filter/design.nim:236-255). */
+double pseg_response(const pseg_filter_spec *spec, double f_hz, double dt_ms);
+
+/* ------------------------------------------------------------- pipeline -- */
+
+/* Pipeline.compile (pipeline.nim:466-523): validates every op for blocks of up
+ * to `samples` samples at dt_ms and builds the device tables (twiddles, gains,
+ * pad tapers, AGC tables) on the current device.  record_samples: 0, or the
+ * trace length a window of `samples` is cut from.  Kinds outside the GPU path
+ * (mutes, IIR, gain curves, calibration) give PSEG_E_UNSUPPORTED. */
+pseg_status pseg_plan_compile(const pseg_op *ops, int32_t n_ops, int32_t dtype, double dt_ms,
+                              int32_t samples, int32_t record_samples, pseg_plan **out);
+pseg_status pseg_plan_destroy(pseg_plan *plan);
+pseg_status pseg_plan_info_get(const pseg_plan *plan, pseg_plan_info *out);
+
+/* Pipeline.requiredMargin(dtMs) without compiling (pipeline.nim:525-548). */
+pseg_status pseg_required_margin(const pseg_op *ops, int32_t n_ops, double dt_ms,
+                                 int32_t *margin);
+
+/* decodeWindow (pipeline.nim:550-560): the samples to decode for a zoom that
+ * shows shown_first .. shown_last (inclusive) of traces trace_len long. */
+void pseg_decode_window(int32_t shown_first, int32_t shown_last, int64_t margin,
+                        int32_t trace_len, int32_t *first, int32_t *last);
+
+/* The device workspace apply needs for `rows` x `cols` (ws.reserve(plan),
+ * pipeline.nim:599-602): 0 unless the plan has a per-panel normalization. */
+pseg_status pseg_plan_workspace_size(const pseg_plan *plan, int32_t rows, int32_t cols,
+                                     size_t *bytes);
+
+/* Plan.apply(src, dst, meta, ws) (pipeline.nim:736-772) and
+ * ProcessPool.apply (core/pool.nim:277-308): runs the plan on every row of
+ * src into dst; rows with row_len < 0 are copied unchanged, samples past a
+ * row's live length too.  src == dst runs in place.  cols <= plan samples.
+ * ws: device memory of pseg_plan_workspace_size bytes (NULL when 0). */
+pseg_status pseg_plan_apply(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
+                            int32_t cols, const int32_t *row_len, void *ws, size_t ws_bytes,
+                            void *stream);
+
+/* ---------------------------------------------------------- correlation -- */
+
+/* What a correlation plan reports (vibroseis/correlate.nim:162-188). */
+typedef struct pseg_corr_info {
+  int32_t dtype;
+  int32_t nfft;
+  int32_t shift;           /* whole samples of the pilot's start time */
+  int32_t out_samples;
+  int32_t pilot_samples;
+  int32_t record_samples;
+  double dt_ms;
+  double scale;            /* normalization factor folded into the multiplier */
+  size_t lds_bytes;
+} pseg_corr_info;
+
+/* initCorrelationPlan (vibroseis/correlate.nim:82-160): pilot samples in
+ * float64 (Pilot[T].samples converted), the pilot's own interval and start
+ * time, records of up to record_samples at dt_ms, out_samples columns (0: the
+ * listen time), mode 0 cmCorrelate / 1 cmDeconvolve, eps_percent, and
+ * normalize.  The pilot spectrum is computed in float64 on the host; the
+ * per-bin multiplier stays on the device. */
+pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t pilot_len,
+                              double pilot_dt_ms, double pilot_t0_ms, double dt_ms,
+                              int32_t record_samples, int32_t out_samples, int32_t mode,
+                              double eps_percent, int32_t normalize, pseg_corr_plan **out);
+pseg_status pseg_corr_destroy(pseg_corr_plan *plan);
+pseg_status pseg_corr_info_get(const pseg_corr_plan *plan, pseg_corr_info *out);
+
+/* correlateInto(plan, src, dst, meta, ws) (vibroseis/correlate.nim:421-464)
+ * and ProcessPool.correlateInto (core/pool.nim:329-347): src is
+ * [rows, src_cols <= record_samples], dst [rows, out_samples]; row_len from
+ * pseg_meta_flatten(for_correlation = 1); rows with -1 give zeros. */
+pseg_status pseg_corr_apply(const pseg_corr_plan *plan, const void *src, int32_t rows,
+                            int32_t src_cols, const int32_t *row_len, void *dst,
+                            int32_t dst_cols, void *stream);
+
+/* ------------------------------------------------------ sweep and pilot -- */
+
+/* linearSweep + generatePilot (vibroseis/sweep.nim:185-192, 292-349;
+ * pilot.nim:115-119) with tpCos2 end tapers (vibroseis/taper.nim:125-179):
+ * n must be samplesInLength(length_ms, dt_ms) (core/types.nim:147-166). */
+pseg_status pseg_linear_sweep(double f1_hz, double f2_hz, double length_ms, double dt_ms,
+                              double taper_start_ms, double taper_end_ms,
+                              double start_phase_deg, double amplitude, double *out,
+                              int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

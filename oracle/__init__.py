@@ -1,0 +1,434 @@
+"""ctypes binding of the oracle (TEST INFRASTRUCTURE ONLY).
+
+The oracle is a C restatement of processSEG's panel hot path (oracle.c,
+or_tmpl.h).  Only tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg
+of bench.py use it, as the checker; the product (processseg_amd) never imports
+it.  Functions take numpy arrays and op lists built with processseg_amd's op
+builders (any object with the same attribute names works).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+MAX_NOTCH = 8
+
+
+class OrSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("zero_dc", C.c_int32), ("n_notch", C.c_int32),
+                ("reserved", C.c_int32),
+                ("f1", C.c_double), ("f2", C.c_double), ("f3", C.c_double), ("f4", C.c_double),
+                ("low_hz", C.c_double), ("low_slope", C.c_double),
+                ("high_hz", C.c_double), ("high_slope", C.c_double),
+                ("notch_hz", C.c_double * MAX_NOTCH), ("notch_width", C.c_double * MAX_NOTCH)]
+
+
+class OrOp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("trend", C.c_int32), ("mode", C.c_int32),
+                ("stat", C.c_int32), ("scope", C.c_int32), ("reserved", C.c_int32),
+                ("value", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
+                ("window_ms", C.c_double), ("target", C.c_double), ("epsilon", C.c_double),
+                ("percentile", C.c_double), ("spec", OrSpec)]
+
+
+class OrFilterInfo(C.Structure):
+    _fields_ = [("margin", C.c_int64), ("pad", C.c_int64), ("nfft", C.c_int64),
+                ("reflect", C.c_int32), ("zero_dc", C.c_int32), ("dc_gain", C.c_double)]
+
+
+class OrPlanInfo(C.Structure):
+    _fields_ = [("margin", C.c_int64), ("parts", C.c_int32), ("n_steps", C.c_int32),
+                ("nfft", C.c_int64 * 16), ("half", C.c_int64 * 16)]
+
+
+class OrCorrInfo(C.Structure):
+    _fields_ = [("nfft", C.c_int64), ("shift", C.c_int64), ("out_samples", C.c_int64),
+                ("pilot_samples", C.c_int64), ("record_samples", C.c_int64),
+                ("scale", C.c_double)]
+
+
+class OracleError(ValueError):
+    pass
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with make (gcc)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", HERE],
+                       check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        E = [C.c_char_p, C.c_int]
+        L.or_is_fast_len.argtypes = [C.c_int64]
+        L.or_next_fast_len.argtypes = [C.c_int64, C.c_int]
+        L.or_next_fast_len.restype = C.c_int64
+        L.or_factorize.argtypes = [C.c_int64, i32p]
+        for t, p in (("f32", f32p), ("f64", f64p)):
+            getattr(L, f"or_rfft_{t}").argtypes = [p, C.c_int64, p, p]
+            getattr(L, f"or_irfft_{t}").argtypes = [p, p, C.c_int64, p]
+            getattr(L, f"or_demean_{t}").argtypes = [p, C.c_int64]
+            getattr(L, f"or_agc_{t}").argtypes = [p, C.c_int64, C.c_double, C.c_double,
+                                                 C.c_int32, C.c_double, C.c_double]
+            getattr(L, f"or_percentile_abs_{t}").argtypes = [p, C.c_int64, C.c_double]
+            getattr(L, f"or_percentile_abs_{t}").restype = C.c_double
+            getattr(L, f"or_panel_stats_{t}").argtypes = [p, C.c_int32, C.c_int32, i32p,
+                                                         C.c_double, f64p]
+            getattr(L, f"or_filter_{t}").argtypes = [p, C.c_int64, C.POINTER(OrSpec),
+                                                    C.c_double, C.c_int64, C.c_int32] + E
+            getattr(L, f"or_pipeline_apply_{t}").argtypes = [
+                C.POINTER(OrOp), C.c_int32, C.c_double, C.c_int64, C.c_int64, p, p,
+                C.c_int32, C.c_int32, i32p] + E
+            getattr(L, f"or_correlate_{t}").argtypes = [
+                f64p, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64,
+                C.c_int32, C.c_double, C.c_int32, p, C.c_int32, C.c_int32, i32p, p,
+                C.c_int32] + E
+        L.or_response.argtypes = [C.POINTER(OrSpec), C.c_double, C.c_double]
+        L.or_response.restype = C.c_double
+        L.or_narrowest_ramp_hz.argtypes = [C.POINTER(OrSpec)]
+        L.or_narrowest_ramp_hz.restype = C.c_double
+        L.or_impulse_length.argtypes = [C.POINTER(OrSpec), C.c_double,
+                                        C.POINTER(C.c_int64)] + E
+        L.or_filter_info_of.argtypes = [C.POINTER(OrSpec), C.c_double, C.c_int64, C.c_int32,
+                                        C.POINTER(OrFilterInfo)] + E
+        L.or_meta_flatten.argtypes = [C.c_int32, C.c_int32, C.c_int64, i64p, u8p, i64p,
+                                      C.c_int32, C.c_int, i32p]
+        L.or_plan_info_of.argtypes = [C.POINTER(OrOp), C.c_int32, C.c_int32, C.c_double,
+                                      C.c_int64, C.c_int64, C.POINTER(OrPlanInfo)] + E
+        L.or_corr_info_of.argtypes = [f64p, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                      C.c_int64, C.c_int64, C.c_int32, C.c_double, C.c_int32,
+                                      C.POINTER(OrCorrInfo)] + E
+        L.or_samples_in_length.argtypes = [C.c_double, C.c_double]
+        L.or_samples_in_length.restype = C.c_int64
+        L.or_linear_sweep.argtypes = [C.c_double] * 8 + [f64p, C.c_int64] + E
+        L.or_pool_create.argtypes = [C.c_int32]
+        L.or_pool_create.restype = C.c_void_p
+        L.or_pool_destroy.argtypes = [C.c_void_p]
+        L.or_pool_apply_f32.argtypes = [C.c_void_p, C.POINTER(OrOp), C.c_int32, C.c_double,
+                                        C.c_int64, f32p, f32p, C.c_int32, C.c_int32, i32p,
+                                        C.c_int32] + E
+        _lib = L
+    return _lib
+
+
+def _err():
+    return C.create_string_buffer(512)
+
+
+def _check(rc, err):
+    if rc != 0:
+        raise OracleError(err.value.decode())
+
+
+def _sfx(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "f32"
+    if dt == np.float64:
+        return "f64"
+    raise TypeError(f"sample type must be float32 or float64, got {dt}")
+
+
+# ------------------------------------------------------------------ specs --
+
+def spec_struct(spec) -> OrSpec:
+    """An OrSpec from a filter spec object (processseg_amd.FilterSpec)."""
+    s = OrSpec()
+    s.kind = int(spec.kind)
+    s.zero_dc = int(bool(spec.zero_dc))
+    notches = list(spec.notches)
+    if len(notches) > MAX_NOTCH:
+        raise ValueError("too many notches")
+    s.n_notch = len(notches)
+    s.f1, s.f2, s.f3, s.f4 = spec.f1, spec.f2, spec.f3, spec.f4
+    s.low_hz, s.low_slope = spec.low_hz, spec.low_slope
+    s.high_hz, s.high_slope = spec.high_hz, spec.high_slope
+    for i, (f, w) in enumerate(notches):
+        s.notch_hz[i] = f
+        s.notch_width[i] = w
+    return s
+
+
+def op_struct(op) -> OrOp:
+    o = OrOp()
+    o.kind = int(op.kind)
+    o.trend = int(getattr(op, "trend", 2))
+    o.mode = int(getattr(op, "mode", 0))
+    o.stat = int(getattr(op, "stat", 0))
+    o.scope = int(getattr(op, "scope", 0))
+    o.value = float(getattr(op, "value", 0.0))
+    o.lo = float(getattr(op, "lo", 0.0))
+    o.hi = float(getattr(op, "hi", 0.0))
+    o.window_ms = float(getattr(op, "window_ms", 0.0))
+    o.target = float(getattr(op, "target", 1.0))
+    o.epsilon = float(getattr(op, "epsilon", 1e-9))
+    o.percentile = float(getattr(op, "percentile", 99.0))
+    spec = getattr(op, "spec", None)
+    if spec is not None:
+        o.spec = spec_struct(spec)
+    return o
+
+
+def _ops_array(ops):
+    arr = (OrOp * max(1, len(ops)))()
+    for i, op in enumerate(ops):
+        arr[i] = op_struct(op)
+    return arr
+
+
+# ----------------------------------------------------------------- lengths --
+
+def is_fast_len(n: int) -> bool:
+    return bool(lib().or_is_fast_len(n))
+
+
+def next_fast_len(n: int, even: bool = True) -> int:
+    return int(lib().or_next_fast_len(n, 1 if even else 0))
+
+
+def factorize(n: int) -> list:
+    r = np.zeros(64, np.int32)
+    c = lib().or_factorize(n, r)
+    return [int(v) for v in r[:c]]
+
+
+# --------------------------------------------------------------------- FFT --
+
+def rfft(x: np.ndarray):
+    x = np.ascontiguousarray(x)
+    n = x.shape[0]
+    re = np.zeros(n // 2 + 1, x.dtype)
+    im = np.zeros(n // 2 + 1, x.dtype)
+    if getattr(lib(), f"or_rfft_{_sfx(x.dtype)}")(x, n, re, im):
+        raise OracleError(f"{n} is not a 2·3·5-smooth length")
+    return re, im
+
+
+def irfft(re: np.ndarray, im: np.ndarray, n: int):
+    re = np.ascontiguousarray(re)
+    im = np.ascontiguousarray(im, dtype=re.dtype)
+    x = np.zeros(n, re.dtype)
+    if getattr(lib(), f"or_irfft_{_sfx(re.dtype)}")(re, im, n, x):
+        raise OracleError(f"{n} is not a 2·3·5-smooth length")
+    return x
+
+
+# ------------------------------------------------------------------ design --
+
+def response(spec, f: float, dt_ms: float) -> float:
+    s = spec_struct(spec)
+    return float(lib().or_response(C.byref(s), f, dt_ms))
+
+
+def narrowest_ramp_hz(spec) -> float:
+    s = spec_struct(spec)
+    return float(lib().or_narrowest_ramp_hz(C.byref(s)))
+
+
+def impulse_length(spec, dt_ms: float) -> int:
+    s = spec_struct(spec)
+    L = C.c_int64()
+    e = _err()
+    _check(lib().or_impulse_length(C.byref(s), dt_ms, C.byref(L), e, 512), e)
+    return int(L.value)
+
+
+def filter_info(spec, dt_ms: float, samples: int, trend: int = 2) -> dict:
+    s = spec_struct(spec)
+    out = OrFilterInfo()
+    e = _err()
+    _check(lib().or_filter_info_of(C.byref(s), dt_ms, samples, trend, C.byref(out), e, 512), e)
+    return dict(margin=out.margin, pad=out.pad, nfft=out.nfft, reflect=bool(out.reflect),
+                zero_dc=bool(out.zero_dc), dc_gain=out.dc_gain)
+
+
+# -------------------------------------------------------------- kernels --
+
+def filter_trace(x: np.ndarray, spec, dt_ms: float, samples: int, trend: int = 2):
+    y = np.ascontiguousarray(x).copy()
+    s = spec_struct(spec)
+    e = _err()
+    _check(getattr(lib(), f"or_filter_{_sfx(y.dtype)}")(y, y.shape[0], C.byref(s), dt_ms,
+                                                         samples, trend, e, 512), e)
+    return y
+
+
+def demean(x: np.ndarray):
+    y = np.ascontiguousarray(x).copy()
+    getattr(lib(), f"or_demean_{_sfx(y.dtype)}")(y, y.shape[0])
+    return y
+
+
+def agc(x: np.ndarray, window_ms: float, dt_ms: float, mode: int = 0, target: float = 1.0,
+        epsilon: float = 1e-9):
+    y = np.ascontiguousarray(x).copy()
+    rc = getattr(lib(), f"or_agc_{_sfx(y.dtype)}")(y, y.shape[0], window_ms, dt_ms, mode,
+                                                    target, epsilon)
+    if rc:
+        raise OracleError("invalid AGC parameters")
+    return y
+
+
+def percentile_abs(x: np.ndarray, p: float) -> float:
+    x = np.ascontiguousarray(x)
+    return float(getattr(lib(), f"or_percentile_abs_{_sfx(x.dtype)}")(x, x.shape[0], p))
+
+
+def panel_stats(x: np.ndarray, row_len: np.ndarray, clip_percentile: float = 99.0) -> dict:
+    x = np.ascontiguousarray(x)
+    out = np.zeros(5)
+    getattr(lib(), f"or_panel_stats_{_sfx(x.dtype)}")(
+        x, x.shape[0], x.shape[1], np.ascontiguousarray(row_len, np.int32), clip_percentile, out)
+    return dict(rms=out[0], max_abs=out[1], clip=out[2], rows=int(out[3]), samples=int(out[4]))
+
+
+# ---------------------------------------------------------------- meta --
+
+def meta_flatten(cols: int, num_samples, status, failed=(), first_sample: int = 0,
+                 used_for_corr: bool = False) -> np.ndarray:
+    ns = np.ascontiguousarray(num_samples, np.int64)
+    st = np.ascontiguousarray(status, np.uint8)
+    fl = np.ascontiguousarray(np.array(sorted(failed), np.int64).reshape(-1))
+    if fl.size == 0:
+        fl = np.zeros(1, np.int64)
+        nfl = 0
+    else:
+        nfl = int(fl.size)
+    out = np.zeros(ns.shape[0], np.int32)
+    lib().or_meta_flatten(ns.shape[0], cols, first_sample, ns, st, fl, nfl,
+                          1 if used_for_corr else 0, out)
+    return out
+
+
+# ------------------------------------------------------------ pipeline --
+
+def plan_info(ops, dtype, dt_ms: float, samples: int, record_samples: int = 0) -> dict:
+    arr = _ops_array(ops)
+    out = OrPlanInfo()
+    e = _err()
+    dt = 0 if np.dtype(dtype) == np.float32 else 1
+    _check(lib().or_plan_info_of(arr, len(ops), dt, dt_ms, samples, record_samples,
+                                 C.byref(out), e, 512), e)
+    return dict(margin=out.margin, parts=out.parts, n_steps=out.n_steps,
+                nfft=[out.nfft[i] for i in range(out.n_steps)],
+                half=[out.half[i] for i in range(out.n_steps)])
+
+
+def pipeline_apply(ops, src: np.ndarray, row_len, dt_ms: float, samples: int | None = None,
+                   record_samples: int = 0, dst: np.ndarray | None = None) -> np.ndarray:
+    """compile + apply: src (rows x cols, float32 or float64) -> a new array."""
+    src = np.ascontiguousarray(src)
+    rows, cols = src.shape
+    out = np.empty_like(src) if dst is None else dst
+    arr = _ops_array(ops)
+    e = _err()
+    rl = np.ascontiguousarray(row_len, np.int32)
+    _check(getattr(lib(), f"or_pipeline_apply_{_sfx(src.dtype)}")(
+        arr, len(ops), dt_ms, samples if samples is not None else cols, record_samples,
+        src, out, rows, cols, rl, e, 512), e)
+    return out
+
+
+class Pool:
+    """The ProcessPool clone: persistent worker threads."""
+
+    def __init__(self, threads: int):
+        self.threads = int(threads)
+        self._p = lib().or_pool_create(self.threads)
+
+    def apply_f32(self, ops, src, dst, row_len, dt_ms, samples=None, repeats: int = 1):
+        arr = _ops_array(ops)
+        e = _err()
+        rows, cols = src.shape
+        _check(lib().or_pool_apply_f32(self._p, arr, len(ops), dt_ms,
+                                       samples if samples is not None else cols, src, dst,
+                                       rows, cols, np.ascontiguousarray(row_len, np.int32),
+                                       repeats, e, 512), e)
+        return dst
+
+    def close(self):
+        if self._p:
+            lib().or_pool_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------- correlation --
+
+def corr_info(pilot, dt_ms, record_samples, out_samples=0, mode=0, eps_percent=1.0,
+              normalize=True, pilot_dt_ms=None, pilot_t0_ms=0.0) -> dict:
+    p = np.ascontiguousarray(pilot, np.float64)
+    out = OrCorrInfo()
+    e = _err()
+    _check(lib().or_corr_info_of(p, p.shape[0], dt_ms if pilot_dt_ms is None else pilot_dt_ms,
+                                 pilot_t0_ms, dt_ms, record_samples, out_samples, mode,
+                                 eps_percent, 1 if normalize else 0, C.byref(out), e, 512), e)
+    return dict(nfft=out.nfft, shift=out.shift, out_samples=out.out_samples,
+                pilot_samples=out.pilot_samples, record_samples=out.record_samples,
+                scale=out.scale)
+
+
+def correlate(src: np.ndarray, row_len, pilot, dt_ms, record_samples, out_samples=0, mode=0,
+              eps_percent=1.0, normalize=True, pilot_dt_ms=None, pilot_t0_ms=0.0):
+    src = np.ascontiguousarray(src)
+    info = corr_info(pilot, dt_ms, record_samples, out_samples, mode, eps_percent, normalize,
+                     pilot_dt_ms, pilot_t0_ms)
+    rows, cols = src.shape
+    dst = np.zeros((rows, info["out_samples"]), src.dtype)
+    p = np.ascontiguousarray(pilot, np.float64)
+    e = _err()
+    _check(getattr(lib(), f"or_correlate_{_sfx(src.dtype)}")(
+        p, p.shape[0], dt_ms if pilot_dt_ms is None else pilot_dt_ms, pilot_t0_ms, dt_ms,
+        record_samples, out_samples, mode, eps_percent, 1 if normalize else 0, src, rows, cols,
+        np.ascontiguousarray(row_len, np.int32), dst, dst.shape[1], e, 512), e)
+    return dst
+
+
+# ------------------------------------------------------------- sweeps --
+
+def samples_in_length(length_ms: float, dt_ms: float) -> int:
+    return int(lib().or_samples_in_length(length_ms, dt_ms))
+
+
+def linear_sweep(f1, f2, length_ms, dt_ms, taper_start_ms=0.0, taper_end_ms=0.0,
+                 start_phase_deg=0.0, amplitude=1.0) -> np.ndarray:
+    n = samples_in_length(length_ms, dt_ms)
+    x = np.zeros(max(n, 1), np.float64)
+    e = _err()
+    _check(lib().or_linear_sweep(f1, f2, length_ms, dt_ms, taper_start_ms, taper_end_ms,
+                                 start_phase_deg, amplitude, x, n, e, 512), e)
+    return x[:n]
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = math.sqrt(float(np.sum((a - b) ** 2)))
+    r = math.sqrt(float(np.sum(b ** 2)))
+    return d / r if r > 0 else d

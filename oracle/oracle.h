@@ -1,0 +1,194 @@
+/*
+ * Oracle: a CPU restatement of processSEG's compiled-pipeline hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (processseg_amd/, the
+ * C-ABI library) links, loads or calls this code.  Only tests/, the smoke
+ * check in __graft_entry__.py and bench.py's cpu_baseline leg use it, as the
+ * checker and as the CPU baseline ("port").
+ *
+ * The reference is Nim (processSEG 0.14.1) and cannot be compiled here: no Nim
+ * toolchain, and its dependencies (parseSEG, Arraymancer) are not vendored.
+ * This file restates its algorithm in C, following the reference file by file
+ * (each function cites the file:line it follows).  Every kernel exists twice,
+ * with the sample type T = float ("FP32 mirror": float64 tables and sums, T
+ * butterflies, exactly the reference's precisions) and T = double ("FP64
+ * oracle").  Built with -ffp-contract=off, so no multiply-add is fused.
+ *
+ * Parity pins (see tests/test_oracle_*.py): nextFastLen table and FFT vs a
+ * naive DFT and numpy.fft (tfftplan.nim); response closed forms and
+ * impulse-length known answers L=110 / nfft=2250, L=881 / nfft=3840
+ * (bench/README.md:116-117); AGC vs brute-force window sums (tgain.nim:
+ * 105-121), h=125 / window 251 (tgain.nim:160); percentile vs sort
+ * (tstats.nim:15); correlation vs a direct sum and nfft=16200
+ * (tcorrelate.nim:15-64, bench/README.md:369); decodeWindow table
+ * (tpipeline.nim:320-324); filter invariants (tfilterapply.nim).
+ */
+#ifndef PSEG_ORACLE_H
+#define PSEG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_NOTCH 8
+
+/* FilterSpec (filter/design.nim:78-90).  kind 0 = rkTrapezoid, 1 = rkButterworth. */
+typedef struct or_spec {
+  int32_t kind;
+  int32_t zero_dc;
+  int32_t n_notch;
+  int32_t reserved;
+  double f1, f2, f3, f4;                  /* trapezoid corners, Hz (Inf: none) */
+  double low_hz, low_slope;               /* Butterworth low cut (0: none) */
+  double high_hz, high_slope;             /* Butterworth high cut (Inf: none) */
+  double notch_hz[OR_MAX_NOTCH];
+  double notch_width[OR_MAX_NOTCH];
+} or_spec;
+
+/* Op (pipeline.nim:158-174), as plain data.  kind follows OpKind
+ * (pipeline.nim:92-108): 0 demean, 1 detrend, 2 scale, 3 gainDb, 4 clip,
+ * 5 polarity, 8 filter, 10 agc, 11 normalize.  Other kinds are refused. */
+typedef struct or_op {
+  int32_t kind;
+  int32_t trend;      /* filter: 0 trNone, 1 trMean, 2 trLinear */
+  int32_t mode;       /* agc: 0 agcRms, 1 agcMeanAbs */
+  int32_t stat;       /* normalize: 0 normMax, 1 normRms, 2 normPercentile */
+  int32_t scope;      /* normalize: 0 perTrace, 1 perPanel */
+  int32_t reserved;
+  double value;       /* scale: factor; gainDb: dB */
+  double lo, hi;      /* clip */
+  double window_ms;   /* agc */
+  double target;      /* agc, normalize */
+  double epsilon;     /* agc */
+  double percentile;  /* normalize */
+  or_spec spec;       /* filter */
+} or_op;
+
+/* ------------------------------------------------------------- lengths -- */
+int or_is_fast_len(int64_t n);
+int64_t or_next_fast_len(int64_t n, int even);
+int or_factorize(int64_t n, int32_t *radices); /* returns the count */
+
+/* ----------------------------------------------------------------- FFT -- */
+/* Real forward transform (unscaled) and inverse (scaled 1/n) of one trace. */
+int or_rfft_f32(const float *x, int64_t n, float *re, float *im);
+int or_rfft_f64(const double *x, int64_t n, double *re, double *im);
+int or_irfft_f32(const float *re, const float *im, int64_t n, float *x);
+int or_irfft_f64(const double *re, const double *im, int64_t n, double *x);
+
+/* -------------------------------------------------------------- design -- */
+double or_response(const or_spec *spec, double f, double dt_ms);
+double or_narrowest_ramp_hz(const or_spec *spec);
+/* 0 ok; -1 invalid spec (message in err). */
+int or_impulse_length(const or_spec *spec, double dt_ms, int64_t *L, char *err, int errlen);
+
+/* Filter plan decisions (filter/apply.nim:118-151). */
+typedef struct or_filter_info {
+  int64_t margin, pad, nfft;
+  int32_t reflect, zero_dc;
+  double dc_gain;
+} or_filter_info;
+int or_filter_info_of(const or_spec *spec, double dt_ms, int64_t samples, int32_t trend,
+                      or_filter_info *out, char *err, int errlen);
+
+/* ---------------------------------------------------------------- meta -- */
+/* isLive / liveSamples (core/types.nim:212-230) flattened to one int32 per
+ * row: the live length, or -1 for a row that is not live.  status: 0 tsLive,
+ * 1 tsUnknown, 2 tsDead, 3 tsAuxiliary.  used_for_corr != 0 gives the rows
+ * correlation uses instead (vibroseis/correlate.nim:442-443: not dead and not
+ * failed). */
+int or_meta_flatten(int32_t rows, int32_t cols, int64_t first_sample,
+                    const int64_t *num_samples, const uint8_t *status,
+                    const int64_t *failed, int32_t n_failed, int used_for_corr,
+                    int32_t *row_len);
+
+/* ------------------------------------------------------------ pipeline -- */
+typedef struct or_plan_info {
+  int64_t margin;
+  int32_t parts;
+  int32_t n_steps;
+  int64_t nfft[16];   /* per step: the FFT length of a filter step, else 0 */
+  int64_t half[16];   /* per step: the AGC half window h, else 0 */
+} or_plan_info;
+
+int or_plan_info_of(const or_op *ops, int32_t n_ops, int32_t dtype, double dt_ms,
+                    int64_t samples, int64_t record_samples, or_plan_info *out,
+                    char *err, int errlen);
+/* compile + apply (pipeline.nim:466-523, 725-772).  src may equal dst (in
+ * place).  row_len as or_meta_flatten.  Returns 0, or -1 with a message. */
+int or_pipeline_apply_f32(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
+                          int64_t record_samples, const float *src, float *dst,
+                          int32_t rows, int32_t cols, const int32_t *row_len,
+                          char *err, int errlen);
+int or_pipeline_apply_f64(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
+                          int64_t record_samples, const double *src, double *dst,
+                          int32_t rows, int32_t cols, const int32_t *row_len,
+                          char *err, int errlen);
+/* The same through a ProcessPool clone (core/pool.nim:145-308): `threads`
+ * persistent workers taking runs of ~256 KiB of rows (multiples of 8) from an
+ * atomic counter, one workspace each, every step on a tile while it is in
+ * cache.  Bit-identical to the single-thread form.  Used as the CPU baseline. */
+typedef struct or_pool or_pool;
+or_pool *or_pool_create(int32_t threads);
+void or_pool_destroy(or_pool *pool);
+int or_pool_apply_f32(or_pool *pool, const or_op *ops, int32_t n_ops, double dt_ms,
+                      int64_t samples, const float *src, float *dst, int32_t rows,
+                      int32_t cols, const int32_t *row_len, int32_t repeats,
+                      char *err, int errlen);
+
+/* ------------------------------------------------------- single kernels -- */
+int or_demean_f32(float *x, int64_t n);
+int or_demean_f64(double *x, int64_t n);
+int or_agc_f32(float *x, int64_t n, double window_ms, double dt_ms, int32_t mode,
+               double target, double epsilon);
+int or_agc_f64(double *x, int64_t n, double window_ms, double dt_ms, int32_t mode,
+               double target, double epsilon);
+double or_percentile_abs_f32(const float *x, int64_t n, double p);
+double or_percentile_abs_f64(const double *x, int64_t n, double p);
+/* panelStats (gain/stats.nim:253-304): out = {rms, maxAbs, clip, rows, samples} */
+int or_panel_stats_f32(const float *x, int32_t rows, int32_t cols, const int32_t *row_len,
+                       double clip_percentile, double *out);
+int or_panel_stats_f64(const double *x, int32_t rows, int32_t cols, const int32_t *row_len,
+                       double clip_percentile, double *out);
+/* Single-trace applyFilter (filter/apply.nim:340-363). */
+int or_filter_f32(float *x, int64_t n, const or_spec *spec, double dt_ms, int64_t samples,
+                  int32_t trend, char *err, int errlen);
+int or_filter_f64(double *x, int64_t n, const or_spec *spec, double dt_ms, int64_t samples,
+                  int32_t trend, char *err, int errlen);
+
+/* --------------------------------------------------------- correlation -- */
+typedef struct or_corr_info {
+  int64_t nfft, shift, out_samples, pilot_samples, record_samples;
+  double scale;
+} or_corr_info;
+int or_corr_info_of(const double *pilot, int64_t pilot_len, double pilot_dt_ms,
+                    double pilot_t0_ms, double dt_ms, int64_t record_samples,
+                    int64_t out_samples, int32_t mode, double eps_percent,
+                    int32_t normalize, or_corr_info *out, char *err, int errlen);
+int or_correlate_f32(const double *pilot, int64_t pilot_len, double pilot_dt_ms,
+                     double pilot_t0_ms, double dt_ms, int64_t record_samples,
+                     int64_t out_samples, int32_t mode, double eps_percent,
+                     int32_t normalize, const float *src, int32_t rows, int32_t src_cols,
+                     const int32_t *row_len, float *dst, int32_t dst_cols,
+                     char *err, int errlen);
+int or_correlate_f64(const double *pilot, int64_t pilot_len, double pilot_dt_ms,
+                     double pilot_t0_ms, double dt_ms, int64_t record_samples,
+                     int64_t out_samples, int32_t mode, double eps_percent,
+                     int32_t normalize, const double *src, int32_t rows, int32_t src_cols,
+                     const int32_t *row_len, double *dst, int32_t dst_cols,
+                     char *err, int errlen);
+
+/* ------------------------------------------------------ sweep / pilot -- */
+int64_t or_samples_in_length(double length_ms, double dt_ms);
+/* linearSweep + sweepInto + applyTaper(tpCos2) (vibroseis/sweep.nim:185-342,
+ * taper.nim:125-179), as float64 samples (generatePilot's default type). */
+int or_linear_sweep(double f1_hz, double f2_hz, double length_ms, double dt_ms,
+                    double taper_start_ms, double taper_end_ms, double start_phase_deg,
+                    double amplitude, double *out, int64_t n, char *err, int errlen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

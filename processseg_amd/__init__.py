@@ -1,0 +1,20 @@
+"""processseg_amd: processSEG's compiled-pipeline panel hot path on MI355X.
+
+The panel path (batched mixed-radix real FFT, zero-phase FFT filters with the
+reference's edge handling, demean, AGC, normalization and vibroseis
+correlation) runs in hand-written gfx950 kernels behind the C ABI of
+include/pseg.h (libpseg.so).  This package mirrors the reference's Nim API
+over that ABI: initPipeline / compile / PipelinePlan.apply and
+initCorrelationPlan / correlateInto.
+"""
+from ._lib import HipError, UnsupportedOp, WorkspaceError, build, lib  # noqa: F401
+from .pipeline import (  # noqa: F401
+    ChunkMeta, Corners, FilterSpec, Inf, NaN, Op, Pipeline, PipelinePlan, agcMeanAbs, agcOp,
+    agcRms, bandPass, bandPassOp, butterBandPass, butterHighCut, butterLowCut, clipOp,
+    decodeWindow, demeanOp, detrendOp, filterOp, float32, float64, gainDbOp, highCut, iirOp,
+    initPipeline, lowCut, nextFastLen, normalizeOp, normMax, normPercentile, normRms, perPanel,
+    perTrace, polarityOp, sameInterval, scaleOp, trLinear, trMean, trNone, tsAuxiliary, tsDead,
+    tsLive, tsUnknown)
+from .correlate import (  # noqa: F401
+    CorrelationPlan, Pilot, SweepSpec, SweepTaper, cmCorrelate, cmDeconvolve, generatePilot,
+    initCorrelationPlan, initSweepTaper, linearSweep, samplesInLength, sweepSamples)

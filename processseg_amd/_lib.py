@@ -1,0 +1,145 @@
+"""ctypes binding of libpseg.so (the C ABI of include/pseg.h).
+
+The library is built in-tree (``make -C processseg_amd``, or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no CPU fallback for the panel path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpseg.so")
+
+MAX_NOTCH = 8
+MAX_OPS = 16
+
+OK, E_INVALID, E_UNSUPPORTED, E_HIP, E_WORKSPACE, E_BUSY = range(6)
+
+
+class FilterSpecC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("zero_dc", C.c_int32), ("n_notch", C.c_int32),
+                ("reserved", C.c_int32),
+                ("f1", C.c_double), ("f2", C.c_double), ("f3", C.c_double), ("f4", C.c_double),
+                ("low_hz", C.c_double), ("low_slope", C.c_double),
+                ("high_hz", C.c_double), ("high_slope", C.c_double),
+                ("notch_hz", C.c_double * MAX_NOTCH), ("notch_width", C.c_double * MAX_NOTCH)]
+
+
+class OpC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("trend", C.c_int32), ("mode", C.c_int32),
+                ("stat", C.c_int32), ("scope", C.c_int32), ("reserved", C.c_int32),
+                ("value", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
+                ("window_ms", C.c_double), ("target", C.c_double), ("epsilon", C.c_double),
+                ("percentile", C.c_double), ("spec", FilterSpecC)]
+
+
+class PlanInfoC(C.Structure):
+    _fields_ = [("generation", C.c_uint64), ("dtype", C.c_int32), ("n_steps", C.c_int32),
+                ("parts", C.c_int32), ("samples", C.c_int32), ("record_samples", C.c_int32),
+                ("margin", C.c_int32), ("dt_ms", C.c_double),
+                ("step_kind", C.c_int32 * MAX_OPS), ("step_nfft", C.c_int32 * MAX_OPS),
+                ("step_pad", C.c_int32 * MAX_OPS), ("step_margin", C.c_int32 * MAX_OPS),
+                ("step_reflect", C.c_int32 * MAX_OPS), ("lds_bytes", C.c_size_t)]
+
+
+class CorrInfoC(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("nfft", C.c_int32), ("shift", C.c_int32),
+                ("out_samples", C.c_int32), ("pilot_samples", C.c_int32),
+                ("record_samples", C.c_int32), ("dt_ms", C.c_double), ("scale", C.c_double),
+                ("lds_bytes", C.c_size_t)]
+
+
+class UnsupportedOp(ValueError):
+    """An op outside the GPU path: keep the CPU plan for that flow."""
+
+
+class WorkspaceError(AssertionError):
+    """A workspace missing or smaller than the plan needs (the reference's
+    AssertionDefect for an unreserved Workspace)."""
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def build(force: bool = False) -> str:
+    """Compile libpseg.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    args = ["make", "-s", "-C", HERE]
+    if force:
+        args.insert(2, "-B")
+    subprocess.run(args, check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+SYMBOLS = [
+    "pseg_last_error", "pseg_device_count", "pseg_meta_flatten", "pseg_next_fast_len",
+    "pseg_impulse_length", "pseg_response", "pseg_plan_compile", "pseg_plan_destroy",
+    "pseg_plan_info_get", "pseg_required_margin", "pseg_decode_window",
+    "pseg_plan_workspace_size", "pseg_plan_apply", "pseg_corr_compile", "pseg_corr_destroy",
+    "pseg_corr_info_get", "pseg_corr_apply", "pseg_linear_sweep",
+]
+
+
+def lib():
+    """The loaded library; raises ImportError when it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it first (make -C processseg_amd "
+                          "or __graft_entry__.build()); the panel path has no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    L.pseg_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    L.pseg_last_error.restype = C.c_size_t
+    L.pseg_device_count.restype = i32
+    L.pseg_meta_flatten.argtypes = [i32, i32, i64, vp, vp, vp, i32, i32, vp]
+    L.pseg_next_fast_len.argtypes = [i64, i32]
+    L.pseg_next_fast_len.restype = i64
+    L.pseg_impulse_length.argtypes = [C.POINTER(FilterSpecC), f64, C.POINTER(i64)]
+    L.pseg_response.argtypes = [C.POINTER(FilterSpecC), f64, f64]
+    L.pseg_response.restype = f64
+    L.pseg_plan_compile.argtypes = [C.POINTER(OpC), i32, i32, f64, i32, i32, C.POINTER(vp)]
+    L.pseg_plan_destroy.argtypes = [vp]
+    L.pseg_plan_info_get.argtypes = [vp, C.POINTER(PlanInfoC)]
+    L.pseg_required_margin.argtypes = [C.POINTER(OpC), i32, f64, C.POINTER(i32)]
+    L.pseg_decode_window.argtypes = [i32, i32, i64, i32, C.POINTER(i32), C.POINTER(i32)]
+    L.pseg_decode_window.restype = None
+    L.pseg_plan_workspace_size.argtypes = [vp, i32, i32, C.POINTER(C.c_size_t)]
+    L.pseg_plan_apply.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_size_t, vp]
+    L.pseg_corr_compile.argtypes = [i32, vp, i32, f64, f64, f64, i32, i32, i32, f64, i32,
+                                    C.POINTER(vp)]
+    L.pseg_corr_destroy.argtypes = [vp]
+    L.pseg_corr_info_get.argtypes = [vp, C.POINTER(CorrInfoC)]
+    L.pseg_corr_apply.argtypes = [vp, vp, i32, i32, vp, vp, i32, vp]
+    L.pseg_linear_sweep.argtypes = [f64] * 8 + [vp, i32]
+    for name in SYMBOLS:
+        if not hasattr(L, name):
+            raise ImportError(f"{LIB_PATH} does not export {name}")
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    lib().pseg_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> None:
+    """Raise the exception the reference raises for a status."""
+    if rc == OK:
+        return
+    msg = last_error()
+    if rc == E_INVALID:
+        raise ValueError(msg)
+    if rc == E_UNSUPPORTED:
+        raise UnsupportedOp(msg)
+    if rc == E_WORKSPACE:
+        raise WorkspaceError(msg)
+    raise HipError(msg)

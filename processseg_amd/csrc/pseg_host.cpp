@@ -1,0 +1,1106 @@
+// Host side of libpseg: the plan compiler (float64 design and tables, as the
+// reference builds its plans) and the C ABI of include/pseg.h.
+//
+// A compiled plan is one device allocation: the DevPlan (or DevCorr) the
+// kernels read, followed by its tables (stage twiddles, split table, gains,
+// pad tapers, AGC tables, the pilot multiplier), built on the host and copied
+// once.  apply only launches kernels: it allocates nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/pseg.h"
+#include "pseg_internal.hpp"
+
+using namespace pseg;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+// ------------------------------------------------------------------ errors
+
+thread_local std::string g_error;
+
+pseg_status fail(pseg_status st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return st;
+}
+
+pseg_status hip_fail(hipError_t e, const char *what) {
+  return fail(PSEG_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+struct Invalid {
+  std::string msg;
+};
+
+[[noreturn]] void invalid(const char *fmt, ...) {
+  char buf[400];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  throw Invalid{buf};
+}
+
+bool finite_positive(double x) { return std::isfinite(x) && x > 0; }
+
+template <typename T>
+bool fits_in(double x) {
+  return finite_positive(x) && finite_positive((double)(T)x);
+}
+
+// ---------------------------------------------------------------- lengths
+
+bool is_fast(int64_t n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+int64_t next_fast(int64_t n, bool even) {
+  int64_t r = std::max<int64_t>(n, 1);
+  while (!is_fast(r) || (even && (r & 1))) r++;
+  return r;
+}
+
+// Stage radices of a complex length: 4s, then a 2, then 3s and 5s
+// (fft/plan.nim:120-132).
+std::vector<int> radices_of(int64_t m) {
+  std::vector<int> r;
+  while (m % 4 == 0) { r.push_back(4); m /= 4; }
+  if (m % 2 == 0) { r.push_back(2); m /= 2; }
+  while (m % 3 == 0) { r.push_back(3); m /= 3; }
+  while (m % 5 == 0) { r.push_back(5); m /= 5; }
+  return r;
+}
+
+FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  f.shift = l;
+  f.mul = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+  return f;
+}
+
+// ------------------------------------------------- float64 host transform
+//
+// A recursive decimation-in-time complex FFT over std::complex<double>, for
+// plan building only (impulse lengths, the pilot spectrum).  Twiddles come
+// from one table of exp(-2 pi i k / N) indexed exactly (k mod N).
+
+using cd = std::complex<double>;
+
+struct HostFft {
+  int64_t N;
+  std::vector<cd> W;
+  explicit HostFft(int64_t n) : N(n), W(n) {
+    for (int64_t k = 0; k < n; k++) {
+      double a = -2 * kPi * (double)k / (double)n;
+      W[k] = cd(std::cos(a), std::sin(a));
+    }
+  }
+  void rec(cd *out, const cd *in, int64_t n, int64_t stride, int64_t wstep) const {
+    if (n == 1) {
+      out[0] = in[0];
+      return;
+    }
+    int r = n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : 5;
+    int64_t q = n / r;
+    for (int j = 0; j < r; j++) rec(out + j * q, in + j * stride, q, stride * r, wstep * r);
+    cd tmp[5];
+    for (int64_t k = 0; k < q; k++) {
+      for (int j = 0; j < r; j++) tmp[j] = out[j * q + k];
+      for (int t = 0; t < r; t++) {
+        cd acc = 0;
+        for (int j = 0; j < r; j++) {
+          int64_t e = ((int64_t)j * (k + t * q)) % n;
+          acc += tmp[j] * W[(e * wstep) % N];
+        }
+        out[t * q + k] = acc;
+      }
+    }
+  }
+  // forward: X[k] = sum x[j] exp(-2 pi i jk / N)
+  std::vector<cd> forward(const std::vector<cd> &x) const {
+    std::vector<cd> y(N);
+    rec(y.data(), x.data(), N, 1, 1);
+    return y;
+  }
+};
+
+// irfft of a real, even spectrum given by bins 0 .. n/2 (imaginary parts 0),
+// scaled by 1/n.
+std::vector<double> inverse_real(const std::vector<double> &re, int64_t n) {
+  HostFft f(n);
+  std::vector<cd> X(n);
+  for (int64_t k = 0; k < n; k++) {
+    int64_t kk = k <= n / 2 ? k : n - k;
+    X[k] = cd(re[kk], 0.0);  // conj of a real value: itself
+  }
+  std::vector<cd> y = f.forward(X);  // conj(fft(conj(X))) of a real even X
+  std::vector<double> h(n);
+  for (int64_t j = 0; j < n; j++) h[j] = y[j].real() / (double)n;
+  return h;
+}
+
+// ------------------------------------------------------------ filter design
+
+bool has_low(const pseg_filter_spec &s) { return s.f2 > 0; }
+bool has_high(const pseg_filter_spec &s) { return s.f3 < kInf; }
+
+double warp(double f, double dt) {
+  double x = kPi * f * dt / 1000;
+  return x >= kPi / 2 ? kInf : std::tan(x);
+}
+
+double ramp(double x) {
+  if (x <= 0) return 0.0;
+  if (x >= 1) return 1.0;
+  double s = std::sin(kPi / 2 * x);
+  return s * s;
+}
+
+// response (filter/design.nim:236-255)
+double response(const pseg_filter_spec &s, double f, double dt) {
+  double r = 1.0;
+  if (s.kind == 0) {
+    if (has_low(s)) r *= ramp((f - s.f1) / (s.f2 - s.f1));
+    if (has_high(s)) r *= 1 - ramp((f - s.f3) / (s.f4 - s.f3));
+  } else {
+    double wf = warp(f, dt);
+    if (s.low_hz > 0)
+      r *= f <= 0 ? 0.0 : 1 / (1 + std::pow(warp(s.low_hz, dt) / wf, s.low_slope / 6));
+    if (s.high_hz < kInf)
+      r *= wf == kInf ? 0.0 : 1 / (1 + std::pow(wf / warp(s.high_hz, dt), s.high_slope / 6));
+  }
+  for (int i = 0; i < s.n_notch; i++)
+    r *= ramp(std::fabs(f - s.notch_hz[i]) / (s.notch_width[i] / 2));
+  return r;
+}
+
+// narrowestRampHz (filter/design.nim:267-282)
+double narrowest(const pseg_filter_spec &s) {
+  double r = kInf;
+  if (s.kind == 0) {
+    if (has_low(s)) r = std::min(r, s.f2 - s.f1);
+    if (has_high(s)) r = std::min(r, s.f4 - s.f3);
+  } else {
+    if (s.low_hz > 0) r = std::min(r, s.low_hz * std::min(1.0, 24 / s.low_slope) / 2);
+    if (s.high_hz < kInf) r = std::min(r, s.high_hz * std::min(1.0, 24 / s.high_slope) / 2);
+  }
+  for (int i = 0; i < s.n_notch; i++) r = std::min(r, s.notch_width[i] / 2);
+  return r;
+}
+
+// validate (filter/design.nim:100-173)
+void validate(const pseg_filter_spec &s, double dt) {
+  if (!(dt > 0 && dt < kInf)) invalid("dtMs must be finite and > 0, got %g", dt);
+  if (s.kind != 0 && s.kind != 1) invalid("unknown response kind %d", s.kind);
+  if (s.n_notch < 0 || s.n_notch > PSEG_MAX_NOTCH) invalid("at most %d notches", PSEG_MAX_NOTCH);
+  double nyq = 500 / dt;
+  if (s.kind == 0) {
+    for (double f : {s.f1, s.f2, s.f3, s.f4})
+      if (std::isnan(f)) invalid("a corner is NaN");
+    if (has_low(s) && !(s.f1 >= 0 && s.f1 < s.f2))
+      invalid("a low cut needs 0 <= f1 < f2 (a sloped ramp), got f1 = %g, f2 = %g", s.f1, s.f2);
+    if (!has_low(s) && (s.f1 != 0 || s.f2 != 0)) invalid("no low cut needs f1 = f2 = 0");
+    if (has_high(s) && !(s.f3 < s.f4 && s.f4 < kInf))
+      invalid("a high cut needs f3 < f4 < Inf (a sloped ramp), got f3 = %g, f4 = %g", s.f3, s.f4);
+    if (!has_high(s) && s.f4 != kInf) invalid("no high cut needs f3 = f4 = Inf");
+    if (has_low(s) && has_high(s) && !(s.f2 <= s.f3)) invalid("a band-pass needs f2 <= f3");
+    if (has_high(s) && s.f4 > nyq)
+      invalid("f4 = %g Hz is above Nyquist (%g Hz at %g ms)", s.f4, nyq, dt);
+    if (has_low(s) && s.f2 >= nyq) invalid("f2 = %g Hz is not below Nyquist (%g Hz)", s.f2, nyq);
+  } else {
+    if (!(s.low_hz >= 0 && s.low_hz < kInf))
+      invalid("a low-cut corner must be finite and >= 0 (0: none), got %g", s.low_hz);
+    if (!(s.high_hz > 0)) invalid("a high-cut corner must be > 0 (Inf: none), got %g", s.high_hz);
+    if (s.low_hz > 0 && !(s.low_slope > 0 && s.low_slope < kInf))
+      invalid("a slope must be finite and > 0 dB/octave, got %g", s.low_slope);
+    if (s.high_hz < kInf && !(s.high_slope > 0 && s.high_slope < kInf))
+      invalid("a slope must be finite and > 0 dB/octave, got %g", s.high_slope);
+    if (!(s.low_hz < s.high_hz))
+      invalid("the low-cut corner must be below the high-cut one, got %g and %g", s.low_hz,
+              s.high_hz);
+    if (s.high_hz < kInf && s.high_hz >= nyq)
+      invalid("the high-cut corner %g Hz is not below Nyquist (%g Hz)", s.high_hz, nyq);
+    if (s.low_hz >= nyq) invalid("the low-cut corner %g Hz is not below Nyquist (%g Hz)", s.low_hz, nyq);
+  }
+  for (int i = 0; i < s.n_notch; i++) {
+    if (!(s.notch_hz[i] > 0 && s.notch_hz[i] < kInf))
+      invalid("a notch frequency must be finite and > 0, got %g", s.notch_hz[i]);
+    if (!(s.notch_width[i] > 0 && s.notch_width[i] < kInf))
+      invalid("a notch width must be finite and > 0, got %g", s.notch_width[i]);
+    if (s.notch_hz[i] >= nyq)
+      invalid("a notch at %g Hz is not below Nyquist (%g Hz)", s.notch_hz[i], nyq);
+  }
+}
+
+// energyLength (filter/design.nim:288-298)
+int64_t energy_length(const std::vector<double> &h, double centre) {
+  const int64_t n = (int64_t)h.size();
+  double total = centre * centre;
+  for (int64_t j = 1; j < n; j++) total += h[j] * h[j];
+  double e = centre * centre;
+  int64_t L = 0;
+  while (e < 0.999 * total && L < n / 2) {
+    L++;
+    e += h[L] * h[L] + (n - L != L ? h[n - L] * h[n - L] : 0.0);
+  }
+  return L;
+}
+
+constexpr int64_t kMaxGrid = 1 << 24;
+
+// featureLength (filter/design.nim:300-325)
+int64_t feature_length(const pseg_filter_spec &s, double dt) {
+  double feature = narrowest(s);
+  if (feature == kInf) return 0;
+  double fs = 1000 / dt;
+  double start = std::ceil(16 * fs / feature);
+  int64_t n = start > (double)kMaxGrid ? kMaxGrid + 1
+                                       : next_fast(std::max<int64_t>(64, (int64_t)start), true);
+  for (;;) {
+    if (n > kMaxGrid)
+      invalid("the filter's impulse response is longer than %lld samples at %g ms "
+              "(narrowest ramp %g Hz)",
+              (long long)(kMaxGrid / 8), dt, feature);
+    std::vector<double> re(n / 2 + 1);
+    double df = 1000 / ((double)n * dt);
+    for (int64_t k = 0; k <= n / 2; k++) re[k] = response(s, (double)k * df, dt);
+    if (s.zero_dc) re[0] = 0;
+    std::vector<double> h = inverse_real(re, n);
+    int64_t L = std::max(energy_length(h, h[0]), energy_length(h, 1 - h[0]));
+    if (L <= n / 8) return L;
+    n = next_fast(2 * n, true);
+  }
+}
+
+// impulseLength + features (filter/design.nim:327-366)
+int64_t impulse_length(const pseg_filter_spec &s, double dt) {
+  validate(s, dt);
+  std::vector<pseg_filter_spec> feats;
+  pseg_filter_spec flat;
+  memset(&flat, 0, sizeof flat);
+  flat.kind = 0;
+  flat.f3 = flat.f4 = kInf;
+  if (s.kind == 0) {
+    if (has_low(s)) { pseg_filter_spec f = flat; f.f1 = s.f1; f.f2 = s.f2; feats.push_back(f); }
+    if (has_high(s)) { pseg_filter_spec f = flat; f.f3 = s.f3; f.f4 = s.f4; feats.push_back(f); }
+  } else {
+    pseg_filter_spec b;
+    memset(&b, 0, sizeof b);
+    b.kind = 1;
+    if (s.low_hz > 0) {
+      pseg_filter_spec f = b;
+      f.low_hz = s.low_hz; f.low_slope = s.low_slope; f.high_hz = kInf; f.high_slope = s.high_slope;
+      feats.push_back(f);
+    }
+    if (s.high_hz < kInf) {
+      pseg_filter_spec f = b;
+      f.low_hz = 0; f.low_slope = s.low_slope; f.high_hz = s.high_hz; f.high_slope = s.high_slope;
+      feats.push_back(f);
+    }
+  }
+  for (int i = 0; i < s.n_notch; i++) {
+    pseg_filter_spec f = flat;
+    f.n_notch = 1;
+    f.notch_hz[0] = s.notch_hz[i];
+    f.notch_width[0] = s.notch_width[i];
+    feats.push_back(f);
+  }
+  int64_t L = 0;
+  for (auto &f : feats) L = std::max(L, feature_length(f, dt));
+  return L;
+}
+
+// --------------------------------------------------------- device blobs
+
+// A host image of one device allocation: structs and tables appended with
+// 256-byte alignment; pointers inside are offsets until patched.
+struct Blob {
+  std::vector<char> bytes;
+  size_t add(const void *p, size_t n) {
+    size_t off = (bytes.size() + 255) / 256 * 256;
+    bytes.resize(off + n);
+    if (n) memcpy(bytes.data() + off, p, n);
+    return off;
+  }
+  template <typename V>
+  size_t add_vec(const std::vector<V> &v) {
+    return add(v.data(), v.size() * sizeof(V));
+  }
+};
+
+// Pointer fields hold blob offsets + 1 until the device base is known.
+template <typename P>
+void set_off(P &field, size_t off) {
+  field = (P)(uintptr_t)(off + 1);
+}
+template <typename P>
+void patch(P &field, char *base) {
+  uintptr_t v = (uintptr_t)field;
+  if (v) field = (P)(base + (v - 1));
+}
+
+// Stage twiddles and split table of a real FFT of n points (fft/plan.nim:
+// 136-160): float64, stored as T.
+template <typename T>
+void build_fft(int64_t n, DevFft &f, Blob &blob) {
+  memset(&f, 0, sizeof f);
+  f.n = (int32_t)n;
+  f.m = (int32_t)(n / 2);
+  std::vector<int> rad = radices_of(f.m);
+  if ((int)rad.size() > kMaxRadices) invalid("an FFT of %lld points has too many stages", (long long)n);
+  f.nrad = (int32_t)rad.size();
+  std::vector<T> tw;
+  int64_t cur = f.m, s = 1;
+  for (int i = 0; i < f.nrad; i++) {
+    int r = rad[i];
+    int64_t sub = cur / r;
+    f.rad[i] = r;
+    f.sub[i] = (int32_t)sub;
+    f.tw_off[i] = (int32_t)(tw.size() / 2);
+    f.sdiv[i] = make_fastdiv((uint32_t)s);
+    for (int64_t p = 0; p < sub; p++)
+      for (int k = 1; k < r; k++) {
+        double a = -2 * kPi * (double)((p * k) % cur) / (double)cur;
+        tw.push_back((T)std::cos(a));
+        tw.push_back((T)std::sin(a));
+      }
+    cur = sub;
+    s *= r;
+  }
+  if (tw.empty()) tw.assign(2, (T)0);
+  std::vector<T> sp(2 * (f.m + 1));
+  for (int64_t k = 0; k <= f.m; k++) {
+    double a = -2 * kPi * (double)k / (double)n;
+    sp[2 * k] = (T)std::cos(a);
+    sp[2 * k + 1] = (T)std::sin(a);
+  }
+  set_off(f.tw, blob.add_vec(tw));
+  set_off(f.split, blob.add_vec(sp));
+}
+
+void patch_fft(DevFft &f, char *base) {
+  patch(f.tw, base);
+  patch(f.split, base);
+}
+
+const char *op_name(int kind) {
+  static const char *names[] = {"opDemean", "opDetrend", "opScale", "opGainDb", "opClip",
+                                "opPolarity", "opMuteTop", "opMuteBottom", "opFilter",
+                                "opIir", "opAgc", "opNormalize", "opGainCurve",
+                                "opCalibrate", "opUncalibrate"};
+  return kind >= 0 && kind <= 14 ? names[kind] : "op?";
+}
+
+std::atomic<uint64_t> g_generation{0};
+
+}  // namespace
+
+// --------------------------------------------------------------- the plans
+
+struct pseg_plan {
+  int dtype;
+  double dt_ms;
+  int32_t samples, record, margin;
+  int n_steps;
+  int n_parts;
+  int part_a[kMaxSteps + 1], part_b[kMaxSteps + 1];
+  uint64_t generation;
+  DevPlan host;          // with device pointers
+  DevPlan *dev;          // the device copy (start of the allocation)
+  char *dev_base;
+  size_t step_lds[kMaxSteps];
+  int32_t step_kind[kMaxSteps];
+  int32_t step_nfft[kMaxSteps], step_pad[kMaxSteps], step_margin[kMaxSteps],
+      step_reflect[kMaxSteps];
+  int32_t norm_stat[kMaxSteps];
+  double norm_target[kMaxSteps], norm_pct[kMaxSteps];
+};
+
+struct pseg_corr_plan {
+  int dtype;
+  double dt_ms, scale;
+  int32_t nfft, shift, out, pilot, record;
+  size_t lds;
+  DevCorr *dev;
+};
+
+namespace {
+
+template <typename T>
+void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Blob &blob,
+                  pseg_plan &P, int i) {
+  memset(&st, 0, sizeof st);
+  P.step_kind[i] = op.kind;
+  size_t lds = 0;
+  switch (op.kind) {
+    case PSEG_OP_DEMEAN: st.kind = kDemean; break;
+    case PSEG_OP_DETREND: st.kind = kDetrend; break;
+    case PSEG_OP_POLARITY: st.kind = kPolarity; break;
+    case PSEG_OP_SCALE:
+      if (!std::isfinite(op.value)) invalid("the factor must be finite, got %g", op.value);
+      if (!std::isfinite((double)(T)op.value))
+        invalid("a factor of %g overflows %s", op.value, sizeof(T) == 4 ? "float32" : "float64");
+      st.kind = kScale;
+      st.factor = op.value;
+      break;
+    case PSEG_OP_GAIN_DB: {
+      if (!std::isfinite(op.value)) invalid("the gain must be finite, got %g dB", op.value);
+      double f = std::pow(10.0, op.value / 20);
+      if (!std::isfinite(f) || !std::isfinite((double)(T)f))
+        invalid("a gain of %g dB overflows %s", op.value, sizeof(T) == 4 ? "float32" : "float64");
+      st.kind = kScale;
+      st.factor = f;
+      break;
+    }
+    case PSEG_OP_CLIP:
+      if (std::isnan(op.lo) || std::isnan(op.hi) || op.lo > op.hi)
+        invalid("clip limits must satisfy lo <= hi, got %g .. %g", op.lo, op.hi);
+      st.kind = kClip;
+      st.lo = op.lo;
+      st.hi = op.hi;
+      break;
+    case PSEG_OP_FILTER: {
+      // initFilterPlan (filter/apply.nim:118-151)
+      const pseg_filter_spec &spec = op.spec;
+      if (op.trend < 0 || op.trend > 2) invalid("unknown trend removal %d", op.trend);
+      validate(spec, dt);
+      int64_t margin = impulse_length(spec, dt);
+      double dc = response(spec, 0, dt);
+      bool reflect = dc >= 0.5 || op.trend != 2;
+      int64_t pad = reflect ? margin : std::min<int64_t>(margin, samples);
+      int64_t n = samples + (reflect ? 4 : 2) * pad;
+      double rmp = narrowest(spec);
+      if (rmp < kInf && pad == margin) n = std::max<int64_t>(n, (int64_t)std::ceil(2000 / (rmp * dt)));
+      n = next_fast(n, true);
+      if (n > (1 << 20)) invalid("an FFT of %lld points is beyond this GPU path", (long long)n);
+      DevFilter &F = st.filter;
+      build_fft<T>(n, F.fft, blob);
+      std::vector<T> gain(n / 2 + 1);
+      double df = 1000 / ((double)n * dt);
+      for (int64_t k = 0; k <= n / 2; k++) gain[k] = (T)response(spec, (double)k * df, dt);
+      if (spec.zero_dc) gain[0] = (T)0;
+      std::vector<T> fall(std::max<int64_t>(pad, 1));
+      for (int64_t k = 1; k <= pad; k++) {
+        double c = std::cos(kPi / 2 * (double)k / (double)(pad + 1));
+        fall[k - 1] = (T)(c * c);
+      }
+      set_off(F.gain, blob.add_vec(gain));
+      set_off(F.fall, blob.add_vec(fall));
+      F.pad = (int32_t)pad;
+      F.reflect = reflect;
+      F.zero_dc = spec.zero_dc != 0;
+      F.trend = op.trend;
+      F.dc_gain = dc;
+      st.kind = kFilter;
+      P.step_nfft[i] = (int32_t)n;
+      P.step_pad[i] = (int32_t)pad;
+      P.step_margin[i] = (int32_t)margin;
+      P.step_reflect[i] = reflect;
+      P.margin += (int32_t)margin;
+      lds = 2 * (size_t)n * sizeof(T);
+      break;
+    }
+    case PSEG_OP_AGC: {
+      // initAgcPlan (gain/gain.nim:272-302)
+      if (op.mode != 0 && op.mode != 1) invalid("unknown AGC mode %d", op.mode);
+      if (!finite_positive(op.window_ms))
+        invalid("an AGC window must be finite and positive, got %g ms", op.window_ms);
+      double h = std::round(op.window_ms / (2 * dt));
+      if (h < 1) invalid("an AGC window of %g ms is under 3 samples at %g ms", op.window_ms, dt);
+      if (h > (double)(1 << 26)) invalid("an AGC window of %g ms is too long", op.window_ms);
+      if (!finite_positive(op.target)) invalid("an AGC target must be finite and positive, got %g", op.target);
+      if (!finite_positive(op.epsilon)) invalid("an AGC epsilon must be finite and positive, got %g", op.epsilon);
+      int64_t H = (int64_t)h;
+      std::vector<double> inv(2 * H + 2), rev(2 * H + 1);
+      inv[0] = 0;
+      for (int64_t c = 1; c < 2 * H + 2; c++) inv[c] = 1 / (double)c;
+      for (int64_t m = 0; m < 2 * H + 1; m++) rev[m] = 1 / (double)(2 * H + 1 - m);
+      DevAgc &G = st.agc;
+      G.half = (int32_t)H;
+      G.mode = op.mode;
+      G.target = op.target;
+      G.epsilon = op.epsilon;
+      set_off(G.inv, blob.add_vec(inv));
+      set_off(G.rev, blob.add_vec(rev));
+      st.kind = kAgc;
+      P.step_margin[i] = (int32_t)H;
+      P.margin += (int32_t)H;
+      lds = (2 * (size_t)samples + 2) * sizeof(T);
+      break;
+    }
+    case PSEG_OP_NORMALIZE: {
+      // initNormalizePlan (gain/gain.nim:153-167)
+      if (op.stat < 0 || op.stat > 2) invalid("unknown statistic %d", op.stat);
+      if (!finite_positive(op.target))
+        invalid("a normalize target must be finite and positive, got %g", op.target);
+      if (!(op.percentile >= 0 && op.percentile <= 100))
+        invalid("a percentile must be in 0 .. 100, got %g", op.percentile);
+      st.kind = op.scope == 1 ? kNormPanel : kNormTrace;
+      st.stat = op.stat;
+      st.target = op.target;
+      st.percentile = op.percentile;
+      P.norm_stat[i] = op.stat;
+      P.norm_target[i] = op.target;
+      P.norm_pct[i] = op.percentile;
+      lds = 256 * sizeof(uint32_t);
+      break;
+    }
+    case PSEG_OP_MUTE_TOP:
+    case PSEG_OP_MUTE_BOTTOM:
+    case PSEG_OP_IIR:
+    case PSEG_OP_GAIN_CURVE:
+    case PSEG_OP_CALIBRATE:
+    case PSEG_OP_UNCALIBRATE:
+      throw Invalid{"UNSUPPORTED"};
+    default:
+      invalid("unknown op kind %d", op.kind);
+  }
+  P.step_lds[i] = lds;
+}
+
+void patch_step(DevStep &st, char *base) {
+  if (st.kind == kFilter) {
+    patch_fft(st.filter.fft, base);
+    patch(st.filter.gain, base);
+    patch(st.filter.fall, base);
+  } else if (st.kind == kAgc) {
+    patch(st.agc.inv, base);
+    patch(st.agc.rev, base);
+  }
+}
+
+template <typename T>
+pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t samples,
+                         int32_t record, pseg_plan **out) {
+  if (!std::isfinite(dt) || dt <= 0) return fail(PSEG_E_INVALID, "dtMs must be finite and positive, got %g", dt);
+  if (samples < 1) return fail(PSEG_E_INVALID, "a pipeline needs samples >= 1, got %d", samples);
+  if (record != 0 && record < samples)
+    return fail(PSEG_E_INVALID,
+                "recordSamples (%d) must be 0 or >= samples (%d): it is the length of the "
+                "traces a window of `samples` is cut from", record, samples);
+  if (n_ops < 0 || n_ops > PSEG_MAX_OPS) return fail(PSEG_E_INVALID, "at most %d ops", PSEG_MAX_OPS);
+  if (n_ops > 0 && !ops) return fail(PSEG_E_INVALID, "ops is NULL");
+  if (samples > pipeline_max_cols())
+    return fail(PSEG_E_UNSUPPORTED, "rows of %d samples are beyond this GPU path (at most %d)",
+                samples, pipeline_max_cols());
+  pseg_plan *P = new pseg_plan;
+  memset(P, 0, sizeof *P);
+  P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
+  P->dt_ms = dt;
+  P->samples = samples;
+  P->record = std::max(samples, record);
+  Blob blob;
+  blob.bytes.resize(sizeof(DevPlan));
+  DevPlan &H = P->host;
+  memset(&H, 0, sizeof H);
+  for (int i = 0; i < n_ops; i++) {
+    try {
+      compile_step<T>(ops[i], dt, samples, H.steps[i], blob, *P, i);
+    } catch (const Invalid &e) {
+      delete P;
+      if (e.msg == "UNSUPPORTED")
+        return fail(PSEG_E_UNSUPPORTED, "op %d (%s): not part of the GPU path; keep the CPU plan "
+                    "for this flow", i, op_name(ops[i].kind));
+      return fail(PSEG_E_INVALID, "op %d (%s): %s", i, op_name(ops[i].kind), e.msg.c_str());
+    }
+  }
+  H.n_steps = n_ops;
+  P->n_steps = n_ops;
+  // parts: a per-panel normalization after the first op starts one
+  // (pipeline.nim:515-522)
+  int start = 0;
+  for (int i = 0; i < n_ops; i++) {
+    if (H.steps[i].kind == kNormPanel && i > start) {
+      P->part_a[P->n_parts] = start;
+      P->part_b[P->n_parts] = i;
+      P->n_parts++;
+      start = i;
+    }
+  }
+  P->part_a[P->n_parts] = start;
+  P->part_b[P->n_parts] = n_ops;
+  P->n_parts++;
+  // the device copy
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
+  if (e != hipSuccess) {
+    delete P;
+    return hip_fail(e, "hipMalloc of a plan");
+  }
+  for (int i = 0; i < n_ops; i++) patch_step(H.steps[i], base);
+  memcpy(blob.bytes.data(), &H, sizeof H);
+  e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    delete P;
+    return hip_fail(e, "hipMemcpy of a plan");
+  }
+  P->dev = (DevPlan *)base;
+  P->dev_base = base;
+  P->generation = ++g_generation;
+  *out = P;
+  return PSEG_OK;
+}
+
+template <typename F>
+pseg_status guarded(F f) {
+  try {
+    return f();
+  } catch (const Invalid &e) {
+    return fail(PSEG_E_INVALID, "%s", e.msg.c_str());
+  } catch (const std::bad_alloc &) {
+    return fail(PSEG_E_INVALID, "out of host memory");
+  }
+}
+
+bool overlap(const void *a, const void *b, size_t bytes) {
+  uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x != y && x < y + bytes && y < x + bytes;
+}
+
+// The scratch of a per-panel normalization: row sums, row maxima, the
+// histogram and the factor.
+size_t panel_ws_bytes(int32_t rows) {
+  size_t b = 0;
+  b += ((size_t)rows * 8 + 255) / 256 * 256;       // row_sq
+  b += ((size_t)rows * 8 + 255) / 256 * 256;       // row_max (2 x u32)
+  b += 65536 * 4;                                   // histogram
+  b += 256;                                         // factor
+  return b;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+
+extern "C" {
+
+size_t pseg_last_error(char *buf, size_t n) {
+  if (buf && n) {
+    size_t k = std::min(n - 1, g_error.size());
+    memcpy(buf, g_error.data(), k);
+    buf[k] = 0;
+  }
+  return g_error.size();
+}
+
+int32_t pseg_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+pseg_status pseg_meta_flatten(int32_t rows, int32_t cols, int64_t first_sample,
+                              const int64_t *num_samples, const uint8_t *status,
+                              const int64_t *failed, int32_t n_failed,
+                              int32_t for_correlation, int32_t *row_len) {
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
+  if (rows > 0 && (!num_samples || !status || !row_len)) return fail(PSEG_E_INVALID, "NULL meta array");
+  // initChunkMeta's checks of the failed list (core/types.nim:106-113)
+  for (int32_t i = 0; i < n_failed; i++) {
+    if (failed[i] < 0 || failed[i] >= rows)
+      return fail(PSEG_E_INVALID, "failed row %lld is outside 0 ..< %d", (long long)failed[i], rows);
+    if (i > 0 && failed[i] <= failed[i - 1])
+      return fail(PSEG_E_INVALID, "failed rows must be ascending and unique: %lld is followed by %lld",
+                  (long long)failed[i - 1], (long long)failed[i]);
+  }
+  int32_t f = 0;
+  for (int32_t r = 0; r < rows; r++) {
+    while (f < n_failed && failed[f] < r) f++;
+    bool is_failed = f < n_failed && failed[f] == r;
+    uint8_t s = status[r];
+    bool ok = for_correlation ? (!is_failed && s != 2) : (!is_failed && (s == 0 || s == 1));
+    int64_t live = std::min<int64_t>(std::max<int64_t>(num_samples[r] - first_sample, 0), cols);
+    row_len[r] = ok ? (int32_t)live : -1;
+  }
+  return PSEG_OK;
+}
+
+int64_t pseg_next_fast_len(int64_t n, int32_t even) { return next_fast(n, even != 0); }
+
+pseg_status pseg_impulse_length(const pseg_filter_spec *spec, double dt_ms, int64_t *length) {
+  if (!spec || !length) return fail(PSEG_E_INVALID, "NULL argument");
+  return guarded([&] {
+    *length = impulse_length(*spec, dt_ms);
+    return PSEG_OK;
+  });
+}
+
+double pseg_response(const pseg_filter_spec *spec, double f_hz, double dt_ms) {
+  return spec ? response(*spec, f_hz, dt_ms) : NAN;
+}
+
+pseg_status pseg_plan_compile(const pseg_op *ops, int32_t n_ops, int32_t dtype, double dt_ms,
+                              int32_t samples, int32_t record_samples, pseg_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  return guarded([&] {
+    if (dtype == PSEG_F32) return compile_plan<float>(ops, n_ops, dt_ms, samples, record_samples, out);
+    if (dtype == PSEG_F64) return compile_plan<double>(ops, n_ops, dt_ms, samples, record_samples, out);
+    return fail(PSEG_E_INVALID, "dtype must be PSEG_F32 or PSEG_F64, got %d", dtype);
+  });
+}
+
+pseg_status pseg_plan_destroy(pseg_plan *plan) {
+  if (!plan) return PSEG_OK;
+  if (plan->dev_base) (void)hipFree(plan->dev_base);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_plan_info_get(const pseg_plan *P, pseg_plan_info *o) {
+  if (!P || !o) return fail(PSEG_E_INVALID, "NULL argument");
+  memset(o, 0, sizeof *o);
+  o->generation = P->generation;
+  o->dtype = P->dtype;
+  o->n_steps = P->n_steps;
+  o->parts = P->n_parts;
+  o->samples = P->samples;
+  o->record_samples = P->record;
+  o->margin = P->margin;
+  o->dt_ms = P->dt_ms;
+  size_t lds = 0;
+  for (int i = 0; i < P->n_steps; i++) {
+    o->step_kind[i] = P->step_kind[i];
+    o->step_nfft[i] = P->step_nfft[i];
+    o->step_pad[i] = P->step_pad[i];
+    o->step_margin[i] = P->step_margin[i];
+    o->step_reflect[i] = P->step_reflect[i];
+    lds = std::max(lds, P->step_lds[i]);
+  }
+  o->lds_bytes = lds;
+  return PSEG_OK;
+}
+
+pseg_status pseg_required_margin(const pseg_op *ops, int32_t n_ops, double dt_ms,
+                                 int32_t *margin) {
+  if (!margin) return fail(PSEG_E_INVALID, "margin is NULL");
+  if (!std::isfinite(dt_ms) || dt_ms <= 0)
+    return fail(PSEG_E_INVALID, "dtMs must be finite and positive, got %g", dt_ms);
+  int64_t m = 0;
+  for (int i = 0; i < n_ops; i++) {
+    try {
+      if (ops[i].kind == PSEG_OP_FILTER) {
+        validate(ops[i].spec, dt_ms);
+        m += impulse_length(ops[i].spec, dt_ms);
+      } else if (ops[i].kind == PSEG_OP_AGC) {
+        double h = std::round(ops[i].window_ms / (2 * dt_ms));
+        if (!finite_positive(ops[i].window_ms))
+          invalid("an AGC window must be finite and positive, got %g ms", ops[i].window_ms);
+        if (h < 1) invalid("an AGC window of %g ms is under 3 samples at %g ms", ops[i].window_ms, dt_ms);
+        m += (int64_t)h;
+      } else if (ops[i].kind == PSEG_OP_IIR) {
+        return fail(PSEG_E_UNSUPPORTED, "op %d (%s): not part of the GPU path", i, op_name(ops[i].kind));
+      }
+    } catch (const Invalid &e) {
+      return fail(PSEG_E_INVALID, "op %d (%s): %s", i, op_name(ops[i].kind), e.msg.c_str());
+    }
+  }
+  *margin = (int32_t)m;
+  return PSEG_OK;
+}
+
+void pseg_decode_window(int32_t shown_first, int32_t shown_last, int64_t margin,
+                        int32_t trace_len, int32_t *first, int32_t *last) {
+  int64_t m = std::min<int64_t>(std::max<int64_t>(margin, 0), std::max<int64_t>(trace_len, 0));
+  if (first) *first = (int32_t)std::max<int64_t>(0, (int64_t)shown_first - m);
+  if (last) *last = (int32_t)std::min<int64_t>((int64_t)trace_len - 1, (int64_t)shown_last + m);
+}
+
+pseg_status pseg_plan_workspace_size(const pseg_plan *P, int32_t rows, int32_t cols,
+                                     size_t *bytes) {
+  if (!P || !bytes) return fail(PSEG_E_INVALID, "NULL argument");
+  (void)cols;
+  bool panel = false;
+  for (int i = 0; i < P->n_steps; i++) panel |= P->host.steps[i].kind == kNormPanel;
+  *bytes = panel ? panel_ws_bytes(std::max(rows, 0)) : 0;
+  return PSEG_OK;
+}
+
+pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int32_t rows,
+                            int32_t cols, const int32_t *row_len, void *ws, size_t ws_bytes,
+                            void *stream) {
+  // checkApply (pipeline.nim:619-641): every check before any sample changes
+  if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (rows == 0) return PSEG_OK;
+  if (!src || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  if (cols > P->samples)
+    return fail(PSEG_E_INVALID, "the block has %d columns, the plan takes %d samples", cols, P->samples);
+  size_t esz = P->dtype == PSEG_F32 ? 4 : 8;
+  if (overlap(src, dst, (size_t)rows * (size_t)cols * esz))
+    return fail(PSEG_E_INVALID, "src and dst overlap without being the same block");
+  size_t need = 0;
+  pseg_plan_workspace_size(P, rows, cols, &need);
+  if (need > 0 && (!ws || ws_bytes < need))
+    return fail(PSEG_E_WORKSPACE, "the plan needs a workspace of %zu bytes, got %zu", need,
+                ws ? ws_bytes : (size_t)0);
+  hipStream_t s = (hipStream_t)stream;
+  char *w = (char *)ws;
+  double *row_sq = (double *)w;
+  uint32_t *row_max = (uint32_t *)(w + ((size_t)rows * 8 + 255) / 256 * 256);
+  uint32_t *hist = (uint32_t *)((char *)row_max + ((size_t)rows * 8 + 255) / 256 * 256);
+  double *factor = (double *)((char *)hist + 65536 * 4);
+  for (int part = 0; part < P->n_parts; part++) {
+    const int a = P->part_a[part], b = P->part_b[part];
+    PipeLaunch L;
+    L.first = a;
+    L.last = b;
+    L.cols = cols;
+    L.has_factor = 0;
+    L.factor = nullptr;
+    const bool panel = b > a && P->host.steps[a].kind == kNormPanel;
+    const void *in = part == 0 ? src : dst;
+    if (panel) {
+      // panelFactorOf over the whole block as the previous parts left it
+      hipError_t e = launch_panel_factor(P->dtype, in, rows, cols, row_len, P->norm_stat[a],
+                                         P->norm_target[a], P->norm_pct[a], row_sq, row_max,
+                                         hist, factor, s);
+      if (e != hipSuccess) return hip_fail(e, "panel statistics");
+      L.first = a + 1;
+      L.has_factor = 1;
+      L.factor = factor;
+    }
+    size_t lds = 1024;
+    for (int i = L.first; i < L.last; i++) lds = std::max(lds, P->step_lds[i]);
+    L.lds_bytes = lds;
+    hipError_t e = launch_pipeline(P->dtype, P->dev, L, in, dst, rows, row_len, s);
+    if (e != hipSuccess) return hip_fail(e, "pipeline launch");
+  }
+  return PSEG_OK;
+}
+
+// ------------------------------------------------------------ correlation
+
+pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, double pdt,
+                              double pt0, double dt, int32_t record, int32_t out_samples,
+                              int32_t mode, double eps_percent, int32_t normalize,
+                              pseg_corr_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (dtype != PSEG_F32 && dtype != PSEG_F64)
+    return fail(PSEG_E_INVALID, "dtype must be PSEG_F32 or PSEG_F64, got %d", dtype);
+  // Pilot.validate (vibroseis/pilot.nim:89-106)
+  if (!pilot || plen < 2) return fail(PSEG_E_INVALID, "a pilot needs at least two samples, got %d", plen);
+  if (!(pdt > 0 && pdt < kInf)) return fail(PSEG_E_INVALID, "a pilot's dtMs must be finite and positive, got %g", pdt);
+  if (!std::isfinite(pt0)) return fail(PSEG_E_INVALID, "a pilot's t0Ms must be finite, got %g", pt0);
+  double energy = 0;
+  for (int32_t i = 0; i < plen; i++) {
+    if (!std::isfinite(pilot[i])) return fail(PSEG_E_INVALID, "pilot sample %d is not finite", i);
+    energy += pilot[i] * pilot[i];
+  }
+  if (energy == 0) return fail(PSEG_E_INVALID, "the pilot is all zeros");
+  // initCorrelationPlan (vibroseis/correlate.nim:94-160)
+  if (!(dt > 0 && dt < kInf)) return fail(PSEG_E_INVALID, "dtMs must be finite and positive, got %g", dt);
+  if (!(std::fabs(pdt - dt) <= 1e-9 * dt))
+    return fail(PSEG_E_INVALID, "the pilot is sampled at %g ms, the records at %g ms: resample the pilot first", pdt, dt);
+  if (record < 1) return fail(PSEG_E_INVALID, "recordSamples must be >= 1, got %d", record);
+  int64_t outN = out_samples == 0 ? (int64_t)record - plen : out_samples;
+  if (outN < 1)
+    return fail(PSEG_E_INVALID, "the pilot (%d samples) is not shorter than the record (%d): no listen time; pass outSamples", plen, record);
+  if (out_samples < 0 || outN > record)
+    return fail(PSEG_E_INVALID, "outSamples must be in 0 .. recordSamples, got %d", out_samples);
+  if (mode == 1 && !(eps_percent > 0 && eps_percent < kInf))
+    return fail(PSEG_E_INVALID, "epsPercent must be finite and positive, got %g", eps_percent);
+  if (mode != 0 && mode != 1) return fail(PSEG_E_INVALID, "unknown correlation mode %d", mode);
+  double sft = pt0 / dt;
+  if (std::fabs(sft) >= (double)((int64_t)record + plen))
+    return fail(PSEG_E_INVALID, "the pilot starts %g ms from the time break: no lag of a %d-sample record reaches the output", pt0, record);
+  int64_t whole = (int64_t)std::round(sft);
+  double frac = sft - (double)whole;
+  int64_t n = next_fast((int64_t)record + plen + 64 + std::llabs(whole), true);
+  size_t esz = dtype == PSEG_F32 ? 4 : 8;
+  size_t lds = 2 * (size_t)n * esz;
+  if (lds > 160 * 1024 - 2048)
+    return fail(PSEG_E_UNSUPPORTED, "a correlation FFT of %lld points does not fit one workgroup's LDS", (long long)n);
+  // The pilot's spectrum in float64.
+  HostFft hf(n);
+  std::vector<cd> x(n, cd(0, 0));
+  for (int32_t j = 0; j < plen; j++) x[j] = cd(pilot[j], 0);
+  std::vector<cd> P = hf.forward(x);
+  const int64_t bins = n / 2 + 1;
+  double peak_pow = 0;
+  for (int64_t k = 0; k < bins; k++) peak_pow = std::max(peak_pow, std::norm(P[k]));
+  double eps = mode == 1 ? eps_percent / 100 * peak_pow : 0.0;
+  std::vector<double> w(bins);
+  double peak = 0;
+  for (int64_t k = 0; k < bins; k++) {
+    double p2 = P[k].real() * P[k].real() + P[k].imag() * P[k].imag();
+    w[k] = mode == 1 ? 1 / (p2 + eps) : 1.0;
+    double g = p2 * w[k];
+    peak += (k == 0 || (n % 2 == 0 && k == bins - 1)) ? g : 2 * g;
+  }
+  peak /= (double)n;
+  double scale = normalize ? 1 / peak : 1.0;
+  Blob blob;
+  blob.bytes.resize(sizeof(DevCorr));
+  DevCorr C;
+  memset(&C, 0, sizeof C);
+  try {
+    if (dtype == PSEG_F32) build_fft<float>(n, C.fft, blob);
+    else build_fft<double>(n, C.fft, blob);
+  } catch (const Invalid &e) {
+    return fail(PSEG_E_INVALID, "%s", e.msg.c_str());
+  }
+  // conj(P) w scale exp(+2 pi i k frac / n)
+  std::vector<double> mul(2 * bins);
+  for (int64_t k = 0; k < bins; k++) {
+    double ws = w[k] * scale;
+    double ar = P[k].real() * ws, ai = -P[k].imag() * ws;
+    double th = 2 * kPi * (double)k * frac / (double)n;
+    double er = std::cos(th), ei = std::sin(th);
+    mul[2 * k] = ar * er - ai * ei;
+    mul[2 * k + 1] = ar * ei + ai * er;
+  }
+  if (n % 2 == 0) mul[2 * (bins - 1) + 1] = 0;  // the inverse takes Nyquist as real
+  if (dtype == PSEG_F32) {
+    std::vector<float> mf(mul.begin(), mul.end());
+    set_off(C.mul, blob.add_vec(mf));
+  } else {
+    set_off(C.mul, blob.add_vec(mul));
+  }
+  C.shift = (int32_t)whole;
+  C.pilot = plen;
+  C.record = record;
+  C.out = (int32_t)outN;
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc of a correlation plan");
+  patch_fft(C.fft, base);
+  patch(C.mul, base);
+  memcpy(blob.bytes.data(), &C, sizeof C);
+  e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    return hip_fail(e, "hipMemcpy of a correlation plan");
+  }
+  pseg_corr_plan *cp = new pseg_corr_plan;
+  cp->dtype = dtype;
+  cp->dt_ms = dt;
+  cp->scale = scale;
+  cp->nfft = (int32_t)n;
+  cp->shift = (int32_t)whole;
+  cp->out = (int32_t)outN;
+  cp->pilot = plen;
+  cp->record = record;
+  cp->lds = lds;
+  cp->dev = (DevCorr *)base;
+  *out = cp;
+  return PSEG_OK;
+}
+
+pseg_status pseg_corr_destroy(pseg_corr_plan *plan) {
+  if (!plan) return PSEG_OK;
+  if (plan->dev) (void)hipFree(plan->dev);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_corr_info_get(const pseg_corr_plan *c, pseg_corr_info *o) {
+  if (!c || !o) return fail(PSEG_E_INVALID, "NULL argument");
+  memset(o, 0, sizeof *o);
+  o->dtype = c->dtype;
+  o->nfft = c->nfft;
+  o->shift = c->shift;
+  o->out_samples = c->out;
+  o->pilot_samples = c->pilot;
+  o->record_samples = c->record;
+  o->dt_ms = c->dt_ms;
+  o->scale = c->scale;
+  o->lds_bytes = c->lds;
+  return PSEG_OK;
+}
+
+pseg_status pseg_corr_apply(const pseg_corr_plan *c, const void *src, int32_t rows,
+                            int32_t src_cols, const int32_t *row_len, void *dst,
+                            int32_t dst_cols, void *stream) {
+  // checkCorrelate (vibroseis/correlate.nim:328-347)
+  if (!c) return fail(PSEG_E_INVALID, "a plan must come from pseg_corr_compile");
+  if (rows < 0 || src_cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (src_cols > c->record)
+    return fail(PSEG_E_INVALID, "the records are %d samples, the plan takes %d", src_cols, c->record);
+  if (dst_cols != c->out)
+    return fail(PSEG_E_INVALID, "dst must be %d samples wide, got %d", c->out, dst_cols);
+  if (rows == 0) return PSEG_OK;
+  if (!src || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  hipError_t e = launch_correlate(c->dtype, c->dev, c->lds, src, rows, src_cols, row_len, dst,
+                                  (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "correlation launch");
+  return PSEG_OK;
+}
+
+pseg_status pseg_linear_sweep(double f1, double f2, double length_ms, double dt_ms, double ts,
+                              double te, double phase_deg, double amp, double *x, int32_t n) {
+  // validate (vibroseis/sweep.nim:132-183) for a linear law
+  if (!(dt_ms > 0 && dt_ms < kInf)) return fail(PSEG_E_INVALID, "dtMs must be finite and positive, got %g", dt_ms);
+  if (!(length_ms >= dt_ms && length_ms < kInf))
+    return fail(PSEG_E_INVALID, "lengthMs must be at least one sample interval, got %g", length_ms);
+  double steps = length_ms / dt_ms;
+  if (std::fabs(steps - std::round(steps)) > 1e-6)
+    return fail(PSEG_E_INVALID, "lengthMs = %g must be a whole number of samples of %g ms", length_ms, dt_ms);
+  if (!std::isfinite(amp)) return fail(PSEG_E_INVALID, "the amplitude must be finite, got %g", amp);
+  if (!std::isfinite(phase_deg)) return fail(PSEG_E_INVALID, "startPhaseDeg must be finite, got %g", phase_deg);
+  double nyq = 500 / dt_ms, limit = 0.8 * nyq;
+  for (double f : {f1, f2}) {
+    if (!(f > 0 && f < kInf)) return fail(PSEG_E_INVALID, "a sweep frequency must be > 0 Hz, got %g", f);
+    if (f > nyq) return fail(PSEG_E_INVALID, "%g Hz is above Nyquist (%g Hz): it would alias", f, nyq);
+    if (f > limit * (1 + 1e-12))
+      return fail(PSEG_E_INVALID, "%g Hz is above the anti-alias limit %g Hz", f, limit);
+  }
+  double tsl = std::isnan(ts) ? 0 : ts, tel = std::isnan(te) ? 0 : te;
+  if (!(tsl >= 0 && tsl < kInf) || !(tel >= 0 && tel < kInf))
+    return fail(PSEG_E_INVALID, "a taper length must be >= 0 and finite");
+  if (tsl + tel > length_ms * (1 + 1e-9))
+    return fail(PSEG_E_INVALID, "the sweep tapers overlap: %g + %g ms is longer than the %g ms sweep", tsl, tel, length_ms);
+  // samplesInLength (core/types.nim:147-166)
+  double q = length_ms / dt_ms;
+  double r = std::round(q);
+  int64_t len = std::fabs(q - r) <= 1e-6 ? (int64_t)r : (int64_t)std::floor(q);
+  if (!x || n != len) return fail(PSEG_E_INVALID, "the sweep has %lld samples, the buffer %d", (long long)len, n);
+  // phasesInto (slLinear) and sweepInto (vibroseis/sweep.nim:292-342)
+  const double dt = dt_ms / 1000, T = length_ms / 1000, rate = (f2 - f1) / T;
+  const double p0 = phase_deg * kPi / 180;
+  for (int32_t j = 0; j < n; j++) {
+    double t = (double)j * dt;
+    x[j] = amp * std::sin(p0 + 2 * kPi * (f1 * t + 0.5 * rate * t * t));
+  }
+  // applyTaper with tpCos2 (vibroseis/taper.nim:125-179)
+  auto wgt = [](double v) {
+    v = std::min(1.0, std::max(0.0, v));
+    return 0.5 - 0.5 * std::cos(kPi * v);
+  };
+  const double sweep_ms = (double)n * dt_ms;
+  if (tsl > 0)
+    for (int32_t j = 0; j < n && (double)j * dt_ms < tsl; j++) x[j] = x[j] * wgt((double)j * dt_ms / tsl);
+  if (tel > 0)
+    for (int32_t j = n - 1; j >= 0 && sweep_ms - (double)j * dt_ms < tel; j--)
+      x[j] = x[j] * wgt((sweep_ms - (double)j * dt_ms) / tel);
+  return PSEG_OK;
+}
+
+}  // extern "C"

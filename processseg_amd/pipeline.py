@@ -1,0 +1,484 @@
+"""The reference's pipeline API over the GPU plan (pipeline.nim, filter/design.nim,
+gain/gain.nim, core/types.nim), mirrored name for name.
+
+```python
+flow = initPipeline([demeanOp(), bandPassOp(Corners(4, 8, 60, 80)), agcOp(windowMs=500)])
+plan = flow.compile(float32, dtMs=2, samples=2001)   # ValueError if invalid
+plan.apply(raw, processed, meta)                      # src -> dst, device tensors
+plan.apply(samples, meta)                             # in place
+```
+
+Panels are torch tensors on the GPU ([rows, cols], C-contiguous, float32 or
+float64).  A ``ChunkMeta`` holds the per-row facts the kernels read (liveness
+and live length), flattened once into a device int32 array.  Ops outside the
+GPU path (mutes, IIR, gain curves, calibration) raise ``UnsupportedOp`` at
+compile: keep the CPU plan for such a flow.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import UnsupportedOp, WorkspaceError  # noqa: F401  (re-exported)
+
+Inf = math.inf
+NaN = math.nan
+
+float32 = np.float32
+float64 = np.float64
+
+# TrendRemoval (filter/edges.nim:8-14)
+trNone, trMean, trLinear = 0, 1, 2
+# AgcMode (gain/gain.nim:92-95)
+agcRms, agcMeanAbs = 0, 1
+# NormStat / NormScope (gain/gain.nim:72-81)
+normMax, normRms, normPercentile = 0, 1, 2
+perTrace, perPanel = 0, 1
+# OpKind (pipeline.nim:92-108)
+(opDemean, opDetrend, opScale, opGainDb, opClip, opPolarity, opMuteTop, opMuteBottom,
+ opFilter, opIir, opAgc, opNormalize, opGainCurve, opCalibrate, opUncalibrate) = range(15)
+# TraceStatus of parseSEG as ChunkMeta reads it
+tsLive, tsUnknown, tsDead, tsAuxiliary = 0, 1, 2, 3
+
+_KIND_NAMES = ["opDemean", "opDetrend", "opScale", "opGainDb", "opClip", "opPolarity",
+               "opMuteTop", "opMuteBottom", "opFilter", "opIir", "opAgc", "opNormalize",
+               "opGainCurve", "opCalibrate", "opUncalibrate"]
+
+
+# ------------------------------------------------------------- filter specs
+
+@dataclasses.dataclass(frozen=True)
+class Corners:
+    """The corners of a cos² trapezoid in Hz (filter/design.nim:63-67)."""
+    f1: float
+    f2: float
+    f3: float
+    f4: float
+
+
+@dataclasses.dataclass(frozen=True)
+class FilterSpec:
+    """A zero-phase response (filter/design.nim:78-90): kind 0 trapezoid,
+    1 Butterworth magnitude; notches as (freqHz, widthHz)."""
+    kind: int = 0
+    f1: float = 0.0
+    f2: float = 0.0
+    f3: float = Inf
+    f4: float = Inf
+    low_hz: float = 0.0
+    low_slope: float = 24.0
+    high_hz: float = Inf
+    high_slope: float = 24.0
+    notches: tuple = ()
+    zero_dc: bool = False
+
+    def withNotch(self, freqHz: float, widthHz: float) -> "FilterSpec":
+        """withNotch (filter/design.nim:212-218)."""
+        if not (freqHz > 0 and freqHz < Inf):
+            raise ValueError(f"a notch frequency must be finite and > 0, got {freqHz}")
+        if not (widthHz > 0 and widthHz < Inf):
+            raise ValueError(f"a notch width must be finite and > 0, got {widthHz}")
+        return dataclasses.replace(self, notches=self.notches + ((float(freqHz), float(widthHz)),))
+
+    def withZeroDc(self) -> "FilterSpec":
+        """withZeroDc (filter/design.nim:220-223)."""
+        return dataclasses.replace(self, zero_dc=True)
+
+    def response(self, f: float, dtMs: float) -> float:
+        """response (filter/design.nim:236-255), computed by the library."""
+        s = spec_c(self)
+        return float(_lib.lib().pseg_response(C.byref(s), f, dtMs))
+
+    def impulseLength(self, dtMs: float) -> int:
+        """impulseLength (filter/design.nim:353-366)."""
+        s = spec_c(self)
+        L = C.c_int64()
+        _lib.check(_lib.lib().pseg_impulse_length(C.byref(s), dtMs, C.byref(L)))
+        return int(L.value)
+
+
+def _check_corners(c: Corners) -> None:
+    # checkCorners (filter/design.nim:103-115)
+    for f in (c.f1, c.f2, c.f3, c.f4):
+        if math.isnan(f):
+            raise ValueError(f"a corner is NaN: {c}")
+    has_low, has_high = c.f2 > 0, c.f3 < Inf
+    if has_low and not (c.f1 >= 0 and c.f1 < c.f2):
+        raise ValueError(f"a low cut needs 0 ≤ f1 < f2 (a sloped ramp), got {c}")
+    if not has_low and (c.f1 != 0 or c.f2 != 0):
+        raise ValueError(f"no low cut needs f1 = f2 = 0, got {c}")
+    if has_high and not (c.f3 < c.f4 and c.f4 < Inf):
+        raise ValueError(f"a high cut needs f3 < f4 < Inf (a sloped ramp), got {c}")
+    if not has_high and c.f4 != Inf:
+        raise ValueError(f"no high cut needs f3 = f4 = Inf, got {c}")
+    if has_low and has_high and not (c.f2 <= c.f3):
+        raise ValueError(f"a band-pass needs f2 ≤ f3, got {c}")
+
+
+def bandPass(c: Corners) -> FilterSpec:
+    """bandPass (filter/design.nim:177-181)."""
+    _check_corners(c)
+    return FilterSpec(kind=0, f1=float(c.f1), f2=float(c.f2), f3=float(c.f3), f4=float(c.f4))
+
+
+def lowCut(f1: float, f2: float) -> FilterSpec:
+    return bandPass(Corners(f1, f2, Inf, Inf))
+
+
+def highCut(f3: float, f4: float) -> FilterSpec:
+    return bandPass(Corners(0, 0, f3, f4))
+
+
+def butterBandPass(lowHz: float, highHz: float, slopeDbPerOctave: float = 24.0,
+                   highSlopeDbPerOctave: float = NaN) -> FilterSpec:
+    """butterBandPass (filter/design.nim:191-200)."""
+    hs = slopeDbPerOctave if math.isnan(highSlopeDbPerOctave) else highSlopeDbPerOctave
+    if not (lowHz >= 0 and lowHz < Inf):
+        raise ValueError(f"a low-cut corner must be finite and ≥ 0 (0: none), got {lowHz}")
+    if not (highHz > 0):
+        raise ValueError(f"a high-cut corner must be > 0 (Inf: none), got {highHz}")
+    for on, s in ((lowHz > 0, slopeDbPerOctave), (highHz < Inf, hs)):
+        if on and not (s > 0 and s < Inf):
+            raise ValueError(f"a slope must be finite and > 0 dB/octave, got {s}")
+    if not (lowHz < highHz):
+        raise ValueError(f"the low-cut corner must be below the high-cut one, got {lowHz} and {highHz}")
+    return FilterSpec(kind=1, low_hz=float(lowHz), low_slope=float(slopeDbPerOctave),
+                      high_hz=float(highHz), high_slope=float(hs))
+
+
+def butterLowCut(hz: float, slopeDbPerOctave: float = 24.0) -> FilterSpec:
+    if not (hz > 0):
+        raise ValueError(f"a low-cut corner must be > 0, got {hz}")
+    return butterBandPass(hz, Inf, slopeDbPerOctave, slopeDbPerOctave)
+
+
+def butterHighCut(hz: float, slopeDbPerOctave: float = 24.0) -> FilterSpec:
+    if not (hz < Inf):
+        raise ValueError(f"a high-cut corner must be finite, got {hz}")
+    return butterBandPass(0, hz, slopeDbPerOctave, slopeDbPerOctave)
+
+
+def spec_c(spec: FilterSpec) -> _lib.FilterSpecC:
+    s = _lib.FilterSpecC()
+    s.kind = int(spec.kind)
+    s.zero_dc = int(bool(spec.zero_dc))
+    if len(spec.notches) > _lib.MAX_NOTCH:
+        raise ValueError(f"at most {_lib.MAX_NOTCH} notches")
+    s.n_notch = len(spec.notches)
+    s.f1, s.f2, s.f3, s.f4 = spec.f1, spec.f2, spec.f3, spec.f4
+    s.low_hz, s.low_slope, s.high_hz, s.high_slope = (spec.low_hz, spec.low_slope,
+                                                       spec.high_hz, spec.high_slope)
+    for i, (f, w) in enumerate(spec.notches):
+        s.notch_hz[i] = f
+        s.notch_width[i] = w
+    return s
+
+
+# -------------------------------------------------------------------- ops
+
+@dataclasses.dataclass
+class Op:
+    """One operation as plain data (pipeline.nim:158-174).  Parameters are
+    edited in place and checked at compile."""
+    kind: int
+    value: float = 0.0          # scale factor, or gain in dB
+    lo: float = 0.0             # clip
+    hi: float = 0.0
+    spec: Optional[FilterSpec] = None
+    trend: int = trLinear       # filter
+    window_ms: float = 0.0      # agc
+    mode: int = agcRms
+    target: float = 1.0         # agc, normalize
+    epsilon: float = 1e-9       # agc
+    stat: int = normMax         # normalize
+    scope: int = perTrace
+    percentile: float = 99.0
+
+    def to_c(self) -> _lib.OpC:
+        o = _lib.OpC()
+        o.kind = self.kind
+        o.trend = self.trend
+        o.mode = self.mode
+        o.stat = self.stat
+        o.scope = self.scope
+        o.value = self.value
+        o.lo = self.lo
+        o.hi = self.hi
+        o.window_ms = self.window_ms
+        o.target = self.target
+        o.epsilon = self.epsilon
+        o.percentile = self.percentile
+        if self.spec is not None:
+            o.spec = spec_c(self.spec)
+        return o
+
+
+def demeanOp() -> Op:
+    return Op(opDemean)
+
+
+def detrendOp() -> Op:
+    return Op(opDetrend)
+
+
+def scaleOp(factor: float) -> Op:
+    return Op(opScale, value=float(factor))
+
+
+def gainDbOp(dB: float) -> Op:
+    return Op(opGainDb, value=float(dB))
+
+
+def clipOp(lo: float, hi: float) -> Op:
+    return Op(opClip, lo=float(lo), hi=float(hi))
+
+
+def polarityOp() -> Op:
+    return Op(opPolarity)
+
+
+def filterOp(spec: FilterSpec, trend: int = trLinear) -> Op:
+    return Op(opFilter, spec=spec, trend=trend)
+
+
+def bandPassOp(corners: Corners, trend: int = trLinear) -> Op:
+    return filterOp(bandPass(corners), trend)
+
+
+def agcOp(windowMs: float, mode: int = agcRms, target: float = 1.0,
+          epsilon: float = 1e-9) -> Op:
+    return Op(opAgc, window_ms=float(windowMs), mode=mode, target=float(target),
+              epsilon=float(epsilon))
+
+
+def normalizeOp(stat: int, scope: int = perTrace, target: float = 1.0,
+                percentile: float = 99.0) -> Op:
+    return Op(opNormalize, stat=stat, scope=scope, target=float(target),
+              percentile=float(percentile))
+
+
+def iirOp(*_a, **_k) -> Op:
+    return Op(opIir)
+
+
+def _ops_c(ops: Sequence[Op]):
+    arr = (_lib.OpC * max(1, len(ops)))()
+    for i, op in enumerate(ops):
+        arr[i] = op.to_c()
+    return arr
+
+
+# ------------------------------------------------------------------- meta
+
+class ChunkMeta:
+    """The per-row facts of a block (core/types.nim:69-122): status per row
+    (tsLive, tsUnknown, tsDead, tsAuxiliary), trace length numSamples per
+    row, failed rows, dt and firstSample.  ``row_len(cols)`` flattens
+    isLive/liveSamples to one int32 per row (-1: not live)."""
+
+    def __init__(self, rows: int, dtMs: float, numSamples=None, status=None,
+                 failed: Iterable[int] = (), firstSample: int = 0):
+        if not (isinstance(dtMs, (int, float)) and math.isfinite(dtMs) and dtMs > 0):
+            raise ValueError(f"dtMs must be finite and positive, got {dtMs}")
+        if firstSample < 0:
+            raise ValueError(f"firstSample must be ≥ 0, got {firstSample}")
+        self.rows = int(rows)
+        self.dtMs = float(dtMs)
+        self.firstSample = int(firstSample)
+        big = np.iinfo(np.int64).max // 4
+        self.numSamples = (np.full(rows, big, np.int64) if numSamples is None
+                           else np.ascontiguousarray(numSamples, np.int64))
+        self.status = (np.zeros(rows, np.uint8) if status is None
+                       else np.ascontiguousarray(status, np.uint8))
+        self.failed = np.ascontiguousarray(sorted(int(f) for f in failed), np.int64)
+        if self.numSamples.shape != (rows,) or self.status.shape != (rows,):
+            raise ValueError("numSamples and status need one value per row")
+        self._cache = {}
+
+    def __len__(self):
+        return self.rows
+
+    def row_len(self, cols: int, for_correlation: bool = False) -> np.ndarray:
+        out = np.zeros(self.rows, np.int32)
+        fl = self.failed if self.failed.size else np.zeros(1, np.int64)
+        _lib.check(_lib.lib().pseg_meta_flatten(
+            self.rows, cols, self.firstSample, self.numSamples.ctypes.data,
+            self.status.ctypes.data, fl.ctypes.data, int(self.failed.size),
+            1 if for_correlation else 0, out.ctypes.data))
+        return out
+
+    def device_row_len(self, cols: int, device, for_correlation: bool = False):
+        import torch
+        key = (cols, str(device), for_correlation)
+        t = self._cache.get(key)
+        if t is None:
+            t = torch.from_numpy(self.row_len(cols, for_correlation)).to(device)
+            self._cache[key] = t
+        return t
+
+
+def sameInterval(dtMs: float, planDtMs: float) -> bool:
+    """sameInterval (core/types.nim:136-145)."""
+    return abs(dtMs - planDtMs) <= 1e-9 * planDtMs
+
+
+def decodeWindow(shown: range, margin: int, traceLen: int) -> range:
+    """decodeWindow (pipeline.nim:550-560) for an inclusive `shown` given as
+    range(a, b + 1); returns range(first, last + 1)."""
+    a, b = C.c_int32(), C.c_int32()
+    _lib.lib().pseg_decode_window(shown.start, shown.stop - 1, int(min(margin, 2**62)),
+                                  traceLen, C.byref(a), C.byref(b))
+    return range(a.value, b.value + 1)
+
+
+def nextFastLen(n: int, even: bool = True) -> int:
+    return int(_lib.lib().pseg_next_fast_len(n, 1 if even else 0))
+
+
+# --------------------------------------------------------------- pipeline
+
+class Pipeline:
+    """An ordered list of Ops (pipeline.nim:176-178, 338-380)."""
+
+    def __init__(self, ops: Iterable[Op] = ()):
+        self.ops: List[Op] = list(ops)
+
+    def __len__(self):
+        return len(self.ops)
+
+    def __getitem__(self, i):
+        return self.ops[i]
+
+    def __setitem__(self, i, op):
+        self.ops[i] = op
+
+    def add(self, op: Op) -> None:
+        self.ops.append(op)
+
+    def compile(self, T, dtMs: float, samples: int, recordSamples: int = 0) -> "PipelinePlan":
+        """compile (pipeline.nim:466-523) into a plan on the current GPU."""
+        return PipelinePlan(self, T, dtMs, samples, recordSamples)
+
+    def requiredMargin(self, dtMs: float) -> int:
+        """requiredMargin (pipeline.nim:525-548) without compiling."""
+        m = C.c_int32()
+        arr = _ops_c(self.ops)
+        _lib.check(_lib.lib().pseg_required_margin(arr, len(self.ops), dtMs, C.byref(m)))
+        return int(m.value)
+
+
+def initPipeline(ops: Iterable[Op] = ()) -> Pipeline:
+    return Pipeline(ops)
+
+
+def _dtype_code(T) -> int:
+    dt = np.dtype(T)
+    if dt == np.float32:
+        return 0
+    if dt == np.float64:
+        return 1
+    raise TypeError(f"the sample type must be float32 or float64, got {dt}")
+
+
+def _stream_handle(stream) -> Optional[int]:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+class PipelinePlan:
+    """A compiled pipeline on the GPU (pipeline.nim:200-211).  Immutable; any
+    number of streams may apply it at once."""
+
+    def __init__(self, flow: Pipeline, T, dtMs: float, samples: int, recordSamples: int = 0):
+        self.T = np.dtype(T)
+        self._h = C.c_void_p()
+        arr = _ops_c(flow.ops)
+        _lib.check(_lib.lib().pseg_plan_compile(arr, len(flow.ops), _dtype_code(T), dtMs,
+                                                samples, recordSamples, C.byref(self._h)))
+        info = _lib.PlanInfoC()
+        _lib.check(_lib.lib().pseg_plan_info_get(self._h, C.byref(info)))
+        self.generation = int(info.generation)
+        self.dtMs = float(info.dt_ms)
+        self.samples = int(info.samples)
+        self.recordSamples = int(info.record_samples)
+        self.requiredMargin = int(info.margin)
+        self.partCount = int(info.parts)
+        self.lds_bytes = int(info.lds_bytes)
+        n = info.n_steps
+        self.steps = [dict(kind=_KIND_NAMES[info.step_kind[i]], nfft=info.step_nfft[i],
+                           pad=info.step_pad[i], margin=info.step_margin[i],
+                           reflect=bool(info.step_reflect[i])) for i in range(n)]
+        self._ws = {}
+
+    def __len__(self):
+        return len(self.steps)
+
+    def workspaceSize(self, rows: int, cols: int) -> int:
+        b = C.c_size_t()
+        _lib.check(_lib.lib().pseg_plan_workspace_size(self._h, rows, cols, C.byref(b)))
+        return int(b.value)
+
+    def _workspace(self, rows, cols, device):
+        import torch
+        need = self.workspaceSize(rows, cols)
+        if need == 0:
+            return None, 0
+        key = str(device)
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(need, dtype=torch.uint8, device=device)
+            self._ws[key] = ws
+        return ws, need
+
+    def apply(self, src, dst=None, meta: Optional[ChunkMeta] = None, *, row_len=None,
+              ws=None, stream=None):
+        """apply(src, dst, meta) (src -> dst) or apply(samples, meta) (in
+        place), pipeline.nim:736-772.  `row_len` (a device int32 tensor) may
+        replace `meta`.  Asynchronous on `stream` (default: torch's current)."""
+        import torch
+        if isinstance(dst, ChunkMeta) and meta is None:
+            meta, dst = dst, None
+        if dst is None:
+            dst = src
+        for t, name in ((src, "src"), (dst, "dst")):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda:
+                raise ValueError(f"{name} must be a GPU tensor")
+            if t.dim() != 2:
+                raise ValueError(f"{name} must be a [rows, cols] tensor, got rank {t.dim()}")
+            if not t.is_contiguous():
+                raise ValueError(f"{name} must be C-contiguous (row-major)")
+            if np.dtype(str(t.dtype).replace("torch.", "")) != self.T:
+                raise ValueError(f"{name} holds {t.dtype}, the plan {self.T}")
+        if src.shape != dst.shape:
+            raise ValueError(f"src is {src.shape[0]} × {src.shape[1]}, dst {dst.shape[0]} × {dst.shape[1]}")
+        rows, cols = src.shape
+        if row_len is None:
+            if meta is None:
+                raise ValueError("apply needs a ChunkMeta (or row_len)")
+            if len(meta) != rows:
+                raise ValueError(f"samples have {rows} rows, the meta {len(meta)}")
+            if not sameInterval(meta.dtMs, self.dtMs):
+                raise ValueError(f"the plan is for dtMs = {self.dtMs}, the rows have {meta.dtMs}")
+            row_len = meta.device_row_len(cols, src.device)
+        if ws is None:
+            ws, need = self._workspace(rows, cols, src.device)
+        else:
+            need = ws.numel()
+        _lib.check(_lib.lib().pseg_plan_apply(
+            self._h, src.data_ptr(), dst.data_ptr(), rows, cols, row_len.data_ptr(),
+            ws.data_ptr() if ws is not None else None, need, _stream_handle(stream)))
+        return dst
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().pseg_plan_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
