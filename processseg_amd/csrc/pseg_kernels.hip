@@ -657,7 +657,10 @@ __device__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds
   if (isfinite(level) && level > 0 && fits_in<T>(st.target / level)) {
     const T f = (T)(st.target / level);
 #pragma unroll
-    for (int k = 0; k < PER; k++) v[k] = v[k] * f;
+    for (int k = 0; k < PER; k++) {
+      int j = threadIdx.x + NT * k;
+      if (j < n) v[k] = v[k] * f;
+    }
   }
   __syncthreads();
 }

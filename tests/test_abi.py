@@ -1,0 +1,82 @@
+"""The C ABI on the CPU: the library loads, exports every function
+include/pseg.h declares, and its host logic (meta flattening, lengths,
+margins, sweeps) equals the oracle's.  No kernel runs here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import processseg_amd as P
+from processseg_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for fn in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names += re.findall(r"\b(pseg_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(_lib.SYMBOLS) <= set(names)
+
+
+def test_device_count_never_fails():
+    assert _lib.lib().pseg_device_count() >= 0
+
+
+def test_meta_flatten_equals_oracle(oracle):
+    rng = np.random.default_rng(0)
+    for rows, cols, first in [(1, 10, 0), (50, 2001, 0), (200, 501, 100), (7, 0, 3)]:
+        nums = rng.integers(-5, cols + 300, rows).astype(np.int64)
+        status = rng.integers(0, 4, rows).astype(np.uint8)
+        failed = sorted(set(rng.integers(0, rows, rows // 5).tolist()))
+        meta = P.ChunkMeta(rows, 2.0, numSamples=nums, status=status, failed=failed,
+                           firstSample=first)
+        for corr in (False, True):
+            got = meta.row_len(cols, corr)
+            want = oracle.meta_flatten(cols, nums, status, failed, first, corr)
+            assert np.array_equal(got, want)
+
+
+def test_meta_flatten_rejects_bad_failed_lists():
+    import pytest
+    meta = P.ChunkMeta(4, 2.0)
+    meta.failed = np.array([2, 1], np.int64)
+    with pytest.raises(ValueError):
+        meta.row_len(10)
+    meta.failed = np.array([4], np.int64)
+    with pytest.raises(ValueError):
+        meta.row_len(10)
+
+
+def test_apply_checks_before_touching_samples():
+    """A NULL plan or a block wider than the plan is refused by the ABI
+    itself, with the reference's messages (pipeline.nim:619-641)."""
+    L = _lib.lib()
+    rc = L.pseg_plan_apply(None, None, None, 1, 1, None, None, 0, None)
+    assert rc == _lib.E_INVALID and "plan" in _lib.last_error()
+    rc = L.pseg_corr_apply(None, None, 1, 1, None, None, 1, None)
+    assert rc == _lib.E_INVALID
+
+
+def test_corr_compile_validation():
+    import pytest
+    pilot = P.generatePilot(P.linearSweep(6, 80, 12000, 2, P.initSweepTaper(300, 300)))
+    with pytest.raises(ValueError, match="not shorter than the record"):
+        P.initCorrelationPlan(np.float32, pilot, 2.0, 5000)
+    with pytest.raises(ValueError, match="resample"):
+        P.initCorrelationPlan(np.float32, pilot, 4.0, 20000)
+    zero = P.Pilot(np.zeros(100), 2.0)
+    with pytest.raises(ValueError, match="all zeros"):
+        P.initCorrelationPlan(np.float32, zero, 2.0, 5000)
