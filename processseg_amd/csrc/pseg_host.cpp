@@ -868,28 +868,39 @@ pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int3
   double *factor = (double *)((char *)hist + 65536 * 4);
   for (int part = 0; part < P->n_parts; part++) {
     const int a = P->part_a[part], b = P->part_b[part];
-    PipeLaunch L;
-    L.first = a;
-    L.last = b;
-    L.cols = cols;
-    L.has_factor = 0;
-    L.factor = nullptr;
-    const bool panel = b > a && P->host.steps[a].kind == kNormPanel;
     const void *in = part == 0 ? src : dst;
+    JobLaunch L;
+    memset(&L, 0, sizeof L);
+    Job &J = L.single;
+    J.plan = P->dev;
+    J.src = in;
+    J.dst = dst;
+    J.num = row_len;
+    J.src_pitch = J.dst_pitch = cols;
+    J.rows = rows;
+    J.cols = cols;
+    J.first = 0;
+    J.crop0 = 0;
+    J.crop1 = cols;
+    J.step_first = a;
+    J.step_last = b;
+    const bool panel = b > a && P->host.steps[a].kind == kNormPanel;
     if (panel) {
       // panelFactorOf over the whole block as the previous parts left it
       hipError_t e = launch_panel_factor(P->dtype, in, rows, cols, row_len, P->norm_stat[a],
                                          P->norm_target[a], P->norm_pct[a], row_sq, row_max,
                                          hist, factor, s);
       if (e != hipSuccess) return hip_fail(e, "panel statistics");
-      L.first = a + 1;
-      L.has_factor = 1;
-      L.factor = factor;
+      J.step_first = a + 1;
+      J.has_factor = 1;
+      J.factor = factor;
     }
     size_t lds = 1024;
-    for (int i = L.first; i < L.last; i++) lds = std::max(lds, P->step_lds[i]);
+    for (int i = J.step_first; i < J.step_last; i++) lds = std::max(lds, P->step_lds[i]);
     L.lds_bytes = lds;
-    hipError_t e = launch_pipeline(P->dtype, P->dev, L, in, dst, rows, row_len, s);
+    L.total = rows;
+    L.max_cols = cols;
+    hipError_t e = launch_pipeline(P->dtype, L, s);
     if (e != hipSuccess) return hip_fail(e, "pipeline launch");
   }
   return PSEG_OK;

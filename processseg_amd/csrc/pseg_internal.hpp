@@ -81,13 +81,35 @@ struct DevPlan {
   DevStep steps[kMaxSteps];
 };
 
-// What one launch of the pipeline kernel runs: steps [first, last) of the
-// plan on rows `cols` wide, after scaling the live rows by *factor when
-// has_factor (the start of a part after a per-panel normalization).
-struct PipeLaunch {
-  int32_t first, last, cols, has_factor;
-  size_t lds_bytes;
+// One job of the pipeline kernel: steps [step_first, step_last) of a plan on
+// `rows` rows of a window `cols` wide.  Row r of the window is
+// src[r * src_pitch .. + cols); its trace holds num[r] samples from trace
+// sample 0 (-1: the row is not live), the window starts at trace sample
+// `first`, so its live part is clamp(num[r] - first, 0, cols).  Window columns
+// [crop0, crop1) are stored to dst[r * dst_pitch + 0 ..).  When has_factor,
+// the live part is first scaled by *factor (a part after a per-panel
+// normalization).  Jobs of a batch are sorted by item0, the first work item
+// (row) of the job.
+struct Job {
+  const DevPlan *plan;
+  const void *src;
+  void *dst;
+  const int32_t *num;
   const double *factor;
+  int64_t src_pitch, dst_pitch;
+  int32_t rows, cols, first, crop0, crop1, item0;
+  int32_t step_first, step_last, has_factor, reserved;
+};
+
+// One launch: a single job by value (n_jobs == 0) or n_jobs jobs in device
+// memory; `total` work items; direct (one block per item) or persistent
+// (counter != NULL, `grid` blocks take items from *counter, cleared first).
+struct JobLaunch {
+  Job single;
+  const Job *jobs;
+  int32_t n_jobs, total, grid, max_cols;
+  uint32_t *counter;
+  size_t lds_bytes;
 };
 
 struct DevCorr {
@@ -100,9 +122,7 @@ struct DevCorr {
 };
 
 // Launchers (pseg_kernels.hip).  Return hipSuccess or the launch error.
-hipError_t launch_pipeline(int dtype, const DevPlan *dev_plan, const PipeLaunch &a,
-                           const void *src, void *dst, int32_t rows, const int32_t *row_len,
-                           hipStream_t stream);
+hipError_t launch_pipeline(int dtype, const JobLaunch &a, hipStream_t stream);
 // Per-panel normalization factor (into *factor, NaN: leave the panel alone)
 // of the live rows of x; scratch: rows doubles, 2 rows uint32 and, for a
 // percentile, 65536 uint32 of histogram (cleared here).
