@@ -76,6 +76,35 @@ def cpu_baseline_leg(flow, x, live, gpu_out, seconds=12.0):
              "cpu_mirror_rel_l2": rel_cpu})
 
 
+def kernel_name(cols, dtype="float", threads=128):
+    """The instantiation pseg_plan_apply launches for rows of `cols` samples:
+    the row lives in registers, ceil(cols / threads) samples per thread,
+    rounded up to one of the compiled widths."""
+    per = -(-cols // threads)
+    per = min(w for w in (4, 8, 12, 16, 24) if w >= per)
+    return f"pseg::pipeline_kernel<{dtype}, {per}>"
+
+
+def copy_probe(torch, dev, args):
+    """A device copy of one 2000 x 3001 f32 panel (24.008 MB read, 24.008 MB
+    written) between rotating buffers: the known-size kernel that calibrates
+    the FETCH_SIZE / WRITE_SIZE counters."""
+    srcs = [torch.ones(ROWS, COLS, device=dev) for _ in range(args.pairs)]
+    dsts = [torch.empty_like(srcs[0]) for _ in range(args.pairs)]
+    for i in range(args.warmup):
+        dsts[i % args.pairs].copy_(srcs[i % args.pairs])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dsts[i % args.pairs].copy_(srcs[i % args.pairs])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / args.steps
+    nbytes = ROWS * COLS * 4
+    print(json.dumps({"probe": "device copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "ms_per_copy": round(ms, 5),
+                      "GBps": round(2 * nbytes / (ms / 1000) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +112,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time a plain device copy of the panel instead (the calibration "
+                         "kernel for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the panel path on a single MI355X "
@@ -94,6 +126,9 @@ def main():
     import processseg_amd as P
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        copy_probe(torch, dev, args)
+        return
     from processseg_amd.workloads import big_panel
     log("generating the 2000 x 3001 panel")
     x, live = big_panel(ROWS, COLS, np.float32)
@@ -157,7 +192,7 @@ def main():
                    "buffer_pairs": args.pairs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "pipeline_kernel<float,12>",
+                     "traffic": traffic, "kernel": kernel_name(COLS),
                      "avg_launch_ms": round(avg_ms, 5),
                      "algorithmic_bytes_per_launch": algo_bytes},
     }
