@@ -76,13 +76,14 @@ def cpu_baseline_leg(flow, x, live, gpu_out, seconds=12.0):
              "cpu_mirror_rel_l2": rel_cpu})
 
 
-def kernel_name(cols, dtype="float", threads=128):
-    """The instantiation pseg_plan_apply launches for rows of `cols` samples:
-    the row lives in registers, ceil(cols / threads) samples per thread,
-    rounded up to one of the compiled widths."""
+def kernel_name(cols, threads, dtype="float"):
+    """The instantiation pseg_plan_apply launches for rows of `cols` samples
+    with the plan's `threads` per row: the row lives in registers,
+    ceil(cols / threads) samples per thread, rounded up to a compiled width."""
     per = -(-cols // threads)
-    per = min(w for w in (4, 8, 12, 16, 24) if w >= per)
-    return f"pseg::pipeline_kernel<{dtype}, {per}>"
+    widths = (8, 16, 24) if threads == 512 else (4, 8, 12, 16, 24)
+    per = min(w for w in widths if w >= per)
+    return f"pseg::pipeline_kernel<{dtype}, {threads}, {per}>"
 
 
 def copy_probe(torch, dev, args):
@@ -105,6 +106,121 @@ def copy_probe(torch, dev, args):
                       "GBps": round(2 * nbytes / (ms / 1000) / 1e9, 1)}), flush=True)
 
 
+def correlation_leg(torch, dev, steps, warmup, pairs, with_cpu):
+    """Vibroseis correlation of 1200 records x 10001 samples at 2 ms with a
+    12 s sweep (nfft 16200, 4001 output lags): one correlateInto per step over
+    rotating src/dst pairs.  Algorithmic bytes per record: (10001 + 4001) x 4."""
+    import processseg_amd as P
+    from processseg_amd.workloads import sweep_pilot, vibroseis_panel
+    rows, rec = 1200, 10001
+    pilot = sweep_pilot()
+    x = vibroseis_panel(rows, rec)
+    plan = P.initCorrelationPlan(np.float32, pilot, DT, rec)
+    row_len = torch.full((rows,), rec, dtype=torch.int32, device=dev)
+    srcs = [torch.from_numpy(x).to(dev) for _ in range(pairs)]
+    dsts = [torch.empty(rows, plan.outSamples, device=dev) for _ in range(pairs)]
+    stream = torch.cuda.current_stream()
+    for i in range(warmup):
+        plan.correlateInto(srcs[i % pairs], dsts[i % pairs], row_len=row_len, stream=stream)
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        plan.correlateInto(srcs[i % pairs], dsts[i % pairs], row_len=row_len, stream=stream)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    algo = rows * (rec + plan.outSamples) * 4
+    out = {"workload": "vibroseis correlation 1200x10001 f32, 12 s 6-80 Hz sweep, nfft 16200, "
+                       "4001 lags",
+           "value": round(rows * rec / (wall / steps) / 1e6, 1), "unit": "Msamples/s (input)",
+           "ms_per_step": round(wall * 1000 / steps, 5),
+           "roofline": {"bound": "hbm", "achieved": round(algo / (avg_ms / 1000) / 1e9, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(algo / (avg_ms / 1000) / 1e9 / HBM_PEAK_GBS, 4),
+                        "kernel": "pseg::corr_kernel<float, 1024>",
+                        "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": algo}}
+    if with_cpu:
+        import oracle as O
+        got = dsts[(steps - 1) % pairs][:32].cpu().numpy().astype(np.float64)
+        want = O.correlate(x[:32].astype(np.float64), np.full(32, rec, np.int32), pilot.samples,
+                           DT, rec)
+        sub = x[:400]
+        t1 = time.perf_counter()
+        O.correlate(sub.astype(np.float32), np.full(400, rec, np.int32), pilot.samples, DT, rec)
+        el32 = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": round(400 * rec / el32 / 1e6, 2), "unit": "Msamples/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"400 records in float32, {el32:.2f} s on one core"}
+        out["accuracy"] = {"vs": "float64 oracle, 32 records",
+                           "rel_l2": float(np.linalg.norm(got - want) / np.linalg.norm(want)),
+                           "max_abs_over_peak": float(np.max(np.abs(got - want))
+                                                      / np.max(np.abs(want)))}
+    return out
+
+
+def batch_leg(torch, dev, steps, warmup):
+    """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
+    6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
+    (dt, samples)) as one PipelineBatch: persistent launches replayed from a
+    captured graph, against one Plan.apply launch per record."""
+    import processseg_amd as P
+    from processseg_amd import synth
+    from processseg_amd.workloads import mixed_flow, mixed_shapes
+    shapes = mixed_shapes(64)
+    R = max(r for r, _, _ in shapes)
+    Cm = max(c for _, c, _ in shapes)
+    base, _ = synth.gather(R, Cm, 2.0, 9)   # records are cut from one seeded gather
+    base = torch.from_numpy(base).to(dev)
+    flow = mixed_flow()
+    plans, srcs, dsts, lens = {}, [], [], []
+    for r, c, dt in shapes:
+        if (dt, c) not in plans:
+            plans[(dt, c)] = flow.compile(np.float32, dt, c)
+        srcs.append(base[:r, :c].contiguous())
+        dsts.append(torch.empty_like(srcs[-1]))
+        lens.append(torch.full((r,), c, dtype=torch.int32, device=dev))
+    del base
+    items = [P.BatchItem(plans[(dt, c)], s, d, row_len=l)
+             for (r, c, dt), s, d, l in zip(shapes, srcs, dsts, lens)]
+    batch = P.PipelineBatch(items)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, n):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    t_eager = timed(lambda: batch.launch(stream), steps)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batch.launch()
+    t_graph = timed(g.replay, steps)
+
+    def per_record():
+        for it in items:
+            it.plan.apply(it.src, it.dst, row_len=it.row_len, stream=stream)
+    t_loop = timed(per_record, steps)
+    samples = batch.samples
+    return {"workload": "64 records, rows U[240, 2400], samples U[1000, 6001], dt 1/2/4 ms, "
+                        "band-pass 5-10-70-90 Hz + AGC 500 ms, f32",
+            "records": 64, "samples": samples, "plans": len(plans),
+            "launches": batch.launches, "threads_per_row": batch.threads, "grid": batch.grid,
+            "value": round(samples / t_graph / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t_graph * 1000, 4),
+            "ms_eager": round(t_eager * 1000, 4),
+            "ms_one_launch_per_record": round(t_loop * 1000, 4),
+            "achieved_GBps": round(samples * 8 / t_graph / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +228,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="only the headline panel (no correlation leg)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of the panel instead (the calibration "
                          "kernel for the HBM byte counters)")
@@ -192,7 +310,7 @@ def main():
                    "buffer_pairs": args.pairs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": kernel_name(COLS),
+                     "traffic": traffic, "kernel": kernel_name(COLS, plan.threadsPerRow),
                      "avg_launch_ms": round(avg_ms, 5),
                      "algorithmic_bytes_per_launch": algo_bytes},
     }
@@ -201,6 +319,15 @@ def main():
         cpu, acc = cpu_baseline_leg(flow, x, live, out)
         line["cpu_baseline"] = cpu
         line["accuracy"] = acc
+    if not args.no_secondary:
+        del srcs, dsts
+        torch.cuda.empty_cache()
+        log("correlation leg")
+        line["secondary"] = {"correlation": correlation_leg(
+            torch, dev, max(10, args.steps // 4), 3, 4, not args.no_cpu)}
+        torch.cuda.empty_cache()
+        log("uneven batch leg")
+        line["secondary"]["uneven_batch"] = batch_leg(torch, dev, max(5, args.steps // 20), 2)
     print(json.dumps(line), flush=True)
 
 

@@ -107,6 +107,9 @@ typedef struct pseg_plan_info {
   int32_t step_margin[PSEG_MAX_OPS]; /* impulse length L, or AGC half window h */
   int32_t step_reflect[PSEG_MAX_OPS];
   size_t lds_bytes;        /* LDS one row of the fused kernel holds */
+  int32_t threads_per_row; /* workgroup size of the fused kernel for this plan
+                              (128, 256 or 512: rows up to 3072, 6144, 12288) */
+  int32_t reserved;
 } pseg_plan_info;
 
 /* --------------------------------------------------------------- errors -- */
@@ -180,6 +183,43 @@ pseg_status pseg_plan_workspace_size(const pseg_plan *plan, int32_t rows, int32_
 pseg_status pseg_plan_apply(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
                             int32_t cols, const int32_t *row_len, void *ws, size_t ws_bytes,
                             void *stream);
+
+/* ---------------------------------------------------------------- batch -- */
+
+/* Many blocks, each with its own plan (different sample intervals, widths
+ * and row counts), applied as one unit of work: what a caller otherwise does
+ * as one ProcessPool.apply(plan_i, src_i, dst_i, meta_i) per record
+ * (core/pool.nim:277-308, in a loop over records as bench/viewersim.nim
+ * does).  Each item's result equals pseg_plan_apply of that item alone, bit
+ * for bit.  The batch runs as one persistent launch per (dtype, threads per
+ * row) class present, rows handed out heaviest first from a device counter. */
+typedef struct pseg_batch_item {
+  const pseg_plan *plan;   /* one part (no per-panel normalization) */
+  const void *src;         /* device [rows, cols], C-contiguous */
+  void *dst;               /* device, same shape; == src runs in place */
+  int32_t rows, cols;      /* cols <= the plan's samples */
+  const int32_t *row_len;  /* device, rows int32 from pseg_meta_flatten */
+} pseg_batch_item;
+typedef struct pseg_batch pseg_batch;
+
+typedef struct pseg_batch_info {
+  int32_t items;
+  int32_t launches;        /* persistent launches per pseg_batch_launch */
+  int64_t rows;            /* rows of all items */
+  int64_t samples;         /* rows x cols of all items */
+  int32_t threads[4];      /* threads per row of each launch */
+  int32_t grid[4];         /* workgroups of each launch */
+} pseg_batch_info;
+
+/* Validates every item as pseg_plan_apply would (before any sample changes;
+ * the first bad item is named: "item i: ...") and keeps the work table on the
+ * device.  The plans and buffers must outlive the batch. */
+pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n_items, pseg_batch **out);
+/* Runs the batch on `stream`.  No allocation, no host-device copy and no
+ * memset node: the call can be captured in a graph and replayed. */
+pseg_status pseg_batch_launch(const pseg_batch *batch, void *stream);
+pseg_status pseg_batch_info_get(const pseg_batch *batch, pseg_batch_info *out);
+pseg_status pseg_batch_destroy(pseg_batch *batch);
 
 /* ---------------------------------------------------------- correlation -- */
 

@@ -5,7 +5,8 @@ reference's edge handling, demean, AGC, normalization and vibroseis
 correlation) runs in hand-written gfx950 kernels behind the C ABI of
 include/pseg.h (libpseg.so).  This package mirrors the reference's Nim API
 over that ABI: initPipeline / compile / PipelinePlan.apply and
-initCorrelationPlan / correlateInto.
+initCorrelationPlan / correlateInto, plus PipelineBatch (many records with
+their own plans as one persistent launch).
 """
 from ._lib import HipError, UnsupportedOp, WorkspaceError, build, lib  # noqa: F401
 from .pipeline import (  # noqa: F401
@@ -18,3 +19,4 @@ from .pipeline import (  # noqa: F401
 from .correlate import (  # noqa: F401
     CorrelationPlan, Pilot, SweepSpec, SweepTaper, cmCorrelate, cmDeconvolve, generatePilot,
     initCorrelationPlan, initSweepTaper, linearSweep, samplesInLength, sweepSamples)
+from .batch import BatchItem, PipelineBatch  # noqa: F401

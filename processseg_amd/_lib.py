@@ -42,7 +42,8 @@ class PlanInfoC(C.Structure):
                 ("margin", C.c_int32), ("dt_ms", C.c_double),
                 ("step_kind", C.c_int32 * MAX_OPS), ("step_nfft", C.c_int32 * MAX_OPS),
                 ("step_pad", C.c_int32 * MAX_OPS), ("step_margin", C.c_int32 * MAX_OPS),
-                ("step_reflect", C.c_int32 * MAX_OPS), ("lds_bytes", C.c_size_t)]
+                ("step_reflect", C.c_int32 * MAX_OPS), ("lds_bytes", C.c_size_t),
+                ("threads_per_row", C.c_int32), ("reserved", C.c_int32)]
 
 
 class CorrInfoC(C.Structure):
@@ -50,6 +51,16 @@ class CorrInfoC(C.Structure):
                 ("out_samples", C.c_int32), ("pilot_samples", C.c_int32),
                 ("record_samples", C.c_int32), ("dt_ms", C.c_double), ("scale", C.c_double),
                 ("lds_bytes", C.c_size_t)]
+
+
+class BatchItemC(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("src", C.c_void_p), ("dst", C.c_void_p),
+                ("rows", C.c_int32), ("cols", C.c_int32), ("row_len", C.c_void_p)]
+
+
+class BatchInfoC(C.Structure):
+    _fields_ = [("items", C.c_int32), ("launches", C.c_int32), ("rows", C.c_int64),
+                ("samples", C.c_int64), ("threads", C.c_int32 * 4), ("grid", C.c_int32 * 4)]
 
 
 class UnsupportedOp(ValueError):
@@ -81,7 +92,8 @@ SYMBOLS = [
     "pseg_impulse_length", "pseg_response", "pseg_plan_compile", "pseg_plan_destroy",
     "pseg_plan_info_get", "pseg_required_margin", "pseg_decode_window",
     "pseg_plan_workspace_size", "pseg_plan_apply", "pseg_corr_compile", "pseg_corr_destroy",
-    "pseg_corr_info_get", "pseg_corr_apply", "pseg_linear_sweep",
+    "pseg_corr_info_get", "pseg_corr_apply", "pseg_linear_sweep", "pseg_batch_create",
+    "pseg_batch_launch", "pseg_batch_info_get", "pseg_batch_destroy",
 ]
 
 
@@ -118,6 +130,10 @@ def lib():
     L.pseg_corr_info_get.argtypes = [vp, C.POINTER(CorrInfoC)]
     L.pseg_corr_apply.argtypes = [vp, vp, i32, i32, vp, vp, i32, vp]
     L.pseg_linear_sweep.argtypes = [f64] * 8 + [vp, i32]
+    L.pseg_batch_create.argtypes = [C.POINTER(BatchItemC), i32, C.POINTER(vp)]
+    L.pseg_batch_launch.argtypes = [vp, vp]
+    L.pseg_batch_info_get.argtypes = [vp, C.POINTER(BatchInfoC)]
+    L.pseg_batch_destroy.argtypes = [vp]
     for name in SYMBOLS:
         if not hasattr(L, name):
             raise ImportError(f"{LIB_PATH} does not export {name}")

@@ -423,6 +423,7 @@ std::atomic<uint64_t> g_generation{0};
 
 struct pseg_plan {
   int dtype;
+  int threads;           // workgroup size of the fused kernel (pipeline_threads_for)
   double dt_ms;
   int32_t samples, record, margin;
   int n_steps;
@@ -604,12 +605,13 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
                 "traces a window of `samples` is cut from", record, samples);
   if (n_ops < 0 || n_ops > PSEG_MAX_OPS) return fail(PSEG_E_INVALID, "at most %d ops", PSEG_MAX_OPS);
   if (n_ops > 0 && !ops) return fail(PSEG_E_INVALID, "ops is NULL");
-  if (samples > pipeline_max_cols())
+  if (pipeline_threads_for(samples) == 0)
     return fail(PSEG_E_UNSUPPORTED, "rows of %d samples are beyond this GPU path (at most %d)",
-                samples, pipeline_max_cols());
+                samples, pipeline_max_cols(512));
   pseg_plan *P = new pseg_plan;
   memset(P, 0, sizeof *P);
   P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
+  P->threads = pipeline_threads_for(samples);
   P->dt_ms = dt;
   P->samples = samples;
   P->record = std::max(samples, record);
@@ -620,11 +622,18 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   for (int i = 0; i < n_ops; i++) {
     try {
       compile_step<T>(ops[i], dt, samples, H.steps[i], blob, *P, i);
+      if (P->step_lds[i] > kMaxRowLds)
+        invalid("%zu bytes of LDS per row (an FFT of %d points) do not fit one workgroup "
+                "(at most %zu); UNSUPPORTED", P->step_lds[i], P->step_nfft[i], kMaxRowLds);
     } catch (const Invalid &e) {
       delete P;
       if (e.msg == "UNSUPPORTED")
         return fail(PSEG_E_UNSUPPORTED, "op %d (%s): not part of the GPU path; keep the CPU plan "
                     "for this flow", i, op_name(ops[i].kind));
+      const std::string tag = "; UNSUPPORTED";
+      if (e.msg.size() > tag.size() && e.msg.compare(e.msg.size() - tag.size(), tag.size(), tag) == 0)
+        return fail(PSEG_E_UNSUPPORTED, "op %d (%s): %s", i, op_name(ops[i].kind),
+                    e.msg.substr(0, e.msg.size() - tag.size()).c_str());
       return fail(PSEG_E_INVALID, "op %d (%s): %s", i, op_name(ops[i].kind), e.msg.c_str());
     }
   }
@@ -691,6 +700,32 @@ size_t panel_ws_bytes(int32_t rows) {
   b += 65536 * 4;                                   // histogram
   b += 256;                                         // factor
   return b;
+}
+
+}  // namespace
+
+struct pseg_batch {
+  int32_t n_items;
+  int64_t rows, samples;
+  int n_launches;
+  int dtype[4], threads[4];
+  JobLaunch launch[4];
+  char *dev;  // the job tables, then one 256-byte counter per launch
+};
+
+namespace {
+
+// Relative cost of one row of a plan (FFT work dominates): heavier rows are
+// handed out first, so the batch does not end on a long row started late.
+double row_cost(const pseg_plan *P, int32_t cols) {
+  double c = cols;
+  for (int i = 0; i < P->n_steps; i++) {
+    const int nf = P->step_nfft[i];
+    if (nf > 0) c += 2.5 * nf * std::log2((double)nf);
+    else if (P->host.steps[i].kind == kAgc) c += 8.0 * cols;
+    else c += cols;
+  }
+  return c;
 }
 
 }  // namespace
@@ -794,6 +829,7 @@ pseg_status pseg_plan_info_get(const pseg_plan *P, pseg_plan_info *o) {
     lds = std::max(lds, P->step_lds[i]);
   }
   o->lds_bytes = lds;
+  o->threads_per_row = P->threads;
   return PSEG_OK;
 }
 
@@ -900,9 +936,177 @@ pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int3
     L.lds_bytes = lds;
     L.total = rows;
     L.max_cols = cols;
-    hipError_t e = launch_pipeline(P->dtype, L, s);
+    hipError_t e = launch_pipeline(P->dtype, P->threads, L, s);
     if (e != hipSuccess) return hip_fail(e, "pipeline launch");
   }
+  return PSEG_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batch **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (n < 0) return fail(PSEG_E_INVALID, "n_items must be >= 0, got %d", n);
+  if (n > 0 && !items) return fail(PSEG_E_INVALID, "items is NULL");
+  // every check before anything runs (pipeline.nim:619-641, per item)
+  for (int32_t i = 0; i < n; i++) {
+    const pseg_batch_item &it = items[i];
+    const pseg_plan *P = it.plan;
+    if (!P) return fail(PSEG_E_INVALID, "item %d: a plan must come from pseg_plan_compile", i);
+    if (it.rows < 0 || it.cols < 0)
+      return fail(PSEG_E_INVALID, "item %d: rows and cols must be >= 0", i);
+    if (it.rows == 0) continue;
+    if (!it.src || !it.dst || !it.row_len)
+      return fail(PSEG_E_INVALID, "item %d: NULL panel or row_len", i);
+    if (it.cols > P->samples)
+      return fail(PSEG_E_INVALID, "item %d: the block has %d columns, the plan takes %d samples",
+                  i, it.cols, P->samples);
+    const size_t esz = P->dtype == PSEG_F32 ? 4 : 8;
+    if (overlap(it.src, it.dst, (size_t)it.rows * (size_t)it.cols * esz))
+      return fail(PSEG_E_INVALID, "item %d: src and dst overlap without being the same block", i);
+    if (P->n_parts != 1)
+      return fail(PSEG_E_UNSUPPORTED, "item %d: a plan with a per-panel normalization runs on its "
+                  "own (pseg_plan_apply)", i);
+  }
+  // no item writes where another reads or writes
+  for (int32_t i = 0; i < n; i++) {
+    const pseg_batch_item &a = items[i];
+    if (a.rows == 0 || a.cols == 0) continue;
+    const size_t ab = (size_t)a.rows * a.cols * (a.plan->dtype == PSEG_F32 ? 4 : 8);
+    for (int32_t j = 0; j < n; j++) {
+      const pseg_batch_item &b = items[j];
+      if (j == i || b.rows == 0 || b.cols == 0) continue;
+      const size_t bb = (size_t)b.rows * b.cols * (b.plan->dtype == PSEG_F32 ? 4 : 8);
+      const uintptr_t d = (uintptr_t)a.dst;
+      const uintptr_t s2 = (uintptr_t)b.src, d2 = (uintptr_t)b.dst;
+      if ((d < d2 + bb && d2 < d + ab) || (d < s2 + bb && s2 < d + ab))
+        return fail(PSEG_E_INVALID, "items %d and %d: item %d's dst overlaps item %d's samples",
+                    std::min(i, j), std::max(i, j), i, j);
+    }
+  }
+  pseg_batch *B = new pseg_batch;
+  memset(B, 0, sizeof *B);
+  B->n_items = n;
+  // one launch per (dtype, threads per row)
+  std::vector<std::vector<int>> groups;
+  for (int32_t i = 0; i < n; i++) {
+    const pseg_batch_item &it = items[i];
+    if (it.rows == 0) continue;
+    B->rows += it.rows;
+    B->samples += (int64_t)it.rows * it.cols;
+    int g = 0;
+    for (; g < B->n_launches; g++)
+      if (B->dtype[g] == it.plan->dtype && B->threads[g] == it.plan->threads) break;
+    if (g == B->n_launches) {
+      if (g == 4) {
+        delete B;
+        return fail(PSEG_E_UNSUPPORTED, "more than 4 (dtype, threads) classes in one batch");
+      }
+      B->dtype[g] = it.plan->dtype;
+      B->threads[g] = it.plan->threads;
+      B->n_launches++;
+      groups.emplace_back();
+    }
+    groups[g].push_back(i);
+  }
+  std::vector<std::vector<Job>> tables(B->n_launches);
+  size_t bytes = 0;
+  for (int g = 0; g < B->n_launches; g++) {
+    std::vector<int> &ids = groups[g];
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
+      return row_cost(items[a].plan, items[a].cols) > row_cost(items[b].plan, items[b].cols);
+    });
+    JobLaunch &L = B->launch[g];
+    size_t lds = 1024;
+    int32_t item0 = 0;
+    for (int id : ids) {
+      const pseg_batch_item &it = items[id];
+      const pseg_plan *P = it.plan;
+      Job J;
+      memset(&J, 0, sizeof J);
+      J.plan = P->dev;
+      J.src = it.src;
+      J.dst = it.dst;
+      J.num = it.row_len;
+      J.src_pitch = J.dst_pitch = it.cols;
+      J.rows = it.rows;
+      J.cols = it.cols;
+      J.crop1 = it.cols;
+      J.item0 = item0;
+      J.step_first = 0;
+      J.step_last = P->n_steps;
+      tables[g].push_back(J);
+      if ((int64_t)item0 + it.rows > INT32_MAX) {
+        delete B;
+        return fail(PSEG_E_INVALID, "a batch holds at most %d rows per class", INT32_MAX);
+      }
+      item0 += it.rows;
+      for (int s = 0; s < P->n_steps; s++) lds = std::max(lds, P->step_lds[s]);
+      L.max_cols = std::max(L.max_cols, it.cols);
+    }
+    L.n_jobs = (int32_t)ids.size();
+    L.total = item0;
+    L.lds_bytes = lds;
+    int resident = 0;
+    hipError_t e = pipeline_resident_blocks(B->dtype[g], B->threads[g], L.max_cols, lds, &resident);
+    if (e != hipSuccess) {
+      delete B;
+      return hip_fail(e, "occupancy of the pipeline kernel");
+    }
+    L.grid = std::max(1, std::min(L.total, resident));
+    bytes += (tables[g].size() * sizeof(Job) + 255) / 256 * 256;
+  }
+  bytes += 256 * (size_t)std::max(B->n_launches, 1);
+  hipError_t e = hipMalloc((void **)&B->dev, bytes);
+  if (e != hipSuccess) {
+    delete B;
+    return hip_fail(e, "hipMalloc of a batch");
+  }
+  size_t off = 0;
+  for (int g = 0; g < B->n_launches; g++) {
+    const size_t tb = tables[g].size() * sizeof(Job);
+    e = hipMemcpy(B->dev + off, tables[g].data(), tb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(B->dev);
+      delete B;
+      return hip_fail(e, "hipMemcpy of a batch");
+    }
+    B->launch[g].jobs = (const Job *)(B->dev + off);
+    off += (tb + 255) / 256 * 256;
+  }
+  for (int g = 0; g < B->n_launches; g++) B->launch[g].counter = (uint32_t *)(B->dev + off + 256 * g);
+  *out = B;
+  return PSEG_OK;
+}
+
+pseg_status pseg_batch_launch(const pseg_batch *B, void *stream) {
+  if (!B) return fail(PSEG_E_INVALID, "a batch must come from pseg_batch_create");
+  for (int g = 0; g < B->n_launches; g++) {
+    hipError_t e = launch_pipeline(B->dtype[g], B->threads[g], B->launch[g], (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "batch launch");
+  }
+  return PSEG_OK;
+}
+
+pseg_status pseg_batch_info_get(const pseg_batch *B, pseg_batch_info *o) {
+  if (!B || !o) return fail(PSEG_E_INVALID, "NULL argument");
+  memset(o, 0, sizeof *o);
+  o->items = B->n_items;
+  o->launches = B->n_launches;
+  o->rows = B->rows;
+  o->samples = B->samples;
+  for (int g = 0; g < B->n_launches; g++) {
+    o->threads[g] = B->threads[g];
+    o->grid[g] = B->launch[g].grid;
+  }
+  return PSEG_OK;
+}
+
+pseg_status pseg_batch_destroy(pseg_batch *B) {
+  if (!B) return PSEG_OK;
+  if (B->dev) (void)hipFree(B->dev);
+  delete B;
   return PSEG_OK;
 }
 

@@ -121,8 +121,20 @@ struct DevCorr {
   int32_t out;
 };
 
-// Launchers (pseg_kernels.hip).  Return hipSuccess or the launch error.
-hipError_t launch_pipeline(int dtype, const JobLaunch &a, hipStream_t stream);
+// Launchers.  Return hipSuccess or the launch error.
+// The pipeline kernel with `threads` (128, 256 or 512) threads per row
+// (pseg_kernels.hip, which dispatches to pipeline_nt<threads>.hip).
+hipError_t launch_pipeline(int dtype, int threads, const JobLaunch &a, hipStream_t stream);
+hipError_t launch_pipeline_nt128(int dtype, const JobLaunch &a, hipStream_t stream);
+hipError_t launch_pipeline_nt256(int dtype, const JobLaunch &a, hipStream_t stream);
+hipError_t launch_pipeline_nt512(int dtype, const JobLaunch &a, hipStream_t stream);
+// Workgroups of the kernel launch_pipeline would pick for (dtype, threads,
+// max_cols) that fit on the device at once with `lds` bytes each.
+hipError_t pipeline_resident_blocks(int dtype, int threads, int max_cols, size_t lds,
+                                    int *blocks);
+hipError_t resident_pipeline_nt128(int dtype, int max_cols, size_t lds, int *blocks);
+hipError_t resident_pipeline_nt256(int dtype, int max_cols, size_t lds, int *blocks);
+hipError_t resident_pipeline_nt512(int dtype, int max_cols, size_t lds, int *blocks);
 // Per-panel normalization factor (into *factor, NaN: leave the panel alone)
 // of the live rows of x; scratch: rows doubles, 2 rows uint32 and, for a
 // percentile, 65536 uint32 of histogram (cleared here).
@@ -133,6 +145,21 @@ hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t c
 hipError_t launch_correlate(int dtype, const DevCorr *dev_plan, size_t lds_bytes,
                             const void *src, int32_t rows, int32_t src_cols,
                             const int32_t *row_len, void *dst, hipStream_t stream);
-int pipeline_max_cols();
+// Dynamic LDS one row of the pipeline kernel may hold (the workgroup's
+// 160 KiB less room for the kernel's static LDS).
+constexpr size_t kMaxRowLds = 160 * 1024 - 2048;
+// The widest row each thread count serves (samples per thread x threads).
+constexpr int pipeline_max_cols(int threads) {
+  return threads == 128 ? 128 * 24 : threads == 256 ? 256 * 24 : threads == 512 ? 512 * 24 : 0;
+}
+// The thread count of a plan compiled for rows of `samples` samples: the
+// fewest threads whose widest row holds them (0: wider than any).  A plan
+// keeps it for every apply, so a row's arithmetic (the order of its block
+// reductions) depends on its plan alone, never on the batch around it.
+constexpr int pipeline_threads_for(int samples) {
+  return samples <= pipeline_max_cols(128) ? 128
+         : samples <= pipeline_max_cols(256) ? 256
+         : samples <= pipeline_max_cols(512) ? 512 : 0;
+}
 
 }  // namespace pseg

@@ -1,21 +1,5 @@
-// gfx950 kernels of the processSEG panel hot path.
-//
-// pipeline_kernel: one workgroup (256 threads, 4 wave64) per row.  The row
-// crosses HBM once: it is loaded into registers (sample j on thread j mod 256,
-// slot j / 256, so every load and store is coalesced), every step of the plan
-// runs on it there and in LDS, and it is stored once.  Steps:
-//   demean / detrend / scale / clip / polarity   core/basic.nim:52-110
-//   FFT filter (trend, pads, rFFT, gain, irFFT)  filter/apply.nim:194-363
-//   AGC (block prefix/suffix sums, gain step)     gain/gain.nim:351-461
-//   per-trace normalization (max, rms, exact percentile by radix select)
-//                                                 gain/gain.nim:198-219, gain/stats.nim:159-182
-// The real FFT of n points is a complex Stockham FFT of m = n/2 points on the
-// interleaved samples (fft/plan.nim:501-594); its stages ping-pong between two
-// LDS buffers of m complex values.  Split, gain and the inverse's unsplit are
-// one fused pass over the bins, and the inverse transform is the forward one
-// on swapped (im, re) pairs, so the swap back is an XOR of the sample index.
-// Sums (demean, trend, rms, AGC windows) run in float64 as in the reference;
-// the AGC window sums are block prefix and suffix sums without subtraction.
+// gfx950 kernels beside the row pipeline: vibroseis correlation and the
+// per-panel normalization statistics, and the launch dispatch.
 //
 // corr_kernel: one workgroup (1024 threads) per record: gather to the live
 // length, rFFT, times the device-resident pilot multiplier, irFFT, lag spans
@@ -25,1106 +9,9 @@
 // float32 |x| bits >> 15 with integer atomics (bit-exact counts), and a
 // one-block kernel for the per-panel factor (gain/stats.nim:253-304,
 // gain/gain.nim:221-240).
-#include "pseg_internal.hpp"
-
-#include <math.h>
+#include "pseg_device.hpp"
 
 namespace pseg {
-
-namespace {
-
-#ifndef PSEG_NT
-#define PSEG_NT 128
-#endif
-constexpr int NT = PSEG_NT;       // threads (one row) of the pipeline kernel
-constexpr int NW = NT / 64;       // waves of the pipeline kernel
-
-// Plan tables live in global memory; loading them through global (not
-// flat) pointers keeps them off the LDS path.
-template <typename X>
-using gptr = const X __attribute__((address_space(1))) *;
-template <typename X>
-__device__ __forceinline__ gptr<X> as_global(const void *p) {
-  return (gptr<X>)(const X *)p;
-}
-
-template <typename T> struct V2T;
-template <> struct V2T<float> { using type = float2; };
-template <> struct V2T<double> { using type = double2; };
-
-template <typename T> using V2 = typename V2T<T>::type;
-
-template <typename T>
-__device__ __forceinline__ V2<T> mk(T x, T y) {
-  V2<T> r;
-  r.x = x;
-  r.y = y;
-  return r;
-}
-
-// threadIdx.x behind an empty asm barrier: values computed from it inside a
-// step cannot be hoisted out of the loop over the plan's steps (which would
-// keep a dozen per-sample indices live across every step and spill them).
-__device__ __forceinline__ int opaque_tid() {
-  int t = threadIdx.x;
-  asm volatile("" : "+v"(t));
-  return t;
-}
-
-// Plan parameters are the same on every lane; reading them through
-// readfirstlane puts them (and everything computed from them) in SGPRs
-// instead of a VGPR per lane.
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint32_t uni(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
-}
-__device__ __forceinline__ double uni(double v) {
-  long long b = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xFFFFFFFF));
-  int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <typename P>
-__device__ __forceinline__ P *uni_ptr(P *p) {
-  unsigned long long b = (unsigned long long)p;
-  unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xFFFFFFFFull));
-  unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
-  return (P *)(((unsigned long long)hi << 32) | lo);
-}
-__device__ __forceinline__ FastDiv uni(FastDiv d) {
-  FastDiv r;
-  r.d = uni(d.d);
-  r.mul = uni(d.mul);
-  r.shift = uni(d.shift);
-  return r;
-}
-
-// Ends one iteration of an unrolled per-sample loop: the scheduler may not
-// interleave iterations, so a heavy loop body holds one sample's temporaries
-// at a time instead of PER of them.
-#define PSEG_ONE_AT_A_TIME() __builtin_amdgcn_sched_barrier(0)
-
-__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv d) {
-  return (__umulhi(n, d.mul) + n) >> d.shift;
-}
-
-// --------------------------------------------------------------- reductions
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// The sum of v over the block, the same value on every thread, in a fixed
-// order (lane 0 of each wave, then the waves in order): a row's result does
-// not depend on anything but the row.
-template <int NTH>
-__device__ __forceinline__ double block_sum(double v, double *red) {
-  const int tid_ = opaque_tid();
-  v = wave_sum(v);
-  __syncthreads();
-  if ((tid_ & 63) == 0) red[tid_ >> 6] = v;
-  __syncthreads();
-  double t = 0;
-#pragma unroll
-  for (int i = 0; i < NTH / 64; i++) t += red[i];
-  return t;
-}
-
-template <int NTH>
-__device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
-  const int tid_ = opaque_tid();
-  a = wave_sum(a);
-  b = wave_sum(b);
-  __syncthreads();
-  if ((tid_ & 63) == 0) {
-    red[tid_ >> 6] = a;
-    red[NTH / 64 + (tid_ >> 6)] = b;
-  }
-  __syncthreads();
-  double ta = 0, tb = 0;
-#pragma unroll
-  for (int i = 0; i < NTH / 64; i++) {
-    ta += red[i];
-    tb += red[NTH / 64 + i];
-  }
-  a = ta;
-  b = tb;
-}
-
-template <typename U>
-__device__ __forceinline__ U wave_max_u(U v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    U w = __shfl_xor(v, o, 64);
-    v = w > v ? w : v;
-  }
-  return v;
-}
-
-// ------------------------------------------------------------ bit helpers
-
-template <typename T> struct Bits;
-template <> struct Bits<float> {
-  using S = int32_t;
-  using U = uint32_t;
-  static constexpr int width = 32;
-  __device__ static S abs_bits(float x) { return (S)(__float_as_uint(x) & 0x7FFFFFFFu); }
-  __device__ static double value(S b) { return (double)__uint_as_float((uint32_t)b); }
-};
-template <> struct Bits<double> {
-  using S = long long;
-  using U = unsigned long long;
-  static constexpr int width = 64;
-  __device__ static S abs_bits(double x) {
-    return (S)(__double_as_longlong(x) & 0x7FFFFFFFFFFFFFFFll);
-  }
-  __device__ static double value(S b) { return __longlong_as_double(b); }
-};
-
-// fitsIn (gain/gain.nim:112-118): finite and positive, also in T.
-template <typename T>
-__device__ __forceinline__ bool fits_in(double x) {
-  if (!(isfinite(x) && x > 0)) return false;
-  double t = (double)(T)x;
-  return isfinite(t) && t > 0;
-}
-
-// --------------------------------------------------------------- FFT stage
-
-// A table of complex T (interleaved re, im) in global memory.
-template <typename T> struct CTab {
-  gptr<T> p;
-  __device__ __forceinline__ V2<T> operator[](int i) const { return mk<T>(p[2 * i], p[2 * i + 1]); }
-  __device__ __forceinline__ CTab operator+(int o) const { return CTab{p + 2 * o}; }
-  __device__ __forceinline__ CTab operator-(int o) const { return CTab{p - 2 * o}; }
-};
-template <typename T>
-__device__ __forceinline__ CTab<T> ctab(const void *p) { return CTab<T>{as_global<T>(p)}; }
-
-template <typename T>
-__device__ __forceinline__ V2<T> cmul(V2<T> a, V2<T> w) {
-  return mk<T>(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
-}
-
-// Radix-R butterflies of the Stockham DIF stage (fft/plan.nim:225-366),
-// in place on v[0 .. R).
-template <typename T, int R> struct Bfly;
-
-template <typename T> struct Bfly<T, 2> {
-  __device__ static void run(V2<T> *v) {
-    V2<T> u = v[0], w = v[1];
-    v[0] = mk<T>(u.x + w.x, u.y + w.y);
-    v[1] = mk<T>(u.x - w.x, u.y - w.y);
-  }
-};
-
-template <typename T> struct Bfly<T, 3> {
-  __device__ static void run(V2<T> *v) {
-    const T c = (T)(-0.5);
-    const T s = (T)0.8660254037844386467637231707529362;
-    V2<T> a = v[0];
-    T t1r = v[1].x + v[2].x, t1i = v[1].y + v[2].y;
-    T t2r = v[1].x - v[2].x, t2i = v[1].y - v[2].y;
-    T m1r = a.x + c * t1r, m1i = a.y + c * t1i;
-    T m2r = s * t2i, m2i = -s * t2r;
-    v[0] = mk<T>(a.x + t1r, a.y + t1i);
-    v[1] = mk<T>(m1r + m2r, m1i + m2i);
-    v[2] = mk<T>(m1r - m2r, m1i - m2i);
-  }
-};
-
-template <typename T> struct Bfly<T, 4> {
-  __device__ static void run(V2<T> *v) {
-    T t0r = v[0].x + v[2].x, t0i = v[0].y + v[2].y;
-    T t1r = v[0].x - v[2].x, t1i = v[0].y - v[2].y;
-    T t2r = v[1].x + v[3].x, t2i = v[1].y + v[3].y;
-    T t3r = v[1].y - v[3].y, t3i = v[3].x - v[1].x;  // (x1 - x3)(-i)
-    v[0] = mk<T>(t0r + t2r, t0i + t2i);
-    v[1] = mk<T>(t1r + t3r, t1i + t3i);
-    v[2] = mk<T>(t0r - t2r, t0i - t2i);
-    v[3] = mk<T>(t1r - t3r, t1i - t3i);
-  }
-};
-
-template <typename T> struct Bfly<T, 5> {
-  __device__ static void run(V2<T> *v) {
-    const T c1 = (T)0.3090169943749474241022934171828191;
-    const T c2 = (T)(-0.8090169943749474241022934171828191);
-    const T s1 = (T)0.9510565162951535721164393333793821;
-    const T s2 = (T)0.5877852522924731291687059546390728;
-    V2<T> a = v[0];
-    T t1r = v[1].x + v[4].x, t1i = v[1].y + v[4].y;
-    T t2r = v[2].x + v[3].x, t2i = v[2].y + v[3].y;
-    T t3r = v[1].x - v[4].x, t3i = v[1].y - v[4].y;
-    T t4r = v[2].x - v[3].x, t4i = v[2].y - v[3].y;
-    T b1r = a.x + c1 * t1r + c2 * t2r, b1i = a.y + c1 * t1i + c2 * t2i;
-    T b2r = a.x + c2 * t1r + c1 * t2r, b2i = a.y + c2 * t1i + c1 * t2i;
-    T d1r = s1 * t3r + s2 * t4r, d1i = s1 * t3i + s2 * t4i;
-    T d2r = s2 * t3r - s1 * t4r, d2i = s2 * t3i - s1 * t4i;
-    v[0] = mk<T>(a.x + t1r + t2r, a.y + t1i + t2i);
-    v[1] = mk<T>(b1r + d1i, b1i - d1r);
-    v[4] = mk<T>(b1r - d1i, b1i + d1r);
-    v[2] = mk<T>(b2r + d2i, b2i - d2r);
-    v[3] = mk<T>(b2r - d2i, b2i + d2r);
-  }
-};
-
-// One Stockham stage x -> y: y[q + s(rp + k)] = (sum_j x[q + s(p + j sub)]
-// w_r^{jk}) w_cur^{pk}, butterfly b = p s + q.
-template <typename T, int R, int NTH>
-__device__ __forceinline__ void stage(const V2<T> *__restrict__ x, V2<T> *__restrict__ y,
-                                      CTab<T> tw, int m, int s, int sub, FastDiv sd) {
-  const int tid_ = opaque_tid();
-  const int nb = m / R;
-  const bool twiddle = sub > 1;
-  const int in_step = s * sub;   // between the inputs of a butterfly
-#pragma clang loop unroll(disable)
-  for (int b = tid_; b < nb; b += NTH) {
-    const int p = (int)fdiv((uint32_t)b, sd);
-    const int q = b - p * s;
-    const V2<T> *xi = x + (q + s * p);
-    V2<T> *yo = y + (q + s * R * p);
-    V2<T> v[R];
-#pragma unroll
-    for (int j = 0; j < R; j++) v[j] = xi[j * in_step];
-    Bfly<T, R>::run(v);
-    if (twiddle) {
-      CTab<T> t = tw + p * (R - 1) - 1;
-#pragma unroll
-      for (int k = 1; k < R; k++) v[k] = cmul<T>(v[k], t[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < R; k++) yo[k * s] = v[k];
-  }
-}
-
-// Two consecutive Stockham stages (radix R1 at stride s, then R2 at stride
-// s R1) in one pass through LDS.  Butterfly b = p2 s + q loads the R1 R2
-// inputs x[q + s(p2 + j2 sub2 + j sub1)], runs the R2 radix-R1 butterflies of
-// the first stage (p = p2 + j2 sub2) with their twiddles, then the R1
-// radix-R2 butterflies of the second (p2) with theirs, and stores
-// y[q + s k + s R1 (R2 p2 + k2)]: the same operations, in the same order, as
-// the two stages one after the other, without the round trip between them.
-template <typename T, int R1, int R2, int NTH>
-__device__ __forceinline__ void stage2(const V2<T> *__restrict__ x, V2<T> *__restrict__ y,
-                                       CTab<T> tw1, CTab<T> tw2, int m, int s, int sub1,
-                                       FastDiv sd) {
-  const int tid_ = opaque_tid();
-  const int sub2 = sub1 / R2;
-  const int nb = m / (R1 * R2);
-  const bool twiddle2 = sub2 > 1;
-  const int in1 = s * sub1, in2 = s * sub2;  // input strides of j and j2
-  const int out2 = s * R1;                   // output stride of k2
-#pragma clang loop unroll(disable)
-  for (int b = tid_; b < nb; b += NTH) {
-    const int p2 = (int)fdiv((uint32_t)b, sd);
-    const int q = b - p2 * s;
-    const V2<T> *xi = x + (q + s * p2);
-    V2<T> *yo = y + (q + s * R1 * R2 * p2);
-    V2<T> v[R2][R1];
-#pragma unroll
-    for (int j2 = 0; j2 < R2; j2++)
-#pragma unroll
-      for (int j = 0; j < R1; j++) v[j2][j] = xi[j2 * in2 + j * in1];
-    CTab<T> t1 = tw1 + p2 * (R1 - 1) - 1;
-    const int t1_step = sub2 * (R1 - 1);
-#pragma unroll
-    for (int j2 = 0; j2 < R2; j2++) {
-      Bfly<T, R1>::run(v[j2]);
-#pragma unroll
-      for (int k = 1; k < R1; k++) v[j2][k] = cmul<T>(v[j2][k], t1[j2 * t1_step + k]);
-    }
-    CTab<T> t2 = tw2 + p2 * (R2 - 1) - 1;
-#pragma unroll
-    for (int k = 0; k < R1; k++) {
-      V2<T> u[R2];
-#pragma unroll
-      for (int j2 = 0; j2 < R2; j2++) u[j2] = v[j2][k];
-      Bfly<T, R2>::run(u);
-      if (twiddle2) {
-#pragma unroll
-        for (int k2 = 1; k2 < R2; k2++) u[k2] = cmul<T>(u[k2], t2[k2]);
-      }
-#pragma unroll
-      for (int k2 = 0; k2 < R2; k2++) yo[k * s + k2 * out2] = u[k2];
-    }
-  }
-}
-
-// Stage pairs whose butterfly holds at most 16 points run fused; the
-// register footprint of larger ones would cost more occupancy than the
-// LDS round trip they save.
-__host__ __device__ constexpr bool fusable(int r1, int r2) { return r1 * r2 <= 16; }
-
-template <typename T, int R1, int NTH>
-__device__ __forceinline__ void stage2_second(int r2, const V2<T> *x, V2<T> *y, CTab<T> t1,
-                                              CTab<T> t2, int m, int s, int sub1,
-                                              FastDiv sd) {
-  switch (r2) {
-    case 4: if constexpr (fusable(R1, 4)) stage2<T, R1, 4, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-    case 2: if constexpr (fusable(R1, 2)) stage2<T, R1, 2, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-    case 3: if constexpr (fusable(R1, 3)) stage2<T, R1, 3, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-    default: if constexpr (fusable(R1, 5)) stage2<T, R1, 5, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-  }
-}
-
-// The complex forward FFT of a (m points) with b as the other buffer: the
-// plan's stages in pairs; returns the buffer holding the result.
-template <typename T, int NTH>
-__device__ __forceinline__ V2<T> *complex_forward(const DevFft &f, V2<T> *a, V2<T> *b) {
-  CTab<T> tw = ctab<T>(uni_ptr(f.tw));
-  V2<T> *x = a, *y = b;
-  int s = 1;
-  const int nrad = uni(f.nrad), m = uni(f.m);
-  for (int i = 0; i < nrad;) {
-    const int r = uni(f.rad[i]), sub = uni(f.sub[i]);
-    const FastDiv sd = uni(f.sdiv[i]);
-    CTab<T> t = tw + uni(f.tw_off[i]);
-    const int r2 = i + 1 < nrad ? uni(f.rad[i + 1]) : 0;
-    if (i + 1 < nrad && fusable(r, r2)) {
-      CTab<T> t2 = tw + uni(f.tw_off[i + 1]);
-      switch (r) {
-        case 4: stage2_second<T, 4, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-        case 2: stage2_second<T, 2, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-        case 3: stage2_second<T, 3, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-        default: stage2_second<T, 5, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-      }
-      s *= r * r2;
-      i += 2;
-    } else {
-      switch (r) {
-        case 4: stage<T, 4, NTH>(x, y, t, m, s, sub, sd); break;
-        case 2: stage<T, 2, NTH>(x, y, t, m, s, sub, sd); break;
-        case 3: stage<T, 3, NTH>(x, y, t, m, s, sub, sd); break;
-        default: stage<T, 5, NTH>(x, y, t, m, s, sub, sd); break;
-      }
-      s *= r;
-      i += 1;
-    }
-    __syncthreads();
-    V2<T> *tmp = x;
-    x = y;
-    y = tmp;
-  }
-  return x;
-}
-
-// splitForward (fft/plan.nim:470-482): bin k of the real transform from
-// Z[k] = a and Z[m-k] = b.
-template <typename T>
-__device__ __forceinline__ V2<T> split_fwd(V2<T> a, V2<T> b, V2<T> w) {
-  T er = (T)0.5 * (a.x + b.x);
-  T ei = (T)0.5 * (a.y - b.y);
-  T orr = (T)0.5 * (a.y + b.y);
-  T oi = (T)0.5 * (b.x - a.x);
-  return mk<T>(er + w.x * orr - w.y * oi, ei + w.x * oi + w.y * orr);
-}
-
-// splitInverse (fft/plan.nim:484-499) with w = (cos, -sin): the swapped
-// (Im Z', Re Z') of the inverse's input from X[k] = a and X[m-k] = b.
-template <typename T>
-__device__ __forceinline__ V2<T> split_inv(V2<T> a, V2<T> b, V2<T> w, T h) {
-  T er = a.x + b.x, ei = a.y - b.y, dr = a.x - b.x, di = a.y + b.y;
-  T wr = w.x, wi = -w.y;
-  T orr = dr * wr - di * wi;
-  T oi = dr * wi + di * wr;
-  return mk<T>((ei + orr) * h, (er - oi) * h);
-}
-
-// Bin multipliers of the fused split / multiply / unsplit pass.
-template <typename T> struct GainMul {
-  gptr<T> g;
-  __device__ V2<T> operator()(V2<T> x, int k) const {
-    T w = g[k];
-    return mk<T>(x.x * w, x.y * w);
-  }
-};
-template <typename T> struct CplxMul {
-  CTab<T> mul;
-  __device__ V2<T> operator()(V2<T> x, int k) const {
-    V2<T> w = mul[k];
-    return mk<T>(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
-  }
-};
-
-// Z (the forward complex FFT of the packed samples) -> W, the swapped input
-// of the inverse: split into bins, multiply, unsplit with the 1/n scale.
-template <typename T, int NTH, typename Mul>
-__device__ __forceinline__ void combine(const DevFft &f, const V2<T> *__restrict__ Z, V2<T> *__restrict__ W,
-                        Mul mul) {
-  const int tid_ = opaque_tid();
-  const int m = uni(f.m);
-  CTab<T> sp = ctab<T>(uni_ptr(f.split));
-  const T scale = (T)1 / (T)uni(f.n);
-  for (int k = tid_; k <= m / 2; k += NTH) {
-    if (k == 0) {
-      V2<T> z = Z[0];
-      V2<T> x0 = mul(mk<T>(z.x + z.y, (T)0), 0);
-      V2<T> xm = mul(mk<T>(z.x - z.y, (T)0), m);
-      W[0] = mk<T>((x0.x - xm.x) * scale, (x0.x + xm.x) * scale);
-    } else {
-      const int kk = m - k;
-      V2<T> a = Z[k], b = Z[kk];
-      V2<T> xk = mul(split_fwd<T>(a, b, sp[k]), k);
-      if (kk != k) {
-        V2<T> xkk = mul(split_fwd<T>(b, a, sp[kk]), kk);
-        W[k] = split_inv<T>(xk, xkk, sp[k], scale);
-        W[kk] = split_inv<T>(xkk, xk, sp[kk], scale);
-      } else {
-        W[k] = split_inv<T>(xk, xk, sp[k], scale);
-      }
-    }
-  }
-}
-
-// ------------------------------------------------------------ row helpers
-
-template <typename T, int PER>
-__device__ __forceinline__ double row_sum(const T (&v)[PER], int n) {
-  const int tid_ = opaque_tid();
-  double s = 0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) s += (double)v[k];
-  }
-  return s;
-}
-
-// Least-squares line (filter/edges.nim:16-30, core/basic.nim:61-81 share it).
-template <typename T, int PER>
-__device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, double *red, double &a,
-                         double &b) {
-  const int tid_ = opaque_tid();
-  if (trend == 0) {
-    a = 0;
-    b = 0;
-    return;
-  }
-  double sx = 0, sxi = 0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) {
-      sx += (double)v[k];
-      sxi += (double)j * (double)v[k];
-    }
-  }
-  if (trend == 1) {
-    a = block_sum<NT>(sx, red) / (double)n;
-    b = 0;
-    return;
-  }
-  block_sum2<NT>(sx, sxi, red);
-  double nf = (double)n, mean = sx / nf;
-  if (n == 1) {
-    a = mean;
-    b = 0;
-    return;
-  }
-  double c = (nf - 1) / 2;
-  double slope = (sxi - c * sx) / (nf * (nf * nf - 1) / 12);
-  a = mean - slope * c;
-  b = slope;
-}
-
-// One FFT filter step (filter/apply.nim:340-363) on the row in registers.
-template <typename T, int PER>
-__device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int n, char *lds, double *red) {
-  const int tid_ = opaque_tid();
-  const DevFft &f = F.fft;
-  T *buf = (T *)lds;                          // nfft samples = m complex
-  V2<T> *A = (V2<T> *)lds;
-  const int fn = uni(f.n);
-  V2<T> *B = (V2<T> *)(lds + sizeof(T) * (size_t)fn);
-  double a, b;
-  trend_of<T, PER>(v, n, uni(F.trend), red, a, b);
-  const int p = uni(F.pad), nf = fn;
-  const int reflect = uni(F.reflect);
-  // pack (packInto, filter/apply.nim:194-216): detrended data ...
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) buf[p + j] = (T)((double)v[k] - (a + b * (double)j));
-    PSEG_ONE_AT_A_TIME();
-  }
-  if (!reflect) {
-    for (int j = tid_; j < p; j += NT) buf[j] = (T)0;
-    for (int j = p + n + tid_; j < nf; j += NT) buf[j] = (T)0;
-  } else {
-    for (int j = 2 * p + n + tid_; j < nf; j += NT) buf[j] = (T)0;
-  }
-  __syncthreads();
-  if (reflect) {
-    // ... and odd reflections 2 x_edge - x_k tapered by fall[k - 1]
-    gptr<T> fall = as_global<T>(uni_ptr(F.fall));
-    const T d0 = buf[p], dn = buf[p + n - 1];
-    for (int k = 1 + tid_; k <= p; k += NT) {
-      int mk_ = k < n - 1 ? k : n - 1;
-      T fk = fall[k - 1];
-      T rl = buf[p + mk_];
-      T rr = buf[p + n - 1 - mk_];
-      buf[p - k] = (d0 + d0 - rl) * fk;
-      buf[p + n - 1 + k] = (dn + dn - rr) * fk;
-    }
-    __syncthreads();
-  }
-  // forward FFT, split * gain * unsplit, inverse FFT: one copy of the FFT
-  // code serves both directions
-  V2<T> *R = A;
-#pragma nounroll
-  for (int dir = 0; dir < 2; dir++) {
-    R = complex_forward<T, NT>(f, R, R == A ? B : A);
-    if (dir == 0) {
-      V2<T> *W = R == A ? B : A;
-      combine<T, NT>(f, R, W, GainMul<T>{as_global<T>(uni_ptr(F.gain))});
-      __syncthreads();
-      R = W;
-    }
-  }
-  const T *out = (const T *)R;
-  // unpack (unpackFrom, filter/apply.nim:218-231): sample i of the inverse is
-  // element i ^ 1 of the swapped result.
-  const double dcg = uni(F.dc_gain);
-  double gb = b * dcg;
-  double ga = uni(F.zero_dc) ? -gb * (double)(n - 1) / 2 : a * dcg;
-  const bool plain = ga == 0 && gb == 0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) {
-      T y = out[(p + j) ^ 1];
-      v[k] = plain ? y : (T)((double)y + (ga + gb * (double)j));
-    }
-    PSEG_ONE_AT_A_TIME();
-  }
-  __syncthreads();
-}
-
-// A segmented scan value: a float64 sum and a flag.  Forward (prefix)
-// combine(L, R) = (R.f ? R.v : L.v + R.v, L.f | R.f): R's flag means a block
-// starts in R, so nothing left of it carries over.  Backward (suffix)
-// combine(L, R) = (L.v + (L.f ? 0 : R.v), L.f | R.f).
-struct Seg {
-  double v;
-  int f;
-};
-
-// Exclusive segmented scan over the NT threads of the block in thread order
-// (dir 0: from thread 0 up, prefix; dir 1: from the last thread down,
-// suffix).  red: 2 NW doubles of LDS.
-__device__ __forceinline__ Seg block_seg_scan_excl(Seg x, int dir, double *red) {
-  const int tid_ = opaque_tid();
-  const int lane = tid_ & 63, wave = tid_ >> 6;
-  Seg incl = x;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    double ov = dir == 0 ? __shfl_up(incl.v, d, 64) : __shfl_down(incl.v, d, 64);
-    int of = dir == 0 ? __shfl_up(incl.f, d, 64) : __shfl_down(incl.f, d, 64);
-    bool take = dir == 0 ? lane >= d : lane + d < 64;
-    if (take) {
-      if (!incl.f) incl.v = ov + incl.v;
-      incl.f |= of;
-    }
-  }
-  // this lane's exclusive value within the wave
-  Seg ex;
-  ex.v = dir == 0 ? __shfl_up(incl.v, 1, 64) : __shfl_down(incl.v, 1, 64);
-  ex.f = dir == 0 ? __shfl_up(incl.f, 1, 64) : __shfl_down(incl.f, 1, 64);
-  const bool first = dir == 0 ? lane == 0 : lane == 63;
-  if (first) {
-    ex.v = 0;
-    ex.f = 0;
-  }
-  // the wave aggregates, then the waves before this one (in scan order)
-  __syncthreads();
-  if ((dir == 0 && lane == 63) || (dir == 1 && lane == 0)) {
-    red[wave] = incl.v;
-    red[NW + wave] = (double)incl.f;
-  }
-  __syncthreads();
-  Seg carry;
-  carry.v = 0;
-  carry.f = 0;
-  if (dir == 0) {
-    for (int w = 0; w < wave; w++) {
-      int wf = red[NW + w] != 0;
-      carry.v = wf ? red[w] : carry.v + red[w];
-      carry.f |= wf;
-    }
-    // combine(carry, ex)
-    Seg r;
-    r.v = ex.f ? ex.v : carry.v + ex.v;
-    r.f = carry.f | ex.f;
-    return r;
-  }
-  for (int w = NW - 1; w > wave; w--) {
-    int wf = red[NW + w] != 0;
-    carry.v = red[w] + (wf ? 0.0 : carry.v);
-    carry.f |= wf;
-  }
-  // combine(ex, carry)
-  Seg r;
-  r.v = ex.v + (ex.f ? 0.0 : carry.v);
-  r.f = ex.f | carry.f;
-  return r;
-}
-
-// AGC (gain/gain.nim:404-461) on the row in registers.  The window sums are
-// the reference's block prefix and suffix sums (blocks of w = 2h + 1 samples,
-// gain/gain.nim:351-390) computed by the whole block: each thread takes a
-// contiguous run of samples, segmented scans carry the float64 sums across
-// runs, and no sum ever subtracts.
-template <typename T, int PER>
-__device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, char *lds,
-                                         double *red) {
-  const int tid_ = opaque_tid();
-  const int mode = uni(G.mode);
-  const double gtarget = uni(G.target), geps = uni(G.epsilon);
-  double sq = 0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) sq += (double)v[k] * (double)v[k];
-  }
-  const double rms = sqrt(block_sum<NT>(sq, red) / (double)n);
-  if (rms == 0) return;  // all zero: nothing to scale
-  if (!fits_in<T>(gtarget / rms / geps)) return;
-  const int h = uni(G.half), w = 2 * h + 1;
-  T *s = (T *)lds;  // x, then the suffix sums in place
-  T *pp = s + n;    // prefix sums, n + 1 values
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) s[j] = v[k];
-  }
-  __syncthreads();
-  const int c = (n + NT - 1) / NT;  // <= PER
-  const int c0 = min((int)tid_ * c, n), c1 = min(c0 + c, n);
-  double tv[PER];
-  Seg pre, suf;
-  pre.v = 0;
-  pre.f = 0;
-  suf.v = 0;
-  suf.f = 0;
-  bool past_start = false;  // a block start seen after c0 (suffix)
-  // e % w, kept incrementally (no division per sample)
-  const int r0 = c0 < n ? c0 % w : 0;
-  int r = r0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int e = c0 + k;
-    double t = 0;
-    if (e < c1) {
-      const double x = (double)s[e];
-      t = mode == 0 ? x * x : fabs(x);
-      const bool start = r == 0;
-      // prefix: the sum since the last block start in the run
-      pre.v = start ? t : pre.v + t;
-      pre.f |= start;
-      // suffix: the run's terms before the first block start after c0
-      if (start && k > 0) past_start = true;
-      if (!past_start) suf.v += t;
-      r = r + 1 == w ? 0 : r + 1;
-    }
-    tv[k] = t;
-    PSEG_ONE_AT_A_TIME();
-  }
-  suf.f = (c0 < c1) && (r0 == 0 || past_start);
-  if (c0 < c1 && r0 == 0) suf.v = 0;
-  Seg cpre = block_seg_scan_excl(pre, 0, red);
-  Seg csuf = block_seg_scan_excl(suf, 1, red);
-  // prefix sums p[e + 1] (0 at a block start) ...
-  double run = cpre.v;
-  r = r0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int e = c0 + k;
-    if (e < c1) {
-      if (r == 0) run = 0;
-      run += tv[k];
-      r = r + 1 == w ? 0 : r + 1;
-      pp[e + 1] = r == 0 ? (T)0 : (T)run;
-    }
-  }
-  if (tid_ == 0) pp[0] = (T)0;
-  // ... and suffix sums s[e], over x in place (this thread's run only);
-  // the run's last sample has remainder (c1 - 1) % w = (r0 + len - 1) % w
-  const int len = c1 - c0;
-  run = (c1 < n && (r0 + len) % w != 0) ? csuf.v : 0.0;
-  r = len > 0 ? (r0 + len - 1) % w : 0;
-#pragma unroll
-  for (int k = PER - 1; k >= 0; k--) {
-    const int e = c0 + k;
-    if (e < c1) {
-      run += tv[k];
-      s[e] = (T)run;
-      if (r == 0) run = 0;
-      r = r == 0 ? w - 1 : r - 1;
-    }
-  }
-  __syncthreads();
-  // The gain step in T on means scaled by the trace level, five ranges of i.
-  const double unit = mode == 0 ? 1 / (rms * rms) : 1 / rms;
-  const T kk = (T)(gtarget / rms);
-  const T eps = (T)geps;
-  const int first = min(h, n);
-  const int last = max(first, n - h);
-  const int headCut = min(max(n - h, 0), first);
-  const int lastBlock = (n - 1) / w * w;
-  const int tailCut = min(max(lastBlock + h, last), n);
-  const double pn = (double)pp[n];
-  // 1 / count for a window of `count` samples: inv[c] = 1/c; the tail's
-  // rev[m] = 1/(2h+1-m) is inv[2h+1-m], the same float64 value.
-  gptr<double> inv = as_global<double>(uni_ptr(G.inv));
-  const double unitW = unit / (double)w;
-  const double headWhole = headCut < first ? pn * unit * inv[n] : 0.0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int i = tid_ + NT * k;
-    if (i < n) {
-      // the reference's five ranges as one expression: mean = (sv + pv) * M
-      // with sv, pv 0 where a range does not add them (x + 0 = x exactly)
-      const bool head = i < headCut, mid = i >= first && i < last;
-      const bool whole = i >= headCut && i < first, tail1 = i >= last && i < tailCut;
-      const double sv = i >= first ? (double)s[max(i - h, 0)] : 0.0;
-      const double pv = (head || mid) ? (double)pp[min(i + h + 1, n)] : (tail1 ? pn : 0.0);
-      double M = unitW;
-      if (!mid) {
-        const int c = head ? i + h + 1 : n - i + h;
-        M = unit * inv[min(max(c, 1), w)];
-      }
-      const double mean = whole ? headWhole : (sv + pv) * M;
-      const T lvl = mode == 0 ? (T)sqrt((T)mean) : (T)mean;
-      const T gi = kk / (lvl + eps);
-      v[k] = (T)(v[k] * gi);
-    }
-    PSEG_ONE_AT_A_TIME();
-  }
-  __syncthreads();
-}
-
-// The k-th smallest of the row's |x| bit patterns (exact), by radix select
-// over 8-bit digits with an LDS histogram.
-template <typename T, int PER>
-__device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], int n, int64_t rank,
-                                           uint32_t *hist, int64_t *shared_i) {
-  const int tid_ = opaque_tid();
-  using S = typename Bits<T>::S;
-  using U = typename Bits<T>::U;
-  U prefix = 0, mask = 0;
-  int64_t r = rank;
-  for (int shift = Bits<T>::width - 8; shift >= 0; shift -= 8) {
-    for (int i = tid_; i < 256; i += NT) hist[i] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      int j = tid_ + NT * k;
-      if (j < n) {
-        U key = (U)Bits<T>::abs_bits(v[k]);
-        if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-      }
-    }
-    __syncthreads();
-    if (tid_ == 0) {
-      int64_t below = 0;
-      int d = 0;
-      for (; d < 256; d++) {
-        int64_t c = hist[d];
-        if (below + c > r) break;
-        below += c;
-      }
-      shared_i[0] = d;
-      shared_i[1] = r - below;
-    }
-    __syncthreads();
-    prefix |= (U)shared_i[0] << shift;
-    mask |= (U)255 << shift;
-    r = shared_i[1];
-    __syncthreads();
-  }
-  return (S)prefix;
-}
-
-// Per-trace normalization (gain/gain.nim:198-219).
-template <typename T, int PER>
-__device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds,
-                                double *red) {
-  const int tid_ = opaque_tid();
-  using S = typename Bits<T>::S;
-  __shared__ S smax[NW];
-  __shared__ int64_t sh_i[2];
-  // max |x| as the integer maximum of the sign-cleared bits (a NaN sorts
-  // above Inf: gain/stats.nim:97-117)
-  S mb = 0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    if (j < n) {
-      S b = Bits<T>::abs_bits(v[k]);
-      mb = b > mb ? b : mb;
-    }
-  }
-  mb = wave_max_u<S>(mb);
-  __syncthreads();
-  if ((tid_ & 63) == 0) smax[tid_ >> 6] = mb;
-  __syncthreads();
-  mb = smax[0];
-  for (int i = 1; i < NW; i++) mb = smax[i] > mb ? smax[i] : mb;
-  const double mx = Bits<T>::value(mb);
-  double level;
-  const int stat = uni(st.stat);
-  const double starget = uni(st.target);
-  if (stat == 0) {
-    level = mx;
-  } else if (stat == 1) {
-    double sq = 0;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      int j = tid_ + NT * k;
-      if (j < n) sq += (double)v[k] * (double)v[k];
-    }
-    level = sqrt(block_sum<NT>(sq, red) / (double)n);
-  } else {
-    // percentileAbs (gain/stats.nim:159-182): linear between order
-    // statistics k and k + 1 of |x|, h = (n - 1) p / 100.
-    if (isnan(mx)) return;
-    double hh = (double)(n - 1) * uni(st.percentile) / 100;
-    int64_t k = (int64_t)floor(hh);
-    if (k > n - 1) k = n - 1;
-    uint32_t *hist = (uint32_t *)lds;
-    S lo = select_bits<T, PER>(v, n, k, hist, sh_i);
-    double lower = Bits<T>::value(lo);
-    if (k + 1 >= n) {
-      level = lower;
-    } else {
-      // the next order statistic: lower again if it is tied, else the
-      // smallest value above it
-      double cnt = 0;
-      S up = (S)0x7FFFFFFFFFFFFFFFll;
-      if (sizeof(S) == 4) up = (S)0x7FFFFFFF;
-#pragma unroll
-      for (int q = 0; q < PER; q++) {
-        int j = tid_ + NT * q;
-        if (j < n) {
-          S b = Bits<T>::abs_bits(v[q]);
-          if (b <= lo) cnt += 1;
-          else up = b < up ? b : up;
-        }
-      }
-      cnt = block_sum<NT>(cnt, red);
-      // block minimum of up
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        S w2 = __shfl_xor(up, o, 64);
-        up = w2 < up ? w2 : up;
-      }
-      __syncthreads();
-      if ((tid_ & 63) == 0) smax[tid_ >> 6] = up;
-      __syncthreads();
-      up = smax[0];
-      for (int i = 1; i < NW; i++) up = smax[i] < up ? smax[i] : up;
-      double upper = cnt >= (double)(k + 2) ? lower : Bits<T>::value(up);
-      double f = hh - (double)k;
-      level = f == 0 ? lower : lower + f * (upper - lower);
-    }
-  }
-  // scaleToTarget (gain/gain.nim:208-210)
-  if (isfinite(level) && level > 0 && fits_in<T>(starget / level)) {
-    const T f = (T)(starget / level);
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      int j = tid_ + NT * k;
-      if (j < n) v[k] = v[k] * f;
-    }
-  }
-  __syncthreads();
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------- kernels
-
-// One row of one job: load the row's window into registers, run the job's
-// steps on its live part, store the job's crop.  Every thread of the block
-// runs it with the same (job, row).
-template <typename T, int PER>
-__device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, double *red) {
-  const int tid_ = opaque_tid();
-  const DevPlan *plan = J.plan;
-  const int cols = J.cols;
-  const T *s = (const T *)J.src + r * J.src_pitch;
-  T *d = (T *)J.dst + r * J.dst_pitch;
-  const int num = J.num[r];
-  const int n = num < 0 ? -1 : min(max(num - J.first, 0), cols);
-  T v[PER];
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NT * k;
-    v[k] = j < cols ? s[j] : (T)0;
-  }
-  if (n > 0) {
-    if (J.has_factor) {
-      // a per-panel factor at the start of the part (pipeline.nim:717-718)
-      const double f = *J.factor;
-      if (!isnan(f)) {
-        const T kf = (T)f;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-          int j = tid_ + NT * k;
-          if (j < n) v[k] = v[k] * kf;
-        }
-      }
-    }
-    for (int i = J.step_first; i < J.step_last; i++) {
-      const DevStep &st = plan->steps[i];
-      switch (uni(st.kind)) {
-        case kDemean: {
-          const int tid_ = opaque_tid();
-          // demean (core/basic.nim:52-59)
-          const T m = (T)(block_sum<NT>(row_sum<T, PER>(v, n), red) / (double)n);
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NT * k;
-            if (j < n) v[k] = v[k] - m;
-          }
-          break;
-        }
-        case kDetrend: {
-          const int tid_ = opaque_tid();
-          // detrend (core/basic.nim:61-81)
-          double a, b;
-          trend_of<T, PER>(v, n, 2, red, a, b);
-          const T ta = (T)a, tb = (T)b;
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NT * k;
-            if (j < n) v[k] = v[k] - (ta + tb * (T)j);
-          }
-          break;
-        }
-        case kScale: {
-          const int tid_ = opaque_tid();
-          const T kf = (T)uni(st.factor);
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NT * k;
-            if (j < n) v[k] = v[k] * kf;
-          }
-          break;
-        }
-        case kClip: {
-          const int tid_ = opaque_tid();
-          const T lo = (T)uni(st.lo), hi = (T)uni(st.hi);
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NT * k;
-            if (j < n) {
-              T x = v[k];
-              v[k] = x > hi ? hi : (x < lo ? lo : x);
-            }
-          }
-          break;
-        }
-        case kPolarity: {
-          const int tid_ = opaque_tid();
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NT * k;
-            if (j < n) v[k] = -v[k];
-          }
-          break;
-        }
-        case kFilter: filter_step<T, PER>(st.filter, v, n, lds, red); break;
-        case kAgc: agc_step<T, PER>(st.agc, v, n, lds, red); break;
-        case kNormTrace: norm_trace_step<T, PER>(st, v, n, lds, red); break;
-        default: break;
-      }
-    }
-  }
-  const int c0 = J.crop0, c1 = J.crop1;
-  {
-    const int tid_ = opaque_tid();
-    T *dd = d - c0;
-#pragma unroll
-    for (int k = 0; k < PER; k++) {
-      int j = tid_ + NT * k;
-      if (j >= c0 && j < c1) dd[j] = v[k];
-    }
-  }
-}
-
-// The job holding work item `item`: the last with item0 <= item.
-__device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int n_jobs, int item) {
-  int lo = 0, hi = n_jobs - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].item0 <= item) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// The pipeline over work items (one item = one row of one job).  Direct:
-// item = blockIdx.x.  Persistent (counter != NULL): every block takes items
-// from the counter until they run out, so rows of very different cost
-// balance across the chip; each block exits once the counter passes `total`.
-// n_jobs == 0: the single job passed by value.
-template <typename T, int PER>
-__global__ __launch_bounds__(NT, 2) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
-                                                      int32_t n_jobs, int32_t total,
-                                                      uint32_t *__restrict__ counter) {
-  extern __shared__ __align__(16) char lds[];
-  __shared__ double red[2 * NW];
-  __shared__ int s_item;
-  int item = blockIdx.x;
-  for (;;) {
-    if (counter != nullptr) {
-      if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
-      __syncthreads();
-      item = __builtin_amdgcn_readfirstlane(s_item);
-      __syncthreads();
-    }
-    if (item >= total) break;
-    int64_t row = item;
-    const Job *jp = &single;
-    Job J;
-    if (n_jobs == 0) {
-      J = single;
-    } else {
-      const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, n_jobs, item));
-      const Job &G = jobs[j];
-      J.plan = uni_ptr(G.plan);
-      J.src = uni_ptr(G.src);
-      J.dst = uni_ptr(G.dst);
-      J.num = uni_ptr(G.num);
-      J.factor = uni_ptr(G.factor);
-      J.src_pitch = ((int64_t)uni((int)(G.src_pitch >> 32)) << 32) | (uint32_t)uni((int)G.src_pitch);
-      J.dst_pitch = ((int64_t)uni((int)(G.dst_pitch >> 32)) << 32) | (uint32_t)uni((int)G.dst_pitch);
-      J.rows = uni(G.rows);
-      J.cols = uni(G.cols);
-      J.first = uni(G.first);
-      J.crop0 = uni(G.crop0);
-      J.crop1 = uni(G.crop1);
-      J.item0 = uni(G.item0);
-      J.step_first = uni(G.step_first);
-      J.step_last = uni(G.step_last);
-      J.has_factor = uni(G.has_factor);
-      row = item - J.item0;
-    }
-    (void)jp;
-    run_row<T, PER>(J, row, lds, red);
-    if (counter == nullptr) break;
-  }
-}
-
-// Clears n uint32 (a kernel, not hipMemsetAsync, so that a captured graph
-// holds no memset node).
-__global__ void zero_u32_kernel(uint32_t *__restrict__ p, int32_t n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
 
 // Per-row sum of squares (float64) and maximum |x| bits (as float32 for the
 // histogram key and in T for the maximum), for the rows a panel statistic
@@ -1361,38 +248,6 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
 
 namespace {
 
-template <typename T, int PER>
-hipError_t launch_one(const JobLaunch &a, hipStream_t stream) {
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)pipeline_kernel<T, PER>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       160 * 1024 - 2048);
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
-  if (a.counter) {
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, a.counter, 1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  const int grid = a.counter ? a.grid : a.total;
-  hipLaunchKernelGGL((pipeline_kernel<T, PER>), dim3(grid), dim3(NT), a.lds_bytes, stream,
-                     a.single, a.jobs, a.n_jobs, a.total, a.counter);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_typed(const JobLaunch &a, hipStream_t stream) {
-  const int per = (a.max_cols + NT - 1) / NT;
-  if (per <= 4) return launch_one<T, 4>(a, stream);
-  if (per <= 8) return launch_one<T, 8>(a, stream);
-  if (per <= 12) return launch_one<T, 12>(a, stream);
-  if (per <= 16) return launch_one<T, 16>(a, stream);
-  if (per <= 24) return launch_one<T, 24>(a, stream);
-  return hipErrorInvalidValue;
-}
-
 template <typename T>
 hipError_t panel_typed(const T *x, int32_t rows, int32_t cols, const int32_t *row_len,
                        int32_t stat, double target, double percentile, double *row_sq,
@@ -1433,12 +288,24 @@ hipError_t corr_typed(const DevCorr *dplan, size_t lds, const T *src, int32_t ro
 
 }  // namespace
 
-int pipeline_max_cols() { return NT * 24; }
-
-hipError_t launch_pipeline(int dtype, const JobLaunch &a, hipStream_t stream) {
+hipError_t launch_pipeline(int dtype, int threads, const JobLaunch &a, hipStream_t stream) {
   if (a.total <= 0) return hipSuccess;
-  if (dtype == 0) return launch_typed<float>(a, stream);
-  return launch_typed<double>(a, stream);
+  switch (threads) {
+    case 128: return launch_pipeline_nt128(dtype, a, stream);
+    case 256: return launch_pipeline_nt256(dtype, a, stream);
+    case 512: return launch_pipeline_nt512(dtype, a, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t pipeline_resident_blocks(int dtype, int threads, int max_cols, size_t lds,
+                                    int *blocks) {
+  switch (threads) {
+    case 128: return resident_pipeline_nt128(dtype, max_cols, lds, blocks);
+    case 256: return resident_pipeline_nt256(dtype, max_cols, lds, blocks);
+    case 512: return resident_pipeline_nt512(dtype, max_cols, lds, blocks);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t cols,

@@ -1,0 +1,248 @@
+// The fused row kernel of the panel pipeline, templated on the row width:
+// NTH threads (NTH / 64 wave64) per row, PER samples per thread.
+//
+// One workgroup per row.  The row crosses HBM once: it is loaded into
+// registers (sample j on thread j mod NTH, slot j / NTH, so every load and
+// store is coalesced), every step of the plan runs on it there and in LDS,
+// and it is stored once.  Steps:
+//   demean / detrend / scale / clip / polarity   core/basic.nim:52-110
+//   FFT filter (trend, pads, rFFT, gain, irFFT)  filter/apply.nim:194-363
+//   AGC (block prefix/suffix sums, gain step)     gain/gain.nim:351-461
+//   per-trace normalization (max, rms, exact percentile by radix select)
+//                                                 gain/gain.nim:198-219, gain/stats.nim:159-182
+// Each pipeline_nt*.hip instantiates one NTH with its PER widths.
+#pragma once
+#include "pseg_device.hpp"
+
+#include <algorithm>
+
+namespace pseg {
+
+namespace {
+// ---------------------------------------------------------------- kernels
+
+// One row of one job: load the row's window into registers, run the job's
+// steps on its live part, store the job's crop.  Every thread of the block
+// runs it with the same (job, row).
+template <typename T, int NTH, int PER>
+__device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, double *red) {
+  const int tid_ = opaque_tid();
+  const DevPlan *plan = J.plan;
+  const int cols = J.cols;
+  const T *s = (const T *)J.src + r * J.src_pitch;
+  T *d = (T *)J.dst + r * J.dst_pitch;
+  const int num = J.num[r];
+  const int n = num < 0 ? -1 : min(max(num - J.first, 0), cols);
+  T v[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    int j = tid_ + NTH * k;
+    v[k] = j < cols ? s[j] : (T)0;
+  }
+  if (n > 0) {
+    if (J.has_factor) {
+      // a per-panel factor at the start of the part (pipeline.nim:717-718)
+      const double f = *J.factor;
+      if (!isnan(f)) {
+        const T kf = (T)f;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+          int j = tid_ + NTH * k;
+          if (j < n) v[k] = v[k] * kf;
+        }
+      }
+    }
+    for (int i = J.step_first; i < J.step_last; i++) {
+      const DevStep &st = plan->steps[i];
+      switch (uni(st.kind)) {
+        case kDemean: {
+          const int tid_ = opaque_tid();
+          // demean (core/basic.nim:52-59)
+          const T m = (T)(block_sum<NTH>(row_sum<T, NTH, PER>(v, n), red) / (double)n);
+#pragma unroll
+          for (int k = 0; k < PER; k++) {
+            int j = tid_ + NTH * k;
+            if (j < n) v[k] = v[k] - m;
+          }
+          break;
+        }
+        case kDetrend: {
+          const int tid_ = opaque_tid();
+          // detrend (core/basic.nim:61-81)
+          double a, b;
+          trend_of<T, NTH, PER>(v, n, 2, red, a, b);
+          const T ta = (T)a, tb = (T)b;
+#pragma unroll
+          for (int k = 0; k < PER; k++) {
+            int j = tid_ + NTH * k;
+            if (j < n) v[k] = v[k] - (ta + tb * (T)j);
+          }
+          break;
+        }
+        case kScale: {
+          const int tid_ = opaque_tid();
+          const T kf = (T)uni(st.factor);
+#pragma unroll
+          for (int k = 0; k < PER; k++) {
+            int j = tid_ + NTH * k;
+            if (j < n) v[k] = v[k] * kf;
+          }
+          break;
+        }
+        case kClip: {
+          const int tid_ = opaque_tid();
+          const T lo = (T)uni(st.lo), hi = (T)uni(st.hi);
+#pragma unroll
+          for (int k = 0; k < PER; k++) {
+            int j = tid_ + NTH * k;
+            if (j < n) {
+              T x = v[k];
+              v[k] = x > hi ? hi : (x < lo ? lo : x);
+            }
+          }
+          break;
+        }
+        case kPolarity: {
+          const int tid_ = opaque_tid();
+#pragma unroll
+          for (int k = 0; k < PER; k++) {
+            int j = tid_ + NTH * k;
+            if (j < n) v[k] = -v[k];
+          }
+          break;
+        }
+        case kFilter: filter_step<T, NTH, PER>(st.filter, v, n, lds, red); break;
+        case kAgc: agc_step<T, NTH, PER>(st.agc, v, n, lds, red); break;
+        case kNormTrace: norm_trace_step<T, NTH, PER>(st, v, n, lds, red); break;
+        default: break;
+      }
+    }
+  }
+  const int c0 = J.crop0, c1 = J.crop1;
+  {
+    const int tid_ = opaque_tid();
+    T *dd = d - c0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      int j = tid_ + NTH * k;
+      if (j >= c0 && j < c1) dd[j] = v[k];
+    }
+  }
+}
+
+// The job holding work item `item`: the last with item0 <= item.
+__device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int n_jobs, int item) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].item0 <= item) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace
+
+// The pipeline over work items (one item = one row of one job).  Direct:
+// item = blockIdx.x.  Persistent (counter != NULL): every block takes items
+// from the counter until they run out, so rows of very different cost
+// balance across the chip; each block exits once the counter passes `total`.
+// n_jobs == 0: the single job passed by value.
+template <typename T, int NTH, int PER>
+__global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 2 : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
+                                                      int32_t n_jobs, int32_t total,
+                                                      uint32_t *__restrict__ counter) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red[2 * (NTH / 64)];
+  __shared__ int s_item;
+  int item = blockIdx.x;
+  for (;;) {
+    if (counter != nullptr) {
+      if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+      __syncthreads();
+      item = __builtin_amdgcn_readfirstlane(s_item);
+      __syncthreads();
+    }
+    if (item >= total) break;
+    int64_t row = item;
+    const Job *jp = &single;
+    Job J;
+    if (n_jobs == 0) {
+      J = single;
+    } else {
+      const int j = __builtin_amdgcn_readfirstlane(find_job(jobs, n_jobs, item));
+      const Job &G = jobs[j];
+      J.plan = uni_ptr(G.plan);
+      J.src = uni_ptr(G.src);
+      J.dst = uni_ptr(G.dst);
+      J.num = uni_ptr(G.num);
+      J.factor = uni_ptr(G.factor);
+      J.src_pitch = ((int64_t)uni((int)(G.src_pitch >> 32)) << 32) | (uint32_t)uni((int)G.src_pitch);
+      J.dst_pitch = ((int64_t)uni((int)(G.dst_pitch >> 32)) << 32) | (uint32_t)uni((int)G.dst_pitch);
+      J.rows = uni(G.rows);
+      J.cols = uni(G.cols);
+      J.first = uni(G.first);
+      J.crop0 = uni(G.crop0);
+      J.crop1 = uni(G.crop1);
+      J.item0 = uni(G.item0);
+      J.step_first = uni(G.step_first);
+      J.step_last = uni(G.step_last);
+      J.has_factor = uni(G.has_factor);
+      row = item - J.item0;
+    }
+    (void)jp;
+    run_row<T, NTH, PER>(J, row, lds, red);
+    if (counter == nullptr) break;
+  }
+}
+
+namespace {
+
+// Lets the kernel take up to kMaxRowLds bytes of dynamic LDS (once).
+template <typename T, int NTH, int PER>
+hipError_t prepare_kernel() {
+  static const hipError_t e = hipFuncSetAttribute(
+      (const void *)pipeline_kernel<T, NTH, PER>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  return e;
+}
+
+// One launch of pipeline_kernel<T, NTH, PER>: clears the work counter of a
+// persistent launch first (a kernel, not a memset, so the launch can be
+// captured in a graph).
+template <typename T, int NTH, int PER>
+hipError_t launch_one(const JobLaunch &a, hipStream_t stream) {
+  hipError_t e = prepare_kernel<T, NTH, PER>();
+  if (e != hipSuccess) return e;
+  if (a.counter) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, a.counter, 1);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int grid = a.counter ? a.grid : a.total;
+  hipLaunchKernelGGL((pipeline_kernel<T, NTH, PER>), dim3(grid), dim3(NTH), a.lds_bytes, stream,
+                     a.single, a.jobs, a.n_jobs, a.total, a.counter);
+  return hipGetLastError();
+}
+
+// Workgroups of pipeline_kernel<T, NTH, PER> resident on the whole device at
+// once with `lds` bytes of dynamic LDS each (the grid of a persistent launch).
+template <typename T, int NTH, int PER>
+hipError_t resident_one(size_t lds, int *blocks) {
+  hipError_t e = prepare_kernel<T, NTH, PER>();
+  if (e != hipSuccess) return e;
+  int per_cu = 0, dev = 0, cus = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, (const void *)pipeline_kernel<T, NTH, PER>, NTH, lds);
+  if (e != hipSuccess) return e;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  *blocks = std::max(per_cu, 1) * cus;
+  return hipSuccess;
+}
+
+}  // namespace
+
+}  // namespace pseg

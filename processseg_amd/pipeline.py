@@ -410,6 +410,7 @@ class PipelinePlan:
         self.requiredMargin = int(info.margin)
         self.partCount = int(info.parts)
         self.lds_bytes = int(info.lds_bytes)
+        self.threadsPerRow = int(info.threads_per_row)
         n = info.n_steps
         self.steps = [dict(kind=_KIND_NAMES[info.step_kind[i]], nfft=info.step_nfft[i],
                            pad=info.step_pad[i], margin=info.step_margin[i],

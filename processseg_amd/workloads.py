@@ -1,8 +1,10 @@
 """The named workloads: the flows and seeded panels the benchmarks and the
-parity tests run (48 x 2001 plain plan; 2000 x 3001 band-pass + AGC)."""
+parity tests run (48 x 2001 plain plan; 2000 x 3001 band-pass + AGC;
+1200 x 10001 vibroseis records correlated with a 12 s sweep)."""
 import numpy as np
 
 from . import synth
+from .correlate import generatePilot, initSweepTaper, linearSweep
 from .pipeline import (Corners, agcOp, bandPass, bandPassOp, butterHighCut, demeanOp, filterOp,
                        initPipeline)
 
@@ -38,3 +40,36 @@ def big_panel(rows=2000, cols=3001, dtype=np.float32, seed=2):
     """2000 x 3001, live lengths uniform in [2400, 3001], zeros past them."""
     live = synth.live_lengths(seed, rows, 2400, cols)
     return synth.gather(rows, cols, 2.0, seed, live=live, dtype=dtype)
+
+
+def sweep_pilot(dt_ms=2.0):
+    """Linear sweep 6 -> 80 Hz, 12 s, cos² tapers of 300 ms at both ends
+    (6000 samples at 2 ms)."""
+    return generatePilot(linearSweep(6, 80, 12000, dt_ms, initSweepTaper(300, 300)))
+
+
+def vibroseis_panel(rows=1200, cols=10001, dtype=np.float32, seed=4):
+    """Uncorrelated records: the sweep convolved with sparse reflectivity
+    (Bernoulli 0.01, N(0, 1)) plus N(0, 0.1), `cols` samples at 2 ms."""
+    x, _ = synth.vibroseis_records(rows, cols, sweep_pilot().samples, seed, dtype=dtype)
+    return x
+
+
+def mixed_flow():
+    """Band-pass 5-10-70-90 Hz and AGC 500 ms: every corner is under the
+    Nyquist frequency of 1, 2 and 4 ms sampling."""
+    return initPipeline([bandPassOp(Corners(5, 10, 70, 90)), agcOp(500)])
+
+
+def mixed_shapes(n=64, rows=(240, 2400), cols=(1000, 6001), dts=(1.0, 2.0, 4.0), seed=3):
+    """Record shapes of an uneven batch: rows and cols uniform in the given
+    inclusive ranges, the sample interval uniform over `dts`.  Returns a list
+    of (rows, cols, dt_ms)."""
+    u = synth.uniforms(seed, (3, n), stream=31)
+    out = []
+    for i in range(n):
+        r = rows[0] + int(u[0, i] * (rows[1] - rows[0] + 1))
+        c = cols[0] + int(u[1, i] * (cols[1] - cols[0] + 1))
+        out.append((min(r, rows[1]), min(c, cols[1]), dts[min(int(u[2, i] * len(dts)),
+                                                              len(dts) - 1)]))
+    return out

@@ -80,3 +80,37 @@ def test_corr_compile_validation():
     zero = P.Pilot(np.zeros(100), 2.0)
     with pytest.raises(ValueError, match="all zeros"):
         P.initCorrelationPlan(np.float32, zero, 2.0, 5000)
+
+
+def test_corr_float64_full_size_is_refused():
+    """16200 complex float64 points do not fit one workgroup's LDS: the plan
+    says so at compile (PSEG_E_UNSUPPORTED) instead of failing at apply."""
+    import pytest
+    pilot = P.generatePilot(P.linearSweep(6, 80, 12000, 2, P.initSweepTaper(300, 300)))
+    with pytest.raises(P.UnsupportedOp, match="LDS"):
+        P.initCorrelationPlan(np.float64, pilot, 2.0, 10001)
+
+
+def test_pipeline_lds_limit_is_unsupported():
+    """A float64 filter on 10001-sample rows needs more LDS than one workgroup
+    has: compile refuses it with PSEG_E_UNSUPPORTED, naming the op."""
+    import pytest
+    flow = P.initPipeline([P.demeanOp(), P.bandPassOp(P.Corners(5, 10, 70, 90))])
+    with pytest.raises(P.UnsupportedOp, match=r"op 1 \(.*LDS"):
+        flow.compile(np.float64, 2.0, 10001)
+    with pytest.raises(P.UnsupportedOp, match="beyond"):
+        flow.compile(np.float32, 2.0, 12289)
+
+
+def test_batch_checks_before_running():
+    """pseg_batch_create validates every item as pseg_plan_apply would and
+    names the item (no GPU needed: nothing is allocated before the checks)."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.pseg_batch_create(None, 1, C.byref(h)) == _lib.E_INVALID
+    items = (_lib.BatchItemC * 2)()
+    items[0] = _lib.BatchItemC(None, 16, 16, 1, 1, 16)
+    assert L.pseg_batch_create(items, 2, C.byref(h)) == _lib.E_INVALID
+    assert "item 0" in _lib.last_error()
+    assert L.pseg_batch_launch(None, None) == _lib.E_INVALID
+    assert L.pseg_batch_destroy(None) == _lib.OK

@@ -1,0 +1,127 @@
+"""Uneven batches: many records with their own plans (sample intervals 1, 2
+and 4 ms; 1000 to 6001 samples; a few rows each) as persistent launches.
+Each record must equal plan.apply of that record alone bit for bit, and the
+float64 oracle within the float32 tolerance; a captured graph replays to the
+same bits."""
+import numpy as np
+import pytest
+
+import processseg_amd as P
+from processseg_amd import synth, workloads as W
+from tests.common import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+F32_TOL = 1e-5
+
+
+def make_records(n=24, seed=11):
+    shapes = W.mixed_shapes(n, rows=(1, 9), cols=(1000, 6001), seed=seed)
+    recs = []
+    for i, (rows, cols, dt) in enumerate(shapes):
+        x, _ = synth.gather(rows, cols, dt, seed + i)
+        live = synth.live_lengths(seed + i, rows, cols // 2, cols).astype(np.int32)
+        if rows > 2:
+            live[1] = -1          # a dead or failed row: copied unchanged
+        if rows > 3:
+            live[2] = 0
+        recs.append((x, live, dt))
+    return recs
+
+
+def plans_for(recs, flow):
+    cache = {}
+    out = []
+    for x, _, dt in recs:
+        key = (dt, x.shape[1])
+        if key not in cache:
+            cache[key] = flow.compile(np.float32, dt, x.shape[1])
+        out.append(cache[key])
+    return out
+
+
+def test_batch_equals_single_applies_and_oracle(gpu, oracle):
+    import torch
+    recs = make_records()
+    flow = W.mixed_flow()
+    plans = plans_for(recs, flow)
+    srcs = [torch.from_numpy(x).cuda() for x, _, _ in recs]
+    dsts = [torch.full_like(s, 9.0) for s in srcs]
+    lens = [torch.from_numpy(l).cuda() for _, l, _ in recs]
+    batch = P.PipelineBatch([P.BatchItem(p, s, d, row_len=l)
+                             for p, s, d, l in zip(plans, srcs, dsts, lens)])
+    assert batch.launches == 2 and sorted(batch.threads) == [128, 256]
+    batch.launch()
+    alone = [torch.empty_like(s) for s in srcs]
+    for p, s, a, l in zip(plans, srcs, alone, lens):
+        p.apply(s, a, row_len=l)
+    torch.cuda.synchronize()
+    worst = 0.0
+    for i, ((x, live, dt), d, a) in enumerate(zip(recs, dsts, alone)):
+        got = d.cpu().numpy()
+        assert np.array_equal(got, a.cpu().numpy()), f"record {i}: batch != alone"
+        want = oracle.pipeline_apply(flow.ops, x.astype(np.float64), live, dt)
+        e = rel_l2(got, want)
+        worst = max(worst, e)
+        assert e <= F32_TOL, f"record {i} {x.shape} dt {dt}: rel L2 {e}"
+        if x.shape[0] > 2:
+            assert np.array_equal(got[1], x[1])
+    print("batch of", len(recs), "records: worst rel L2 vs FP64 oracle", worst)
+
+
+def test_batch_in_place(gpu):
+    import torch
+    recs = make_records(8, seed=5)
+    flow = W.mixed_flow()
+    plans = plans_for(recs, flow)
+    bufs = [torch.from_numpy(x).cuda() for x, _, _ in recs]
+    ref = [torch.empty_like(b) for b in bufs]
+    lens = [torch.from_numpy(l).cuda() for _, l, _ in recs]
+    for p, b, r, l in zip(plans, bufs, ref, lens):
+        p.apply(b, r, row_len=l)
+    batch = P.PipelineBatch([P.BatchItem(p, b, row_len=l) for p, b, l in zip(plans, bufs, lens)])
+    batch.launch()
+    torch.cuda.synchronize()
+    for b, r in zip(bufs, ref):
+        assert torch.equal(b, r)
+
+
+def test_batch_graph_replay(gpu):
+    """The batch captured in a graph: every replay (with eager work and a
+    synchronization between replays, and dst cleared first) gives the eager
+    bits."""
+    import torch
+    recs = make_records(12, seed=7)
+    flow = W.mixed_flow()
+    plans = plans_for(recs, flow)
+    srcs = [torch.from_numpy(x).cuda() for x, _, _ in recs]
+    dsts = [torch.empty_like(s) for s in srcs]
+    lens = [torch.from_numpy(l).cuda() for _, l, _ in recs]
+    batch = P.PipelineBatch([P.BatchItem(p, s, d, row_len=l)
+                             for p, s, d, l in zip(plans, srcs, dsts, lens)])
+    batch.launch()
+    torch.cuda.synchronize()
+    eager = [d.clone() for d in dsts]
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batch.launch()
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for d in dsts:
+            d.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for i, (d, e) in enumerate(zip(dsts, eager)):
+            assert torch.equal(d, e), f"replay {rep}, record {i}"
+        _ = sum(float(s.sum()) for s in srcs)  # eager work between replays
+
+
+def test_batch_refuses_overlapping_items(gpu):
+    import torch
+    flow = W.mixed_flow()
+    plan = flow.compile(np.float32, 2.0, 1000)
+    a = torch.zeros(8, 1000, device="cuda")
+    l = torch.full((8,), 1000, dtype=torch.int32, device="cuda")
+    with pytest.raises(ValueError, match="overlaps"):
+        P.PipelineBatch([P.BatchItem(plan, a, row_len=l), P.BatchItem(plan, a[:4], row_len=l)])

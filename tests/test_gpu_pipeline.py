@@ -195,3 +195,37 @@ def test_correlation(gpu, oracle):
     print("correlation max abs / peak", e, "rel L2", rel_l2(got, want))
     assert e <= 2e-5
     assert np.all(got[3] == 0) and np.all(got[6] == 0)
+
+
+@pytest.mark.parametrize("cols,dtype,threads", [
+    (3073, np.float32, 256),
+    (5001, np.float32, 256),
+    (5001, np.float64, 256),
+    (6144, np.float32, 256),
+    (6145, np.float32, 512),
+    (10001, np.float32, 512),
+])
+def test_wide_rows(gpu, oracle, cols, dtype, threads):
+    """Rows past 3072 samples run on 256 or 512 threads per row (the plan's
+    choice, from its sample count): the same flow as the 2000 x 3001 panel,
+    live lengths from 80 % of the row to its end."""
+    rows = 24
+    x, _ = synth.gather(rows, cols, 2.0, 7, dtype=dtype)
+    live = synth.live_lengths(7, rows, int(0.8 * cols), cols).astype(np.int32)
+    for r in range(rows):
+        x[r, live[r]:] = 0
+    flow = config2_flow()
+    plan = flow.compile(dtype, 2.0, cols)
+    assert plan.threadsPerRow == threads
+    check(flow, x, live, oracle, label=f"wide-{cols}")
+
+
+def test_wide_rows_percentile(gpu, oracle):
+    """The exact per-trace percentile on 512 threads per row."""
+    rows, cols = 8, 9000
+    x, _ = synth.gather(rows, cols, 2.0, 8)
+    live = np.full(rows, cols, np.int32)
+    live[3] = 4500
+    check(P.initPipeline([P.bandPassOp(P.Corners(4, 8, 60, 80)),
+                          P.normalizeOp(P.normPercentile, percentile=99)]),
+          x, live, oracle, label="wide-p99")
