@@ -221,6 +221,76 @@ def batch_leg(torch, dev, steps, warmup):
             "achieved_GBps": round(samples * 8 / t_graph / 1e9, 1)}
 
 
+def serve_leg(torch, dev, seconds, rate_hz):
+    """Zoom windows of a resident 20000 x 4001 f32 survey (320 MB; a seeded
+    2000 x 4001 gather tiled 10 times) under a Poisson stream of `rate_hz`
+    requests/s for `seconds` (100-2000 traces x 200-4001 shown samples, three
+    flows, plans compiled per window width on first use).  The continuously
+    batched server against one launch per request in arrival order; then the
+    same stream again with every plan cached, and the whole stream submitted
+    at once (throughput)."""
+    import queue
+    import threading
+    import processseg_amd as P
+    from processseg_amd import synth
+    from processseg_amd.workloads import serve_flows, serve_requests
+    tile, _ = synth.gather(2000, 4001, 2.0, 12)
+    survey = torch.from_numpy(tile).to(dev).repeat(10, 1).contiguous()
+    trace_len = torch.full((survey.shape[0],), survey.shape[1], dtype=torch.int32, device=dev)
+    reqs = serve_requests(int(seconds * rate_hz), survey.shape[0], survey.shape[1], rate_hz)
+    flows = serve_flows()
+
+    def run(srv, ids, paced):
+        q = queue.Queue()
+        res = []
+
+        def collect():
+            while True:
+                t = q.get()
+                if t is None:
+                    return
+                r = srv.wait(t)
+                res.append(r)
+                srv.release(t)
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        for at, f, tr0, ntr, a, b in reqs:
+            if paced:
+                while time.perf_counter() - t0 < at:
+                    time.sleep(min(0.0005, max(0.0, at - (time.perf_counter() - t0))))
+            q.put(srv.submit(ids[f], tr0, ntr, a, b))
+        q.put(None)
+        th.join()
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency_ms for r in res])
+        samples = sum(r.rows * r.window for r in res)
+        return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(float(lat.mean()), 3),
+                "compiled_frac": round(float(np.mean([r.plan_compiled for r in res])), 3),
+                "mean_batch": round(float(np.mean([r.batch_requests for r in res])), 2),
+                "requests_per_s": round(len(res) / wall, 1),
+                "Msamples_per_s": round(samples / wall / 1e6, 1)}
+
+    out = {"workload": f"{len(reqs)} zoom-window requests, Poisson {rate_hz:g}/s, survey "
+                       f"{survey.shape[0]}x{survey.shape[1]} f32 resident, 3 flows",
+           "unit": "ms (submit to result on the device)"}
+    for name, mb, inflight in (("batched", 256, 2), ("one_launch_per_request", 1, 1)):
+        srv = P.Server(survey, trace_len, 2.0, arena_bytes=4 << 30, max_batch_requests=mb,
+                       max_inflight=inflight)
+        ids = [srv.add_flow(f) for f in flows]
+        log(f"serving: {name}, cold plan cache")
+        out[name] = {"cold": run(srv, ids, True)}
+        log(f"serving: {name}, warm plan cache")
+        out[name]["warm"] = run(srv, ids, True)
+        log(f"serving: {name}, burst")
+        out[name]["burst"] = run(srv, ids, False)
+        out[name]["server"] = srv.info()
+        srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true",
                     help="only the headline panel (no correlation leg)")
+    ap.add_argument("--serve-seconds", type=float, default=8.0,
+                    help="length of the Poisson request stream of the serving leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of the panel instead (the calibration "
                          "kernel for the HBM byte counters)")
@@ -328,6 +400,11 @@ def main():
         torch.cuda.empty_cache()
         log("uneven batch leg")
         line["secondary"]["uneven_batch"] = batch_leg(torch, dev, max(5, args.steps // 20), 2)
+        torch.cuda.empty_cache()
+        sv = serve_leg(torch, dev, args.serve_seconds, 200.0)
+        line["secondary"]["serving"] = sv
+        line["p50_ms"] = sv["batched"]["warm"]["p50_ms"]
+        line["p99_ms"] = sv["batched"]["warm"]["p99_ms"]
     print(json.dumps(line), flush=True)
 
 

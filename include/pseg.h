@@ -221,6 +221,76 @@ pseg_status pseg_batch_launch(const pseg_batch *batch, void *stream);
 pseg_status pseg_batch_info_get(const pseg_batch *batch, pseg_batch_info *out);
 pseg_status pseg_batch_destroy(pseg_batch *batch);
 
+/* -------------------------------------------------------------- serving -- */
+
+/* A viewer's zoom windows served from a survey resident in device memory.
+ * Each request is the reference's zoomWindow (tests/utils/viewersim.nim:
+ * 167-191): margin = the flow's requiredMargin, window = decodeWindow(shown,
+ * margin, samples), the plan for (flow, window width, samples) compiled on
+ * first use (planFor, viewersim.nim:53-62), apply on the window's rows, crop
+ * to the shown samples.  Requests queue and are served by continuous
+ * batching: a worker thread packs every admitted request (admission: room
+ * for the result in the server's device arena) into persistent launches, two
+ * batches in flight on their own streams.  A request's samples equal
+ * pseg_plan_apply of its window alone, bit for bit, whatever else shares its
+ * launch. */
+typedef struct pseg_server pseg_server;
+
+typedef struct pseg_server_config {
+  int32_t dtype;               /* PSEG_F32 or PSEG_F64 */
+  int32_t traces;              /* rows of the survey */
+  int32_t samples;             /* samples per trace (the record length) */
+  int32_t max_batch_requests;  /* requests per launch; 1 = one launch per request */
+  int64_t pitch;               /* elements between trace starts, >= samples */
+  const void *survey;          /* device [traces, pitch] */
+  const int32_t *trace_len;    /* device, traces int32: the trace's numSamples,
+                                  -1 for a trace that is not live */
+  double dt_ms;
+  size_t arena_bytes;          /* device memory for results */
+  int32_t max_batch_rows;      /* rows per launch (0: no limit) */
+  int32_t max_inflight;        /* launches in flight, 1 or 2 */
+} pseg_server_config;
+
+typedef struct pseg_serve_result {
+  int32_t status;              /* PSEG_OK or the request's error code */
+  int32_t rows, cols;          /* traces x shown samples */
+  int32_t first, window;       /* the decoded window: samples first .. first+window-1 */
+  int32_t batch_requests;      /* requests in the launch that served it */
+  int32_t plan_compiled;       /* 1: its plan was compiled for this request */
+  int32_t reserved;
+  const void *samples;         /* device [rows, cols]; valid until release */
+  uint64_t generation;         /* the plan's generation (staleness tag) */
+  int64_t submit_ns, start_ns, done_ns;  /* steady clock of the host */
+} pseg_serve_result;
+
+typedef struct pseg_server_info {
+  int64_t requests, batches, launches, plans_compiled;
+  int64_t arena_peak_bytes;
+} pseg_server_info;
+
+/* The survey and trace_len must outlive the server. */
+pseg_status pseg_server_create(const pseg_server_config *cfg, pseg_server **out);
+/* Registers a flow (the ops of a Pipeline); *flow is its index.  A flow with
+ * a per-panel normalization is PSEG_E_UNSUPPORTED here. */
+pseg_status pseg_server_add_flow(pseg_server *srv, const pseg_op *ops, int32_t n_ops,
+                                 int32_t *flow);
+/* Queues the zoom window of traces trace0 .. trace0+ntraces-1, samples
+ * shown_first .. shown_last (inclusive).  Thread-safe. */
+pseg_status pseg_server_submit(pseg_server *srv, int32_t flow, int32_t trace0, int32_t ntraces,
+                               int32_t shown_first, int32_t shown_last, uint64_t *ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request; PSEG_E_BUSY when it
+ * is not done yet.  out->status holds the request's own result. */
+pseg_status pseg_server_wait(pseg_server *srv, uint64_t ticket, int32_t timeout_ms,
+                             pseg_serve_result *out);
+/* Copies a finished request's samples into dst (device, rows x cols). */
+pseg_status pseg_server_read(pseg_server *srv, uint64_t ticket, void *dst, void *stream);
+/* Frees a finished request's result memory and forgets the ticket. */
+pseg_status pseg_server_release(pseg_server *srv, uint64_t ticket);
+pseg_status pseg_server_info_get(pseg_server *srv, pseg_server_info *out);
+/* Finishes the queued requests, then stops the worker and frees the server's
+ * memory (plans compiled by the server included). */
+pseg_status pseg_server_destroy(pseg_server *srv);
+
 /* ---------------------------------------------------------- correlation -- */
 
 /* What a correlation plan reports (vibroseis/correlate.nim:162-188). */

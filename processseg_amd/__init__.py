@@ -6,7 +6,8 @@ correlation) runs in hand-written gfx950 kernels behind the C ABI of
 include/pseg.h (libpseg.so).  This package mirrors the reference's Nim API
 over that ABI: initPipeline / compile / PipelinePlan.apply and
 initCorrelationPlan / correlateInto, plus PipelineBatch (many records with
-their own plans as one persistent launch).
+their own plans as one persistent launch) and Server (zoom windows of a
+device-resident survey served by continuous batching).
 """
 from ._lib import HipError, UnsupportedOp, WorkspaceError, build, lib  # noqa: F401
 from .pipeline import (  # noqa: F401
@@ -20,3 +21,4 @@ from .correlate import (  # noqa: F401
     CorrelationPlan, Pilot, SweepSpec, SweepTaper, cmCorrelate, cmDeconvolve, generatePilot,
     initCorrelationPlan, initSweepTaper, linearSweep, samplesInLength, sweepSamples)
 from .batch import BatchItem, PipelineBatch  # noqa: F401
+from .serve import ServeResult, Server  # noqa: F401

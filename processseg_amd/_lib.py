@@ -63,6 +63,26 @@ class BatchInfoC(C.Structure):
                 ("samples", C.c_int64), ("threads", C.c_int32 * 4), ("grid", C.c_int32 * 4)]
 
 
+class ServerConfigC(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("traces", C.c_int32), ("samples", C.c_int32),
+                ("max_batch_requests", C.c_int32), ("pitch", C.c_int64), ("survey", C.c_void_p),
+                ("trace_len", C.c_void_p), ("dt_ms", C.c_double), ("arena_bytes", C.c_size_t),
+                ("max_batch_rows", C.c_int32), ("max_inflight", C.c_int32)]
+
+
+class ServeResultC(C.Structure):
+    _fields_ = [("status", C.c_int32), ("rows", C.c_int32), ("cols", C.c_int32),
+                ("first", C.c_int32), ("window", C.c_int32), ("batch_requests", C.c_int32),
+                ("plan_compiled", C.c_int32), ("reserved", C.c_int32), ("samples", C.c_void_p),
+                ("generation", C.c_uint64), ("submit_ns", C.c_int64), ("start_ns", C.c_int64),
+                ("done_ns", C.c_int64)]
+
+
+class ServerInfoC(C.Structure):
+    _fields_ = [("requests", C.c_int64), ("batches", C.c_int64), ("launches", C.c_int64),
+                ("plans_compiled", C.c_int64), ("arena_peak_bytes", C.c_int64)]
+
+
 class UnsupportedOp(ValueError):
     """An op outside the GPU path: keep the CPU plan for that flow."""
 
@@ -74,6 +94,10 @@ class WorkspaceError(AssertionError):
 
 class HipError(RuntimeError):
     pass
+
+
+class BusyError(TimeoutError):
+    """A request that is not done yet (PSEG_E_BUSY)."""
 
 
 def build(force: bool = False) -> str:
@@ -93,7 +117,9 @@ SYMBOLS = [
     "pseg_plan_info_get", "pseg_required_margin", "pseg_decode_window",
     "pseg_plan_workspace_size", "pseg_plan_apply", "pseg_corr_compile", "pseg_corr_destroy",
     "pseg_corr_info_get", "pseg_corr_apply", "pseg_linear_sweep", "pseg_batch_create",
-    "pseg_batch_launch", "pseg_batch_info_get", "pseg_batch_destroy",
+    "pseg_batch_launch", "pseg_batch_info_get", "pseg_batch_destroy", "pseg_server_create",
+    "pseg_server_add_flow", "pseg_server_submit", "pseg_server_wait", "pseg_server_read",
+    "pseg_server_release", "pseg_server_info_get", "pseg_server_destroy",
 ]
 
 
@@ -134,6 +160,14 @@ def lib():
     L.pseg_batch_launch.argtypes = [vp, vp]
     L.pseg_batch_info_get.argtypes = [vp, C.POINTER(BatchInfoC)]
     L.pseg_batch_destroy.argtypes = [vp]
+    L.pseg_server_create.argtypes = [C.POINTER(ServerConfigC), C.POINTER(vp)]
+    L.pseg_server_add_flow.argtypes = [vp, C.POINTER(OpC), i32, C.POINTER(i32)]
+    L.pseg_server_submit.argtypes = [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_uint64)]
+    L.pseg_server_wait.argtypes = [vp, C.c_uint64, i32, C.POINTER(ServeResultC)]
+    L.pseg_server_read.argtypes = [vp, C.c_uint64, vp, vp]
+    L.pseg_server_release.argtypes = [vp, C.c_uint64]
+    L.pseg_server_info_get.argtypes = [vp, C.POINTER(ServerInfoC)]
+    L.pseg_server_destroy.argtypes = [vp]
     for name in SYMBOLS:
         if not hasattr(L, name):
             raise ImportError(f"{LIB_PATH} does not export {name}")
@@ -158,4 +192,6 @@ def check(rc: int) -> None:
         raise UnsupportedOp(msg)
     if rc == E_WORKSPACE:
         raise WorkspaceError(msg)
+    if rc == E_BUSY:
+        raise BusyError(msg)
     raise HipError(msg)

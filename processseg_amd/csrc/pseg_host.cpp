@@ -15,11 +15,13 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/pseg.h"
-#include "pseg_internal.hpp"
+#include "pseg_plan.hpp"
 
 using namespace pseg;
 
@@ -297,7 +299,7 @@ int64_t feature_length(const pseg_filter_spec &s, double dt) {
 }
 
 // impulseLength + features (filter/design.nim:327-366)
-int64_t impulse_length(const pseg_filter_spec &s, double dt) {
+int64_t impulse_length_uncached(const pseg_filter_spec &s, double dt) {
   validate(s, dt);
   std::vector<pseg_filter_spec> feats;
   pseg_filter_spec flat;
@@ -331,6 +333,29 @@ int64_t impulse_length(const pseg_filter_spec &s, double dt) {
   }
   int64_t L = 0;
   for (auto &f : feats) L = std::max(L, feature_length(f, dt));
+  return L;
+}
+
+// impulse_length is a pure function of (spec, dt) and costs float64 FFTs of
+// up to 2^24 points: plans compiled for many window widths (a viewer's zoom
+// windows) share one evaluation per spec.
+int64_t impulse_length(const pseg_filter_spec &s, double dt) {
+  pseg_filter_spec k = s;
+  k.reserved = 0;
+  for (int i = std::max(0, std::min(k.n_notch, PSEG_MAX_NOTCH)); i < PSEG_MAX_NOTCH; i++)
+    k.notch_hz[i] = k.notch_width[i] = 0;
+  std::string key((const char *)&k, sizeof k);
+  key.append((const char *)&dt, sizeof dt);
+  static std::mutex mu;
+  static std::map<std::string, int64_t> memo;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = memo.find(key);
+    if (it != memo.end()) return it->second;
+  }
+  const int64_t L = impulse_length_uncached(s, dt);  // throws for an invalid spec
+  std::lock_guard<std::mutex> g(mu);
+  memo.emplace(key, L);
   return L;
 }
 
@@ -420,26 +445,6 @@ std::atomic<uint64_t> g_generation{0};
 }  // namespace
 
 // --------------------------------------------------------------- the plans
-
-struct pseg_plan {
-  int dtype;
-  int threads;           // workgroup size of the fused kernel (pipeline_threads_for)
-  double dt_ms;
-  int32_t samples, record, margin;
-  int n_steps;
-  int n_parts;
-  int part_a[kMaxSteps + 1], part_b[kMaxSteps + 1];
-  uint64_t generation;
-  DevPlan host;          // with device pointers
-  DevPlan *dev;          // the device copy (start of the allocation)
-  char *dev_base;
-  size_t step_lds[kMaxSteps];
-  int32_t step_kind[kMaxSteps];
-  int32_t step_nfft[kMaxSteps], step_pad[kMaxSteps], step_margin[kMaxSteps],
-      step_reflect[kMaxSteps];
-  int32_t norm_stat[kMaxSteps];
-  double norm_target[kMaxSteps], norm_pct[kMaxSteps];
-};
 
 struct pseg_corr_plan {
   int dtype;
@@ -729,6 +734,17 @@ double row_cost(const pseg_plan *P, int32_t cols) {
 }
 
 }  // namespace
+
+namespace pseg {
+
+pseg_status set_error(pseg_status st, const std::string &msg) {
+  g_error = msg;
+  return st;
+}
+
+pseg_status hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
+
+}  // namespace pseg
 
 // ==================================================================== C ABI
 

@@ -73,3 +73,36 @@ def mixed_shapes(n=64, rows=(240, 2400), cols=(1000, 6001), dts=(1.0, 2.0, 4.0),
         out.append((min(r, rows[1]), min(c, cols[1]), dts[min(int(u[2, i] * len(dts)),
                                                               len(dts) - 1)]))
     return out
+
+
+def serve_flows():
+    """The three flows a viewer serves: band-pass + AGC; band-pass + 99th
+    percentile normalization per trace; demean + a band-pass with a 50 Hz
+    notch + AGC."""
+    from .pipeline import normalizeOp, normPercentile
+    return [
+        initPipeline([bandPassOp(Corners(4, 8, 60, 80)), agcOp(500)]),
+        initPipeline([bandPassOp(Corners(4, 8, 60, 80)), normalizeOp(normPercentile)]),
+        initPipeline([demeanOp(), filterOp(bandPass(Corners(2, 4, 100, 120)).withNotch(50, 4)),
+                      agcOp(250)]),
+    ]
+
+
+def serve_requests(n, traces=20000, samples=4001, rate_hz=200.0, seed=5):
+    """A Poisson stream of zoom-window requests: arrival times (s) at
+    `rate_hz`, flow uniform over the three, 100-2000 traces, a shown window
+    of 200-4001 samples, both placed uniformly.  Returns a list of
+    (arrival_s, flow, trace0, ntraces, shown_first, shown_last)."""
+    u = synth.uniforms(seed, (6, n), stream=51)
+    gaps = -np.log1p(-np.minimum(u[0], 1 - 1e-12)) / rate_hz
+    t = np.cumsum(gaps)
+    out = []
+    for i in range(n):
+        ntr = 100 + int(u[1, i] * 1901)
+        width = 200 + int(u[2, i] * (samples - 200 + 1))
+        ntr, width = min(ntr, 2000), min(width, samples)
+        t0 = int(u[3, i] * (traces - ntr + 1))
+        s0 = int(u[4, i] * (samples - width + 1))
+        out.append((float(t[i]), min(int(u[5, i] * 3), 2), min(t0, traces - ntr), ntr,
+                    min(s0, samples - width), min(s0, samples - width) + width - 1))
+    return out

@@ -114,3 +114,18 @@ def test_batch_checks_before_running():
     assert "item 0" in _lib.last_error()
     assert L.pseg_batch_launch(None, None) == _lib.E_INVALID
     assert L.pseg_batch_destroy(None) == _lib.OK
+
+
+def test_server_checks_arguments():
+    """pseg_server_* refuse NULL and malformed arguments before touching a
+    device."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.pseg_server_create(None, C.byref(h)) == _lib.E_INVALID
+    cfg = _lib.ServerConfigC()
+    cfg.dtype, cfg.traces, cfg.samples, cfg.pitch = 0, 10, 100, 50
+    assert L.pseg_server_create(C.byref(cfg), C.byref(h)) == _lib.E_INVALID
+    assert "pitch" in _lib.last_error()
+    t = C.c_uint64()
+    assert L.pseg_server_submit(None, 0, 0, 1, 0, 1, C.byref(t)) == _lib.E_INVALID
+    assert L.pseg_server_destroy(None) == _lib.OK
