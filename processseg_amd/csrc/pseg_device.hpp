@@ -53,6 +53,13 @@ __device__ __forceinline__ int opaque_tid() {
   return t;
 }
 
+// A per-lane index behind an empty asm barrier: element offsets added to it
+// stay uniform (SGPR) adds instead of being folded into a per-lane multiply.
+__device__ __forceinline__ int opaque_v(int i) {
+  asm volatile("" : "+v"(i));
+  return i;
+}
+
 // Plan parameters are the same on every lane; reading them through
 // readfirstlane puts them (and everything computed from them) in SGPRs
 // instead of a VGPR per lane.
@@ -255,19 +262,26 @@ template <typename T> struct Bfly<T, 5> {
 
 // One Stockham stage x -> y: y[q + s(rp + k)] = (sum_j x[q + s(p + j sub)]
 // w_r^{jk}) w_cur^{pk}, butterfly b = p s + q.
+// pad_in: x is the padded output of a first pass of s points (element i at
+// i + i / s); pad_out (s == 1): write y padded (one spare element after
+// every R).  Both only change uniform strides and the per-butterfly bases.
 template <typename T, int R, int NTH>
 __device__ __forceinline__ void stage(const V2<T> *__restrict__ x, V2<T> *__restrict__ y,
-                                      CTab<T> tw, int m, int s, int sub, FastDiv sd) {
+                                      CTab<T> tw, int m, int s, int sub, FastDiv sd,
+                                      int pad_in, int pad_out) {
   const int tid_ = opaque_tid();
-  const int nb = m / R;
+  // the stride is uniform: keep it (and every product of it) in SGPRs
+  s = uni(s);
+  const int nb = uni(m / R);
   const bool twiddle = sub > 1;
-  const int in_step = s * sub;   // between the inputs of a butterfly
+  const int in_step = uni(s * sub + pad_in * sub);   // between the inputs of a butterfly
+  const int in_p = uni(s + pad_in), out_p = uni(s * R + pad_out);
 #pragma clang loop unroll(disable)
   for (int b = tid_; b < nb; b += NTH) {
     const int p = (int)fdiv((uint32_t)b, sd);
     const int q = b - p * s;
-    const V2<T> *xi = x + (q + s * p);
-    V2<T> *yo = y + (q + s * R * p);
+    const V2<T> *xi = x + opaque_v(q + in_p * p);
+    V2<T> *yo = y + opaque_v(q + out_p * p);
     V2<T> v[R];
 #pragma unroll
     for (int j = 0; j < R; j++) v[j] = xi[j * in_step];
@@ -292,26 +306,30 @@ __device__ __forceinline__ void stage(const V2<T> *__restrict__ x, V2<T> *__rest
 template <typename T, int R1, int R2, int NTH>
 __device__ __forceinline__ void stage2(const V2<T> *__restrict__ x, V2<T> *__restrict__ y,
                                        CTab<T> tw1, CTab<T> tw2, int m, int s, int sub1,
-                                       FastDiv sd) {
+                                       FastDiv sd, int pad_in, int pad_out) {
   const int tid_ = opaque_tid();
-  const int sub2 = sub1 / R2;
-  const int nb = m / (R1 * R2);
+  // the strides are uniform: keep them (and their multiples) in SGPRs
+  s = uni(s);
+  const int sub2 = uni(sub1 / R2);
+  const int nb = uni(m / (R1 * R2));
   const bool twiddle2 = sub2 > 1;
-  const int in1 = s * sub1, in2 = s * sub2;  // input strides of j and j2
-  const int out2 = s * R1;                   // output stride of k2
+  // input strides of j and j2 (in the padded layout when pad_in)
+  const int in1 = uni(s * sub1 + pad_in * sub1), in2 = uni(s * sub2 + pad_in * sub2);
+  const int out2 = uni(s * R1);                        // output stride of k2
+  const int in_p = uni(s + pad_in), out_p = uni(s * R1 * R2 + pad_out);
 #pragma clang loop unroll(disable)
   for (int b = tid_; b < nb; b += NTH) {
     const int p2 = (int)fdiv((uint32_t)b, sd);
     const int q = b - p2 * s;
-    const V2<T> *xi = x + (q + s * p2);
-    V2<T> *yo = y + (q + s * R1 * R2 * p2);
+    const V2<T> *xi = x + opaque_v(q + in_p * p2);
+    V2<T> *yo = y + opaque_v(q + out_p * p2);
     V2<T> v[R2][R1];
 #pragma unroll
     for (int j2 = 0; j2 < R2; j2++)
 #pragma unroll
       for (int j = 0; j < R1; j++) v[j2][j] = xi[j2 * in2 + j * in1];
     CTab<T> t1 = tw1 + p2 * (R1 - 1) - 1;
-    const int t1_step = sub2 * (R1 - 1);
+    const int t1_step = uni(sub2 * (R1 - 1));
 #pragma unroll
     for (int j2 = 0; j2 < R2; j2++) {
       Bfly<T, R1>::run(v[j2]);
@@ -335,20 +353,19 @@ __device__ __forceinline__ void stage2(const V2<T> *__restrict__ x, V2<T> *__res
   }
 }
 
-// Stage pairs whose butterfly holds at most 16 points run fused; the
-// register footprint of larger ones would cost more occupancy than the
-// LDS round trip they save.
-__host__ __device__ constexpr bool fusable(int r1, int r2) { return r1 * r2 <= 16; }
+// Stage pairs whose butterfly holds at most 16 points run fused (fusable,
+// shared with the host); the register footprint of larger ones would cost
+// more occupancy than the LDS round trip they save.
 
 template <typename T, int R1, int NTH>
 __device__ __forceinline__ void stage2_second(int r2, const V2<T> *x, V2<T> *y, CTab<T> t1,
                                               CTab<T> t2, int m, int s, int sub1,
-                                              FastDiv sd) {
+                                              FastDiv sd, int pi, int po) {
   switch (r2) {
-    case 4: if constexpr (fusable(R1, 4)) stage2<T, R1, 4, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-    case 2: if constexpr (fusable(R1, 2)) stage2<T, R1, 2, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-    case 3: if constexpr (fusable(R1, 3)) stage2<T, R1, 3, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
-    default: if constexpr (fusable(R1, 5)) stage2<T, R1, 5, NTH>(x, y, t1, t2, m, s, sub1, sd); break;
+    case 4: if constexpr (fusable(R1, 4)) stage2<T, R1, 4, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 2: if constexpr (fusable(R1, 2)) stage2<T, R1, 2, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 3: if constexpr (fusable(R1, 3)) stage2<T, R1, 3, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
+    default: if constexpr (fusable(R1, 5)) stage2<T, R1, 5, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
   }
 }
 
@@ -360,27 +377,30 @@ __device__ __forceinline__ V2<T> *complex_forward(const DevFft &f, V2<T> *a, V2<
   V2<T> *x = a, *y = b;
   int s = 1;
   const int nrad = uni(f.nrad), m = uni(f.m);
+  const int padded = uni(f.pad) != 0;
   for (int i = 0; i < nrad;) {
     const int r = uni(f.rad[i]), sub = uni(f.sub[i]);
     const FastDiv sd = uni(f.sdiv[i]);
     CTab<T> t = tw + uni(f.tw_off[i]);
     const int r2 = i + 1 < nrad ? uni(f.rad[i + 1]) : 0;
+    // the first pass writes the padded layout, the second reads it
+    const int po = padded && i == 0, pi = padded && s != 1 && s == uni(f.pad);
     if (i + 1 < nrad && fusable(r, r2)) {
       CTab<T> t2 = tw + uni(f.tw_off[i + 1]);
       switch (r) {
-        case 4: stage2_second<T, 4, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-        case 2: stage2_second<T, 2, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-        case 3: stage2_second<T, 3, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
-        default: stage2_second<T, 5, NTH>(r2, x, y, t, t2, m, s, sub, sd); break;
+        case 4: stage2_second<T, 4, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
+        case 2: stage2_second<T, 2, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
+        case 3: stage2_second<T, 3, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
+        default: stage2_second<T, 5, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
       }
       s *= r * r2;
       i += 2;
     } else {
       switch (r) {
-        case 4: stage<T, 4, NTH>(x, y, t, m, s, sub, sd); break;
-        case 2: stage<T, 2, NTH>(x, y, t, m, s, sub, sd); break;
-        case 3: stage<T, 3, NTH>(x, y, t, m, s, sub, sd); break;
-        default: stage<T, 5, NTH>(x, y, t, m, s, sub, sd); break;
+        case 4: stage<T, 4, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
+        case 2: stage<T, 2, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
+        case 3: stage<T, 3, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
+        default: stage<T, 5, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
       }
       s *= r;
       i += 1;
@@ -520,7 +540,7 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   T *buf = (T *)lds;                          // nfft samples = m complex
   V2<T> *A = (V2<T> *)lds;
   const int fn = uni(f.n);
-  V2<T> *B = (V2<T> *)(lds + sizeof(T) * (size_t)fn);
+  V2<T> *B = A + uni(f.stride);
   double a, b;
   trend_of<T, NTH, PER>(v, n, uni(F.trend), red, a, b);
   const int p = uni(F.pad), nf = fn;
@@ -573,13 +593,14 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   double gb = b * dcg;
   double ga = uni(F.zero_dc) ? -gb * (double)(n - 1) / 2 : a * dcg;
   const bool plain = ga == 0 && gb == 0;
+  // Every slot is written (past the live end with a value the row's store
+  // never uses: those samples were stored unchanged before the steps), so
+  // the row's registers are dead during the transforms.
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     int j = tid_ + NTH * k;
-    if (j < n) {
-      T y = out[(p + j) ^ 1];
-      v[k] = plain ? y : (T)((double)y + (ga + gb * (double)j));
-    }
+    T y = out[(j < n ? p + j : p) ^ 1];
+    v[k] = plain ? y : (T)((double)y + (ga + gb * (double)j));
     PSEG_ONE_AT_A_TIME();
   }
   __syncthreads();

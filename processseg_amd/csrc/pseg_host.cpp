@@ -390,6 +390,28 @@ void patch(P &field, char *base) {
 
 // Stage twiddles and split table of a real FFT of n points (fft/plan.nim:
 // 136-160): float64, stored as T.
+// The passes of an m-point complex transform as the device runs them
+// (consecutive stages fused while fusable): the first pass's butterfly size
+// when a second pass reads its padded output, else 0.
+int32_t first_pass_pad(int64_t m) {
+  std::vector<int> rad = radices_of(m);
+  int passes = 0, first = 0;
+  for (size_t i = 0; i < rad.size();) {
+    const bool pair = i + 1 < rad.size() && fusable(rad[i], rad[i + 1]);
+    if (passes == 0) first = pair ? rad[i] * rad[i + 1] : rad[i];
+    i += pair ? 2 : 1;
+    passes++;
+  }
+  return passes >= 2 ? first : 0;
+}
+
+// LDS bytes of the two ping-pong buffers of an n-point real transform in T.
+size_t fft_lds_bytes(int64_t n, size_t esz) {
+  const int64_t m = n / 2;
+  const int32_t pad = first_pass_pad(m);
+  return 2 * (size_t)(m + (pad ? m / pad : 0)) * 2 * esz;
+}
+
 template <typename T>
 void build_fft(int64_t n, DevFft &f, Blob &blob) {
   memset(&f, 0, sizeof f);
@@ -417,6 +439,8 @@ void build_fft(int64_t n, DevFft &f, Blob &blob) {
     s *= r;
   }
   if (tw.empty()) tw.assign(2, (T)0);
+  f.pad = first_pass_pad(f.m);
+  f.stride = f.m + (f.pad ? f.m / f.pad : 0);
   std::vector<T> sp(2 * (f.m + 1));
   for (int64_t k = 0; k <= f.m; k++) {
     double a = -2 * kPi * (double)k / (double)n;
@@ -527,7 +551,7 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       P.step_margin[i] = (int32_t)margin;
       P.step_reflect[i] = reflect;
       P.margin += (int32_t)margin;
-      lds = 2 * (size_t)n * sizeof(T);
+      lds = fft_lds_bytes(n, sizeof(T));
       break;
     }
     case PSEG_OP_AGC: {
@@ -1166,8 +1190,8 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   double frac = sft - (double)whole;
   int64_t n = next_fast((int64_t)record + plen + 64 + std::llabs(whole), true);
   size_t esz = dtype == PSEG_F32 ? 4 : 8;
-  size_t lds = 2 * (size_t)n * esz;
-  if (lds > 160 * 1024 - 2048)
+  size_t lds = fft_lds_bytes(n, esz);
+  if (lds > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "a correlation FFT of %lld points does not fit one workgroup's LDS", (long long)n);
   // The pilot's spectrum in float64.
   HostFft hf(n);

@@ -41,9 +41,21 @@ struct DevFft {
   int32_t sub[kMaxRadices];
   int32_t tw_off[kMaxRadices];
   FastDiv sdiv[kMaxRadices];
+  // The first pass (butterflies of `pad` points, stride 1) writes its
+  // output with one spare element after every `pad`, so that the lanes of
+  // a store hit different LDS banks; the second pass reads that layout.
+  // 0: no padding (a single pass).  `stride`: complex elements one LDS
+  // buffer of the transform holds (m + m / pad).
+  int32_t pad;
+  int32_t stride;
   const void *tw;     // T2 twiddles of every stage
   const void *split;  // T2 (cos, sin)(-2 pi k / n), k = 0 .. m
 };
+
+// Two consecutive Stockham stages whose butterfly holds at most 16 points
+// run as one pass through LDS (host and device share the rule: the host
+// sizes the first pass's padded layout from it).
+__host__ __device__ constexpr bool fusable(int r1, int r2) { return r1 * r2 <= 16; }
 
 struct DevFilter {
   DevFft fft;

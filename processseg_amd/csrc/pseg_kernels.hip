@@ -222,7 +222,7 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
   const int n = f.n;
   T *buf = (T *)lds;
   V2<T> *A = (V2<T> *)lds;
-  V2<T> *B = (V2<T> *)(lds + sizeof(T) * (size_t)n);
+  V2<T> *B = A + f.stride;
   const T *s = src + row * (int64_t)src_cols;
   for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
   __syncthreads();

@@ -39,6 +39,19 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, doub
     int j = tid_ + NTH * k;
     v[k] = j < cols ? s[j] : (T)0;
   }
+  // Samples past the live end leave now, unchanged: every step changes live
+  // samples only and may leave junk in the slots past n, so those registers
+  // are free for the steps.
+  const int c0 = J.crop0, c1 = J.crop1;
+  if (J.src != J.dst || c0 != 0) {
+    const int live = max(n, 0);
+    T *dd = d - c0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      int j = tid_ + NTH * k;
+      if (j >= live && j >= c0 && j < c1) dd[j] = v[k];
+    }
+  }
   if (n > 0) {
     if (J.has_factor) {
       // a per-panel factor at the start of the part (pipeline.nim:717-718)
@@ -118,14 +131,14 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, doub
       }
     }
   }
-  const int c0 = J.crop0, c1 = J.crop1;
+  // the live samples
   {
     const int tid_ = opaque_tid();
     T *dd = d - c0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       int j = tid_ + NTH * k;
-      if (j >= c0 && j < c1) dd[j] = v[k];
+      if (j < n && j >= c0 && j < c1) dd[j] = v[k];
     }
   }
 }
