@@ -676,6 +676,16 @@ __device__ __forceinline__ Seg block_seg_scan_excl(Seg x, int dir, double *red) 
   return r;
 }
 
+// The AGC gain step's level and quotient.  float32: the hardware square
+// root and reciprocal (within 1 ulp each; the gain then differs from the
+// correctly rounded one by at most a few ulp, far inside the float32
+// tolerance) instead of the correctly rounded sequences, which cost about
+// ten instructions each per sample.  float64: correctly rounded.
+__device__ __forceinline__ float agc_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double agc_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float agc_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double agc_div(double a, double b) { return a / b; }
+
 // AGC (gain/gain.nim:404-461) on the row in registers.  The window sums are
 // the reference's block prefix and suffix sums (blocks of w = 2h + 1 samples,
 // gain/gain.nim:351-390) computed by the whole block: each thread takes a
@@ -801,8 +811,8 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
         M = unit * inv[min(max(c, 1), w)];
       }
       const double mean = whole ? headWhole : (sv + pv) * M;
-      const T lvl = mode == 0 ? (T)sqrt((T)mean) : (T)mean;
-      const T gi = kk / (lvl + eps);
+      const T lvl = mode == 0 ? agc_sqrt((T)mean) : (T)mean;
+      const T gi = agc_div(kk, lvl + eps);
       v[k] = (T)(v[k] * gi);
     }
     PSEG_ONE_AT_A_TIME();
