@@ -105,14 +105,29 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// LDS for block reductions: two slots used in turn.  A reduction writes its
+// slot, waits at one barrier and reads it; the next one uses the other slot,
+// so none needs a barrier before its write (every wave has finished reading
+// a slot before any wave passes the barrier of the reduction after it).
+// Every thread takes the slots in the same order.
+struct RedSlots {
+  double *base;  // 2 * size doubles
+  int size;
+  int flip;
+  __device__ __forceinline__ double *next() {
+    double *p = base + (flip ? size : 0);
+    flip ^= 1;
+    return p;
+  }
+};
+
 // The sum of v over the block, the same value on every thread, in a fixed
 // order (lane 0 of each wave, then the waves in order): a row's result does
-// not depend on anything but the row.
+// not depend on anything but the row.  `red`: a slot no wave still reads.
 template <int NTH>
 __device__ __forceinline__ double block_sum(double v, double *red) {
   const int tid_ = opaque_tid();
   v = wave_sum(v);
-  __syncthreads();
   if ((tid_ & 63) == 0) red[tid_ >> 6] = v;
   __syncthreads();
   double t = 0;
@@ -126,7 +141,6 @@ __device__ __forceinline__ void block_sum2(double &a, double &b, double *red) {
   const int tid_ = opaque_tid();
   a = wave_sum(a);
   b = wave_sum(b);
-  __syncthreads();
   if ((tid_ & 63) == 0) {
     red[tid_ >> 6] = a;
     red[NTH / 64 + (tid_ >> 6)] = b;
@@ -413,26 +427,29 @@ __device__ __forceinline__ V2<T> *complex_forward(const DevFft &f, V2<T> *a, V2<
   return x;
 }
 
-// splitForward (fft/plan.nim:470-482): bin k of the real transform from
-// Z[k] = a and Z[m-k] = b.
+// splitForward (fft/plan.nim:470-482), times 2: bin k of the real transform
+// from Z[k] = a and Z[m-k] = b.  The factor 1/2 of the split, and the
+// inverse's 1/n, are folded into the plan's bin multipliers (gain or pilot
+// multiplier times 1/(2n), computed in float64 and rounded once).
 template <typename T>
-__device__ __forceinline__ V2<T> split_fwd(V2<T> a, V2<T> b, V2<T> w) {
-  T er = (T)0.5 * (a.x + b.x);
-  T ei = (T)0.5 * (a.y - b.y);
-  T orr = (T)0.5 * (a.y + b.y);
-  T oi = (T)0.5 * (b.x - a.x);
+__device__ __forceinline__ V2<T> split_fwd2(V2<T> a, V2<T> b, V2<T> w) {
+  T er = a.x + b.x;
+  T ei = a.y - b.y;
+  T orr = a.y + b.y;
+  T oi = b.x - a.x;
   return mk<T>(er + w.x * orr - w.y * oi, ei + w.x * oi + w.y * orr);
 }
 
 // splitInverse (fft/plan.nim:484-499) with w = (cos, -sin): the swapped
-// (Im Z', Re Z') of the inverse's input from X[k] = a and X[m-k] = b.
+// (Im Z', Re Z') of the inverse's input from X[k] = a and X[m-k] = b (the
+// 1/n scale is in the multipliers).
 template <typename T>
-__device__ __forceinline__ V2<T> split_inv(V2<T> a, V2<T> b, V2<T> w, T h) {
+__device__ __forceinline__ V2<T> split_inv(V2<T> a, V2<T> b, V2<T> w) {
   T er = a.x + b.x, ei = a.y - b.y, dr = a.x - b.x, di = a.y + b.y;
   T wr = w.x, wi = -w.y;
   T orr = dr * wr - di * wi;
   T oi = dr * wi + di * wr;
-  return mk<T>((ei + orr) * h, (er - oi) * h);
+  return mk<T>(ei + orr, er - oi);
 }
 
 // Bin multipliers of the fused split / multiply / unsplit pass.
@@ -452,30 +469,33 @@ template <typename T> struct CplxMul {
 };
 
 // Z (the forward complex FFT of the packed samples) -> W, the swapped input
-// of the inverse: split into bins, multiply, unsplit with the 1/n scale.
+// of the inverse: split into bins, multiply, unsplit.  The split table is
+// read once per bin pair: exp(-2 pi i (m - k) / n) = -conj(exp(-2 pi i k / n))
+// for n = 2m.
 template <typename T, int NTH, typename Mul>
 __device__ __forceinline__ void combine(const DevFft &f, const V2<T> *__restrict__ Z, V2<T> *__restrict__ W,
                         Mul mul) {
   const int tid_ = opaque_tid();
   const int m = uni(f.m);
   CTab<T> sp = ctab<T>(uni_ptr(f.split));
-  const T scale = (T)1 / (T)uni(f.n);
   for (int k = tid_; k <= m / 2; k += NTH) {
     if (k == 0) {
       V2<T> z = Z[0];
       V2<T> x0 = mul(mk<T>(z.x + z.y, (T)0), 0);
       V2<T> xm = mul(mk<T>(z.x - z.y, (T)0), m);
-      W[0] = mk<T>((x0.x - xm.x) * scale, (x0.x + xm.x) * scale);
+      W[0] = mk<T>((x0.x - xm.x) * (T)2, (x0.x + xm.x) * (T)2);
     } else {
       const int kk = m - k;
       V2<T> a = Z[k], b = Z[kk];
-      V2<T> xk = mul(split_fwd<T>(a, b, sp[k]), k);
+      const V2<T> w = sp[k];
+      V2<T> xk = mul(split_fwd2<T>(a, b, w), k);
       if (kk != k) {
-        V2<T> xkk = mul(split_fwd<T>(b, a, sp[kk]), kk);
-        W[k] = split_inv<T>(xk, xkk, sp[k], scale);
-        W[kk] = split_inv<T>(xkk, xk, sp[kk], scale);
+        const V2<T> wk = mk<T>(-w.x, w.y);
+        V2<T> xkk = mul(split_fwd2<T>(b, a, wk), kk);
+        W[k] = split_inv<T>(xk, xkk, w);
+        W[kk] = split_inv<T>(xkk, xk, wk);
       } else {
-        W[k] = split_inv<T>(xk, xk, sp[k], scale);
+        W[k] = split_inv<T>(xk, xk, w);
       }
     }
   }
@@ -497,7 +517,7 @@ __device__ __forceinline__ double row_sum(const T (&v)[PER], int n) {
 
 // Least-squares line (filter/edges.nim:16-30, core/basic.nim:61-81 share it).
 template <typename T, int NTH, int PER>
-__device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, double *red, double &a,
+__device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, RedSlots &red, double &a,
                          double &b) {
   const int tid_ = opaque_tid();
   if (trend == 0) {
@@ -515,11 +535,11 @@ __device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, do
     }
   }
   if (trend == 1) {
-    a = block_sum<NTH>(sx, red) / (double)n;
+    a = block_sum<NTH>(sx, red.next()) / (double)n;
     b = 0;
     return;
   }
-  block_sum2<NTH>(sx, sxi, red);
+  block_sum2<NTH>(sx, sxi, red.next());
   double nf = (double)n, mean = sx / nf;
   if (n == 1) {
     a = mean;
@@ -534,7 +554,7 @@ __device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, do
 
 // One FFT filter step (filter/apply.nim:340-363) on the row in registers.
 template <typename T, int NTH, int PER>
-__device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int n, char *lds, double *red) {
+__device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int n, char *lds, RedSlots &red) {
   const int tid_ = opaque_tid();
   const DevFft &f = F.fft;
   T *buf = (T *)lds;                          // nfft samples = m complex
@@ -595,10 +615,13 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   const bool plain = ga == 0 && gb == 0;
   // Every slot is written (past the live end with a value the row's store
   // never uses: those samples were stored unchanged before the steps), so
-  // the row's registers are dead during the transforms.
+  // the row's registers are dead during the transforms.  The indices come
+  // from a fresh thread id: reusing the pack's would keep them live across
+  // both transforms.
+  const int tid2 = opaque_tid();
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    int j = tid_ + NTH * k;
+    int j = tid2 + NTH * k;
     T y = out[(j < n ? p + j : p) ^ 1];
     v[k] = plain ? y : (T)((double)y + (ga + gb * (double)j));
     PSEG_ONE_AT_A_TIME();
@@ -643,7 +666,6 @@ __device__ __forceinline__ Seg block_seg_scan_excl(Seg x, int dir, double *red) 
     ex.f = 0;
   }
   // the wave aggregates, then the waves before this one (in scan order)
-  __syncthreads();
   if ((dir == 0 && lane == 63) || (dir == 1 && lane == 0)) {
     red[wave] = incl.v;
     red[(NTH / 64) + wave] = (double)incl.f;
@@ -693,7 +715,7 @@ __device__ __forceinline__ double agc_div(double a, double b) { return a / b; }
 // runs, and no sum ever subtracts.
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, char *lds,
-                                         double *red) {
+                                         RedSlots &red) {
   const int tid_ = opaque_tid();
   const int mode = uni(G.mode);
   const double gtarget = uni(G.target), geps = uni(G.epsilon);
@@ -703,7 +725,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
     int j = tid_ + NTH * k;
     if (j < n) sq += (double)v[k] * (double)v[k];
   }
-  const double rms = sqrt(block_sum<NTH>(sq, red) / (double)n);
+  const double rms = sqrt(block_sum<NTH>(sq, red.next()) / (double)n);
   if (rms == 0) return;  // all zero: nothing to scale
   if (!fits_in<T>(gtarget / rms / geps)) return;
   const int h = uni(G.half), w = 2 * h + 1;
@@ -748,8 +770,8 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   }
   suf.f = (c0 < c1) && (r0 == 0 || past_start);
   if (c0 < c1 && r0 == 0) suf.v = 0;
-  Seg cpre = block_seg_scan_excl<NTH>(pre, 0, red);
-  Seg csuf = block_seg_scan_excl<NTH>(suf, 1, red);
+  Seg cpre = block_seg_scan_excl<NTH>(pre, 0, red.next());
+  Seg csuf = block_seg_scan_excl<NTH>(suf, 1, red.next());
   // prefix sums p[e + 1] (0 at a block start) ...
   double run = cpre.v;
   r = r0;
@@ -795,9 +817,10 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   gptr<double> inv = as_global<double>(uni_ptr(G.inv));
   const double unitW = unit / (double)w;
   const double headWhole = headCut < first ? pn * unit * inv[n] : 0.0;
+  const int tid3 = opaque_tid();  // fresh: nothing per slot stays live across the scans
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const int i = tid_ + NTH * k;
+    const int i = tid3 + NTH * k;
     if (i < n) {
       // the reference's five ranges as one expression: mean = (sv + pv) * M
       // with sv, pv 0 where a range does not add them (x + 0 = x exactly)
@@ -865,7 +888,7 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
 // Per-trace normalization (gain/gain.nim:198-219).
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds,
-                                double *red) {
+                                RedSlots &red) {
   const int tid_ = opaque_tid();
   using S = typename Bits<T>::S;
   __shared__ S smax[(NTH / 64)];
@@ -900,7 +923,7 @@ __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], 
       int j = tid_ + NTH * k;
       if (j < n) sq += (double)v[k] * (double)v[k];
     }
-    level = sqrt(block_sum<NTH>(sq, red) / (double)n);
+    level = sqrt(block_sum<NTH>(sq, red.next()) / (double)n);
   } else {
     // percentileAbs (gain/stats.nim:159-182): linear between order
     // statistics k and k + 1 of |x|, h = (n - 1) p / 100.
@@ -928,7 +951,7 @@ __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], 
           else up = b < up ? b : up;
         }
       }
-      cnt = block_sum<NTH>(cnt, red);
+      cnt = block_sum<NTH>(cnt, red.next());
       // block minimum of up
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {

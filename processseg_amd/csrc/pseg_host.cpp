@@ -531,7 +531,10 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       build_fft<T>(n, F.fft, blob);
       std::vector<T> gain(n / 2 + 1);
       double df = 1000 / ((double)n * dt);
-      for (int64_t k = 0; k <= n / 2; k++) gain[k] = (T)response(spec, (double)k * df, dt);
+      // the device folds the split's 1/2 and the inverse's 1/n into the gain
+      const double fold = 0.5 / (double)n;
+      for (int64_t k = 0; k <= n / 2; k++)
+        gain[k] = (T)(response(spec, (double)k * df, dt) * fold);
       if (spec.zero_dc) gain[0] = (T)0;
       std::vector<T> fall(std::max<int64_t>(pad, 1));
       for (int64_t k = 1; k <= pad; k++) {
@@ -1224,8 +1227,10 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   }
   // conj(P) w scale exp(+2 pi i k frac / n)
   std::vector<double> mul(2 * bins);
+  // the device folds the split's 1/2 and the inverse's 1/n into the multiplier
+  const double fold = 0.5 / (double)n;
   for (int64_t k = 0; k < bins; k++) {
-    double ws = w[k] * scale;
+    double ws = w[k] * scale * fold;
     double ar = P[k].real() * ws, ai = -P[k].imag() * ws;
     double th = 2 * kPi * (double)k * frac / (double)n;
     double er = std::cos(th), ei = std::sin(th);

@@ -25,7 +25,7 @@ namespace {
 // steps on its live part, store the job's crop.  Every thread of the block
 // runs it with the same (job, row).
 template <typename T, int NTH, int PER>
-__device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, double *red) {
+__device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedSlots &red) {
   const int tid_ = opaque_tid();
   const DevPlan *plan = J.plan;
   const int cols = J.cols;
@@ -71,7 +71,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, doub
         case kDemean: {
           const int tid_ = opaque_tid();
           // demean (core/basic.nim:52-59)
-          const T m = (T)(block_sum<NTH>(row_sum<T, NTH, PER>(v, n), red) / (double)n);
+          const T m = (T)(block_sum<NTH>(row_sum<T, NTH, PER>(v, n), red.next()) / (double)n);
 #pragma unroll
           for (int k = 0; k < PER; k++) {
             int j = tid_ + NTH * k;
@@ -162,11 +162,11 @@ __device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int n_jobs
 // balance across the chip; each block exits once the counter passes `total`.
 // n_jobs == 0: the single job passed by value.
 template <typename T, int NTH, int PER>
-__global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 2 : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
+__global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 3 : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
                                                       int32_t n_jobs, int32_t total,
                                                       uint32_t *__restrict__ counter) {
   extern __shared__ __align__(16) char lds[];
-  __shared__ double red[2 * (NTH / 64)];
+  __shared__ double red_lds[4 * (NTH / 64)];
   __shared__ int s_item;
   int item = blockIdx.x;
   for (;;) {
@@ -204,6 +204,7 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 2 : 1) void pipeline_kernel(J
       row = item - J.item0;
     }
     (void)jp;
+    RedSlots red{red_lds, 2 * (NTH / 64), 0};
     run_row<T, NTH, PER>(J, row, lds, red);
     if (counter == nullptr) break;
   }
