@@ -79,11 +79,12 @@ def cpu_baseline_leg(flow, x, live, gpu_out, seconds=12.0):
 def kernel_name(cols, threads, dtype="float"):
     """The instantiation pseg_plan_apply launches for rows of `cols` samples
     with the plan's `threads` per row: the row lives in registers,
-    ceil(cols / threads) samples per thread, rounded up to a compiled width."""
+    ceil(cols / threads) samples per thread, rounded up to a compiled width;
+    the form with demean / filter / AGC only (`false`)."""
     per = -(-cols // threads)
     widths = (8, 16, 24) if threads == 512 else (4, 8, 12, 16, 24)
     per = min(w for w in widths if w >= per)
-    return f"pseg::pipeline_kernel<{dtype}, {threads}, {per}>"
+    return f"pseg::pipeline_kernel<{dtype}, {threads}, {per}, false>"
 
 
 def copy_probe(torch, dev, args):

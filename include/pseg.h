@@ -207,8 +207,8 @@ typedef struct pseg_batch_info {
   int32_t launches;        /* persistent launches per pseg_batch_launch */
   int64_t rows;            /* rows of all items */
   int64_t samples;         /* rows x cols of all items */
-  int32_t threads[4];      /* threads per row of each launch */
-  int32_t grid[4];         /* workgroups of each launch */
+  int32_t threads[12];     /* threads per row of each launch */
+  int32_t grid[12];        /* workgroups of each launch */
 } pseg_batch_info;
 
 /* Validates every item as pseg_plan_apply would (before any sample changes;

@@ -60,7 +60,7 @@ class BatchItemC(C.Structure):
 
 class BatchInfoC(C.Structure):
     _fields_ = [("items", C.c_int32), ("launches", C.c_int32), ("rows", C.c_int64),
-                ("samples", C.c_int64), ("threads", C.c_int32 * 4), ("grid", C.c_int32 * 4)]
+                ("samples", C.c_int64), ("threads", C.c_int32 * 12), ("grid", C.c_int32 * 12)]
 
 
 class ServerConfigC(C.Structure):

@@ -671,6 +671,7 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   }
   H.n_steps = n_ops;
   P->n_steps = n_ops;
+  for (int i = 0; i < n_ops; i++) P->full |= !core_step(H.steps[i].kind);
   // parts: a per-panel normalization after the first op starts one
   // (pipeline.nim:515-522)
   int start = 0;
@@ -740,8 +741,9 @@ struct pseg_batch {
   int32_t n_items;
   int64_t rows, samples;
   int n_launches;
-  int dtype[4], threads[4];
-  JobLaunch launch[4];
+  // one launch per (dtype, threads per row, kernel form)
+  int dtype[kMaxBatchClasses], threads[kMaxBatchClasses], full[kMaxBatchClasses];
+  JobLaunch launch[kMaxBatchClasses];
   char *dev;  // the job tables, then one 256-byte counter per launch
 };
 
@@ -979,6 +981,7 @@ pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int3
     L.lds_bytes = lds;
     L.total = rows;
     L.max_cols = cols;
+    for (int i = J.step_first; i < J.step_last; i++) L.full |= !core_step(P->host.steps[i].kind);
     hipError_t e = launch_pipeline(P->dtype, P->threads, L, s);
     if (e != hipSuccess) return hip_fail(e, "pipeline launch");
   }
@@ -1040,14 +1043,13 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
     B->samples += (int64_t)it.rows * it.cols;
     int g = 0;
     for (; g < B->n_launches; g++)
-      if (B->dtype[g] == it.plan->dtype && B->threads[g] == it.plan->threads) break;
+      if (B->dtype[g] == it.plan->dtype && B->threads[g] == it.plan->threads &&
+          B->full[g] == it.plan->full)
+        break;
     if (g == B->n_launches) {
-      if (g == 4) {
-        delete B;
-        return fail(PSEG_E_UNSUPPORTED, "more than 4 (dtype, threads) classes in one batch");
-      }
       B->dtype[g] = it.plan->dtype;
       B->threads[g] = it.plan->threads;
+      B->full[g] = it.plan->full;
       B->n_launches++;
       groups.emplace_back();
     }
@@ -1091,8 +1093,10 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
     L.n_jobs = (int32_t)ids.size();
     L.total = item0;
     L.lds_bytes = lds;
+    L.full = B->full[g];
     int resident = 0;
-    hipError_t e = pipeline_resident_blocks(B->dtype[g], B->threads[g], L.max_cols, lds, &resident);
+    hipError_t e = pipeline_resident_blocks(B->dtype[g], B->threads[g], L.max_cols, lds, L.full,
+                                            &resident);
     if (e != hipSuccess) {
       delete B;
       return hip_fail(e, "occupancy of the pipeline kernel");

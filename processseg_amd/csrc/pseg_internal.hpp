@@ -116,12 +116,16 @@ struct Job {
 // One launch: a single job by value (n_jobs == 0) or n_jobs jobs in device
 // memory; `total` work items; direct (one block per item) or persistent
 // (counter != NULL, `grid` blocks take items from *counter, cleared first).
+// `full`: some job runs a step outside demean / filter / AGC, so the launch
+// takes the kernel compiled with every step (the other, smaller one holds
+// only those three, for the instruction cache).
 struct JobLaunch {
   Job single;
   const Job *jobs;
   int32_t n_jobs, total, grid, max_cols;
   uint32_t *counter;
   size_t lds_bytes;
+  int32_t full;
 };
 
 struct DevCorr {
@@ -141,12 +145,20 @@ hipError_t launch_pipeline_nt128(int dtype, const JobLaunch &a, hipStream_t stre
 hipError_t launch_pipeline_nt256(int dtype, const JobLaunch &a, hipStream_t stream);
 hipError_t launch_pipeline_nt512(int dtype, const JobLaunch &a, hipStream_t stream);
 // Workgroups of the kernel launch_pipeline would pick for (dtype, threads,
-// max_cols) that fit on the device at once with `lds` bytes each.
-hipError_t pipeline_resident_blocks(int dtype, int threads, int max_cols, size_t lds,
+// max_cols, full) that fit on the device at once with `lds` bytes each.
+hipError_t pipeline_resident_blocks(int dtype, int threads, int max_cols, size_t lds, int full,
                                     int *blocks);
-hipError_t resident_pipeline_nt128(int dtype, int max_cols, size_t lds, int *blocks);
-hipError_t resident_pipeline_nt256(int dtype, int max_cols, size_t lds, int *blocks);
-hipError_t resident_pipeline_nt512(int dtype, int max_cols, size_t lds, int *blocks);
+hipError_t resident_pipeline_nt128(int dtype, int max_cols, size_t lds, int full, int *blocks);
+hipError_t resident_pipeline_nt256(int dtype, int max_cols, size_t lds, int full, int *blocks);
+hipError_t resident_pipeline_nt512(int dtype, int max_cols, size_t lds, int full, int *blocks);
+// Launch classes of a batch at most: 2 dtypes x 3 thread counts x 2 forms.
+constexpr int kMaxBatchClasses = 12;
+
+// The steps the smaller kernel runs (see JobLaunch::full); a per-panel
+// normalization's factor is applied by the job itself, not by a step.
+constexpr bool core_step(int kind) {
+  return kind == kDemean || kind == kFilter || kind == kAgc || kind == kNormPanel;
+}
 // Per-panel normalization factor (into *factor, NaN: leave the panel alone)
 // of the live rows of x; scratch: rows doubles, 2 rows uint32 and, for a
 // percentile, 65536 uint32 of histogram (cleared here).

@@ -298,12 +298,12 @@ hipError_t launch_pipeline(int dtype, int threads, const JobLaunch &a, hipStream
   }
 }
 
-hipError_t pipeline_resident_blocks(int dtype, int threads, int max_cols, size_t lds,
+hipError_t pipeline_resident_blocks(int dtype, int threads, int max_cols, size_t lds, int full,
                                     int *blocks) {
   switch (threads) {
-    case 128: return resident_pipeline_nt128(dtype, max_cols, lds, blocks);
-    case 256: return resident_pipeline_nt256(dtype, max_cols, lds, blocks);
-    case 512: return resident_pipeline_nt512(dtype, max_cols, lds, blocks);
+    case 128: return resident_pipeline_nt128(dtype, max_cols, lds, full, blocks);
+    case 256: return resident_pipeline_nt256(dtype, max_cols, lds, full, blocks);
+    case 512: return resident_pipeline_nt512(dtype, max_cols, lds, full, blocks);
     default: return hipErrorInvalidValue;
   }
 }

@@ -24,7 +24,7 @@ namespace {
 // One row of one job: load the row's window into registers, run the job's
 // steps on its live part, store the job's crop.  Every thread of the block
 // runs it with the same (job, row).
-template <typename T, int NTH, int PER>
+template <typename T, int NTH, int PER, bool FULL>
 __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedSlots &red) {
   const int tid_ = opaque_tid();
   const DevPlan *plan = J.plan;
@@ -79,54 +79,60 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
           }
           break;
         }
-        case kDetrend: {
-          const int tid_ = opaque_tid();
-          // detrend (core/basic.nim:61-81)
-          double a, b;
-          trend_of<T, NTH, PER>(v, n, 2, red, a, b);
-          const T ta = (T)a, tb = (T)b;
+        case kFilter: filter_step<T, NTH, PER>(st.filter, v, n, lds, red); break;
+        case kAgc: agc_step<T, NTH, PER>(st.agc, v, n, lds, red); break;
+        case kDetrend:
+          if constexpr (FULL) {
+            const int tid_ = opaque_tid();
+            // detrend (core/basic.nim:61-81)
+            double a, b;
+            trend_of<T, NTH, PER>(v, n, 2, red, a, b);
+            const T ta = (T)a, tb = (T)b;
 #pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NTH * k;
-            if (j < n) v[k] = v[k] - (ta + tb * (T)j);
-          }
-          break;
-        }
-        case kScale: {
-          const int tid_ = opaque_tid();
-          const T kf = (T)uni(st.factor);
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NTH * k;
-            if (j < n) v[k] = v[k] * kf;
-          }
-          break;
-        }
-        case kClip: {
-          const int tid_ = opaque_tid();
-          const T lo = (T)uni(st.lo), hi = (T)uni(st.hi);
-#pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NTH * k;
-            if (j < n) {
-              T x = v[k];
-              v[k] = x > hi ? hi : (x < lo ? lo : x);
+            for (int k = 0; k < PER; k++) {
+              int j = tid_ + NTH * k;
+              if (j < n) v[k] = v[k] - (ta + tb * (T)j);
             }
           }
           break;
-        }
-        case kPolarity: {
-          const int tid_ = opaque_tid();
+        case kScale:
+          if constexpr (FULL) {
+            const int tid_ = opaque_tid();
+            const T kf = (T)uni(st.factor);
 #pragma unroll
-          for (int k = 0; k < PER; k++) {
-            int j = tid_ + NTH * k;
-            if (j < n) v[k] = -v[k];
+            for (int k = 0; k < PER; k++) {
+              int j = tid_ + NTH * k;
+              if (j < n) v[k] = v[k] * kf;
+            }
           }
           break;
-        }
-        case kFilter: filter_step<T, NTH, PER>(st.filter, v, n, lds, red); break;
-        case kAgc: agc_step<T, NTH, PER>(st.agc, v, n, lds, red); break;
-        case kNormTrace: norm_trace_step<T, NTH, PER>(st, v, n, lds, red); break;
+        case kClip:
+          if constexpr (FULL) {
+            const int tid_ = opaque_tid();
+            const T lo = (T)uni(st.lo), hi = (T)uni(st.hi);
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+              int j = tid_ + NTH * k;
+              if (j < n) {
+                T x = v[k];
+                v[k] = x > hi ? hi : (x < lo ? lo : x);
+              }
+            }
+          }
+          break;
+        case kPolarity:
+          if constexpr (FULL) {
+            const int tid_ = opaque_tid();
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+              int j = tid_ + NTH * k;
+              if (j < n) v[k] = -v[k];
+            }
+          }
+          break;
+        case kNormTrace:
+          if constexpr (FULL) norm_trace_step<T, NTH, PER>(st, v, n, lds, red);
+          break;
         default: break;
       }
     }
@@ -161,7 +167,7 @@ __device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int n_jobs
 // from the counter until they run out, so rows of very different cost
 // balance across the chip; each block exits once the counter passes `total`.
 // n_jobs == 0: the single job passed by value.
-template <typename T, int NTH, int PER>
+template <typename T, int NTH, int PER, bool FULL>
 __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 3 : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
                                                       int32_t n_jobs, int32_t total,
                                                       uint32_t *__restrict__ counter) {
@@ -205,7 +211,7 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 3 : 1) void pipeline_kernel(J
     }
     (void)jp;
     RedSlots red{red_lds, 2 * (NTH / 64), 0};
-    run_row<T, NTH, PER>(J, row, lds, red);
+    run_row<T, NTH, PER, FULL>(J, row, lds, red);
     if (counter == nullptr) break;
   }
 }
@@ -213,20 +219,20 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 3 : 1) void pipeline_kernel(J
 namespace {
 
 // Lets the kernel take up to kMaxRowLds bytes of dynamic LDS (once).
-template <typename T, int NTH, int PER>
+template <typename T, int NTH, int PER, bool FULL>
 hipError_t prepare_kernel() {
   static const hipError_t e = hipFuncSetAttribute(
-      (const void *)pipeline_kernel<T, NTH, PER>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (const void *)pipeline_kernel<T, NTH, PER, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize,
       (int)kMaxRowLds);
   return e;
 }
 
-// One launch of pipeline_kernel<T, NTH, PER>: clears the work counter of a
+// One launch of pipeline_kernel<T, NTH, PER, FULL>: clears the work counter of a
 // persistent launch first (a kernel, not a memset, so the launch can be
 // captured in a graph).
-template <typename T, int NTH, int PER>
+template <typename T, int NTH, int PER, bool FULL>
 hipError_t launch_one(const JobLaunch &a, hipStream_t stream) {
-  hipError_t e = prepare_kernel<T, NTH, PER>();
+  hipError_t e = prepare_kernel<T, NTH, PER, FULL>();
   if (e != hipSuccess) return e;
   if (a.counter) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, a.counter, 1);
@@ -234,20 +240,20 @@ hipError_t launch_one(const JobLaunch &a, hipStream_t stream) {
     if (e != hipSuccess) return e;
   }
   const int grid = a.counter ? a.grid : a.total;
-  hipLaunchKernelGGL((pipeline_kernel<T, NTH, PER>), dim3(grid), dim3(NTH), a.lds_bytes, stream,
+  hipLaunchKernelGGL((pipeline_kernel<T, NTH, PER, FULL>), dim3(grid), dim3(NTH), a.lds_bytes, stream,
                      a.single, a.jobs, a.n_jobs, a.total, a.counter);
   return hipGetLastError();
 }
 
-// Workgroups of pipeline_kernel<T, NTH, PER> resident on the whole device at
+// Workgroups of pipeline_kernel<T, NTH, PER, FULL> resident on the whole device at
 // once with `lds` bytes of dynamic LDS each (the grid of a persistent launch).
-template <typename T, int NTH, int PER>
+template <typename T, int NTH, int PER, bool FULL>
 hipError_t resident_one(size_t lds, int *blocks) {
-  hipError_t e = prepare_kernel<T, NTH, PER>();
+  hipError_t e = prepare_kernel<T, NTH, PER, FULL>();
   if (e != hipSuccess) return e;
   int per_cu = 0, dev = 0, cus = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void *)pipeline_kernel<T, NTH, PER>, NTH, lds);
+      &per_cu, (const void *)pipeline_kernel<T, NTH, PER, FULL>, NTH, lds);
   if (e != hipSuccess) return e;
   e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;

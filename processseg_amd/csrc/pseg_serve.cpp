@@ -143,7 +143,7 @@ struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   Job *host_jobs = nullptr;      // pinned staging
-  char *dev = nullptr;           // job table + counters
+  char *dev = nullptr;           // job table + one counter per launch (at most 6)
   std::vector<uint64_t> tickets;
 };
 
@@ -160,7 +160,8 @@ struct pseg_server {
   std::mutex plan_mu;
   std::vector<Flow> flows;
   std::map<std::tuple<int, int, int>, pseg_plan *> plans;
-  std::map<std::tuple<int, int, int, size_t>, int> resident;  // (threads, per, dtype, lds)
+  // (threads, per, dtype + 2 x form, lds) -> resident workgroups
+  std::map<std::tuple<int, int, int, size_t>, int> resident;
   Slot slots[kSlots];
   std::mutex mu;
   std::condition_variable work_cv, done_cv;
@@ -274,8 +275,8 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     ok.push_back({r, p});
   }
   // one persistent launch per threads-per-row class; heavier rows first
-  std::map<int, std::vector<Item>> classes;
-  for (const Item &it : ok) classes[it.plan->threads].push_back(it);
+  std::map<std::pair<int, int>, std::vector<Item>> classes;  // (threads, kernel form)
+  for (const Item &it : ok) classes[{it.plan->threads, it.plan->full}].push_back(it);
   size_t n_jobs = 0;
   std::vector<JobLaunch> launches;
   std::vector<int> launch_threads;
@@ -315,13 +316,15 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     L.n_jobs = (int32_t)v.size();
     L.total = item0;
     L.lds_bytes = lds;
-    const int threads = kv.first;
+    const int threads = kv.first.first;
+    L.full = kv.first.second;
     const int per = (L.max_cols + threads - 1) / threads;
-    auto key = std::make_tuple(threads, per, (int)cfg.dtype, lds);
+    auto key = std::make_tuple(threads, per, (int)cfg.dtype + 2 * L.full, lds);
     auto rit = resident.find(key);
     int blocks = 0;
     if (rit == resident.end()) {
-      if (pipeline_resident_blocks(cfg.dtype, threads, L.max_cols, lds, &blocks) != hipSuccess)
+      if (pipeline_resident_blocks(cfg.dtype, threads, L.max_cols, lds, L.full, &blocks) !=
+          hipSuccess)
         blocks = 256;
       resident[key] = blocks;
     } else {
@@ -434,7 +437,7 @@ pseg_status pseg_server_create(const pseg_server_config *cfg, pseg_server **out)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess)
       e = hipHostMalloc((void **)&sl.host_jobs, kMaxJobs * sizeof(Job), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc((void **)&sl.dev, kMaxJobs * sizeof(Job) + 256 * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&sl.dev, kMaxJobs * sizeof(Job) + 256 * 8);
   }
   if (e != hipSuccess) {
     pseg_status st = hip_error(e, "server set-up");
