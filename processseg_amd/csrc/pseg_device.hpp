@@ -745,31 +745,58 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   pre.f = 0;
   suf.v = 0;
   suf.f = 0;
-  bool past_start = false;  // a block start seen after c0 (suffix)
   // e % w, kept incrementally (no division per sample)
   const int r0 = c0 < n ? c0 % w : 0;
   int r = r0;
+  if (c <= w) {
+    // A run holds at most one block start, at ks: the suffix part is the
+    // terms before it, the prefix part the terms from it (the same sums, in
+    // the same order, as the general loop below).
+    const int len = c1 - c0;
+    const int ks = r0 == 0 ? 0 : w - r0;
+    double lo = 0, hi = 0;
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int e = c0 + k;
-    double t = 0;
-    if (e < c1) {
-      const double x = (double)s[e];
-      t = mode == 0 ? x * x : fabs(x);
-      const bool start = r == 0;
-      // prefix: the sum since the last block start in the run
-      pre.v = start ? t : pre.v + t;
-      pre.f |= start;
-      // suffix: the run's terms before the first block start after c0
-      if (start && k > 0) past_start = true;
-      if (!past_start) suf.v += t;
-      r = r + 1 == w ? 0 : r + 1;
+    for (int k = 0; k < PER; k++) {
+      const int e = c0 + k;
+      double t = 0;
+      if (e < c1) {
+        const double x = (double)s[e];
+        t = mode == 0 ? x * x : fabs(x);
+      }
+      tv[k] = t;
+      if (k < ks) lo += t;
+      else hi += t;
+      PSEG_ONE_AT_A_TIME();
     }
-    tv[k] = t;
-    PSEG_ONE_AT_A_TIME();
+    const bool has = ks < len;
+    pre.v = has ? hi : lo;
+    pre.f = has;
+    suf.v = lo;
+    suf.f = has;
+  } else {
+    bool past_start = false;  // a block start seen after c0 (suffix)
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int e = c0 + k;
+      double t = 0;
+      if (e < c1) {
+        const double x = (double)s[e];
+        t = mode == 0 ? x * x : fabs(x);
+        const bool start = r == 0;
+        // prefix: the sum since the last block start in the run
+        pre.v = start ? t : pre.v + t;
+        pre.f |= start;
+        // suffix: the run's terms before the first block start after c0
+        if (start && k > 0) past_start = true;
+        if (!past_start) suf.v += t;
+        r = r + 1 == w ? 0 : r + 1;
+      }
+      tv[k] = t;
+      PSEG_ONE_AT_A_TIME();
+    }
+    suf.f = (c0 < c1) && (r0 == 0 || past_start);
+    if (c0 < c1 && r0 == 0) suf.v = 0;
   }
-  suf.f = (c0 < c1) && (r0 == 0 || past_start);
-  if (c0 < c1 && r0 == 0) suf.v = 0;
   Seg cpre = block_seg_scan_excl<NTH>(pre, 0, red.next());
   Seg csuf = block_seg_scan_excl<NTH>(suf, 1, red.next());
   // prefix sums p[e + 1] (0 at a block start) ...
@@ -818,10 +845,18 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   const double unitW = unit / (double)w;
   const double headWhole = headCut < first ? pn * unit * inv[n] : 0.0;
   const int tid3 = opaque_tid();  // fresh: nothing per slot stays live across the scans
+  const int wave0 = __builtin_amdgcn_readfirstlane(tid3);  // the wave's first thread
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int i = tid3 + NTH * k;
-    if (i < n) {
+    const int lo = wave0 + NTH * k;  // the wave's samples: lo .. lo + 63
+    if (lo >= first && lo + 63 < last) {
+      // the whole wave in the interior range (most of the row): full windows
+      const double mean = ((double)s[i - h] + (double)pp[i + h + 1]) * unitW;
+      const T lvl = mode == 0 ? agc_sqrt((T)mean) : (T)mean;
+      const T gi = agc_div(kk, lvl + eps);
+      v[k] = (T)(v[k] * gi);
+    } else if (i < n) {
       // the reference's five ranges as one expression: mean = (sv + pv) * M
       // with sv, pv 0 where a range does not add them (x + 0 = x exactly)
       const bool head = i < headCut, mid = i >= first && i < last;

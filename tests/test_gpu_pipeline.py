@@ -66,6 +66,8 @@ def check(flow, x, row_len, oracle, tol=None, dt=2.0, label=""):
     ("trmean", [P.filterOp(P.bandPass(P.Corners(4, 8, 60, 80)), trend=P.trMean)]),
     ("agc500", [P.agcOp(500)]),
     ("agc-meanabs", [P.agcOp(300, mode=P.agcMeanAbs, target=2.0)]),
+    ("agc-short", [P.agcOp(20)]),          # 11-sample windows: shorter than a thread's run
+    ("agc-run", [P.agcOp(30)]),            # 17-sample windows: just longer than a run
     ("norm-max", [P.normalizeOp(P.normMax)]),
     ("norm-rms", [P.normalizeOp(P.normRms, target=3.0)]),
     ("norm-p99", [P.normalizeOp(P.normPercentile)]),
