@@ -231,3 +231,40 @@ def test_wide_rows_percentile(gpu, oracle):
     check(P.initPipeline([P.bandPassOp(P.Corners(4, 8, 60, 80)),
                           P.normalizeOp(P.normPercentile, percentile=99)]),
           x, live, oracle, label="wide-p99")
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_config2_full_panel(gpu, oracle, dtype):
+    """The whole 2000 x 3001 panel at its real shape: every row against the
+    FP32 mirror and 48 rows spread over the panel against the FP64 oracle;
+    in place equals src -> dst bit for bit; a row's bits do not depend on
+    the rows launched with it (the same rows in reverse order)."""
+    import torch
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel(2000, 3001, dtype)
+    flow = config2_flow()
+    plan = flow.compile(dtype, 2.0, 3001)
+    rl = torch.from_numpy(live.astype(np.int32)).cuda()
+    src = torch.from_numpy(x).cuda()
+    dst = torch.empty_like(src)
+    plan.apply(src, dst, row_len=rl)
+    inplace = src.clone()
+    plan.apply(inplace, row_len=rl)
+    rev = torch.flip(src, [0]).contiguous()
+    rev_out = torch.empty_like(rev)
+    plan.apply(rev, rev_out, row_len=torch.flip(rl, [0]).contiguous())
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy()
+    assert np.array_equal(got, inplace.cpu().numpy())
+    assert np.array_equal(got, torch.flip(rev_out, [0]).cpu().numpy())
+    idx = np.linspace(0, 1999, 48).astype(int)
+    want = oracle.pipeline_apply(flow.ops, x[idx].astype(np.float64), live[idx].astype(np.int32),
+                                 2.0)
+    e = rel_l2(got[idx], want)
+    print(f"full panel {dtype.__name__}: rel L2 {e:.3e} vs FP64 oracle on 48 rows")
+    assert e <= (F32_TOL if dtype == np.float32 else F64_TOL)
+    if dtype == np.float32:
+        mirror = oracle.pipeline_apply(flow.ops, x, live.astype(np.int32), 2.0)
+        e32 = rel_l2(got, mirror)
+        print(f"full panel: rel L2 {e32:.3e} vs the FP32 mirror on every row")
+        assert e32 <= F32_TOL
