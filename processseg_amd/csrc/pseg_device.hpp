@@ -503,15 +503,11 @@ __device__ __forceinline__ void combine(const DevFft &f, const V2<T> *__restrict
 
 // ------------------------------------------------------------ row helpers
 
-template <typename T, int NTH, int PER>
-__device__ __forceinline__ double row_sum(const T (&v)[PER], int n) {
-  const int tid_ = opaque_tid();
-  double s = 0;
+template <typename T, int PER>
+__device__ __forceinline__ double row_sum(const T (&v)[PER]) {
+  double s = 0;  // slots past the live end hold +0: adding them changes nothing
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NTH * k;
-    if (j < n) s += (double)v[k];
-  }
+  for (int k = 0; k < PER; k++) s += (double)v[k];
   return s;
 }
 
@@ -525,14 +521,12 @@ __device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, Re
     b = 0;
     return;
   }
-  double sx = 0, sxi = 0;
+  double sx = 0, sxi = 0;  // slots past the live end hold +0
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     int j = tid_ + NTH * k;
-    if (j < n) {
-      sx += (double)v[k];
-      sxi += (double)j * (double)v[k];
-    }
+    sx += (double)v[k];
+    sxi += (double)j * (double)v[k];
   }
   if (trend == 1) {
     a = block_sum<NTH>(sx, red.next()) / (double)n;
@@ -613,17 +607,19 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   double gb = b * dcg;
   double ga = uni(F.zero_dc) ? -gb * (double)(n - 1) / 2 : a * dcg;
   const bool plain = ga == 0 && gb == 0;
-  // Every slot is written (past the live end with a value the row's store
-  // never uses: those samples were stored unchanged before the steps), so
-  // the row's registers are dead during the transforms.  The indices come
-  // from a fresh thread id: reusing the pack's would keep them live across
-  // both transforms.
+  // Every slot is written (+0 past the live end: those samples were stored
+  // unchanged before the steps), so the row's registers are dead during the
+  // transforms.  The indices come from a fresh thread id: reusing the pack's
+  // would keep them live across both transforms.
   const int tid2 = opaque_tid();
+  const int last_slot = 2 * uni(f.stride) - 2;  // an even index inside the buffer
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     int j = tid2 + NTH * k;
-    T y = out[(j < n ? p + j : p) ^ 1];
-    v[k] = plain ? y : (T)((double)y + (ga + gb * (double)j));
+    // slots past the live end read any sample of the two transform buffers
+    T y = out[min(p + j, last_slot) ^ 1];
+    y = plain ? y : (T)((double)y + (ga + gb * (double)j));
+    v[k] = j < n ? y : (T)0;
     PSEG_ONE_AT_A_TIME();
   }
   __syncthreads();
@@ -722,8 +718,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   double sq = 0;
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    int j = tid_ + NTH * k;
-    if (j < n) sq += (double)v[k] * (double)v[k];
+    sq += (double)v[k] * (double)v[k];  // +0 past the live end
   }
   const double rms = sqrt(block_sum<NTH>(sq, red.next()) / (double)n);
   if (rms == 0) return;  // all zero: nothing to scale

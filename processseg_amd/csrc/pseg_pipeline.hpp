@@ -40,8 +40,8 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
     v[k] = j < cols ? s[j] : (T)0;
   }
   // Samples past the live end leave now, unchanged: every step changes live
-  // samples only and may leave junk in the slots past n, so those registers
-  // are free for the steps.
+  // samples only.  Their slots then hold +0 for the steps (sums over every
+  // slot need no test of the live length; steps keep them +0).
   const int c0 = J.crop0, c1 = J.crop1;
   if (J.src != J.dst || c0 != 0) {
     const int live = max(n, 0);
@@ -51,6 +51,11 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
       int j = tid_ + NTH * k;
       if (j >= live && j >= c0 && j < c1) dd[j] = v[k];
     }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    int j = tid_ + NTH * k;
+    v[k] = j < n ? v[k] : (T)0;
   }
   if (n > 0) {
     if (J.has_factor) {
@@ -71,7 +76,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
         case kDemean: {
           const int tid_ = opaque_tid();
           // demean (core/basic.nim:52-59)
-          const T m = (T)(block_sum<NTH>(row_sum<T, NTH, PER>(v, n), red.next()) / (double)n);
+          const T m = (T)(block_sum<NTH>(row_sum<T, PER>(v), red.next()) / (double)n);
 #pragma unroll
           for (int k = 0; k < PER; k++) {
             int j = tid_ + NTH * k;
