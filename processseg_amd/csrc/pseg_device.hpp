@@ -3,8 +3,8 @@
 //
 // The real FFT of n points is a complex Stockham FFT of m = n/2 points on the
 // interleaved samples (fft/plan.nim:501-594); its stages ping-pong between two
-// LDS buffers of m complex values, two stages per pass where the fused
-// butterfly holds at most 16 points.  Split, gain and the inverse's unsplit
+// LDS buffers (m complex values, plus one spare per first-pass butterfly),
+// two stages per pass where the fused butterfly holds at most 16 points.  Split, gain and the inverse's unsplit
 // are one fused pass over the bins, and the inverse transform is the forward
 // one on swapped (im, re) pairs, so the swap back is an XOR of the sample
 // index.  Sums (demean, trend, rms, AGC windows) run in float64 as in the

@@ -1,5 +1,7 @@
 // The fused row kernel of the panel pipeline, templated on the row width:
-// NTH threads (NTH / 64 wave64) per row, PER samples per thread.
+// NTH threads (NTH / 64 wave64) per row, PER samples per thread; FULL: every
+// step kind, else demean / filter / AGC only (less code for the instruction
+// cache, the form most flows take).
 //
 // One workgroup per row.  The row crosses HBM once: it is loaded into
 // registers (sample j on thread j mod NTH, slot j / NTH, so every load and
@@ -19,6 +21,7 @@
 namespace pseg {
 
 namespace {
+
 // ---------------------------------------------------------------- kernels
 
 // One row of one job: load the row's window into registers, run the job's
