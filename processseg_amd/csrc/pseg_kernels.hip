@@ -273,14 +273,11 @@ hipError_t panel_typed(const T *x, int32_t rows, int32_t cols, const int32_t *ro
 template <typename T>
 hipError_t corr_typed(const DevCorr *dplan, size_t lds, const T *src, int32_t rows,
                       int32_t src_cols, const int32_t *row_len, T *dst, hipStream_t stream) {
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute((const void *)corr_kernel<T, 1024>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       160 * 1024 - 2048);
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
+  // once per process (a thread-safe static), like the pipeline kernels
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)corr_kernel<T, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (attr != hipSuccess) return attr;
   hipLaunchKernelGGL((corr_kernel<T, 1024>), dim3(rows), dim3(1024), lds, stream, dplan, src,
                      src_cols, row_len, dst);
   return hipGetLastError();
