@@ -163,6 +163,28 @@ def correlation_leg(torch, dev, steps, warmup, pairs, with_cpu):
     return out
 
 
+def panels_batch_leg(torch, P, plan, srcs, dsts, row_len, steps, warmup):
+    """The same plan on the 8 rotating panels as one PipelineBatch (16000
+    rows, one persistent launch): what a caller with many panels at hand
+    gets, the ends of consecutive panels overlapping.  Reported beside the
+    headline, which applies one panel per step as Plan.apply does."""
+    batch = P.PipelineBatch([P.BatchItem(plan, s, d, row_len=row_len)
+                             for s, d in zip(srcs, dsts)])
+    for _ in range(warmup):
+        batch.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.launch()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    samples = batch.samples
+    return {"workload": f"{len(srcs)} panels of 2000x3001 f32, the headline flow, one batch",
+            "value": round(samples / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_batch": round(t * 1000, 4), "grid": batch.grid,
+            "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -393,11 +415,14 @@ def main():
         line["cpu_baseline"] = cpu
         line["accuracy"] = acc
     if not args.no_secondary:
+        log("panels as one batch")
+        line["secondary"] = {"panels_as_batch": panels_batch_leg(
+            torch, P, plan, srcs, dsts, row_len, max(10, args.steps // 10), 3)}
         del srcs, dsts
         torch.cuda.empty_cache()
         log("correlation leg")
-        line["secondary"] = {"correlation": correlation_leg(
-            torch, dev, max(10, args.steps // 4), 3, 4, not args.no_cpu)}
+        line["secondary"]["correlation"] = correlation_leg(
+            torch, dev, max(10, args.steps // 4), 3, 4, not args.no_cpu)
         torch.cuda.empty_cache()
         log("uneven batch leg")
         line["secondary"]["uneven_batch"] = batch_leg(torch, dev, max(5, args.steps // 20), 2)
