@@ -274,121 +274,122 @@ template <typename T> struct Bfly<T, 5> {
   }
 };
 
-// One Stockham stage x -> y: y[q + s(rp + k)] = (sum_j x[q + s(p + j sub)]
-// w_r^{jk}) w_cur^{pk}, butterfly b = p s + q.
-// pad_in: x is the padded output of a first pass of s points (element i at
-// i + i / s); pad_out (s == 1): write y padded (one spare element after
-// every R).  Both only change uniform strides and the per-butterfly bases.
-template <typename T, int R, int NTH>
-__device__ __forceinline__ void stage(const V2<T> *__restrict__ x, V2<T> *__restrict__ y,
-                                      CTab<T> tw, int m, int s, int sub, FastDiv sd,
-                                      int pad_in, int pad_out) {
-  const int tid_ = opaque_tid();
-  // the stride is uniform: keep it (and every product of it) in SGPRs
-  s = uni(s);
-  const int nb = uni(m / R);
-  const bool twiddle = sub > 1;
-  const int in_step = uni(s * sub + pad_in * sub);   // between the inputs of a butterfly
-  const int in_p = uni(s + pad_in), out_p = uni(s * R + pad_out);
-#pragma clang loop unroll(disable)
-  for (int b = tid_; b < nb; b += NTH) {
-    const int p = (int)fdiv((uint32_t)b, sd);
-    const int q = b - p * s;
-    const V2<T> *xi = x + opaque_v(q + in_p * p);
-    V2<T> *yo = y + opaque_v(q + out_p * p);
-    V2<T> v[R];
-#pragma unroll
-    for (int j = 0; j < R; j++) v[j] = xi[j * in_step];
-    Bfly<T, R>::run(v);
-    if (twiddle) {
-      CTab<T> t = tw + p * (R - 1) - 1;
-#pragma unroll
-      for (int k = 1; k < R; k++) v[k] = cmul<T>(v[k], t[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < R; k++) yo[k * s] = v[k];
-  }
-}
-
-// Two consecutive Stockham stages (radix R1 at stride s, then R2 at stride
-// s R1) in one pass through LDS.  Butterfly b = p2 s + q loads the R1 R2
+// One pass of the Stockham DIF transform, in place in one LDS buffer: one
+// stage of radix R1 (R2 == 1) or two consecutive stages, radix R1 at stride s
+// then R2 at stride s R1, fused.  Butterfly b = p2 s + q loads the R1 R2
 // inputs x[q + s(p2 + j2 sub2 + j sub1)], runs the R2 radix-R1 butterflies of
-// the first stage (p = p2 + j2 sub2) with their twiddles, then the R1
-// radix-R2 butterflies of the second (p2) with theirs, and stores
-// y[q + s k + s R1 (R2 p2 + k2)]: the same operations, in the same order, as
-// the two stages one after the other, without the round trip between them.
+// the first stage (p = p2 + j2 sub2) with their twiddles w_cur^{pk}, then
+// the R1 radix-R2 butterflies of the second (p2) with theirs, and stores
+// y[q + s k + s R1 (R2 p2 + k2)]: the operations of fft/plan.nim:225-366 in
+// the same order, without a round trip through LDS between the two stages.
+//
+// In place: every thread first loads the inputs of all its butterflies (at
+// most MAXIT of them, in registers), the block waits, then every thread
+// computes and stores; the caller waits again before the next pass.  One
+// buffer per transform instead of two halves the LDS a row holds, so more
+// rows share a CU.
+//
+// pad_in: x is the padded output of a first pass of s points (element i at
+// i + i / s); pad_out (s == 1): y is written padded (one spare element after
+// every R1 R2, so the lanes of a store hit different banks).  Both only
+// change uniform strides and per-butterfly bases.
 template <typename T, int R1, int R2, int NTH>
-__device__ __forceinline__ void stage2(const V2<T> *__restrict__ x, V2<T> *__restrict__ y,
-                                       CTab<T> tw1, CTab<T> tw2, int m, int s, int sub1,
-                                       FastDiv sd, int pad_in, int pad_out) {
-  const int tid_ = opaque_tid();
+__device__ __forceinline__ void pass_inplace(V2<T> *__restrict__ x, CTab<T> tw1, CTab<T> tw2,
+                                             int m, int s, int sub1, FastDiv sd, int pad_in,
+                                             int pad_out) {
+  constexpr int R = R1 * R2;
+  constexpr int MAXIT = pass_iters_max(R);
   // the strides are uniform: keep them (and their multiples) in SGPRs
   s = uni(s);
   const int sub2 = uni(sub1 / R2);
-  const int nb = uni(m / (R1 * R2));
-  const bool twiddle2 = sub2 > 1;
+  const int nb = uni(m / R);
+  const bool twiddle1 = sub1 > 1;
+  const bool twiddle2 = R2 > 1 && sub2 > 1;
   // input strides of j and j2 (in the padded layout when pad_in)
   const int in1 = uni(s * sub1 + pad_in * sub1), in2 = uni(s * sub2 + pad_in * sub2);
-  const int out2 = uni(s * R1);                        // output stride of k2
-  const int in_p = uni(s + pad_in), out_p = uni(s * R1 * R2 + pad_out);
-#pragma clang loop unroll(disable)
-  for (int b = tid_; b < nb; b += NTH) {
-    const int p2 = (int)fdiv((uint32_t)b, sd);
-    const int q = b - p2 * s;
-    const V2<T> *xi = x + opaque_v(q + in_p * p2);
-    V2<T> *yo = y + opaque_v(q + out_p * p2);
-    V2<T> v[R2][R1];
+  const int out2 = uni(s * R1);  // output stride of k2
+  const int in_p = uni(s + pad_in), out_p = uni(s * R + pad_out);
+  const int t1_step = uni(sub2 * (R1 - 1));
+  V2<T> v[MAXIT][R2][R1];
+  {
+    const int tid_ = opaque_tid();
 #pragma unroll
-    for (int j2 = 0; j2 < R2; j2++)
+    for (int it = 0; it < MAXIT; it++) {
+      const int b = tid_ + NTH * it;
+      if (b < nb) {
+        const int p2 = (int)fdiv((uint32_t)b, sd);
+        const int q = b - p2 * s;
+        const V2<T> *xi = x + opaque_v(q + in_p * p2);
 #pragma unroll
-      for (int j = 0; j < R1; j++) v[j2][j] = xi[j2 * in2 + j * in1];
-    CTab<T> t1 = tw1 + p2 * (R1 - 1) - 1;
-    const int t1_step = uni(sub2 * (R1 - 1));
+        for (int j2 = 0; j2 < R2; j2++)
 #pragma unroll
-    for (int j2 = 0; j2 < R2; j2++) {
-      Bfly<T, R1>::run(v[j2]);
-#pragma unroll
-      for (int k = 1; k < R1; k++) v[j2][k] = cmul<T>(v[j2][k], t1[j2 * t1_step + k]);
-    }
-    CTab<T> t2 = tw2 + p2 * (R2 - 1) - 1;
-#pragma unroll
-    for (int k = 0; k < R1; k++) {
-      V2<T> u[R2];
-#pragma unroll
-      for (int j2 = 0; j2 < R2; j2++) u[j2] = v[j2][k];
-      Bfly<T, R2>::run(u);
-      if (twiddle2) {
-#pragma unroll
-        for (int k2 = 1; k2 < R2; k2++) u[k2] = cmul<T>(u[k2], t2[k2]);
+          for (int j = 0; j < R1; j++) v[it][j2][j] = xi[j2 * in2 + j * in1];
       }
-#pragma unroll
-      for (int k2 = 0; k2 < R2; k2++) yo[k * s + k2 * out2] = u[k2];
     }
   }
-}
-
-// Stage pairs whose butterfly holds at most 16 points run fused (fusable,
-// shared with the host); the register footprint of larger ones would cost
-// more occupancy than the LDS round trip they save.
-
-template <typename T, int R1, int NTH>
-__device__ __forceinline__ void stage2_second(int r2, const V2<T> *x, V2<T> *y, CTab<T> t1,
-                                              CTab<T> t2, int m, int s, int sub1,
-                                              FastDiv sd, int pi, int po) {
-  switch (r2) {
-    case 4: if constexpr (fusable(R1, 4)) stage2<T, R1, 4, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
-    case 2: if constexpr (fusable(R1, 2)) stage2<T, R1, 2, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
-    case 3: if constexpr (fusable(R1, 3)) stage2<T, R1, 3, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
-    default: if constexpr (fusable(R1, 5)) stage2<T, R1, 5, NTH>(x, y, t1, t2, m, s, sub1, sd, pi, po); break;
+  __syncthreads();
+  const int tid_ = opaque_tid();
+#pragma unroll
+  for (int it = 0; it < MAXIT; it++) {
+    const int b = tid_ + NTH * it;
+    if (b < nb) {
+      const int p2 = (int)fdiv((uint32_t)b, sd);
+      const int q = b - p2 * s;
+      V2<T> *yo = x + opaque_v(q + out_p * p2);
+      CTab<T> t1 = tw1 + p2 * (R1 - 1) - 1;
+#pragma unroll
+      for (int j2 = 0; j2 < R2; j2++) {
+        Bfly<T, R1>::run(v[it][j2]);
+        if (twiddle1) {
+#pragma unroll
+          for (int k = 1; k < R1; k++)
+            v[it][j2][k] = cmul<T>(v[it][j2][k], t1[j2 * t1_step + k]);
+        }
+      }
+      if constexpr (R2 == 1) {
+#pragma unroll
+        for (int k = 0; k < R1; k++) yo[k * s] = v[it][0][k];
+      } else {
+        CTab<T> t2 = tw2 + p2 * (R2 - 1) - 1;
+#pragma unroll
+        for (int k = 0; k < R1; k++) {
+          V2<T> u[R2];
+#pragma unroll
+          for (int j2 = 0; j2 < R2; j2++) u[j2] = v[it][j2][k];
+          Bfly<T, R2>::run(u);
+          if (twiddle2) {
+#pragma unroll
+            for (int k2 = 1; k2 < R2; k2++) u[k2] = cmul<T>(u[k2], t2[k2]);
+          }
+#pragma unroll
+          for (int k2 = 0; k2 < R2; k2++) yo[k * s + k2 * out2] = u[k2];
+        }
+      }
+    }
+    PSEG_ONE_AT_A_TIME();
   }
 }
 
-// The complex forward FFT of a (m points) with b as the other buffer: the
-// plan's stages in pairs; returns the buffer holding the result.
+// Stage pairs whose butterfly holds at most 9 points run fused (fusable,
+// shared with the host); the registers of larger ones would cost more
+// occupancy than the LDS round trip they save.  The host checks that no
+// pass needs more than pass_iters_max butterflies per thread.
+template <typename T, int R1, int NTH>
+__device__ __forceinline__ void pass_second(int r2, V2<T> *x, CTab<T> t1, CTab<T> t2, int m,
+                                            int s, int sub1, FastDiv sd, int pi, int po) {
+  switch (r2) {
+    case 4: if constexpr (fusable(R1, 4)) pass_inplace<T, R1, 4, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 2: if constexpr (fusable(R1, 2)) pass_inplace<T, R1, 2, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 3: if constexpr (fusable(R1, 3)) pass_inplace<T, R1, 3, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    default: if constexpr (fusable(R1, 5)) pass_inplace<T, R1, 5, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+  }
+}
+
+// The complex forward FFT of the m points at a, in place (a holds f.stride
+// complex values): the plan's stages in passes.
 template <typename T, int NTH>
-__device__ __forceinline__ V2<T> *complex_forward(const DevFft &f, V2<T> *a, V2<T> *b) {
+__device__ __forceinline__ void complex_forward(const DevFft &f, V2<T> *a) {
   CTab<T> tw = ctab<T>(uni_ptr(f.tw));
-  V2<T> *x = a, *y = b;
   int s = 1;
   const int nrad = uni(f.nrad), m = uni(f.m);
   const int padded = uni(f.pad) != 0;
@@ -402,29 +403,25 @@ __device__ __forceinline__ V2<T> *complex_forward(const DevFft &f, V2<T> *a, V2<
     if (i + 1 < nrad && fusable(r, r2)) {
       CTab<T> t2 = tw + uni(f.tw_off[i + 1]);
       switch (r) {
-        case 4: stage2_second<T, 4, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
-        case 2: stage2_second<T, 2, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
-        case 3: stage2_second<T, 3, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
-        default: stage2_second<T, 5, NTH>(r2, x, y, t, t2, m, s, sub, sd, pi, po); break;
+        case 4: pass_second<T, 4, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        case 2: pass_second<T, 2, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        case 3: pass_second<T, 3, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        default: pass_second<T, 5, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
       }
       s *= r * r2;
       i += 2;
     } else {
       switch (r) {
-        case 4: stage<T, 4, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
-        case 2: stage<T, 2, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
-        case 3: stage<T, 3, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
-        default: stage<T, 5, NTH>(x, y, t, m, s, sub, sd, pi, po); break;
+        case 4: pass_inplace<T, 4, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
+        case 2: pass_inplace<T, 2, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
+        case 3: pass_inplace<T, 3, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
+        default: pass_inplace<T, 5, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
       }
       s *= r;
       i += 1;
     }
     __syncthreads();
-    V2<T> *tmp = x;
-    x = y;
-    y = tmp;
   }
-  return x;
 }
 
 // splitForward (fft/plan.nim:470-482), times 2: bin k of the real transform
@@ -469,12 +466,12 @@ template <typename T> struct CplxMul {
 };
 
 // Z (the forward complex FFT of the packed samples) -> W, the swapped input
-// of the inverse: split into bins, multiply, unsplit.  The split table is
+// of the inverse: split into bins, multiply, unsplit.  W may be Z: each
+// thread reads bins k and m - k before it writes them.  The split table is
 // read once per bin pair: exp(-2 pi i (m - k) / n) = -conj(exp(-2 pi i k / n))
 // for n = 2m.
 template <typename T, int NTH, typename Mul>
-__device__ __forceinline__ void combine(const DevFft &f, const V2<T> *__restrict__ Z, V2<T> *__restrict__ W,
-                        Mul mul) {
+__device__ __forceinline__ void combine(const DevFft &f, const V2<T> *Z, V2<T> *W, Mul mul) {
   const int tid_ = opaque_tid();
   const int m = uni(f.m);
   CTab<T> sp = ctab<T>(uni_ptr(f.split));
@@ -552,9 +549,8 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   const int tid_ = opaque_tid();
   const DevFft &f = F.fft;
   T *buf = (T *)lds;                          // nfft samples = m complex
-  V2<T> *A = (V2<T> *)lds;
+  V2<T> *A = (V2<T> *)lds;                   // the transform's buffer (f.stride)
   const int fn = uni(f.n);
-  V2<T> *B = A + uni(f.stride);
   double a, b;
   trend_of<T, NTH, PER>(v, n, uni(F.trend), red, a, b);
   const int p = uni(F.pad), nf = fn;
@@ -587,20 +583,17 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
     }
     __syncthreads();
   }
-  // forward FFT, split * gain * unsplit, inverse FFT: one copy of the FFT
-  // code serves both directions
-  V2<T> *R = A;
+  // forward FFT, split * gain * unsplit, inverse FFT, all in place: one
+  // copy of the FFT code serves both directions
 #pragma nounroll
   for (int dir = 0; dir < 2; dir++) {
-    R = complex_forward<T, NTH>(f, R, R == A ? B : A);
+    complex_forward<T, NTH>(f, A);
     if (dir == 0) {
-      V2<T> *W = R == A ? B : A;
-      combine<T, NTH>(f, R, W, GainMul<T>{as_global<T>(uni_ptr(F.gain))});
+      combine<T, NTH>(f, A, A, GainMul<T>{as_global<T>(uni_ptr(F.gain))});
       __syncthreads();
-      R = W;
     }
   }
-  const T *out = (const T *)R;
+  const T *out = (const T *)A;
   // unpack (unpackFrom, filter/apply.nim:218-231): sample i of the inverse is
   // element i ^ 1 of the swapped result.
   const double dcg = uni(F.dc_gain);
@@ -724,8 +717,21 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   if (rms == 0) return;  // all zero: nothing to scale
   if (!fits_in<T>(gtarget / rms / geps)) return;
   const int h = uni(G.half), w = 2 * h + 1;
-  T *s = (T *)lds;  // x, then the suffix sums in place
-  T *pp = s + n;    // prefix sums, n + 1 values
+  // The reference's five ranges of i (gain/gain.nim:404-461).
+  const int first = min(h, n);
+  const int last = max(first, n - h);
+  const int headCut = min(max(n - h, 0), first);
+  const int lastBlock = (n - 1) / w * w;
+  const int tailCut = min(max(lastBlock + h, last), n);
+  const double unit = mode == 0 ? 1 / (rms * rms) : 1 / rms;
+  const double unitW = unit / (double)w;
+  // LDS, n + first + 1 values: s[0, n) holds x, then the suffix sums S in
+  // place, then over s[0, n - w] the interior means (S[e] + P[e + w]) / w in
+  // T; hp[0, headCut) holds the prefix sums P[h + 1 + i] the head range
+  // reads, hp[first] holds P[n].  The tail reads S[e] at e > n - w, which the
+  // means never overwrite.
+  T *s = (T *)lds;
+  T *hp = s + n;
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     int j = tid_ + NTH * k;
@@ -794,24 +800,10 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   }
   Seg cpre = block_seg_scan_excl<NTH>(pre, 0, red.next());
   Seg csuf = block_seg_scan_excl<NTH>(suf, 1, red.next());
-  // prefix sums p[e + 1] (0 at a block start) ...
-  double run = cpre.v;
-  r = r0;
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int e = c0 + k;
-    if (e < c1) {
-      if (r == 0) run = 0;
-      run += tv[k];
-      r = r + 1 == w ? 0 : r + 1;
-      pp[e + 1] = r == 0 ? (T)0 : (T)run;
-    }
-  }
-  if (tid_ == 0) pp[0] = (T)0;
-  // ... and suffix sums s[e], over x in place (this thread's run only);
-  // the run's last sample has remainder (c1 - 1) % w = (r0 + len - 1) % w
+  // suffix sums S[e], over x in place (this thread's run only); the run's
+  // last sample has remainder (c1 - 1) % w = (r0 + len - 1) % w
   const int len = c1 - c0;
-  run = (c1 < n && (r0 + len) % w != 0) ? csuf.v : 0.0;
+  double run = (c1 < n && (r0 + len) % w != 0) ? csuf.v : 0.0;
   r = len > 0 ? (r0 + len - 1) % w : 0;
 #pragma unroll
   for (int k = PER - 1; k >= 0; k--) {
@@ -824,20 +816,33 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
     }
   }
   __syncthreads();
-  // The gain step in T on means scaled by the trace level, five ranges of i.
-  const double unit = mode == 0 ? 1 / (rms * rms) : 1 / rms;
+  // prefix sums P[j], j = e + 1 (0 at a block end), used as they are made:
+  // the interior mean of sample j - h - 1 replaces S[j - w], which only this
+  // thread reads; the head's and P[n] go to hp.
+  run = cpre.v;
+  r = r0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int e = c0 + k;
+    if (e < c1) {
+      if (r == 0) run = 0;
+      run += tv[k];
+      r = r + 1 == w ? 0 : r + 1;
+      const T pj = r == 0 ? (T)0 : (T)run;
+      const int j = e + 1;
+      if (j >= w) s[j - w] = (T)(((double)s[j - w] + (double)pj) * unitW);
+      if (j - h - 1 >= 0 && j - h - 1 < headCut) hp[j - h - 1] = pj;
+      if (j == n) hp[first] = pj;
+    }
+  }
+  __syncthreads();
+  // The gain step in T on means scaled by the trace level.
   const T kk = (T)(gtarget / rms);
   const T eps = (T)geps;
-  const int first = min(h, n);
-  const int last = max(first, n - h);
-  const int headCut = min(max(n - h, 0), first);
-  const int lastBlock = (n - 1) / w * w;
-  const int tailCut = min(max(lastBlock + h, last), n);
-  const double pn = (double)pp[n];
+  const double pn = (double)hp[first];
   // 1 / count for a window of `count` samples: inv[c] = 1/c; the tail's
   // rev[m] = 1/(2h+1-m) is inv[2h+1-m], the same float64 value.
   gptr<double> inv = as_global<double>(uni_ptr(G.inv));
-  const double unitW = unit / (double)w;
   const double headWhole = headCut < first ? pn * unit * inv[n] : 0.0;
   const int tid3 = opaque_tid();  // fresh: nothing per slot stays live across the scans
   const int wave0 = __builtin_amdgcn_readfirstlane(tid3);  // the wave's first thread
@@ -847,24 +852,27 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
     const int lo = wave0 + NTH * k;  // the wave's samples: lo .. lo + 63
     if (lo >= first && lo + 63 < last) {
       // the whole wave in the interior range (most of the row): full windows
-      const double mean = ((double)s[i - h] + (double)pp[i + h + 1]) * unitW;
-      const T lvl = mode == 0 ? agc_sqrt((T)mean) : (T)mean;
+      const T mean = s[i - h];
+      const T lvl = mode == 0 ? agc_sqrt(mean) : mean;
       const T gi = agc_div(kk, lvl + eps);
       v[k] = (T)(v[k] * gi);
     } else if (i < n) {
-      // the reference's five ranges as one expression: mean = (sv + pv) * M
-      // with sv, pv 0 where a range does not add them (x + 0 = x exactly)
+      // the interior mean as stored; the edge ranges as one expression,
+      // mean = (sv + pv) * M with sv, pv 0 where a range does not add them
+      // (x + 0 = x exactly)
       const bool head = i < headCut, mid = i >= first && i < last;
       const bool whole = i >= headCut && i < first, tail1 = i >= last && i < tailCut;
-      const double sv = i >= first ? (double)s[max(i - h, 0)] : 0.0;
-      const double pv = (head || mid) ? (double)pp[min(i + h + 1, n)] : (tail1 ? pn : 0.0);
-      double M = unitW;
-      if (!mid) {
+      T mean;
+      if (mid) {
+        mean = s[i - h];
+      } else {
+        const double sv = i >= first ? (double)s[max(i - h, 0)] : 0.0;
+        const double pv = head ? (double)hp[i] : (tail1 ? pn : 0.0);
         const int c = head ? i + h + 1 : n - i + h;
-        M = unit * inv[min(max(c, 1), w)];
+        const double M = unit * inv[min(max(c, 1), w)];
+        mean = (T)(whole ? headWhole : (sv + pv) * M);
       }
-      const double mean = whole ? headWhole : (sv + pv) * M;
-      const T lvl = mode == 0 ? agc_sqrt((T)mean) : (T)mean;
+      const T lvl = mode == 0 ? agc_sqrt(mean) : mean;
       const T gi = agc_div(kk, lvl + eps);
       v[k] = (T)(v[k] * gi);
     }

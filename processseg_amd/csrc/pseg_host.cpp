@@ -391,25 +391,39 @@ void patch(P &field, char *base) {
 // Stage twiddles and split table of a real FFT of n points (fft/plan.nim:
 // 136-160): float64, stored as T.
 // The passes of an m-point complex transform as the device runs them
-// (consecutive stages fused while fusable): the first pass's butterfly size
-// when a second pass reads its padded output, else 0.
-int32_t first_pass_pad(int64_t m) {
-  std::vector<int> rad = radices_of(m);
-  int passes = 0, first = 0;
+// (consecutive stages fused while fusable): their butterfly sizes.
+std::vector<int> fft_passes(int64_t m) {
+  std::vector<int> rad = radices_of(m), out;
   for (size_t i = 0; i < rad.size();) {
     const bool pair = i + 1 < rad.size() && fusable(rad[i], rad[i + 1]);
-    if (passes == 0) first = pair ? rad[i] * rad[i + 1] : rad[i];
+    out.push_back(pair ? rad[i] * rad[i + 1] : rad[i]);
     i += pair ? 2 : 1;
-    passes++;
   }
-  return passes >= 2 ? first : 0;
+  return out;
 }
 
-// LDS bytes of the two ping-pong buffers of an n-point real transform in T.
+// The first pass's butterfly size when a second pass reads its padded
+// output, else 0.
+int32_t first_pass_pad(int64_t m) {
+  std::vector<int> passes = fft_passes(m);
+  return passes.size() >= 2 ? passes[0] : 0;
+}
+
+// LDS bytes of the in-place buffer of an n-point real transform in T.
 size_t fft_lds_bytes(int64_t n, size_t esz) {
   const int64_t m = n / 2;
   const int32_t pad = first_pass_pad(m);
-  return 2 * (size_t)(m + (pad ? m / pad : 0)) * 2 * esz;
+  return (size_t)(m + (pad ? m / pad : 0)) * 2 * esz;
+}
+
+// Whether `threads` threads run every pass of an m-point transform in place
+// (no thread holds more than pass_iters_max butterflies of a pass).
+bool fft_fits(int64_t m, int threads) {
+  for (int r : fft_passes(m)) {
+    const int64_t butterflies = m / r;
+    if ((butterflies + threads - 1) / threads > pass_iters_max(r)) return false;
+  }
+  return true;
 }
 
 template <typename T>
@@ -582,7 +596,8 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       st.kind = kAgc;
       P.step_margin[i] = (int32_t)H;
       P.margin += (int32_t)H;
-      lds = (2 * (size_t)samples + 2) * sizeof(T);
+      // x / suffix sums / interior means, plus the head's prefix sums and P[n]
+      lds = ((size_t)samples + (size_t)std::min<int64_t>(H, samples) + 1) * sizeof(T);
       break;
     }
     case PSEG_OP_NORMALIZE: {
@@ -643,7 +658,6 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   pseg_plan *P = new pseg_plan;
   memset(P, 0, sizeof *P);
   P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
-  P->threads = pipeline_threads_for(samples);
   P->dt_ms = dt;
   P->samples = samples;
   P->record = std::max(samples, record);
@@ -672,6 +686,20 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   H.n_steps = n_ops;
   P->n_steps = n_ops;
   for (int i = 0; i < n_ops; i++) P->full |= !core_step(H.steps[i].kind);
+  // the fewest threads per row that hold the row (24 samples a thread) and
+  // run every transform of the plan in place
+  for (int t = pipeline_threads_for(samples); t != 0 && P->threads == 0; t *= 2) {
+    if (t > 512) break;
+    bool fits = true;
+    for (int i = 0; i < n_ops; i++)
+      if (P->step_nfft[i] > 0 && !fft_fits(P->step_nfft[i] / 2, t)) fits = false;
+    if (fits) P->threads = t;
+  }
+  if (P->threads == 0) {
+    delete P;
+    return fail(PSEG_E_UNSUPPORTED, "the plan's FFTs are too long for the in-place transform "
+                "of one workgroup (at most 512 threads)");
+  }
   // parts: a per-panel normalization after the first op starts one
   // (pipeline.nim:515-522)
   int start = 0;
@@ -1198,6 +1226,9 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   int64_t n = next_fast((int64_t)record + plen + 64 + std::llabs(whole), true);
   size_t esz = dtype == PSEG_F32 ? 4 : 8;
   size_t lds = fft_lds_bytes(n, esz);
+  if (!fft_fits(n / 2, 1024))
+    return fail(PSEG_E_UNSUPPORTED, "a correlation FFT of %lld points is too long for the "
+                "in-place transform of one workgroup", (long long)n);
   if (lds > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "a correlation FFT of %lld points does not fit one workgroup's LDS", (long long)n);
   // The pilot's spectrum in float64.

@@ -44,18 +44,23 @@ struct DevFft {
   // The first pass (butterflies of `pad` points, stride 1) writes its
   // output with one spare element after every `pad`, so that the lanes of
   // a store hit different LDS banks; the second pass reads that layout.
-  // 0: no padding (a single pass).  `stride`: complex elements one LDS
-  // buffer of the transform holds (m + m / pad).
+  // 0: no padding (a single pass).  `stride`: complex elements of the
+  // transform's LDS buffer (m + m / pad); the passes run in place in it.
   int32_t pad;
   int32_t stride;
   const void *tw;     // T2 twiddles of every stage
   const void *split;  // T2 (cos, sin)(-2 pi k / n), k = 0 .. m
 };
 
-// Two consecutive Stockham stages whose butterfly holds at most 16 points
+// Two consecutive Stockham stages whose butterfly holds at most 9 points
 // run as one pass through LDS (host and device share the rule: the host
-// sizes the first pass's padded layout from it).
-__host__ __device__ constexpr bool fusable(int r1, int r2) { return r1 * r2 <= 16; }
+// sizes the first pass's padded layout and checks the passes' registers
+// from it).
+__host__ __device__ constexpr bool fusable(int r1, int r2) { return r1 * r2 <= 9; }
+
+// A pass of butterflies of r points runs in place with each thread holding
+// the inputs of at most this many butterflies (32 complex values).
+__host__ __device__ constexpr int pass_iters_max(int r) { return r >= 32 ? 1 : 32 / r; }
 
 struct DevFilter {
   DevFft fft;
@@ -176,10 +181,11 @@ constexpr size_t kMaxRowLds = 160 * 1024 - 2048;
 constexpr int pipeline_max_cols(int threads) {
   return threads == 128 ? 128 * 24 : threads == 256 ? 256 * 24 : threads == 512 ? 512 * 24 : 0;
 }
-// The thread count of a plan compiled for rows of `samples` samples: the
-// fewest threads whose widest row holds them (0: wider than any).  A plan
-// keeps it for every apply, so a row's arithmetic (the order of its block
-// reductions) depends on its plan alone, never on the batch around it.
+// The fewest threads whose widest row holds rows of `samples` samples (0:
+// wider than any).  A plan takes this count, or a larger one when its
+// transforms need more threads to run in place, and keeps it for every
+// apply, so a row's arithmetic (the order of its block reductions) depends
+// on its plan alone, never on the batch around it.
 constexpr int pipeline_threads_for(int samples) {
   return samples <= pipeline_max_cols(128) ? 128
          : samples <= pipeline_max_cols(256) ? 256

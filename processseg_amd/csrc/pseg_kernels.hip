@@ -221,16 +221,15 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
   const DevFft &f = c.fft;
   const int n = f.n;
   T *buf = (T *)lds;
-  V2<T> *A = (V2<T> *)lds;
-  V2<T> *B = A + f.stride;
+  V2<T> *A = (V2<T> *)lds;  // the transform's buffer (f.stride complex)
   const T *s = src + row * (int64_t)src_cols;
   for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
   __syncthreads();
-  V2<T> *Z = complex_forward<T, NTC>(f, A, B);
-  V2<T> *W = Z == A ? B : A;
-  combine<T, NTC>(f, Z, W, CplxMul<T>{ctab<T>(c.mul)});
+  complex_forward<T, NTC>(f, A);
+  combine<T, NTC>(f, A, A, CplxMul<T>{ctab<T>(c.mul)});
   __syncthreads();
-  const T *out = (const T *)complex_forward<T, NTC>(f, W, Z);
+  complex_forward<T, NTC>(f, A);
+  const T *out = (const T *)A;
   // lagSpans (vibroseis/correlate.nim:265-274)
   const int wrapped = min(max(1 - c.pilot - c.shift, 0), c.out);
   const int direct = min(max(-c.shift, wrapped), c.out);

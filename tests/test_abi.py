@@ -92,10 +92,11 @@ def test_corr_float64_full_size_is_refused():
 
 
 def test_pipeline_lds_limit_is_unsupported():
-    """A float64 filter on 10001-sample rows needs more LDS than one workgroup
-    has: compile refuses it with PSEG_E_UNSUPPORTED, naming the op."""
+    """A float64 low-cut at 0.1-0.2 Hz on 10001-sample rows pads to a
+    19200-point FFT, more LDS than one workgroup has: compile refuses it with
+    PSEG_E_UNSUPPORTED, naming the op."""
     import pytest
-    flow = P.initPipeline([P.demeanOp(), P.bandPassOp(P.Corners(5, 10, 70, 90))])
+    flow = P.initPipeline([P.demeanOp(), P.filterOp(P.lowCut(0.1, 0.2))])
     with pytest.raises(P.UnsupportedOp, match=r"op 1 \(.*LDS"):
         flow.compile(np.float64, 2.0, 10001)
     with pytest.raises(P.UnsupportedOp, match="beyond"):
