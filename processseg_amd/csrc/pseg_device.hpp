@@ -370,24 +370,25 @@ __device__ __forceinline__ void pass_inplace(V2<T> *__restrict__ x, CTab<T> tw1,
   }
 }
 
-// Stage pairs whose butterfly holds at most 9 points run fused (fusable,
+// Stage pairs whose butterfly holds at most FUSE points run fused (fusable,
 // shared with the host); the registers of larger ones would cost more
 // occupancy than the LDS round trip they save.  The host checks that no
 // pass needs more than pass_iters_max butterflies per thread.
-template <typename T, int R1, int NTH>
+template <typename T, int R1, int NTH, int FUSE>
 __device__ __forceinline__ void pass_second(int r2, V2<T> *x, CTab<T> t1, CTab<T> t2, int m,
                                             int s, int sub1, FastDiv sd, int pi, int po) {
   switch (r2) {
-    case 4: if constexpr (fusable(R1, 4)) pass_inplace<T, R1, 4, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
-    case 2: if constexpr (fusable(R1, 2)) pass_inplace<T, R1, 2, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
-    case 3: if constexpr (fusable(R1, 3)) pass_inplace<T, R1, 3, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
-    default: if constexpr (fusable(R1, 5)) pass_inplace<T, R1, 5, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 4: if constexpr (fusable(R1, 4, FUSE) && may_follow(R1, 4)) pass_inplace<T, R1, 4, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 2: if constexpr (fusable(R1, 2, FUSE) && may_follow(R1, 2)) pass_inplace<T, R1, 2, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 3: if constexpr (fusable(R1, 3, FUSE) && may_follow(R1, 3)) pass_inplace<T, R1, 3, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    default: if constexpr (fusable(R1, 5, FUSE) && may_follow(R1, 5)) pass_inplace<T, R1, 5, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
   }
 }
 
 // The complex forward FFT of the m points at a, in place (a holds f.stride
-// complex values): the plan's stages in passes.
-template <typename T, int NTH>
+// complex values): the plan's stages in passes, stage pairs of at most FUSE
+// points fused (the host built f's padded layout with the same FUSE).
+template <typename T, int NTH, int FUSE>
 __device__ __forceinline__ void complex_forward(const DevFft &f, V2<T> *a) {
   CTab<T> tw = ctab<T>(uni_ptr(f.tw));
   int s = 1;
@@ -400,13 +401,13 @@ __device__ __forceinline__ void complex_forward(const DevFft &f, V2<T> *a) {
     const int r2 = i + 1 < nrad ? uni(f.rad[i + 1]) : 0;
     // the first pass writes the padded layout, the second reads it
     const int po = padded && i == 0, pi = padded && s != 1 && s == uni(f.pad);
-    if (i + 1 < nrad && fusable(r, r2)) {
+    if (i + 1 < nrad && fusable(r, r2, FUSE)) {
       CTab<T> t2 = tw + uni(f.tw_off[i + 1]);
       switch (r) {
-        case 4: pass_second<T, 4, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-        case 2: pass_second<T, 2, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-        case 3: pass_second<T, 3, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-        default: pass_second<T, 5, NTH>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        case 4: pass_second<T, 4, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        case 2: pass_second<T, 2, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        case 3: pass_second<T, 3, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+        default: pass_second<T, 5, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
       }
       s *= r * r2;
       i += 2;
@@ -587,7 +588,7 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   // copy of the FFT code serves both directions
 #pragma nounroll
   for (int dir = 0; dir < 2; dir++) {
-    complex_forward<T, NTH>(f, A);
+    complex_forward<T, NTH, kFusePipeline>(f, A);
     if (dir == 0) {
       combine<T, NTH>(f, A, A, GainMul<T>{as_global<T>(uni_ptr(F.gain))});
       __syncthreads();

@@ -392,10 +392,10 @@ void patch(P &field, char *base) {
 // 136-160): float64, stored as T.
 // The passes of an m-point complex transform as the device runs them
 // (consecutive stages fused while fusable): their butterfly sizes.
-std::vector<int> fft_passes(int64_t m) {
+std::vector<int> fft_passes(int64_t m, int fuse) {
   std::vector<int> rad = radices_of(m), out;
   for (size_t i = 0; i < rad.size();) {
-    const bool pair = i + 1 < rad.size() && fusable(rad[i], rad[i + 1]);
+    const bool pair = i + 1 < rad.size() && fusable(rad[i], rad[i + 1], fuse);
     out.push_back(pair ? rad[i] * rad[i + 1] : rad[i]);
     i += pair ? 2 : 1;
   }
@@ -404,22 +404,22 @@ std::vector<int> fft_passes(int64_t m) {
 
 // The first pass's butterfly size when a second pass reads its padded
 // output, else 0.
-int32_t first_pass_pad(int64_t m) {
-  std::vector<int> passes = fft_passes(m);
+int32_t first_pass_pad(int64_t m, int fuse) {
+  std::vector<int> passes = fft_passes(m, fuse);
   return passes.size() >= 2 ? passes[0] : 0;
 }
 
 // LDS bytes of the in-place buffer of an n-point real transform in T.
-size_t fft_lds_bytes(int64_t n, size_t esz) {
+size_t fft_lds_bytes(int64_t n, size_t esz, int fuse) {
   const int64_t m = n / 2;
-  const int32_t pad = first_pass_pad(m);
+  const int32_t pad = first_pass_pad(m, fuse);
   return (size_t)(m + (pad ? m / pad : 0)) * 2 * esz;
 }
 
 // Whether `threads` threads run every pass of an m-point transform in place
 // (no thread holds more than pass_iters_max butterflies of a pass).
-bool fft_fits(int64_t m, int threads) {
-  for (int r : fft_passes(m)) {
+bool fft_fits(int64_t m, int threads, int fuse) {
+  for (int r : fft_passes(m, fuse)) {
     const int64_t butterflies = m / r;
     if ((butterflies + threads - 1) / threads > pass_iters_max(r)) return false;
   }
@@ -427,7 +427,7 @@ bool fft_fits(int64_t m, int threads) {
 }
 
 template <typename T>
-void build_fft(int64_t n, DevFft &f, Blob &blob) {
+void build_fft(int64_t n, DevFft &f, Blob &blob, int fuse) {
   memset(&f, 0, sizeof f);
   f.n = (int32_t)n;
   f.m = (int32_t)(n / 2);
@@ -453,7 +453,7 @@ void build_fft(int64_t n, DevFft &f, Blob &blob) {
     s *= r;
   }
   if (tw.empty()) tw.assign(2, (T)0);
-  f.pad = first_pass_pad(f.m);
+  f.pad = first_pass_pad(f.m, fuse);
   f.stride = f.m + (f.pad ? f.m / f.pad : 0);
   std::vector<T> sp(2 * (f.m + 1));
   for (int64_t k = 0; k <= f.m; k++) {
@@ -489,6 +489,7 @@ struct pseg_corr_plan {
   double dt_ms, scale;
   int32_t nfft, shift, out, pilot, record;
   size_t lds;
+  int threads;
   DevCorr *dev;
 };
 
@@ -542,7 +543,7 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       n = next_fast(n, true);
       if (n > (1 << 20)) invalid("an FFT of %lld points is beyond this GPU path", (long long)n);
       DevFilter &F = st.filter;
-      build_fft<T>(n, F.fft, blob);
+      build_fft<T>(n, F.fft, blob, kFusePipeline);
       std::vector<T> gain(n / 2 + 1);
       double df = 1000 / ((double)n * dt);
       // the device folds the split's 1/2 and the inverse's 1/n into the gain
@@ -568,7 +569,7 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       P.step_margin[i] = (int32_t)margin;
       P.step_reflect[i] = reflect;
       P.margin += (int32_t)margin;
-      lds = fft_lds_bytes(n, sizeof(T));
+      lds = fft_lds_bytes(n, sizeof(T), kFusePipeline);
       break;
     }
     case PSEG_OP_AGC: {
@@ -692,7 +693,7 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
     if (t > 512) break;
     bool fits = true;
     for (int i = 0; i < n_ops; i++)
-      if (P->step_nfft[i] > 0 && !fft_fits(P->step_nfft[i] / 2, t)) fits = false;
+      if (P->step_nfft[i] > 0 && !fft_fits(P->step_nfft[i] / 2, t, kFusePipeline)) fits = false;
     if (fits) P->threads = t;
   }
   if (P->threads == 0) {
@@ -1225,8 +1226,11 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   double frac = sft - (double)whole;
   int64_t n = next_fast((int64_t)record + plen + 64 + std::llabs(whole), true);
   size_t esz = dtype == PSEG_F32 ? 4 : 8;
-  size_t lds = fft_lds_bytes(n, esz);
-  if (!fft_fits(n / 2, 1024))
+  size_t lds = fft_lds_bytes(n, esz, kFuseCorr);
+  // 512 threads per record when every pass fits them (two records then
+  // share a CU when their LDS allows), else 1024.
+  const int threads = fft_fits(n / 2, 512, kFuseCorr) ? 512 : 1024;
+  if (!fft_fits(n / 2, threads, kFuseCorr))
     return fail(PSEG_E_UNSUPPORTED, "a correlation FFT of %lld points is too long for the "
                 "in-place transform of one workgroup", (long long)n);
   if (lds > kMaxRowLds)
@@ -1255,8 +1259,8 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   DevCorr C;
   memset(&C, 0, sizeof C);
   try {
-    if (dtype == PSEG_F32) build_fft<float>(n, C.fft, blob);
-    else build_fft<double>(n, C.fft, blob);
+    if (dtype == PSEG_F32) build_fft<float>(n, C.fft, blob, kFuseCorr);
+    else build_fft<double>(n, C.fft, blob, kFuseCorr);
   } catch (const Invalid &e) {
     return fail(PSEG_E_INVALID, "%s", e.msg.c_str());
   }
@@ -1304,6 +1308,7 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   cp->pilot = plen;
   cp->record = record;
   cp->lds = lds;
+  cp->threads = threads;
   cp->dev = (DevCorr *)base;
   *out = cp;
   return PSEG_OK;
@@ -1343,7 +1348,7 @@ pseg_status pseg_corr_apply(const pseg_corr_plan *c, const void *src, int32_t ro
     return fail(PSEG_E_INVALID, "dst must be %d samples wide, got %d", c->out, dst_cols);
   if (rows == 0) return PSEG_OK;
   if (!src || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL panel or row_len");
-  hipError_t e = launch_correlate(c->dtype, c->dev, c->lds, src, rows, src_cols, row_len, dst,
+  hipError_t e = launch_correlate(c->dtype, c->threads, c->dev, c->lds, src, rows, src_cols, row_len, dst,
                                   (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "correlation launch");
   return PSEG_OK;

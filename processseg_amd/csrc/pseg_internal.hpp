@@ -52,11 +52,19 @@ struct DevFft {
   const void *split;  // T2 (cos, sin)(-2 pi k / n), k = 0 .. m
 };
 
-// Two consecutive Stockham stages whose butterfly holds at most 9 points
-// run as one pass through LDS (host and device share the rule: the host
-// sizes the first pass's padded layout and checks the passes' registers
-// from it).
-__host__ __device__ constexpr bool fusable(int r1, int r2) { return r1 * r2 <= 9; }
+// Two consecutive Stockham stages whose butterfly holds at most `fuse`
+// points run as one pass through LDS (host and device share the rule: the
+// host sizes the first pass's padded layout and checks the passes' registers
+// from it).  Both kernels fuse up to 16 points.
+constexpr int kFusePipeline = 16;
+constexpr int kFuseCorr = 16;
+__host__ __device__ constexpr bool fusable(int r1, int r2, int fuse) { return r1 * r2 <= fuse; }
+// Whether stage radix r2 can follow r1: a plan's radices come as 4s, at most
+// one 2, 3s, then 5s (the device instantiates only such pairs).
+__host__ __device__ constexpr int radix_rank(int r) { return r == 4 ? 0 : r == 2 ? 1 : r == 3 ? 2 : 3; }
+__host__ __device__ constexpr bool may_follow(int r1, int r2) {
+  return radix_rank(r1) < radix_rank(r2) || (r1 == r2 && r1 != 2);
+}
 
 // A pass of butterflies of r points runs in place with each thread holding
 // the inputs of at most this many butterflies (32 complex values).
@@ -171,7 +179,8 @@ hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t c
                                const int32_t *row_len, int32_t stat, double target,
                                double percentile, double *row_sq, uint32_t *row_max,
                                uint32_t *hist, double *factor, hipStream_t stream);
-hipError_t launch_correlate(int dtype, const DevCorr *dev_plan, size_t lds_bytes,
+// The correlation kernel with `threads` (512 or 1024) threads per record.
+hipError_t launch_correlate(int dtype, int threads, const DevCorr *dev_plan, size_t lds_bytes,
                             const void *src, int32_t rows, int32_t src_cols,
                             const int32_t *row_len, void *dst, hipStream_t stream);
 // Dynamic LDS one row of the pipeline kernel may hold (the workgroup's

@@ -225,10 +225,10 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
   const T *s = src + row * (int64_t)src_cols;
   for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
   __syncthreads();
-  complex_forward<T, NTC>(f, A);
+  complex_forward<T, NTC, kFuseCorr>(f, A);
   combine<T, NTC>(f, A, A, CplxMul<T>{ctab<T>(c.mul)});
   __syncthreads();
-  complex_forward<T, NTC>(f, A);
+  complex_forward<T, NTC, kFuseCorr>(f, A);
   const T *out = (const T *)A;
   // lagSpans (vibroseis/correlate.nim:265-274)
   const int wrapped = min(max(1 - c.pilot - c.shift, 0), c.out);
@@ -269,15 +269,15 @@ hipError_t panel_typed(const T *x, int32_t rows, int32_t cols, const int32_t *ro
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, int NTC>
 hipError_t corr_typed(const DevCorr *dplan, size_t lds, const T *src, int32_t rows,
                       int32_t src_cols, const int32_t *row_len, T *dst, hipStream_t stream) {
   // once per process (a thread-safe static), like the pipeline kernels
   static const hipError_t attr = hipFuncSetAttribute(
-      (const void *)corr_kernel<T, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (const void *)corr_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
       (int)kMaxRowLds);
   if (attr != hipSuccess) return attr;
-  hipLaunchKernelGGL((corr_kernel<T, 1024>), dim3(rows), dim3(1024), lds, stream, dplan, src,
+  hipLaunchKernelGGL((corr_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dplan, src,
                      src_cols, row_len, dst);
   return hipGetLastError();
 }
@@ -315,15 +315,24 @@ hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t c
                              row_sq, row_max, hist, factor, stream);
 }
 
-hipError_t launch_correlate(int dtype, const DevCorr *dev_plan, size_t lds_bytes,
+hipError_t launch_correlate(int dtype, int threads, const DevCorr *dev_plan, size_t lds_bytes,
                             const void *src, int32_t rows, int32_t src_cols,
                             const int32_t *row_len, void *dst, hipStream_t stream) {
   if (rows <= 0) return hipSuccess;
+  const float *sf = (const float *)src;
+  const double *sd = (const double *)src;
+  if (threads == 512) {
+    if (dtype == 0)
+      return corr_typed<float, 512>(dev_plan, lds_bytes, sf, rows, src_cols, row_len,
+                                    (float *)dst, stream);
+    return corr_typed<double, 512>(dev_plan, lds_bytes, sd, rows, src_cols, row_len,
+                                   (double *)dst, stream);
+  }
   if (dtype == 0)
-    return corr_typed<float>(dev_plan, lds_bytes, (const float *)src, rows, src_cols, row_len,
-                             (float *)dst, stream);
-  return corr_typed<double>(dev_plan, lds_bytes, (const double *)src, rows, src_cols, row_len,
-                            (double *)dst, stream);
+    return corr_typed<float, 1024>(dev_plan, lds_bytes, sf, rows, src_cols, row_len,
+                                   (float *)dst, stream);
+  return corr_typed<double, 1024>(dev_plan, lds_bytes, sd, rows, src_cols, row_len,
+                                  (double *)dst, stream);
 }
 
 }  // namespace pseg

@@ -82,13 +82,14 @@ def test_corr_compile_validation():
         P.initCorrelationPlan(np.float32, zero, 2.0, 5000)
 
 
-def test_corr_float64_full_size_is_refused():
-    """16200 complex float64 points do not fit one workgroup's LDS: the plan
-    says so at compile (PSEG_E_UNSUPPORTED) instead of failing at apply."""
+def test_corr_float64_too_long_is_refused():
+    """A 20001-sample float64 record against a 12 s sweep needs an FFT whose
+    float64 buffer does not fit one workgroup's LDS: the plan says so at
+    compile (PSEG_E_UNSUPPORTED) instead of failing at apply."""
     import pytest
     pilot = P.generatePilot(P.linearSweep(6, 80, 12000, 2, P.initSweepTaper(300, 300)))
     with pytest.raises(P.UnsupportedOp, match="LDS"):
-        P.initCorrelationPlan(np.float64, pilot, 2.0, 10001)
+        P.initCorrelationPlan(np.float64, pilot, 2.0, 20001)
 
 
 def test_pipeline_lds_limit_is_unsupported():

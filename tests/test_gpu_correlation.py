@@ -27,7 +27,8 @@ def _meta(rows):
     nums = np.full(rows, REC, np.int64)
     nums[13] = 7000            # a short record: zeros past its live end
     nums[17] = 0
-    return P.ChunkMeta(rows, 2.0, numSamples=nums, status=status, failed=[19, 1100])
+    failed = [r for r in (19, 1100) if r < rows]
+    return P.ChunkMeta(rows, 2.0, numSamples=nums, status=status, failed=failed)
 
 
 def test_full_size_against_oracle(gpu, oracle, records):
@@ -68,3 +69,27 @@ def test_batch_independence(gpu, records):
     torch.cuda.synchronize()
     assert np.array_equal(dst[37].cpu().numpy(), one[0].cpu().numpy())
 
+
+
+def test_full_size_float64_against_oracle(gpu, oracle, records):
+    """The same records in float64 (nfft 16200 in one workgroup's LDS):
+    within float64 rounding of the oracle."""
+    import torch
+    pilot = W.sweep_pilot()
+    plan = P.initCorrelationPlan(np.float64, pilot, 2.0, REC)
+    assert (plan.nfft, plan.outSamples) == (16200, 4001)
+    rows = 256
+    x = records[:rows].astype(np.float64)
+    meta = _meta(rows)
+    dst = torch.full((rows, plan.outSamples), 7.0, dtype=torch.float64, device="cuda")
+    plan.correlateInto(torch.from_numpy(x).cuda(), dst, meta)
+    torch.cuda.synchronize()
+    got = dst.cpu().numpy()
+    rl = meta.row_len(REC, for_correlation=True)
+    want = oracle.correlate(x, rl, pilot.samples, 2.0, REC)
+    e = max_abs(got, want) / np.max(np.abs(want))
+    r = rel_l2(got, want)
+    print("256x10001 float64 correlation: max abs / peak", e, "rel L2", r)
+    assert r <= 1e-12 and e <= 1e-12
+    for row in (7, 19):
+        assert np.all(got[row] == 0)
