@@ -805,37 +805,62 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   // last sample has remainder (c1 - 1) % w = (r0 + len - 1) % w
   const int len = c1 - c0;
   double run = (c1 < n && (r0 + len) % w != 0) ? csuf.v : 0.0;
-  r = len > 0 ? (r0 + len - 1) % w : 0;
+  // A run of c <= w samples holds at most one block start, at k = ks, and
+  // its prefix sums are 0 (a block end) only at k + 1 = zk; longer runs
+  // (windows under c samples) track e % w sample by sample.
+  const int ks = r0 == 0 ? 0 : w - r0;
+  const int zk = ks == 0 ? w : ks;
+  if (c <= w) {
 #pragma unroll
-  for (int k = PER - 1; k >= 0; k--) {
-    const int e = c0 + k;
-    if (e < c1) {
-      run += tv[k];
-      s[e] = (T)run;
-      if (r == 0) run = 0;
-      r = r == 0 ? w - 1 : r - 1;
+    for (int k = PER - 1; k >= 0; k--) {
+      if (k < len) {
+        run += tv[k];
+        s[c0 + k] = (T)run;
+        if (k == ks) run = 0;
+      }
+    }
+  } else {
+    r = len > 0 ? (r0 + len - 1) % w : 0;
+#pragma unroll
+    for (int k = PER - 1; k >= 0; k--) {
+      if (k < len) {
+        run += tv[k];
+        s[c0 + k] = (T)run;
+        if (r == 0) run = 0;
+        r = r == 0 ? w - 1 : r - 1;
+      }
     }
   }
   __syncthreads();
   // prefix sums P[j], j = e + 1 (0 at a block end), used as they are made:
   // the interior mean of sample j - h - 1 replaces S[j - w], which only this
-  // thread reads; the head's and P[n] go to hp.
+  // thread reads; the head's go to hp, and P[n] (the last run's last) to
+  // hp[first].
   run = cpre.v;
   r = r0;
+  T plast = (T)0;
+  const int jh = c0 - h;  // j - h - 1 at k = 0
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    const int e = c0 + k;
-    if (e < c1) {
-      if (r == 0) run = 0;
-      run += tv[k];
-      r = r + 1 == w ? 0 : r + 1;
-      const T pj = r == 0 ? (T)0 : (T)run;
-      const int j = e + 1;
-      if (j >= w) s[j - w] = (T)(((double)s[j - w] + (double)pj) * unitW);
-      if (j - h - 1 >= 0 && j - h - 1 < headCut) hp[j - h - 1] = pj;
-      if (j == n) hp[first] = pj;
+    if (k < len) {
+      T pj;
+      if (c <= w) {
+        if (k == ks) run = 0;
+        run += tv[k];
+        pj = k + 1 == zk ? (T)0 : (T)run;
+      } else {
+        if (r == 0) run = 0;
+        run += tv[k];
+        r = r + 1 == w ? 0 : r + 1;
+        pj = r == 0 ? (T)0 : (T)run;
+      }
+      const int q = c0 + k + 1 - w;  // j - w
+      if (q >= 0) s[q] = (T)(((double)s[q] + (double)pj) * unitW);
+      if ((unsigned)(jh + k) < (unsigned)headCut) hp[jh + k] = pj;
+      plast = pj;
     }
   }
+  if (len > 0 && c1 == n) hp[first] = plast;
   __syncthreads();
   // The gain step in T on means scaled by the trace level.
   const T kk = (T)(gtarget / rms);
