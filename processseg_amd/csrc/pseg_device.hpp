@@ -60,6 +60,15 @@ __device__ __forceinline__ int opaque_v(int i) {
   return i;
 }
 
+// A uniform value behind an empty asm barrier (kept in an SGPR): what the
+// steps compute from the row length (trend denominators, run lengths) is
+// computed again in each step instead of hoisted out of the loop over the
+// steps, where it would stay live across every step and spill.
+__device__ __forceinline__ int opaque_s(int v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
 // Plan parameters are the same on every lane; reading them through
 // readfirstlane puts them (and everything computed from them) in SGPRs
 // instead of a VGPR per lane.

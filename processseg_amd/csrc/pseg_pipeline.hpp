@@ -35,7 +35,8 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   const T *s = (const T *)J.src + r * J.src_pitch;
   T *d = (T *)J.dst + r * J.dst_pitch;
   const int num = J.num[r];
-  const int n = num < 0 ? -1 : min(max(num - J.first, 0), cols);
+  const int n_row = num < 0 ? -1 : min(max(num - J.first, 0), cols);
+  const int n = n_row;
   T v[PER];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
@@ -75,6 +76,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
     }
     for (int i = J.step_first; i < J.step_last; i++) {
       const DevStep &st = plan->steps[i];
+      const int n = opaque_s(uni(n_row));
       switch (uni(st.kind)) {
         case kDemean: {
           const int tid_ = opaque_tid();
