@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Vector-ALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64
+# instruction = 614.4 G wave-instructions/s (78.6 TFLOP/s of non-packed fp32 FMA).
+VALU_PEAK_GWINST = 256 * 4 * 2.4 / 4
 ROWS, COLS, DT = 2000, 3001, 2.0
 
 
@@ -142,7 +145,7 @@ def correlation_leg(torch, dev, steps, warmup, pairs, with_cpu):
            "roofline": {"bound": "hbm", "achieved": round(algo / (avg_ms / 1000) / 1e9, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(algo / (avg_ms / 1000) / 1e9 / HBM_PEAK_GBS, 4),
-                        "kernel": "pseg::corr_kernel<float, 1024>",
+                        "kernel": f"pseg::corr_kernel<float, {plan.threadsPerRecord}>",
                         "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": algo}}
     if with_cpu:
         import oracle as O
@@ -378,10 +381,24 @@ def main():
     algo_bytes = samples * 8  # 4 read + 4 written per sample
     achieved = algo_bytes / (avg_ms / 1000) / 1e9
     traffic = None
+    valu = None
     tpath = os.path.join(ROOT, "profiles", "traffic_bandpass_agc_2000x3001_f32.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("bytes_per_launch")
+        if prof.get("valu_insts_per_wave") and prof.get("waves_per_launch"):
+            # The kernel's second bound: vector-ALU issue.  A wave64 VALU
+            # instruction occupies a 16-lane SIMD for 4 cycles; MI355X has
+            # 256 CUs x 4 SIMDs at 2.4 GHz.  Instructions per launch come
+            # from the committed PMC pass (they depend on the plan, not the
+            # clock); the time is this run's.
+            winst = prof["valu_insts_per_wave"] * prof["waves_per_launch"]
+            rate = winst / (avg_ms / 1000) / 1e9
+            peak = VALU_PEAK_GWINST
+            valu = {"achieved": round(rate, 1), "peak": peak, "unit": "G wave-instructions/s",
+                    "frac": round(rate / peak, 4),
+                    "insts_per_launch": round(winst)}
 
     out = dsts[(args.steps - 1) % args.pairs].cpu().numpy()
     line = {
@@ -407,7 +424,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": kernel_name(COLS, plan.threadsPerRow),
                      "avg_launch_ms": round(avg_ms, 5),
-                     "algorithmic_bytes_per_launch": algo_bytes},
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "valu_issue": valu},
     }
     if not args.no_cpu:
         log("CPU baseline leg")

@@ -304,6 +304,8 @@ typedef struct pseg_corr_info {
   double dt_ms;
   double scale;            /* normalization factor folded into the multiplier */
   size_t lds_bytes;
+  int32_t threads_per_record;  /* workgroup size of the correlation kernel */
+  int32_t reserved;
 } pseg_corr_info;
 
 /* initCorrelationPlan (vibroseis/correlate.nim:82-160): pilot samples in

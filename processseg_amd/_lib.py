@@ -50,7 +50,8 @@ class CorrInfoC(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("nfft", C.c_int32), ("shift", C.c_int32),
                 ("out_samples", C.c_int32), ("pilot_samples", C.c_int32),
                 ("record_samples", C.c_int32), ("dt_ms", C.c_double), ("scale", C.c_double),
-                ("lds_bytes", C.c_size_t)]
+                ("lds_bytes", C.c_size_t), ("threads_per_record", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class BatchItemC(C.Structure):

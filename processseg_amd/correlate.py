@@ -117,6 +117,7 @@ class CorrelationPlan:
         self.recordSamples = int(info.record_samples)
         self.dtMs = float(info.dt_ms)
         self.scale = float(info.scale)
+        self.threadsPerRecord = int(info.threads_per_record)
         self.mode = mode
 
     def correlateInto(self, src, dst, meta: ChunkMeta = None, *, row_len=None, stream=None):

@@ -1333,6 +1333,7 @@ pseg_status pseg_corr_info_get(const pseg_corr_plan *c, pseg_corr_info *o) {
   o->dt_ms = c->dt_ms;
   o->scale = c->scale;
   o->lds_bytes = c->lds;
+  o->threads_per_record = c->threads;
   return PSEG_OK;
 }
 

@@ -37,6 +37,7 @@ def test_full_size_against_oracle(gpu, oracle, records):
     plan = P.initCorrelationPlan(np.float32, pilot, 2.0, REC)
     info = oracle.corr_info(pilot.samples, 2.0, REC)
     assert (plan.nfft, plan.outSamples) == (info["nfft"], info["out_samples"]) == (16200, 4001)
+    assert plan.threadsPerRecord == 512
     meta = _meta(ROWS)
     src = torch.from_numpy(records).cuda()
     dst = torch.full((ROWS, plan.outSamples), 7.0, dtype=torch.float32, device="cuda")
