@@ -916,6 +916,25 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   __syncthreads();
 }
 
+// Exclusive sum of x over the block's threads in thread order (wave
+// shuffles, then the waves' totals through LDS).  wsum: NTH / 64 values.
+template <int NTH>
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t x, uint32_t *wsum) {
+  const int tid_ = opaque_tid();
+  const int lane = tid_ & 63, wave = tid_ >> 6;
+  uint32_t incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wave; w++) before += wsum[w];
+  return before + incl - x;
+}
+
 // The k-th smallest of the row's |x| bit patterns (exact), by radix select
 // over 8-bit digits with an LDS histogram.
 template <typename T, int NTH, int PER>
@@ -924,6 +943,7 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
   const int tid_ = opaque_tid();
   using S = typename Bits<T>::S;
   using U = typename Bits<T>::U;
+  __shared__ uint32_t wsum[NTH / 64];
   U prefix = 0, mask = 0;
   int64_t r = rank;
   for (int shift = Bits<T>::width - 8; shift >= 0; shift -= 8) {
@@ -938,16 +958,29 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
       }
     }
     __syncthreads();
-    if (tid_ == 0) {
-      int64_t below = 0;
-      int d = 0;
-      for (; d < 256; d++) {
-        int64_t c = hist[d];
-        if (below + c > r) break;
-        below += c;
+    // The digit holding rank r: an exclusive scan of the 256 counts over the
+    // block (thread t holds bins t BPT .. t BPT + BPT - 1), and the one bin
+    // with below <= r < below + count reports it.
+    {
+      constexpr int BPT = NTH >= 256 ? 1 : 256 / NTH;
+      uint32_t c[BPT];
+      uint32_t mine = 0;
+#pragma unroll
+      for (int i = 0; i < BPT; i++) {
+        const int bin = tid_ * BPT + i;
+        c[i] = bin < 256 ? hist[bin] : 0u;
+        mine += c[i];
       }
-      shared_i[0] = d;
-      shared_i[1] = r - below;
+      uint32_t below = block_excl_scan_u32<NTH>(mine, wsum);
+      const uint32_t rr = (uint32_t)r;
+#pragma unroll
+      for (int i = 0; i < BPT; i++) {
+        if (below <= rr && rr < below + c[i]) {
+          shared_i[0] = tid_ * BPT + i;
+          shared_i[1] = (int64_t)(rr - below);
+        }
+        below += c[i];
+      }
     }
     __syncthreads();
     prefix |= (U)shared_i[0] << shift;
