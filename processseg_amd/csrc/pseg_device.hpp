@@ -936,7 +936,7 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t x, uint32_t *ws
 }
 
 // The k-th smallest of the row's |x| bit patterns (exact), by radix select
-// over 8-bit digits with an LDS histogram.
+// over 8-bit digits (the last one 7 bits) with an LDS histogram.
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], int n, int64_t rank,
                                            uint32_t *hist, int64_t *shared_i) {
@@ -946,7 +946,12 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
   __shared__ uint32_t wsum[NTH / 64];
   U prefix = 0, mask = 0;
   int64_t r = rank;
-  for (int shift = Bits<T>::width - 8; shift >= 0; shift -= 8) {
+  // digits from just below the cleared sign bit: the first is the exponent,
+  // so each bin holds one binade (fewer colliding atomics than two)
+  for (int top = Bits<T>::width - 1; top > 0;) {
+    const int shift = max(top - 8, 0);
+    const U dmask = ((U)1 << (top - shift)) - 1;
+    top = shift;
     for (int i = tid_; i < 256; i += NTH) hist[i] = 0;
     __syncthreads();
 #pragma unroll
@@ -954,7 +959,7 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
       int j = tid_ + NTH * k;
       if (j < n) {
         U key = (U)Bits<T>::abs_bits(v[k]);
-        if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        if ((key & mask) == prefix) atomicAdd(&hist[(uint32_t)((key >> shift) & dmask)], 1u);
       }
     }
     __syncthreads();
@@ -984,7 +989,7 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
     }
     __syncthreads();
     prefix |= (U)shared_i[0] << shift;
-    mask |= (U)255 << shift;
+    mask |= dmask << shift;
     r = shared_i[1];
     __syncthreads();
   }
