@@ -188,6 +188,30 @@ def panels_batch_leg(torch, P, plan, srcs, dsts, row_len, steps, warmup):
             "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
 
 
+def f64_panel_leg(torch, P, flow, dev, steps, warmup):
+    """The headline flow and panel in float64 (the big-panel configuration
+    is float32 and float64): one Plan.apply per step, 4 buffer pairs."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel(dtype=np.float64)
+    plan = flow.compile(np.float64, DT, COLS)
+    srcs = [torch.from_numpy(x).to(dev) for _ in range(4)]
+    dsts = [torch.empty_like(srcs[0]) for _ in range(4)]
+    rl = torch.from_numpy(live.astype(np.int32)).to(dev)
+    for i in range(warmup):
+        plan.apply(srcs[i % 4], dsts[i % 4], row_len=rl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        plan.apply(srcs[i % 4], dsts[i % 4], row_len=rl)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    samples = ROWS * COLS
+    return {"workload": "the headline flow on the 2000x3001 panel in float64",
+            "value": round(samples / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5), "threads_per_row": plan.threadsPerRow,
+            "achieved_GBps": round(samples * 16 / t / 1e9, 1)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -437,6 +461,10 @@ def main():
         line["secondary"] = {"panels_as_batch": panels_batch_leg(
             torch, P, plan, srcs, dsts, row_len, max(10, args.steps // 10), 3)}
         del srcs, dsts
+        torch.cuda.empty_cache()
+        log("float64 panel leg")
+        line["secondary"]["f64_panel"] = f64_panel_leg(torch, P, flow, dev,
+                                                       max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
         log("correlation leg")
         line["secondary"]["correlation"] = correlation_leg(

@@ -616,14 +616,23 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   // would keep them live across both transforms.
   const int tid2 = opaque_tid();
   const int last_slot = 2 * uni(f.stride) - 2;  // an even index inside the buffer
+  // (slots past the live end read any sample of the transform buffer)
+  if (plain) {  // no trend to add back (a 0 Hz gain of 0, or no trend)
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid2 + NTH * k;
-    // slots past the live end read any sample of the two transform buffers
-    T y = out[min(p + j, last_slot) ^ 1];
-    y = plain ? y : (T)((double)y + (ga + gb * (double)j));
-    v[k] = j < n ? y : (T)0;
-    PSEG_ONE_AT_A_TIME();
+    for (int k = 0; k < PER; k++) {
+      int j = tid2 + NTH * k;
+      T y = out[min(p + j, last_slot) ^ 1];
+      v[k] = j < n ? y : (T)0;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      int j = tid2 + NTH * k;
+      T y = out[min(p + j, last_slot) ^ 1];
+      y = (T)((double)y + (ga + gb * (double)j));
+      v[k] = j < n ? y : (T)0;
+      PSEG_ONE_AT_A_TIME();
+    }
   }
   __syncthreads();
 }
