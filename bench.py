@@ -357,8 +357,8 @@ def main():
                          "kernel for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the panel path on a single MI355X "
-              f"(rows shard across GPUs with no exchange; see DESIGN.md); --gpus {args.gpus} "
+        print(f"bench.py measures one GPU: the panel path runs on a single MI355X "
+              f"(rows shard across GPUs with no exchange); --gpus {args.gpus} "
               "is not run", file=sys.stderr)
         sys.exit(2)
 
