@@ -268,3 +268,40 @@ def test_config2_full_panel(gpu, oracle, dtype):
         e32 = rel_l2(got, mirror)
         print(f"full panel: rel L2 {e32:.3e} vs the FP32 mirror on every row")
         assert e32 <= F32_TOL
+
+
+@pytest.mark.parametrize("panel_stat", [None, P.normPercentile])
+def test_apply_graph_replay(gpu, panel_stat):
+    """Plan.apply captured in a HIP graph (one kernel; with per-panel
+    normalizations, the statistics kernels between the parts, whose
+    histogram is cleared by a kernel): every replay, with dst cleared, eager
+    work and a synchronization between replays, gives the eager bits."""
+    import torch
+    x, live = config2_panel(rows=200, cols=1500)
+    ops = [P.demeanOp(), P.bandPassOp(P.Corners(5, 10, 70, 90)), P.agcOp(250)]
+    if panel_stat is not None:
+        ops += [P.normalizeOp(panel_stat, scope=P.perPanel)]
+    flow = P.initPipeline(ops)
+    plan = flow.compile(np.float32, 2.0, 1500)
+    src = torch.from_numpy(x).cuda()
+    dst = torch.empty_like(src)
+    rl = torch.from_numpy(live.astype(np.int32)).cuda()
+    # the first apply (outside the capture) allocates the plan's workspace
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        plan.apply(src, dst, row_len=rl)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    eager = dst.clone()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan.apply(src, dst, row_len=rl)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        dst.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(dst, eager), f"replay {rep}"
+        _ = float(src.sum())  # eager work between replays
