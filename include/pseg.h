@@ -54,6 +54,7 @@ typedef enum pseg_op_kind {
 
 #define PSEG_MAX_NOTCH 8
 #define PSEG_MAX_OPS 16
+#define PSEG_MAX_KNOTS 32
 
 /* FilterSpec (filter/design.nim:78-90): kind 0 = cos² trapezoid (corners
  * f1 < f2 <= f3 < f4; f2 = 0: no low cut, f3 = f4 = Inf: no high cut),
@@ -86,6 +87,18 @@ typedef struct pseg_op {
   double epsilon;      /* agc */
   double percentile;   /* normalize */
   pseg_filter_spec spec; /* filter */
+  /* muteTop / muteBottom (core/mute.nim:31-98, pipeline.nim:283-289): curve
+   * mute_kind 0 timeMute (mute_t_ms), 1 tableMute (mute_knots knots of
+   * |offset| -> time, offsets strictly ascending), 2 lmoMute (mute_t_ms =
+   * t0Ms, mute_ms_per_dist = 1000 / velocity, mute_pad_ms); the taper in ms. */
+  int32_t mute_kind;
+  int32_t mute_knots;
+  double mute_t_ms;
+  double mute_ms_per_dist;
+  double mute_pad_ms;
+  double taper_ms;
+  double mute_offsets[PSEG_MAX_KNOTS];
+  double mute_times[PSEG_MAX_KNOTS];
 } pseg_op;
 
 typedef struct pseg_plan pseg_plan;
@@ -183,6 +196,17 @@ pseg_status pseg_plan_workspace_size(const pseg_plan *plan, int32_t rows, int32_
 pseg_status pseg_plan_apply(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
                             int32_t cols, const int32_t *row_len, void *ws, size_t ws_bytes,
                             void *stream);
+
+/* The same with the rows' trace keys, which mutes read (muteCol,
+ * core/mute.nim:189-194: the mute time's column (t - t0Ms) / dtMs -
+ * firstSample): row_keys holds (t0Ms, offset) per row as float64 in device
+ * memory (ChunkMeta.keys), or is NULL (t0Ms 0, offset NaN: a timeMute then
+ * applies, an offset-dependent mute leaves the rows alone); first_sample is
+ * ChunkMeta.firstSample.  pseg_plan_apply is this with NULL and 0. */
+pseg_status pseg_plan_apply_keys(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *row_keys,
+                                 int64_t first_sample, void *ws, size_t ws_bytes,
+                                 void *stream);
 
 /* ---------------------------------------------------------------- batch -- */
 

@@ -35,7 +35,11 @@ class OrOp(C.Structure):
                 ("stat", C.c_int32), ("scope", C.c_int32), ("reserved", C.c_int32),
                 ("value", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
                 ("window_ms", C.c_double), ("target", C.c_double), ("epsilon", C.c_double),
-                ("percentile", C.c_double), ("spec", OrSpec)]
+                ("percentile", C.c_double), ("spec", OrSpec),
+                ("mute_kind", C.c_int32), ("mute_knots", C.c_int32),
+                ("mute_t_ms", C.c_double), ("mute_ms_per_dist", C.c_double),
+                ("mute_pad_ms", C.c_double), ("taper_ms", C.c_double),
+                ("mute_offsets", C.c_double * 32), ("mute_times", C.c_double * 32)]
 
 
 class OrFilterInfo(C.Structure):
@@ -82,6 +86,8 @@ def lib():
         u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
         E = [C.c_char_p, C.c_int]
         L.or_is_fast_len.argtypes = [C.c_int64]
+        L.or_mute_time_ms.argtypes = [C.POINTER(OrOp), C.c_double]
+        L.or_mute_time_ms.restype = C.c_double
         L.or_next_fast_len.argtypes = [C.c_int64, C.c_int]
         L.or_next_fast_len.restype = C.c_int64
         L.or_factorize.argtypes = [C.c_int64, i32p]
@@ -100,6 +106,11 @@ def lib():
             getattr(L, f"or_pipeline_apply_{t}").argtypes = [
                 C.POINTER(OrOp), C.c_int32, C.c_double, C.c_int64, C.c_int64, p, p,
                 C.c_int32, C.c_int32, i32p] + E
+            getattr(L, f"or_pipeline_apply_keys_{t}").argtypes = [
+                C.POINTER(OrOp), C.c_int32, C.c_double, C.c_int64, C.c_int64, p, p,
+                C.c_int32, C.c_int32, i32p, C.c_void_p, C.c_int64] + E
+            getattr(L, f"or_mute_top_{t}").argtypes = [p, C.c_int64, C.c_double, C.c_double]
+            getattr(L, f"or_mute_bottom_{t}").argtypes = [p, C.c_int64, C.c_double, C.c_double]
             getattr(L, f"or_correlate_{t}").argtypes = [
                 f64p, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64,
                 C.c_int32, C.c_double, C.c_int32, p, C.c_int32, C.c_int32, i32p, p,
@@ -187,6 +198,17 @@ def op_struct(op) -> OrOp:
     spec = getattr(op, "spec", None)
     if spec is not None:
         o.spec = spec_struct(spec)
+    mute = getattr(op, "mute", None)
+    if mute is not None:
+        o.mute_kind = int(mute.kind)
+        o.mute_knots = len(mute.offsets)
+        o.mute_t_ms = float(mute.t0Ms if mute.kind == 2 else mute.tMs)
+        o.mute_ms_per_dist = float(mute.msPerDist)
+        o.mute_pad_ms = float(mute.padMs)
+        o.taper_ms = float(getattr(op, "taper_ms", 0.0))
+        for i, (a, b) in enumerate(zip(mute.offsets, mute.timesMs)):
+            o.mute_offsets[i] = a
+            o.mute_times[i] = b
     return o
 
 
@@ -336,18 +358,34 @@ def plan_info(ops, dtype, dt_ms: float, samples: int, record_samples: int = 0) -
 
 
 def pipeline_apply(ops, src: np.ndarray, row_len, dt_ms: float, samples: int | None = None,
-                   record_samples: int = 0, dst: np.ndarray | None = None) -> np.ndarray:
-    """compile + apply: src (rows x cols, float32 or float64) -> a new array."""
+                   record_samples: int = 0, dst: np.ndarray | None = None, keys=None,
+                   first_sample: int = 0) -> np.ndarray:
+    """compile + apply: src (rows x cols, float32 or float64) -> a new array.
+    keys: (t0Ms, offset) per row for mutes (None: t0Ms 0, offset NaN);
+    first_sample: ChunkMeta.firstSample."""
     src = np.ascontiguousarray(src)
     rows, cols = src.shape
     out = np.empty_like(src) if dst is None else dst
     arr = _ops_array(ops)
     e = _err()
     rl = np.ascontiguousarray(row_len, np.int32)
-    _check(getattr(lib(), f"or_pipeline_apply_{_sfx(src.dtype)}")(
+    k = None if keys is None else np.ascontiguousarray(keys, np.float64).reshape(rows, 2)
+    _check(getattr(lib(), f"or_pipeline_apply_keys_{_sfx(src.dtype)}")(
         arr, len(ops), dt_ms, samples if samples is not None else cols, record_samples,
-        src, out, rows, cols, rl, e, 512), e)
+        src, out, rows, cols, rl, None if k is None else k.ctypes.data, first_sample, e, 512), e)
     return out
+
+
+def mute_time_ms(op, offset: float) -> float:
+    """muteTimeMs of a mute op (core/mute.nim:100-118)."""
+    return float(lib().or_mute_time_ms(C.byref(op_struct(op)), offset))
+
+
+def mute_row(kind: str, x: np.ndarray, col: float, taper_cols: float) -> np.ndarray:
+    """muteTop / muteBottom of one row in column units, on a copy."""
+    y = np.ascontiguousarray(x).copy()
+    getattr(lib(), f"or_mute_{kind}_{_sfx(y.dtype)}")(y, y.shape[0], col, taper_cols)
+    return y
 
 
 class Pool:

@@ -711,11 +711,48 @@ int SFX(or_panel_stats)(const T *x, int32_t rows, int32_t cols, const int32_t *r
   return 0;
 }
 
+/* ---------------------------------------------------------------- mutes -- */
+
+/* muteTop (core/mute.nim:136-156): x[j] for j >= start_col untouched,
+ * start_col - taper <= j < start_col scaled by sin²(π/2 (j - start_col +
+ * taper) / taper) in float64, earlier columns zeroed; a NaN start leaves
+ * the row alone, an infinite taper is none. */
+void SFX(or_mute_top)(T *x, int64_t n, double start_col, double taper_cols) {
+  if (isnan(start_col)) return;
+  const double taper = taper_cols < INFINITY ? taper_cols : 0.0;
+  const double ramp_start = start_col - taper;
+  const int64_t zero_end = or_first_at_or_after(ramp_start, n);
+  for (int64_t j = 0; j < zero_end; j++) x[j] = (T)0;
+  if (taper > 0) {
+    const int64_t e = or_first_at_or_after(start_col, n);
+    for (int64_t j = zero_end; j < e; j++) {
+      const double s = sin(M_PI / 2 * ((double)j - ramp_start) / taper);
+      x[j] = (T)((double)x[j] * (s * s));
+    }
+  }
+}
+
+/* muteBottom (core/mute.nim:158-180), the mirror image. */
+void SFX(or_mute_bottom)(T *x, int64_t n, double end_col, double taper_cols) {
+  if (isnan(end_col)) return;
+  const double taper = taper_cols < INFINITY ? taper_cols : 0.0;
+  const double ramp_end = end_col + taper;
+  const int64_t zero_start = or_first_after(ramp_end, n);
+  if (taper > 0) {
+    for (int64_t j = or_first_after(end_col, n); j < zero_start; j++) {
+      const double s = sin(M_PI / 2 * (ramp_end - (double)j) / taper);
+      x[j] = (T)((double)x[j] * (s * s));
+    }
+  }
+  for (int64_t j = zero_start; j < n; j++) x[j] = (T)0;
+}
+
 /* ------------------------------------------------------------- pipeline -- */
 
 typedef struct SFX(step) {
   int kind;
   double factor, lo, hi;
+  or_op mute;          /* a mute step: its op (curve and taper) */
   SFX(filterplan) filter;
   SFX(agcplan) agc;
   int stat;
@@ -775,6 +812,12 @@ static int SFX(plan_compile)(SFX(plan) *P, const or_op *ops, int32_t n_ops, doub
       case 4:
         if (isnan(op->lo) || isnan(op->hi) || op->lo > op->hi) { bad = 1; snprintf(msg, sizeof msg, "clip limits must satisfy lo <= hi"); break; }
         s->kind = ST_CLIP; s->lo = op->lo; s->hi = op->hi; break;
+      case 6:
+      case 7:
+        if (or_mute_check(op, msg, (int)sizeof msg)) { bad = 1; break; }
+        s->kind = op->kind == 6 ? ST_MUTE_TOP : ST_MUTE_BOTTOM;
+        s->mute = *op;
+        break;
       case 8:
         s->kind = ST_FILTER;
         if (SFX(filter_init)(&s->filter, &op->spec, dt, samples, op->trend, msg, (int)sizeof msg)) bad = 1;
@@ -826,9 +869,21 @@ static int SFX(plan_compile)(SFX(plan) *P, const or_op *ops, int32_t n_ops, doub
   return 0;
 }
 
-/* runStep (pipeline.nim:671-688) on one live row of n samples. */
-static void SFX(run_step)(const SFX(step) *s, T *x, int64_t n, void *scratch) {
+/* runStep (pipeline.nim:671-688) on one live row of n samples; key: the
+ * row's (t0Ms, offset), first: ChunkMeta.firstSample, dt: the plan's. */
+static void SFX(run_step)(const SFX(step) *s, T *x, int64_t n, void *scratch, const double *key,
+                          int64_t first, double dt) {
   switch (s->kind) {
+    case ST_MUTE_TOP:
+    case ST_MUTE_BOTTOM: {
+      /* muteCol (core/mute.nim:189-194): the row's column of the mute time */
+      const double t0 = key ? key[0] : 0.0, off = key ? key[1] : NAN;
+      const double col = (or_mute_time_ms(&s->mute, off) - t0) / dt - (double)first;
+      const double taper_cols = s->mute.taper_ms / dt;
+      if (s->kind == ST_MUTE_TOP) SFX(or_mute_top)(x, n, col, taper_cols);
+      else SFX(or_mute_bottom)(x, n, col, taper_cols);
+      break;
+    }
     case ST_DEMEAN: SFX(demean)(x, n); break;
     case ST_DETREND: SFX(detrend)(x, n); break;
     case ST_POLARITY: SFX(polarity)(x, n); break;
@@ -864,7 +919,8 @@ static double SFX(panel_factor)(const SFX(step) *s, const T *x, int32_t rows, in
  * tile of `tile` rows. */
 static void SFX(run_part_rows)(const SFX(plan) *P, int part, const T *src, T *dst,
                                int32_t r0, int32_t r1, int32_t cols, const int32_t *row_len,
-                               double factor, int64_t tile, void *scratch) {
+                               double factor, int64_t tile, void *scratch, const double *keys,
+                               int64_t first_sample) {
   int first = P->part_a[part];
   if (P->part_b[part] > first && P->steps[first].kind == ST_NORM_PANEL) first++;
   for (int32_t a = r0; a < r1; a += (int32_t)tile) {
@@ -876,7 +932,9 @@ static void SFX(run_part_rows)(const SFX(plan) *P, int part, const T *src, T *ds
         if (row_len[r] >= 0) SFX(scale)(dst + (int64_t)r * cols, row_len[r], factor);
     for (int i = first; i < P->part_b[part]; i++)
       for (int32_t r = a; r < b; r++)
-        if (row_len[r] >= 0) SFX(run_step)(&P->steps[i], dst + (int64_t)r * cols, row_len[r], scratch);
+        if (row_len[r] >= 0)
+          SFX(run_step)(&P->steps[i], dst + (int64_t)r * cols, row_len[r], scratch,
+                        keys ? keys + 2 * (int64_t)r : NULL, first_sample, P->dt);
   }
 }
 
@@ -893,9 +951,10 @@ static int SFX(check_apply)(const SFX(plan) *P, int32_t cols, char *err, int err
   return 0;
 }
 
-int SFX(or_pipeline_apply)(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
-                           int64_t record_samples, const T *src, T *dst, int32_t rows,
-                           int32_t cols, const int32_t *row_len, char *err, int errlen) {
+int SFX(or_pipeline_apply_keys)(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
+                                int64_t record_samples, const T *src, T *dst, int32_t rows,
+                                int32_t cols, const int32_t *row_len, const double *keys,
+                                int64_t first, char *err, int errlen) {
   SFX(plan) *P = (SFX(plan) *)malloc(sizeof(SFX(plan)));
   if (SFX(plan_compile)(P, ops, n_ops, dt_ms, samples, record_samples, err, errlen)) {
     free(P);
@@ -911,12 +970,19 @@ int SFX(or_pipeline_apply)(const or_op *ops, int32_t n_ops, double dt_ms, int64_
     if (P->part_b[part] > a && P->steps[a].kind == ST_NORM_PANEL)
       factor = SFX(panel_factor)(&P->steps[a], from0 ? src : dst, rows, cols, row_len);
     SFX(run_part_rows)(P, part, from0 ? src : NULL, dst, 0, rows, cols, row_len, factor, tile,
-                       scratch);
+                       scratch, keys, first);
   }
   free(scratch);
   SFX(plan_free)(P);
   free(P);
   return 0;
+}
+
+int SFX(or_pipeline_apply)(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
+                           int64_t record_samples, const T *src, T *dst, int32_t rows,
+                           int32_t cols, const int32_t *row_len, char *err, int errlen) {
+  return SFX(or_pipeline_apply_keys)(ops, n_ops, dt_ms, samples, record_samples, src, dst, rows,
+                                     cols, row_len, NULL, 0, err, errlen);
 }
 
 /* ---------------------------------------------------------- correlation -- */

@@ -25,7 +25,7 @@
 
 /* Step kinds of a compiled plan (pipeline.nim:180-183), the subset restated. */
 enum { ST_DEMEAN, ST_DETREND, ST_SCALE, ST_CLIP, ST_POLARITY, ST_FILTER, ST_AGC,
-       ST_NORM_TRACE, ST_NORM_PANEL };
+       ST_NORM_TRACE, ST_NORM_PANEL, ST_MUTE_TOP, ST_MUTE_BOTTOM };
 
 static int or_seterr(char *err, int errlen, const char *msg) {
   if (err && errlen > 0) {
@@ -45,6 +45,67 @@ static const char *or_op_name(int kind) {
                                 "opCalibrate", "opUncalibrate"};
   if (kind < 0 || kind > 14) return "op?";
   return names[kind];
+}
+
+/* ---------------------------------------------------------------- mutes -- */
+
+/* muteTimeMs (core/mute.nim:100-118): the mute time of a trace at `offset`
+ * (|offset| used; NaN when the curve needs the offset and it is NaN). */
+double or_mute_time_ms(const or_op *c, double offset) {
+  if (c->mute_kind == 0) return c->mute_t_ms;
+  if (c->mute_kind == 2) return c->mute_t_ms + fabs(offset) * c->mute_ms_per_dist + c->mute_pad_ms;
+  const double x = fabs(offset);
+  if (isnan(x)) return NAN;
+  const int last = c->mute_knots - 1;
+  if (x <= c->mute_offsets[0]) return c->mute_times[0];
+  if (x >= c->mute_offsets[last]) return c->mute_times[last];
+  int lo = 0, hi = last;  /* the last knot <= x, by bisection */
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (c->mute_offsets[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  const double u = (x - c->mute_offsets[lo]) / (c->mute_offsets[hi] - c->mute_offsets[lo]);
+  return c->mute_times[lo] + u * (c->mute_times[hi] - c->mute_times[lo]);
+}
+
+/* firstAtOrAfter / firstAfter (core/mute.nim:122-132): the first column
+ * >= c (> c), clamped to 0 .. n (0 for NaN). */
+static int64_t or_first_at_or_after(double c, int64_t n) {
+  if (!(c > 0)) return 0;
+  if (c >= (double)n) return n;
+  return (int64_t)ceil(c);
+}
+static int64_t or_first_after(double c, int64_t n) {
+  if (!(c >= 0)) return 0;
+  if (c >= (double)n) return n;
+  return (int64_t)floor(c) + 1;
+}
+
+/* A mute op's validation (timeMute / tableMute / lmoMute, core/mute.nim:
+ * 58-98, and the taper check of pipeline.nim:417-421). */
+static int or_mute_check(const or_op *op, char *msg, int msglen) {
+  if (!isfinite(op->taper_ms) || op->taper_ms < 0) {
+    snprintf(msg, msglen, "a mute taper must be finite and >= 0 ms, got %g", op->taper_ms);
+    return -1;
+  }
+  if (op->mute_kind == 0) {
+    if (!isfinite(op->mute_t_ms)) { snprintf(msg, msglen, "a mute time must be finite"); return -1; }
+    return 0;
+  }
+  if (op->mute_kind == 2) {
+    if (!or_finite_positive(op->mute_ms_per_dist)) { snprintf(msg, msglen, "an LMO mute velocity must be finite and positive"); return -1; }
+    if (!isfinite(op->mute_t_ms) || !isfinite(op->mute_pad_ms)) { snprintf(msg, msglen, "LMO mute times must be finite"); return -1; }
+    return 0;
+  }
+  if (op->mute_kind != 1) { snprintf(msg, msglen, "unknown mute curve %d", op->mute_kind); return -1; }
+  if (op->mute_knots < 1 || op->mute_knots > OR_MAX_KNOTS) { snprintf(msg, msglen, "a mute table needs 1 .. %d knots", OR_MAX_KNOTS); return -1; }
+  for (int i = 0; i < op->mute_knots; i++) {
+    if (!isfinite(op->mute_offsets[i]) || op->mute_offsets[i] < 0) { snprintf(msg, msglen, "mute table offsets must be finite and >= 0"); return -1; }
+    if (i > 0 && op->mute_offsets[i] <= op->mute_offsets[i - 1]) { snprintf(msg, msglen, "mute table offsets must be strictly ascending"); return -1; }
+    if (!isfinite(op->mute_times[i])) { snprintf(msg, msglen, "mute table times must be finite"); return -1; }
+  }
+  return 0;
 }
 
 /* ------------------------------------------------------------- lengths -- */
@@ -596,7 +657,7 @@ static void or_part_job(or_pool *p, int32_t worker) {
     if (first >= p->rows) break;
     int32_t last = (int32_t)(first + p->grain < p->rows ? first + p->grain : p->rows);
     run_part_rows_f32(p->plan, p->part, p->part == 0 ? p->src : NULL, p->dst, (int32_t)first,
-                      last, p->cols, p->row_len, p->factor, p->grain, scratch);
+                      last, p->cols, p->row_len, p->factor, p->grain, scratch, NULL, 0);
   }
 }
 

@@ -33,6 +33,7 @@ extern "C" {
 #endif
 
 #define OR_MAX_NOTCH 8
+#define OR_MAX_KNOTS 32
 
 /* FilterSpec (filter/design.nim:78-90).  kind 0 = rkTrapezoid, 1 = rkButterworth. */
 typedef struct or_spec {
@@ -49,7 +50,8 @@ typedef struct or_spec {
 
 /* Op (pipeline.nim:158-174), as plain data.  kind follows OpKind
  * (pipeline.nim:92-108): 0 demean, 1 detrend, 2 scale, 3 gainDb, 4 clip,
- * 5 polarity, 8 filter, 10 agc, 11 normalize.  Other kinds are refused. */
+ * 5 polarity, 6 muteTop, 7 muteBottom, 8 filter, 10 agc, 11 normalize.
+ * Other kinds are refused. */
 typedef struct or_op {
   int32_t kind;
   int32_t trend;      /* filter: 0 trNone, 1 trMean, 2 trLinear */
@@ -64,6 +66,17 @@ typedef struct or_op {
   double epsilon;     /* agc */
   double percentile;  /* normalize */
   or_spec spec;       /* filter */
+  /* muteTop / muteBottom (core/mute.nim:31-98): mute_kind 0 timeMute
+   * (mute_t_ms), 1 tableMute (mute_knots |offset| -> time knots), 2 lmoMute
+   * (mute_t_ms = t0Ms, mute_ms_per_dist = 1000 / velocity, mute_pad_ms) */
+  int32_t mute_kind;
+  int32_t mute_knots;
+  double mute_t_ms;
+  double mute_ms_per_dist;
+  double mute_pad_ms;
+  double taper_ms;
+  double mute_offsets[OR_MAX_KNOTS];
+  double mute_times[OR_MAX_KNOTS];
 } or_op;
 
 /* ------------------------------------------------------------- lengths -- */
@@ -126,6 +139,25 @@ int or_pipeline_apply_f64(const or_op *ops, int32_t n_ops, double dt_ms, int64_t
                           int64_t record_samples, const double *src, double *dst,
                           int32_t rows, int32_t cols, const int32_t *row_len,
                           char *err, int errlen);
+/* The same with the rows' trace keys (mutes read them): keys holds
+ * (t0Ms, offset) per row, or is NULL (t0Ms 0, offset NaN); first_sample is
+ * ChunkMeta.firstSample (the column origin of the mute times). */
+int or_pipeline_apply_keys_f32(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
+                               int64_t record_samples, const float *src, float *dst,
+                               int32_t rows, int32_t cols, const int32_t *row_len,
+                               const double *keys, int64_t first_sample, char *err, int errlen);
+int or_pipeline_apply_keys_f64(const or_op *ops, int32_t n_ops, double dt_ms, int64_t samples,
+                               int64_t record_samples, const double *src, double *dst,
+                               int32_t rows, int32_t cols, const int32_t *row_len,
+                               const double *keys, int64_t first_sample, char *err, int errlen);
+/* muteTimeMs (core/mute.nim:100-118) of a mute op at `offset`. */
+double or_mute_time_ms(const or_op *op, double offset);
+/* muteTop / muteBottom of one row in column units (core/mute.nim:136-180). */
+void or_mute_top_f32(float *x, int64_t n, double start_col, double taper_cols);
+void or_mute_top_f64(double *x, int64_t n, double start_col, double taper_cols);
+void or_mute_bottom_f32(float *x, int64_t n, double end_col, double taper_cols);
+void or_mute_bottom_f64(double *x, int64_t n, double end_col, double taper_cols);
+
 /* The same through a ProcessPool clone (core/pool.nim:145-308): `threads`
  * persistent workers taking runs of ~256 KiB of rows (multiples of 8) from an
  * atomic counter, one workspace each, every step on a tile while it is in

@@ -2,7 +2,7 @@
 
 The panel path (batched mixed-radix real FFT, zero-phase FFT filters with the
 reference's edge handling, demean, AGC, normalization and vibroseis
-correlation) runs in hand-written gfx950 kernels behind the C ABI of
+correlation, and the mutes) runs in hand-written gfx950 kernels behind the C ABI of
 include/pseg.h (libpseg.so).  This package mirrors the reference's Nim API
 over that ABI: initPipeline / compile / PipelinePlan.apply and
 initCorrelationPlan / correlateInto, plus PipelineBatch (many records with
@@ -11,7 +11,8 @@ device-resident survey served by continuous batching).
 """
 from ._lib import HipError, UnsupportedOp, WorkspaceError, build, lib  # noqa: F401
 from .pipeline import (  # noqa: F401
-    ChunkMeta, Corners, FilterSpec, Inf, NaN, Op, Pipeline, PipelinePlan, agcMeanAbs, agcOp,
+    ChunkMeta, Corners, FilterSpec, Inf, MuteCurve, NaN, Op, Pipeline, PipelinePlan, agcMeanAbs,
+    agcOp, lmoMute, mkLmo, mkTable, mkTime, muteBottomOp, muteTopOp, tableMute, timeMute,
     agcRms, bandPass, bandPassOp, butterBandPass, butterHighCut, butterLowCut, clipOp,
     decodeWindow, demeanOp, detrendOp, filterOp, float32, float64, gainDbOp, highCut, iirOp,
     initPipeline, lowCut, nextFastLen, normalizeOp, normMax, normPercentile, normRms, perPanel,

@@ -33,7 +33,11 @@ class OpC(C.Structure):
                 ("stat", C.c_int32), ("scope", C.c_int32), ("reserved", C.c_int32),
                 ("value", C.c_double), ("lo", C.c_double), ("hi", C.c_double),
                 ("window_ms", C.c_double), ("target", C.c_double), ("epsilon", C.c_double),
-                ("percentile", C.c_double), ("spec", FilterSpecC)]
+                ("percentile", C.c_double), ("spec", FilterSpecC),
+                ("mute_kind", C.c_int32), ("mute_knots", C.c_int32),
+                ("mute_t_ms", C.c_double), ("mute_ms_per_dist", C.c_double),
+                ("mute_pad_ms", C.c_double), ("taper_ms", C.c_double),
+                ("mute_offsets", C.c_double * 32), ("mute_times", C.c_double * 32)]
 
 
 class PlanInfoC(C.Structure):
@@ -116,7 +120,8 @@ SYMBOLS = [
     "pseg_last_error", "pseg_device_count", "pseg_meta_flatten", "pseg_next_fast_len",
     "pseg_impulse_length", "pseg_response", "pseg_plan_compile", "pseg_plan_destroy",
     "pseg_plan_info_get", "pseg_required_margin", "pseg_decode_window",
-    "pseg_plan_workspace_size", "pseg_plan_apply", "pseg_corr_compile", "pseg_corr_destroy",
+    "pseg_plan_workspace_size", "pseg_plan_apply", "pseg_plan_apply_keys", "pseg_corr_compile",
+    "pseg_corr_destroy",
     "pseg_corr_info_get", "pseg_corr_apply", "pseg_linear_sweep", "pseg_batch_create",
     "pseg_batch_launch", "pseg_batch_info_get", "pseg_batch_destroy", "pseg_server_create",
     "pseg_server_add_flow", "pseg_server_submit", "pseg_server_wait", "pseg_server_read",
@@ -151,6 +156,7 @@ def lib():
     L.pseg_decode_window.restype = None
     L.pseg_plan_workspace_size.argtypes = [vp, i32, i32, C.POINTER(C.c_size_t)]
     L.pseg_plan_apply.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_size_t, vp]
+    L.pseg_plan_apply_keys.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, C.c_size_t, vp]
     L.pseg_corr_compile.argtypes = [i32, vp, i32, f64, f64, f64, i32, i32, i32, f64, i32,
                                     C.POINTER(vp)]
     L.pseg_corr_destroy.argtypes = [vp]

@@ -1005,6 +1005,112 @@ __device__ __forceinline__ typename Bits<T>::S select_bits(const T (&v)[PER], in
   return (S)prefix;
 }
 
+// muteTimeMs (core/mute.nim:100-118) of a trace at `offset`.
+__device__ __forceinline__ double mute_time_ms(const DevMute &M, double offset) {
+  const int kind = uni(M.kind);
+  if (kind == 0) return uni(M.t_ms);
+  if (kind == 2) return uni(M.t_ms) + fabs(offset) * uni(M.ms_per_dist) + uni(M.pad_ms);
+  const double x = fabs(offset);
+  if (isnan(x)) return x;
+  gptr<double> o = as_global<double>(uni_ptr(M.offsets));
+  gptr<double> t = as_global<double>(uni_ptr(M.times));
+  const int last = uni(M.knots) - 1;
+  if (x <= o[0]) return t[0];
+  if (x >= o[last]) return t[last];
+  int lo = 0, hi = last;  // the last knot <= x, by bisection
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (o[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  const double u = (x - o[lo]) / (o[hi] - o[lo]);
+  return t[lo] + u * (t[hi] - t[lo]);
+}
+
+// firstAtOrAfter / firstAfter (core/mute.nim:122-132).
+__device__ __forceinline__ int first_at_or_after(double c, int n) {
+  return !(c > 0) ? 0 : c >= (double)n ? n : (int)ceil(c);
+}
+__device__ __forceinline__ int first_after(double c, int n) {
+  return !(c >= 0) ? 0 : c >= (double)n ? n : (int)floor(c) + 1;
+}
+
+// sin x for 0 <= x <= π/2 (a mute taper's argument): the Taylor series to
+// x^21, whose remainder there is below 2e-18 (under half an ulp of the
+// result), by Horner's rule in x² — a few FMAs and registers where the
+// general sin's range reduction would cost the row kernel its registers.
+// A float64 constant made in SGPRs where it is used (an asm barrier): the
+// compiler would otherwise hoist the coefficients below out of the step loop
+// into twenty VGPRs.
+__device__ __forceinline__ double sconst(double c) {
+  asm volatile("" : "+s"(c));
+  return c;
+}
+__device__ __forceinline__ double taper_sin(double x) {
+  const double z = x * x;
+  double p = sconst(1.0 / 51090942171709440000.0);           // 1/21!
+  p = fma(p, z, sconst(-1.0 / 121645100408832000.0));        // 1/19!
+  p = fma(p, z, sconst(1.0 / 355687428096000.0));            // 1/17!
+  p = fma(p, z, sconst(-1.0 / 1307674368000.0));             // 1/15!
+  p = fma(p, z, sconst(1.0 / 6227020800.0));                 // 1/13!
+  p = fma(p, z, sconst(-1.0 / 39916800.0));                  // 1/11!
+  p = fma(p, z, sconst(1.0 / 362880.0));                     // 1/9!
+  p = fma(p, z, sconst(-1.0 / 5040.0));                      // 1/7!
+  p = fma(p, z, sconst(1.0 / 120.0));                        // 1/5!
+  p = fma(p, z, sconst(-1.0 / 6.0));                         // 1/3!
+  return fma(x * z, p, x);
+}
+
+// muteTop / muteBottom (core/mute.nim:136-180) of the row's live part at
+// column `col` (the row's column of the mute time, muteCol, core/mute.nim:
+// 189-194), the taper in columns, the weights sin² in float64 (taper_sin
+// and the argument as (π/2 / taper) x: within a few ulp of the reference's
+// sin(π/2 x / taper), far inside the float32 rounding of the product).
+template <typename T, int NTH, int PER>
+__device__ __forceinline__ void mute_step(const DevMute &M, bool top, T (&v)[PER], int n,
+                                          double col) {
+  if (isnan(col)) return;  // a NaN time or t0Ms: the row is left alone
+  const int tid_ = opaque_tid();
+  const double dt = uni(M.dt_ms);
+  const double tc = uni(M.taper_ms) / dt;
+  const double taper = tc < INFINITY ? tc : 0.0;
+  constexpr double kHalfPi = 1.5707963267948966;  // π / 2 as the float64 nearest
+  const double arg_per_col = taper > 0 ? kHalfPi / taper : 0.0;  // one division per row
+  if (top) {
+    const double ramp_start = col - taper;
+    const int zero_end = first_at_or_after(ramp_start, n);
+    const int e = taper > 0 ? first_at_or_after(col, n) : zero_end;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int j = tid_ + NTH * k;
+      if (j < zero_end) {
+        v[k] = (T)0;
+      } else if (j < e) {
+        const double s = taper_sin(arg_per_col * ((double)j - ramp_start));
+        v[k] = (T)((double)v[k] * (s * s));
+      }
+      PSEG_ONE_AT_A_TIME();
+    }
+  } else {
+    const double ramp_end = col + taper;
+    const int zero_start = first_after(ramp_end, n);
+    const int b = taper > 0 ? first_after(col, n) : zero_start;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int j = tid_ + NTH * k;
+      if (j < n) {
+        if (j >= zero_start) {
+          v[k] = (T)0;
+        } else if (j >= b) {
+          const double s = taper_sin(arg_per_col * (ramp_end - (double)j));
+          v[k] = (T)((double)v[k] * (s * s));
+        }
+      }
+      PSEG_ONE_AT_A_TIME();
+    }
+  }
+}
+
 // Per-trace normalization (gain/gain.nim:198-219).
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds,

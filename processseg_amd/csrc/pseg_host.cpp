@@ -619,7 +619,46 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       break;
     }
     case PSEG_OP_MUTE_TOP:
-    case PSEG_OP_MUTE_BOTTOM:
+    case PSEG_OP_MUTE_BOTTOM: {
+      // muteTopOp / muteBottomOp (pipeline.nim:417-424) with the curve
+      // checks of timeMute / tableMute / lmoMute (core/mute.nim:58-98)
+      if (!std::isfinite(op.taper_ms) || op.taper_ms < 0)
+        invalid("a mute taper must be finite and >= 0 ms, got %g", op.taper_ms);
+      DevMute &M = st.mute;
+      M.kind = op.mute_kind;
+      M.taper_ms = op.taper_ms;
+      M.dt_ms = dt;
+      if (op.mute_kind == 0) {
+        if (!std::isfinite(op.mute_t_ms)) invalid("a mute time must be finite, got %g", op.mute_t_ms);
+        M.t_ms = op.mute_t_ms;
+      } else if (op.mute_kind == 2) {
+        if (!finite_positive(op.mute_ms_per_dist))
+          invalid("an LMO mute velocity must be finite and positive");
+        if (!std::isfinite(op.mute_t_ms) || !std::isfinite(op.mute_pad_ms))
+          invalid("LMO mute times must be finite, got t0Ms %g, padMs %g", op.mute_t_ms, op.mute_pad_ms);
+        M.t_ms = op.mute_t_ms;
+        M.ms_per_dist = op.mute_ms_per_dist;
+        M.pad_ms = op.mute_pad_ms;
+      } else if (op.mute_kind == 1) {
+        if (op.mute_knots < 1 || op.mute_knots > PSEG_MAX_KNOTS)
+          invalid("a mute table needs 1 .. %d knots, got %d", PSEG_MAX_KNOTS, op.mute_knots);
+        std::vector<double> o(op.mute_offsets, op.mute_offsets + op.mute_knots);
+        std::vector<double> t(op.mute_times, op.mute_times + op.mute_knots);
+        for (int32_t k = 0; k < op.mute_knots; k++) {
+          if (!std::isfinite(o[k]) || o[k] < 0) invalid("mute table offsets must be finite and >= 0, got %g", o[k]);
+          if (k > 0 && o[k] <= o[k - 1]) invalid("mute table offsets must be strictly ascending: %g is followed by %g", o[k - 1], o[k]);
+          if (!std::isfinite(t[k])) invalid("mute table times must be finite, got %g", t[k]);
+        }
+        M.knots = op.mute_knots;
+        set_off(M.offsets, blob.add_vec(o));
+        set_off(M.times, blob.add_vec(t));
+      } else {
+        invalid("unknown mute curve %d", op.mute_kind);
+      }
+      st.kind = op.kind == PSEG_OP_MUTE_TOP ? kMuteTop : kMuteBottom;
+      lds = 0;
+      break;
+    }
     case PSEG_OP_IIR:
     case PSEG_OP_GAIN_CURVE:
     case PSEG_OP_CALIBRATE:
@@ -639,6 +678,9 @@ void patch_step(DevStep &st, char *base) {
   } else if (st.kind == kAgc) {
     patch(st.agc.inv, base);
     patch(st.agc.rev, base);
+  } else if (st.kind == kMuteTop || st.kind == kMuteBottom) {
+    patch(st.mute.offsets, base);
+    patch(st.mute.times, base);
   }
 }
 
@@ -955,9 +997,17 @@ pseg_status pseg_plan_workspace_size(const pseg_plan *P, int32_t rows, int32_t c
 pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int32_t rows,
                             int32_t cols, const int32_t *row_len, void *ws, size_t ws_bytes,
                             void *stream) {
+  return pseg_plan_apply_keys(P, src, dst, rows, cols, row_len, nullptr, 0, ws, ws_bytes, stream);
+}
+
+pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *row_keys,
+                                 int64_t first_sample, void *ws, size_t ws_bytes,
+                                 void *stream) {
   // checkApply (pipeline.nim:619-641): every check before any sample changes
   if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
   if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
   if (rows == 0) return PSEG_OK;
   if (!src || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL panel or row_len");
   if (cols > P->samples)
@@ -986,6 +1036,8 @@ pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int3
     J.src = in;
     J.dst = dst;
     J.num = row_len;
+    J.keys = row_keys;
+    J.col0 = first_sample;
     J.src_pitch = J.dst_pitch = cols;
     J.rows = rows;
     J.cols = cols;

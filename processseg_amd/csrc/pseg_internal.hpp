@@ -22,6 +22,8 @@ enum StepKind : int32_t {
   kAgc = 6,
   kNormTrace = 7,
   kNormPanel = 8,
+  kMuteTop = 9,
+  kMuteBottom = 10,
 };
 
 // Divide by a runtime constant with a multiply and a shift (Granlund-
@@ -90,6 +92,20 @@ struct DevAgc {
   const double *rev;  // 1 / (2h + 1 - m), m = 0 .. 2h
 };
 
+// A mute (core/mute.nim): the curve giving each trace's mute time and the
+// taper, both in ms; the row's column of that time needs its keys.
+struct DevMute {
+  int32_t kind;        // 0 time, 1 table, 2 LMO
+  int32_t knots;
+  double t_ms;         // time; LMO: t0Ms
+  double ms_per_dist;  // LMO
+  double pad_ms;       // LMO
+  double taper_ms;
+  double dt_ms;
+  const double *offsets;  // table knots
+  const double *times;
+};
+
 struct DevStep {
   int32_t kind;
   int32_t stat;       // normalize
@@ -99,6 +115,7 @@ struct DevStep {
   double percentile;  // normalize
   DevFilter filter;
   DevAgc agc;
+  DevMute mute;
 };
 
 struct DevPlan {
@@ -114,14 +131,17 @@ struct DevPlan {
 // [crop0, crop1) are stored to dst[r * dst_pitch + 0 ..).  When has_factor,
 // the live part is first scaled by *factor (a part after a per-panel
 // normalization).  Jobs of a batch are sorted by item0, the first work item
-// (row) of the job.
+// (row) of the job.  Mutes read the rows' trace keys (t0Ms, offset per row;
+// NULL: 0 and NaN) and col0, the trace sample of window column 0.
 struct Job {
   const DevPlan *plan;
   const void *src;
   void *dst;
   const int32_t *num;
   const double *factor;
+  const double *keys;
   int64_t src_pitch, dst_pitch;
+  int64_t col0;
   int32_t rows, cols, first, crop0, crop1, item0;
   int32_t step_first, step_last, has_factor, reserved;
 };

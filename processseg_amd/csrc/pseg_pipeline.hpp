@@ -143,6 +143,20 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
         case kNormTrace:
           if constexpr (FULL) norm_trace_step<T, NTH, PER>(st, v, n, lds, red);
           break;
+        case kMuteTop:
+        case kMuteBottom:
+          if constexpr (FULL) {
+            // muteCol (core/mute.nim:189-194) from the row's keys (the
+            // job's fields through readfirstlane: SGPRs, whatever the
+            // compiler hoists)
+            const double *keys = (const double *)uni_ptr(J.keys);
+            const double col0 = uni((double)J.col0);
+            const double t0 = keys ? uni(keys[2 * r]) : 0.0;
+            const double off = keys ? uni(keys[2 * r + 1]) : (double)NAN;
+            const double col = (mute_time_ms(st.mute, off) - t0) / uni(st.mute.dt_ms) - col0;
+            mute_step<T, NTH, PER>(st.mute, uni(st.kind) == kMuteTop, v, n, col);
+          }
+          break;
         default: break;
       }
     }

@@ -181,6 +181,88 @@ def spec_c(spec: FilterSpec) -> _lib.FilterSpecC:
 
 # -------------------------------------------------------------------- ops
 
+MAX_KNOTS = 32
+mkTime, mkTable, mkLmo = 0, 1, 2
+
+
+@dataclasses.dataclass(frozen=True)
+class MuteCurve:
+    """The mute time of each trace (core/mute.nim:31-52): kind mkTime (one
+    time), mkTable (linear in |offset| between knots, flat beyond them) or
+    mkLmo (t0Ms + |offset| msPerDist + padMs).  Build with timeMute,
+    tableMute or lmoMute, which validate it."""
+    kind: int
+    tMs: float = 0.0
+    offsets: tuple = ()
+    timesMs: tuple = ()
+    msPerDist: float = 0.0
+    t0Ms: float = 0.0
+    padMs: float = 0.0
+
+    def muteTimeMs(self, offset: float) -> float:
+        """muteTimeMs (core/mute.nim:100-118)."""
+        if self.kind == mkTime:
+            return self.tMs
+        if self.kind == mkLmo:
+            return self.t0Ms + abs(offset) * self.msPerDist + self.padMs
+        x = abs(offset)
+        if math.isnan(x):
+            return math.nan
+        o, t = self.offsets, self.timesMs
+        if x <= o[0]:
+            return t[0]
+        if x >= o[-1]:
+            return t[-1]
+        lo, hi = 0, len(o) - 1
+        while hi - lo > 1:
+            mid = (lo + hi) >> 1
+            if o[mid] <= x:
+                lo = mid
+            else:
+                hi = mid
+        u = (x - o[lo]) / (o[hi] - o[lo])
+        return t[lo] + u * (t[hi] - t[lo])
+
+
+def _finite(x) -> bool:
+    return isinstance(x, (int, float)) and math.isfinite(x)
+
+
+def timeMute(tMs: float) -> MuteCurve:
+    """timeMute (core/mute.nim:58-62)."""
+    if not _finite(tMs):
+        raise ValueError(f"a mute time must be finite, got {tMs}")
+    return MuteCurve(mkTime, tMs=float(tMs))
+
+
+def tableMute(offsets, timesMs) -> MuteCurve:
+    """tableMute (core/mute.nim:64-86): |offset| knots strictly ascending."""
+    offsets = tuple(float(v) for v in offsets)
+    timesMs = tuple(float(v) for v in timesMs)
+    if len(offsets) == 0:
+        raise ValueError("a mute table needs at least one knot")
+    if len(offsets) != len(timesMs):
+        raise ValueError(f"a mute table has {len(offsets)} offsets and {len(timesMs)} times")
+    for i, (o, t) in enumerate(zip(offsets, timesMs)):
+        if not _finite(o) or o < 0:
+            raise ValueError(f"mute table offsets must be finite and ≥ 0, got {o}")
+        if i > 0 and o <= offsets[i - 1]:
+            raise ValueError(f"mute table offsets must be strictly ascending: {offsets[i - 1]} "
+                             f"is followed by {o}")
+        if not _finite(t):
+            raise ValueError(f"mute table times must be finite, got {t}")
+    return MuteCurve(mkTable, offsets=offsets, timesMs=timesMs)
+
+
+def lmoMute(velocity: float, t0Ms: float = 0.0, padMs: float = 0.0) -> MuteCurve:
+    """lmoMute (core/mute.nim:88-98): velocity in distance units per second."""
+    if not _finite(velocity) or velocity <= 0:
+        raise ValueError(f"an LMO mute velocity must be finite and positive, got {velocity}")
+    if not _finite(t0Ms) or not _finite(padMs):
+        raise ValueError(f"LMO mute times must be finite, got t0Ms {t0Ms}, padMs {padMs}")
+    return MuteCurve(mkLmo, msPerDist=1000 / float(velocity), t0Ms=float(t0Ms), padMs=float(padMs))
+
+
 @dataclasses.dataclass
 class Op:
     """One operation as plain data (pipeline.nim:158-174).  Parameters are
@@ -198,6 +280,8 @@ class Op:
     stat: int = normMax         # normalize
     scope: int = perTrace
     percentile: float = 99.0
+    mute: Optional[MuteCurve] = None  # muteTop, muteBottom
+    taper_ms: float = 0.0
 
     def to_c(self) -> _lib.OpC:
         o = _lib.OpC()
@@ -215,7 +299,24 @@ class Op:
         o.percentile = self.percentile
         if self.spec is not None:
             o.spec = spec_c(self.spec)
+        if self.mute is not None:
+            mute_fields(o, self.mute, self.taper_ms)
         return o
+
+
+def mute_fields(o, c: MuteCurve, taper_ms: float):
+    """Fill the mute fields of a C op (pseg_op, or the oracle's)."""
+    if c.kind == mkTable and len(c.offsets) > MAX_KNOTS:
+        raise UnsupportedOp(f"a mute table of {len(c.offsets)} knots (at most {MAX_KNOTS})")
+    o.mute_kind = c.kind
+    o.mute_knots = len(c.offsets)
+    o.mute_t_ms = c.t0Ms if c.kind == mkLmo else c.tMs
+    o.mute_ms_per_dist = c.msPerDist
+    o.mute_pad_ms = c.padMs
+    o.taper_ms = float(taper_ms)
+    for i, (a, b) in enumerate(zip(c.offsets, c.timesMs)):
+        o.mute_offsets[i] = a
+        o.mute_times[i] = b
 
 
 def demeanOp() -> Op:
@@ -240,6 +341,16 @@ def clipOp(lo: float, hi: float) -> Op:
 
 def polarityOp() -> Op:
     return Op(opPolarity)
+
+
+def muteTopOp(curve: MuteCurve, taperMs: float = 0.0) -> Op:
+    """muteTopOp (pipeline.nim:283-285): a top mute with a taper before it."""
+    return Op(opMuteTop, mute=curve, taper_ms=float(taperMs))
+
+
+def muteBottomOp(curve: MuteCurve, taperMs: float = 0.0) -> Op:
+    """muteBottomOp (pipeline.nim:287-289): a bottom mute with a taper after it."""
+    return Op(opMuteBottom, mute=curve, taper_ms=float(taperMs))
 
 
 def filterOp(spec: FilterSpec, trend: int = trLinear) -> Op:
@@ -278,11 +389,12 @@ def _ops_c(ops: Sequence[Op]):
 class ChunkMeta:
     """The per-row facts of a block (core/types.nim:69-122): status per row
     (tsLive, tsUnknown, tsDead, tsAuxiliary), trace length numSamples per
-    row, failed rows, dt and firstSample.  ``row_len(cols)`` flattens
+    row, failed rows, dt and firstSample, and the trace keys mutes read
+    (t0Ms, offset per row; default 0 and NaN).  ``row_len(cols)`` flattens
     isLive/liveSamples to one int32 per row (-1: not live)."""
 
     def __init__(self, rows: int, dtMs: float, numSamples=None, status=None,
-                 failed: Iterable[int] = (), firstSample: int = 0):
+                 failed: Iterable[int] = (), firstSample: int = 0, t0Ms=None, offset=None):
         if not (isinstance(dtMs, (int, float)) and math.isfinite(dtMs) and dtMs > 0):
             raise ValueError(f"dtMs must be finite and positive, got {dtMs}")
         if firstSample < 0:
@@ -298,6 +410,10 @@ class ChunkMeta:
         self.failed = np.ascontiguousarray(sorted(int(f) for f in failed), np.int64)
         if self.numSamples.shape != (rows,) or self.status.shape != (rows,):
             raise ValueError("numSamples and status need one value per row")
+        self.t0Ms = (np.zeros(rows) if t0Ms is None
+                     else np.ascontiguousarray(t0Ms, np.float64).reshape(rows))
+        self.offset = (np.full(rows, np.nan) if offset is None
+                       else np.ascontiguousarray(offset, np.float64).reshape(rows))
         self._cache = {}
 
     def __len__(self):
@@ -311,6 +427,19 @@ class ChunkMeta:
             self.status.ctypes.data, fl.ctypes.data, int(self.failed.size),
             1 if for_correlation else 0, out.ctypes.data))
         return out
+
+    def keys(self) -> np.ndarray:
+        """(t0Ms, offset) per row, [rows, 2] float64 (meta.keys(row))."""
+        return np.stack([self.t0Ms, self.offset], axis=1)
+
+    def device_keys(self, device):
+        import torch
+        key = ("keys", str(device))
+        t = self._cache.get(key)
+        if t is None:
+            t = torch.from_numpy(self.keys()).to(device)
+            self._cache[key] = t
+        return t
 
     def device_row_len(self, cols: int, device, for_correlation: bool = False):
         import torch
@@ -415,6 +544,8 @@ class PipelinePlan:
         self.steps = [dict(kind=_KIND_NAMES[info.step_kind[i]], nfft=info.step_nfft[i],
                            pad=info.step_pad[i], margin=info.step_margin[i],
                            reflect=bool(info.step_reflect[i])) for i in range(n)]
+        # mutes read the rows' trace keys (t0Ms, offset) from the meta
+        self.readsKeys = any(op.kind in (opMuteTop, opMuteBottom) for op in flow.ops)
         self._ws = {}
 
     def __len__(self):
@@ -438,10 +569,12 @@ class PipelinePlan:
         return ws, need
 
     def apply(self, src, dst=None, meta: Optional[ChunkMeta] = None, *, row_len=None,
-              ws=None, stream=None):
+              ws=None, stream=None, keys=None, first_sample: int = 0):
         """apply(src, dst, meta) (src -> dst) or apply(samples, meta) (in
         place), pipeline.nim:736-772.  `row_len` (a device int32 tensor) may
-        replace `meta`.  Asynchronous on `stream` (default: torch's current)."""
+        replace `meta`; then `keys` ([rows, 2] float64 device tensor of
+        t0Ms, offset) and `first_sample` give what mutes read from a meta.
+        Asynchronous on `stream` (default: torch's current)."""
         import torch
         if isinstance(dst, ChunkMeta) and meta is None:
             meta, dst = dst, None
@@ -467,12 +600,20 @@ class PipelinePlan:
             if not sameInterval(meta.dtMs, self.dtMs):
                 raise ValueError(f"the plan is for dtMs = {self.dtMs}, the rows have {meta.dtMs}")
             row_len = meta.device_row_len(cols, src.device)
+            if self.readsKeys:
+                keys = meta.device_keys(src.device)
+            first_sample = meta.firstSample
+        if keys is not None:
+            if (not isinstance(keys, torch.Tensor) or not keys.is_cuda or keys.dtype != torch.float64
+                    or tuple(keys.shape) != (rows, 2) or not keys.is_contiguous()):
+                raise ValueError("keys must be a contiguous [rows, 2] float64 GPU tensor")
         if ws is None:
             ws, need = self._workspace(rows, cols, src.device)
         else:
             need = ws.numel()
-        _lib.check(_lib.lib().pseg_plan_apply(
+        _lib.check(_lib.lib().pseg_plan_apply_keys(
             self._h, src.data_ptr(), dst.data_ptr(), rows, cols, row_len.data_ptr(),
+            keys.data_ptr() if keys is not None else None, int(first_sample),
             ws.data_ptr() if ws is not None else None, need, _stream_handle(stream)))
         return dst
 
