@@ -216,6 +216,36 @@ def f64_panel_leg(torch, P, flow, dev, steps, warmup):
             "achieved_GBps": round(samples * 16 / t / 1e9, 1)}
 
 
+def muted_flow_leg(torch, P, dev, steps, warmup):
+    """The headline panel with mutes in the flow (the kernel form with every
+    step): demean > band-pass > top LMO mute (2500 m/s, 20 ms taper) > AGC >
+    bottom table mute (50 ms taper), the rows' offsets -2400 .. 2400 m and
+    t0 0 from the ChunkMeta; one Plan.apply per step, 4 buffer pairs."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live, offset=np.linspace(-2400, 2400, ROWS))
+    flow = P.initPipeline([
+        P.demeanOp(), P.bandPassOp(P.Corners(5, 10, 70, 90)),
+        P.muteTopOp(P.lmoMute(2500.0, padMs=-12.0), taperMs=20), P.agcOp(250),
+        P.muteBottomOp(P.tableMute([0.0, 800, 2400], [2600.0, 3600, 5200]), taperMs=50)])
+    plan = flow.compile(np.float32, DT, COLS)
+    srcs = [torch.from_numpy(x).to(dev) for _ in range(4)]
+    dsts = [torch.empty_like(srcs[0]) for _ in range(4)]
+    for i in range(warmup):
+        plan.apply(srcs[i % 4], dsts[i % 4], meta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        plan.apply(srcs[i % 4], dsts[i % 4], meta)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    samples = ROWS * COLS
+    return {"workload": "demean > band-pass 5-10-70-90 Hz > top LMO mute > AGC 250 ms > "
+                        "bottom table mute, 2000x3001 f32, offsets from the meta",
+            "value": round(samples / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5), "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -473,6 +503,10 @@ def main():
         line["secondary"] = {"panels_as_batch": panels_batch_leg(
             torch, P, plan, srcs, dsts, row_len, max(10, args.steps // 10), 3)}
         del srcs, dsts
+        torch.cuda.empty_cache()
+        log("muted flow leg")
+        line["secondary"]["muted_flow"] = muted_flow_leg(torch, P, dev,
+                                                         max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
         log("float64 panel leg")
         line["secondary"]["f64_panel"] = f64_panel_leg(torch, P, flow, dev,
