@@ -55,6 +55,7 @@ typedef enum pseg_op_kind {
 #define PSEG_MAX_NOTCH 8
 #define PSEG_MAX_OPS 16
 #define PSEG_MAX_KNOTS 32
+#define PSEG_MAX_GAIN_KNOTS (1 << 20)
 
 /* FilterSpec (filter/design.nim:78-90): kind 0 = cos² trapezoid (corners
  * f1 < f2 <= f3 < f4; f2 = 0: no low cut, f3 = f4 = Inf: no high cut),
@@ -99,6 +100,19 @@ typedef struct pseg_op {
   double taper_ms;
   double mute_offsets[PSEG_MAX_KNOTS];
   double mute_times[PSEG_MAX_KNOTS];
+  /* gainCurveOp (pipeline.nim:318-322, gain/curves.nim:61-153): gain_kind 0
+   * knotGain (gain_knots knots: times gain_knot_ms, strictly increasing, and
+   * gains gain_knot_db, read during pseg_plan_compile only; at most
+   * PSEG_MAX_GAIN_KNOTS), 1 powerGain (gain_power, gain_ref_ms, gain_min_ms),
+   * 2 exponentialGain (gain_db_per_s); the table covers gain_tmin_ms ..
+   * gain_tmax_ms (NaN: the end of the record, recordSamples). */
+  int32_t gain_kind;
+  int32_t gain_knots;
+  const double *gain_knot_ms;
+  const double *gain_knot_db;
+  double gain_power, gain_ref_ms, gain_min_ms;
+  double gain_db_per_s;
+  double gain_tmin_ms, gain_tmax_ms;
 } pseg_op;
 
 typedef struct pseg_plan pseg_plan;

@@ -39,7 +39,12 @@ class OrOp(C.Structure):
                 ("mute_kind", C.c_int32), ("mute_knots", C.c_int32),
                 ("mute_t_ms", C.c_double), ("mute_ms_per_dist", C.c_double),
                 ("mute_pad_ms", C.c_double), ("taper_ms", C.c_double),
-                ("mute_offsets", C.c_double * 32), ("mute_times", C.c_double * 32)]
+                ("mute_offsets", C.c_double * 32), ("mute_times", C.c_double * 32),
+                ("gain_kind", C.c_int32), ("gain_knots", C.c_int32),
+                ("gain_knot_ms", C.POINTER(C.c_double)), ("gain_knot_db", C.POINTER(C.c_double)),
+                ("gain_power", C.c_double), ("gain_ref_ms", C.c_double),
+                ("gain_min_ms", C.c_double), ("gain_db_per_s", C.c_double),
+                ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double)]
 
 
 class OrFilterInfo(C.Structure):
@@ -88,6 +93,9 @@ def lib():
         L.or_is_fast_len.argtypes = [C.c_int64]
         L.or_mute_time_ms.argtypes = [C.POINTER(OrOp), C.c_double]
         L.or_mute_time_ms.restype = C.c_double
+        for fn in ("or_gain_db_at", "or_gain_factor_at"):
+            getattr(L, fn).argtypes = [C.POINTER(OrOp), C.c_double]
+            getattr(L, fn).restype = C.c_double
         L.or_next_fast_len.argtypes = [C.c_int64, C.c_int]
         L.or_next_fast_len.restype = C.c_int64
         L.or_factorize.argtypes = [C.c_int64, i32p]
@@ -110,6 +118,9 @@ def lib():
                 C.POINTER(OrOp), C.c_int32, C.c_double, C.c_int64, C.c_int64, p, p,
                 C.c_int32, C.c_int32, i32p, C.c_void_p, C.c_int64] + E
             getattr(L, f"or_mute_top_{t}").argtypes = [p, C.c_int64, C.c_double, C.c_double]
+            getattr(L, f"or_gain_curve_apply_{t}").argtypes = [
+                C.POINTER(OrOp), C.c_double, C.c_int64, p, C.c_int64, C.c_double,
+                C.POINTER(C.c_int64), C.c_char_p, C.c_int]
             getattr(L, f"or_mute_bottom_{t}").argtypes = [p, C.c_int64, C.c_double, C.c_double]
             getattr(L, f"or_correlate_{t}").argtypes = [
                 f64p, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64,
@@ -209,7 +220,27 @@ def op_struct(op) -> OrOp:
         for i, (a, b) in enumerate(zip(mute.offsets, mute.timesMs)):
             o.mute_offsets[i] = a
             o.mute_times[i] = b
+    curve = getattr(op, "curve", None)
+    if curve is not None:
+        gain_fields(o, curve, getattr(op, "t_min_ms", 0.0), getattr(op, "t_max_ms", math.nan))
     return o
+
+
+def gain_fields(o, curve, t_min_ms: float = 0.0, t_max_ms: float = math.nan):
+    """A gain curve (kind 0 knots, 1 power, 2 exponential) into the C op; the
+    struct keeps the knot arrays alive."""
+    o.gain_kind = int(curve.kind)
+    n = len(curve.knots)
+    o.gain_knots = n
+    if n:
+        o.gain_knot_ms = (C.c_double * n)(*[k[0] for k in curve.knots])
+        o.gain_knot_db = (C.c_double * n)(*[k[1] for k in curve.knots])
+    o.gain_power = float(curve.power)
+    o.gain_ref_ms = float(curve.refMs)
+    o.gain_min_ms = float(curve.minMs)
+    o.gain_db_per_s = float(curve.dbPerS)
+    o.gain_tmin_ms = float(t_min_ms)
+    o.gain_tmax_ms = float(t_max_ms)
 
 
 def _ops_array(ops):
@@ -379,6 +410,34 @@ def pipeline_apply(ops, src: np.ndarray, row_len, dt_ms: float, samples: int | N
 def mute_time_ms(op, offset: float) -> float:
     """muteTimeMs of a mute op (core/mute.nim:100-118)."""
     return float(lib().or_mute_time_ms(C.byref(op_struct(op)), offset))
+
+
+def gain_db_at(curve, t_ms: float) -> float:
+    """gainDbAt (gain/curves.nim:134-153)."""
+    o = OrOp()
+    gain_fields(o, curve)
+    return float(lib().or_gain_db_at(C.byref(o), t_ms))
+
+
+def gain_factor_at(curve, t_ms: float) -> float:
+    """factorAt (gain/curves.nim:154-160)."""
+    o = OrOp()
+    gain_fields(o, curve)
+    return float(lib().or_gain_factor_at(C.byref(o), t_ms))
+
+
+def gain_curve_apply(curve, x: np.ndarray, dt_ms: float, samples: int, start_ms: float,
+                     t_min_ms: float = 0.0, t_max_ms: float = math.nan):
+    """initGainCurvePlan + the pipeline's applyGainCurve on one trace (on a
+    copy): returns (trace, table length).  ValueError as the plan raises."""
+    o = OrOp()
+    gain_fields(o, curve, t_min_ms, t_max_ms)
+    y = np.ascontiguousarray(x).copy()
+    tl = C.c_int64(0)
+    e = _err()
+    _check(getattr(lib(), f"or_gain_curve_apply_{_sfx(y.dtype)}")(
+        C.byref(o), dt_ms, samples, y, y.shape[0], start_ms, C.byref(tl), e, 512), e)
+    return y, int(tl.value)
 
 
 def mute_row(kind: str, x: np.ndarray, col: float, taper_cols: float) -> np.ndarray:

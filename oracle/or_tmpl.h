@@ -749,10 +749,99 @@ void SFX(or_mute_bottom)(T *x, int64_t n, double end_col, double taper_cols) {
 
 /* ------------------------------------------------------------- pipeline -- */
 
+/* ---------------------------------------------------------- gain curve -- */
+
+/* GainCurvePlan (gain/curves.nim:214-249): T(factorAt(t)) at
+ * t = tmin + k·dt for k < len. */
+typedef struct SFX(gainplan) {
+  or_op curve;
+  T *table;
+  int64_t len;
+  double tmin, dt;
+} SFX(gainplan);
+
+static void SFX(gain_free)(SFX(gainplan) *g) {
+  free(g->table);
+  g->table = NULL;
+}
+
+/* initGainCurvePlan (gain/curves.nim:214-249) for traces of `samples`. */
+static int SFX(gain_init)(SFX(gainplan) *g, const or_op *op, double dt, int64_t samples,
+                          char *msg, int msglen) {
+  memset(g, 0, sizeof(*g));
+  if (!or_finite_positive(dt)) { snprintf(msg, msglen, "dtMs must be finite and positive, got %g", dt); return -1; }
+  if (samples < 1) { snprintf(msg, msglen, "a gain curve plan needs samples >= 1"); return -1; }
+  if (or_gain_check(op, msg, msglen)) return -1;
+  const double lo = op->gain_tmin_ms;
+  const double hi = isnan(op->gain_tmax_ms) ? lo + (double)(samples - 1) * dt : op->gain_tmax_ms;
+  if (!(isfinite(lo) && isfinite(hi))) { snprintf(msg, msglen, "the table limits must be finite, got %g .. %g", lo, hi); return -1; }
+  if (hi < lo) { snprintf(msg, msglen, "tMaxMs %g is before tMinMs %g", hi, lo); return -1; }
+  const double span = floor((hi - lo) / dt + 1e-9);
+  if (span >= (double)(1 << 26)) { snprintf(msg, msglen, "a gain table of %g samples is too long", span); return -1; }
+  g->len = (int64_t)span + 1;
+  g->table = (T *)malloc(sizeof(T) * (size_t)g->len);
+  g->curve = *op;
+  g->tmin = lo;
+  g->dt = dt;
+  for (int64_t k = 0; k < g->len; k++) {
+    const double t = lo + (double)k * dt;
+    const double f = or_gain_factor_at(op, t);
+    g->table[k] = (T)f;
+    if (isinf((double)g->table[k]) || isnan(f)) {
+      /* 0 · Inf would turn muted zeros into NaN */
+      snprintf(msg, msglen, "the gain at %g ms (%g) overflows the sample type", t, f);
+      SFX(gain_free)(g);
+      return -1;
+    }
+  }
+  return 0;
+}
+
+/* applyGainCurve's workspace form on one row (gain/curves.nim:380-416):
+ * the table on x[j0 ..< j1] (two entries interpolated in T off the grid),
+ * T(factorAt) elsewhere, a zero staying zero; a NaN start leaves the row. */
+static void SFX(gain_apply)(const SFX(gainplan) *g, T *x, int64_t n, double start) {
+  if (n <= 0 || isnan(start)) return;
+  const or_gain_span sp = or_gain_span_of(g->len, g->tmin, g->dt, n, start);
+  for (int64_t j = 0; j < n; j++) {
+    if (j >= sp.j0 && j < sp.j1) continue;
+    const T fac = (T)or_gain_factor_at(&g->curve, start + (double)j * g->dt);
+    if (x[j] != 0) x[j] = x[j] * fac;
+  }
+  if (sp.j1 > sp.j0) {
+    const T *tab = g->table + sp.s;
+    if (sp.on_grid) {
+      for (int64_t j = sp.j0; j < sp.j1; j++) x[j] = x[j] * tab[j];
+    } else {
+      const T f = (T)sp.f;
+      for (int64_t j = sp.j0; j < sp.j1; j++) {
+        const T a = tab[j];
+        x[j] = x[j] * (a + f * (tab[j + 1] - a));
+      }
+    }
+  }
+}
+
+int SFX(or_gain_curve_apply)(const or_op *op, double dt_ms, int64_t samples, T *x, int64_t n,
+                             double start_ms, int64_t *table_len, char *err, int errlen) {
+  SFX(gainplan) g;
+  char msg[256];
+  if (SFX(gain_init)(&g, op, dt_ms, samples, msg, (int)sizeof msg)) return or_seterr(err, errlen, msg);
+  if (table_len) *table_len = g.len;
+  if (n > samples) {
+    SFX(gain_free)(&g);
+    return or_seterr(err, errlen, "a trace longer than the plan's samples");
+  }
+  SFX(gain_apply)(&g, x, n, start_ms);
+  SFX(gain_free)(&g);
+  return 0;
+}
+
 typedef struct SFX(step) {
   int kind;
   double factor, lo, hi;
   or_op mute;          /* a mute step: its op (curve and taper) */
+  SFX(gainplan) gain;
   SFX(filterplan) filter;
   SFX(agcplan) agc;
   int stat;
@@ -773,6 +862,7 @@ static void SFX(plan_free)(SFX(plan) *p) {
   for (int i = 0; i < p->n_steps; i++) {
     if (p->steps[i].kind == ST_FILTER) SFX(filter_free)(&p->steps[i].filter);
     if (p->steps[i].kind == ST_AGC) SFX(agc_free)(&p->steps[i].agc);
+    if (p->steps[i].kind == ST_GAIN_CURVE) SFX(gain_free)(&p->steps[i].gain);
   }
   p->n_steps = 0;
 }
@@ -832,6 +922,11 @@ static int SFX(plan_compile)(SFX(plan) *P, const or_op *ops, int32_t n_ops, doub
         s->kind = op->scope == 1 ? ST_NORM_PANEL : ST_NORM_TRACE;
         s->stat = op->stat; s->target = op->target; s->percentile = op->percentile;
         break;
+      case 12:
+        /* gainCurveOp: tabulated to the end of the record (pipeline.nim:440-442) */
+        if (SFX(gain_init)(&s->gain, op, dt, P->record, msg, (int)sizeof msg)) { bad = 1; break; }
+        s->kind = ST_GAIN_CURVE;
+        break;
       default:
         bad = 1; snprintf(msg, sizeof msg, "not part of the panel hot path");
     }
@@ -884,6 +979,11 @@ static void SFX(run_step)(const SFX(step) *s, T *x, int64_t n, void *scratch, co
       else SFX(or_mute_bottom)(x, n, col, taper_cols);
       break;
     }
+    case ST_GAIN_CURVE:
+      /* sample 0 of the row is at sampleTimeMs(r, 0) = t0Ms + firstSample·dt
+       * (core/types.nim:237-240) */
+      SFX(gain_apply)(&s->gain, x, n, (key ? key[0] : 0.0) + (double)first * dt);
+      break;
     case ST_DEMEAN: SFX(demean)(x, n); break;
     case ST_DETREND: SFX(detrend)(x, n); break;
     case ST_POLARITY: SFX(polarity)(x, n); break;

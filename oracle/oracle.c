@@ -25,7 +25,7 @@
 
 /* Step kinds of a compiled plan (pipeline.nim:180-183), the subset restated. */
 enum { ST_DEMEAN, ST_DETREND, ST_SCALE, ST_CLIP, ST_POLARITY, ST_FILTER, ST_AGC,
-       ST_NORM_TRACE, ST_NORM_PANEL, ST_MUTE_TOP, ST_MUTE_BOTTOM };
+       ST_NORM_TRACE, ST_NORM_PANEL, ST_MUTE_TOP, ST_MUTE_BOTTOM, ST_GAIN_CURVE };
 
 static int or_seterr(char *err, int errlen, const char *msg) {
   if (err && errlen > 0) {
@@ -106,6 +106,94 @@ static int or_mute_check(const or_op *op, char *msg, int msglen) {
     if (!isfinite(op->mute_times[i])) { snprintf(msg, msglen, "mute table times must be finite"); return -1; }
   }
   return 0;
+}
+
+/* ---------------------------------------------------------- gain curves -- */
+
+/* gainDbAt (gain/curves.nim:134-153): linear in dB between knots and flat
+ * outside them; 20·power·log10(max(t, minMs)/refMs); dbPerS·t/1000.  NaN for
+ * a NaN time or a knot curve without knots. */
+double or_gain_db_at(const or_op *c, double t) {
+  if (isnan(t)) return NAN;
+  if (c->gain_kind == 1) return 20 * c->gain_power * log10(fmax(t, c->gain_min_ms) / c->gain_ref_ms);
+  if (c->gain_kind == 2) return c->gain_db_per_s * t / 1000;
+  if (c->gain_knots < 1) return NAN;
+  const double *kt = c->gain_knot_ms, *kd = c->gain_knot_db;
+  const int last = c->gain_knots - 1;
+  if (t <= kt[0]) return kd[0];
+  if (t >= kt[last]) return kd[last];
+  int lo = 0, hi = last;  /* kt[lo] < t < kt[hi] */
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (kt[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const double f = (t - kt[lo]) / (kt[hi] - kt[lo]);
+  return kd[lo] + f * (kd[hi] - kd[lo]);
+}
+
+/* factorAt (gain/curves.nim:154-160): 10^(dB/20), and (max(t, minMs)/refMs)^power
+ * directly for a power curve. */
+double or_gain_factor_at(const or_op *c, double t) {
+  if (isnan(t)) return NAN;
+  if (c->gain_kind == 1) return pow(fmax(t, c->gain_min_ms) / c->gain_ref_ms, c->gain_power);
+  return pow(10.0, or_gain_db_at(c, t) / 20);
+}
+
+/* The curve checks of knotGain / powerGain / exponentialGain
+ * (gain/curves.nim:100-132) and initGainCurvePlan's "no knots"
+ * (gain/curves.nim:225-227). */
+static int or_gain_check(const or_op *op, char *msg, int msglen) {
+  if (op->gain_kind == 1) {
+    if (!isfinite(op->gain_power)) { snprintf(msg, msglen, "a power must be finite, got %g", op->gain_power); return -1; }
+    if (!or_finite_positive(op->gain_ref_ms)) { snprintf(msg, msglen, "refMs must be finite and positive, got %g", op->gain_ref_ms); return -1; }
+    if (!or_finite_positive(op->gain_min_ms)) { snprintf(msg, msglen, "minMs must be finite and positive, got %g", op->gain_min_ms); return -1; }
+    return 0;
+  }
+  if (op->gain_kind == 2) {
+    if (!isfinite(op->gain_db_per_s)) { snprintf(msg, msglen, "dbPerS must be finite, got %g", op->gain_db_per_s); return -1; }
+    return 0;
+  }
+  if (op->gain_kind != 0) { snprintf(msg, msglen, "unknown gain curve %d", op->gain_kind); return -1; }
+  if (op->gain_knots < 1 || !op->gain_knot_ms || !op->gain_knot_db) {
+    snprintf(msg, msglen, "the curve has no knots: build it with knotGain, powerGain or exponentialGain");
+    return -1;
+  }
+  for (int i = 0; i < op->gain_knots; i++) {
+    if (!isfinite(op->gain_knot_ms[i]) || !isfinite(op->gain_knot_db[i])) { snprintf(msg, msglen, "knot %d is not finite", i); return -1; }
+    if (i > 0 && !(op->gain_knot_ms[i] > op->gain_knot_ms[i - 1])) {
+      snprintf(msg, msglen, "knot times must strictly increase: %g ms is followed by %g ms",
+               op->gain_knot_ms[i - 1], op->gain_knot_ms[i]);
+      return -1;
+    }
+  }
+  return 0;
+}
+
+/* spanOf (gain/curves.nim:340-355): where a row of n samples whose sample 0
+ * is at `start` lies on a table of len factors from tmin every dt.
+ * x[j0 ..< j1] takes table entry s + j (interpolated by f unless on_grid);
+ * the rest of the row is off the table. */
+typedef struct or_gain_span {
+  int64_t s, j0, j1;
+  double f;
+  int on_grid;
+} or_gain_span;
+
+static int64_t or_clamp64(int64_t x, int64_t a, int64_t b) { return x < a ? a : (x > b ? b : x); }
+
+static or_gain_span or_gain_span_of(int64_t len, double tmin, double dt, int64_t n, double start) {
+  or_gain_span sp = {0, n, n, 0.0, 0};
+  const double u = (start - tmin) / dt; /* the table position of x[0] */
+  if (!isfinite(u) || fabs(u) > 1e15) return sp;
+  const double r = round(u);
+  sp.on_grid = fabs(u - r) <= 1e-9 * fmax(1.0, fabs(u));
+  sp.s = (int64_t)(sp.on_grid ? r : floor(u));
+  sp.f = u - (double)sp.s;
+  const int64_t last = len - (sp.on_grid ? 1 : 2); /* the last j with s + j on the table */
+  sp.j0 = or_clamp64(-sp.s, 0, n);
+  sp.j1 = or_clamp64(last - sp.s + 1, sp.j0, n);
+  return sp;
 }
 
 /* ------------------------------------------------------------- lengths -- */

@@ -50,7 +50,8 @@ typedef struct or_spec {
 
 /* Op (pipeline.nim:158-174), as plain data.  kind follows OpKind
  * (pipeline.nim:92-108): 0 demean, 1 detrend, 2 scale, 3 gainDb, 4 clip,
- * 5 polarity, 6 muteTop, 7 muteBottom, 8 filter, 10 agc, 11 normalize.
+ * 5 polarity, 6 muteTop, 7 muteBottom, 8 filter, 10 agc, 11 normalize,
+ * 12 gainCurve.
  * Other kinds are refused. */
 typedef struct or_op {
   int32_t kind;
@@ -77,6 +78,19 @@ typedef struct or_op {
   double taper_ms;
   double mute_offsets[OR_MAX_KNOTS];
   double mute_times[OR_MAX_KNOTS];
+  /* gainCurve (gain/curves.nim:61-153, pipeline.nim:318-322): gain_kind 0
+   * knotGain (gain_knots knots, times gain_knot_ms strictly increasing, gains
+   * gain_knot_db), 1 powerGain (gain_power, gain_ref_ms, gain_min_ms), 2
+   * exponentialGain (gain_db_per_s); tabulated over gain_tmin_ms ..
+   * gain_tmax_ms (NaN: the end of the record).  The knot arrays are read
+   * while the plan is built and used. */
+  int32_t gain_kind;
+  int32_t gain_knots;
+  const double *gain_knot_ms;
+  const double *gain_knot_db;
+  double gain_power, gain_ref_ms, gain_min_ms;
+  double gain_db_per_s;
+  double gain_tmin_ms, gain_tmax_ms;
 } or_op;
 
 /* ------------------------------------------------------------- lengths -- */
@@ -152,6 +166,19 @@ int or_pipeline_apply_keys_f64(const or_op *ops, int32_t n_ops, double dt_ms, in
                                const double *keys, int64_t first_sample, char *err, int errlen);
 /* muteTimeMs (core/mute.nim:100-118) of a mute op at `offset`. */
 double or_mute_time_ms(const or_op *op, double offset);
+/* gainDbAt / factorAt (gain/curves.nim:140-160) of a gainCurve op's curve. */
+double or_gain_db_at(const or_op *op, double t_ms);
+double or_gain_factor_at(const or_op *op, double t_ms);
+/* A gainCurve op's plan (initGainCurvePlan, gain/curves.nim:214-249) for
+ * traces of `samples` samples at dt_ms, applied to one trace x[0 ..< n] whose
+ * sample 0 is at start_ms, as a pipeline step applies it (the workspace form,
+ * gain/curves.nim:380-416: off the table the factor is rounded to the sample
+ * type first; a zero stays zero).  table_len (may be NULL) receives the
+ * table's length.  0 ok, -1 invalid (message in err). */
+int or_gain_curve_apply_f32(const or_op *op, double dt_ms, int64_t samples, float *x, int64_t n,
+                            double start_ms, int64_t *table_len, char *err, int errlen);
+int or_gain_curve_apply_f64(const or_op *op, double dt_ms, int64_t samples, double *x, int64_t n,
+                            double start_ms, int64_t *table_len, char *err, int errlen);
 /* muteTop / muteBottom of one row in column units (core/mute.nim:136-180). */
 void or_mute_top_f32(float *x, int64_t n, double start_col, double taper_cols);
 void or_mute_top_f64(double *x, int64_t n, double start_col, double taper_cols);

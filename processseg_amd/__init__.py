@@ -2,7 +2,7 @@
 
 The panel path (batched mixed-radix real FFT, zero-phase FFT filters with the
 reference's edge handling, demean, AGC, normalization and vibroseis
-correlation, and the mutes) runs in hand-written gfx950 kernels behind the C ABI of
+correlation, the mutes and time-variant gain curves) runs in hand-written gfx950 kernels behind the C ABI of
 include/pseg.h (libpseg.so).  This package mirrors the reference's Nim API
 over that ABI: initPipeline / compile / PipelinePlan.apply and
 initCorrelationPlan / correlateInto, plus PipelineBatch (many records with
@@ -13,6 +13,8 @@ from ._lib import HipError, UnsupportedOp, WorkspaceError, build, lib  # noqa: F
 from .pipeline import (  # noqa: F401
     ChunkMeta, Corners, FilterSpec, Inf, MuteCurve, NaN, Op, Pipeline, PipelinePlan, agcMeanAbs,
     agcOp, lmoMute, mkLmo, mkTable, mkTime, muteBottomOp, muteTopOp, tableMute, timeMute,
+    GainCurve, GainKnot, exponentialGain, gainCurveOp, gcExponential, gcKnots, gcPower,
+    heldGain, knotGain, powerGain,
     agcRms, bandPass, bandPassOp, butterBandPass, butterHighCut, butterLowCut, clipOp,
     decodeWindow, demeanOp, detrendOp, filterOp, float32, float64, gainDbOp, highCut, iirOp,
     initPipeline, lowCut, nextFastLen, normalizeOp, normMax, normPercentile, normRms, perPanel,

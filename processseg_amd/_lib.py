@@ -37,7 +37,12 @@ class OpC(C.Structure):
                 ("mute_kind", C.c_int32), ("mute_knots", C.c_int32),
                 ("mute_t_ms", C.c_double), ("mute_ms_per_dist", C.c_double),
                 ("mute_pad_ms", C.c_double), ("taper_ms", C.c_double),
-                ("mute_offsets", C.c_double * 32), ("mute_times", C.c_double * 32)]
+                ("mute_offsets", C.c_double * 32), ("mute_times", C.c_double * 32),
+                ("gain_kind", C.c_int32), ("gain_knots", C.c_int32),
+                ("gain_knot_ms", C.POINTER(C.c_double)), ("gain_knot_db", C.POINTER(C.c_double)),
+                ("gain_power", C.c_double), ("gain_ref_ms", C.c_double),
+                ("gain_min_ms", C.c_double), ("gain_db_per_s", C.c_double),
+                ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double)]
 
 
 class PlanInfoC(C.Structure):

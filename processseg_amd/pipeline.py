@@ -10,9 +10,10 @@ plan.apply(samples, meta)                             # in place
 
 Panels are torch tensors on the GPU ([rows, cols], C-contiguous, float32 or
 float64).  A ``ChunkMeta`` holds the per-row facts the kernels read (liveness
-and live length), flattened once into a device int32 array.  Ops outside the
-GPU path (mutes, IIR, gain curves, calibration) raise ``UnsupportedOp`` at
-compile: keep the CPU plan for such a flow.
+and live length, flattened once into a device int32 array; t0Ms and offset,
+which mutes and gain curves read, into a float64 pair per row).  Ops outside
+the GPU path (IIR, calibration) raise ``UnsupportedOp`` at compile: keep the
+CPU plan for such a flow.
 """
 from __future__ import annotations
 
@@ -263,6 +264,144 @@ def lmoMute(velocity: float, t0Ms: float = 0.0, padMs: float = 0.0) -> MuteCurve
     return MuteCurve(mkLmo, msPerDist=1000 / float(velocity), t0Ms=float(t0Ms), padMs=float(padMs))
 
 
+# GainCurveKind (gain/curves.nim:68-72)
+gcKnots, gcPower, gcExponential = 0, 1, 2
+
+
+@dataclasses.dataclass(frozen=True)
+class GainKnot:
+    """A point of a knotGain curve (gain/curves.nim:63-66)."""
+    tMs: float
+    dB: float
+
+
+@dataclasses.dataclass(frozen=True)
+class GainCurve:
+    """A gain against time (gain/curves.nim:74-84): kind gcKnots (linear in
+    dB between knots, flat outside them), gcPower ((max(t, minMs)/refMs)^power)
+    or gcExponential (dbPerS dB per second).  Build with knotGain, powerGain
+    or exponentialGain, which validate it; ``GainCurve()`` is the unbuilt
+    curve (no knots) that a plan refuses."""
+    kind: int = gcKnots
+    knots: tuple = ()           # ((tMs, dB), ...)
+    power: float = 0.0
+    refMs: float = 1000.0
+    minMs: float = 1.0
+    dbPerS: float = 0.0
+
+    def gainDbAt(self, tMs: float) -> float:
+        """gainDbAt (gain/curves.nim:134-153)."""
+        if math.isnan(tMs):
+            return math.nan
+        if self.kind == gcPower:
+            return 20 * self.power * math.log10(max(tMs, self.minMs) / self.refMs)
+        if self.kind == gcExponential:
+            return self.dbPerS * tMs / 1000
+        k = self.knots
+        if not k:
+            return math.nan
+        if tMs <= k[0][0]:
+            return k[0][1]
+        if tMs >= k[-1][0]:
+            return k[-1][1]
+        lo, hi = 0, len(k) - 1
+        while hi - lo > 1:
+            mid = (lo + hi) >> 1
+            if k[mid][0] <= tMs:
+                lo = mid
+            else:
+                hi = mid
+        f = (tMs - k[lo][0]) / (k[hi][0] - k[lo][0])
+        return k[lo][1] + f * (k[hi][1] - k[lo][1])
+
+    def factorAt(self, tMs: float) -> float:
+        """factorAt (gain/curves.nim:154-160): 10^(gainDbAt/20), computed
+        directly for a power curve."""
+        if math.isnan(tMs):
+            return math.nan
+        if self.kind == gcPower:
+            return math.pow(max(tMs, self.minMs) / self.refMs, self.power)
+        db = self.gainDbAt(tMs)
+        return math.nan if math.isnan(db) else math.pow(10.0, db / 20)
+
+
+def knotGain(knots) -> GainCurve:
+    """knotGain (gain/curves.nim:100-112): at least one knot, every value
+    finite, times strictly increasing.  Knots are GainKnot or (tMs, dB)."""
+    ks = tuple((float(k.tMs), float(k.dB)) if isinstance(k, GainKnot) else
+               (float(k[0]), float(k[1])) for k in knots)
+    if not ks:
+        raise ValueError("a knot gain needs at least one knot")
+    for i, (t, d) in enumerate(ks):
+        if not (math.isfinite(t) and math.isfinite(d)):
+            raise ValueError(f"knot {i} is not finite: ({t}, {d})")
+        if i > 0 and not t > ks[i - 1][0]:
+            raise ValueError(f"knot times must strictly increase: {ks[i - 1][0]} ms is "
+                             f"followed by {t} ms")
+    return GainCurve(gcKnots, knots=ks)
+
+
+def powerGain(power: float, refMs: float = 1000.0, minMs: float = 1.0) -> GainCurve:
+    """powerGain (gain/curves.nim:114-125)."""
+    if not _finite(power):
+        raise ValueError(f"a power must be finite, got {power}")
+    if not (_finite(refMs) and refMs > 0):
+        raise ValueError(f"refMs must be finite and positive, got {refMs}")
+    if not (_finite(minMs) and minMs > 0):
+        raise ValueError(f"minMs must be finite and positive, got {minMs}")
+    return GainCurve(gcPower, power=float(power), refMs=float(refMs), minMs=float(minMs))
+
+
+def exponentialGain(dbPerS: float) -> GainCurve:
+    """exponentialGain (gain/curves.nim:127-131)."""
+    if not _finite(dbPerS):
+        raise ValueError(f"dbPerS must be finite, got {dbPerS}")
+    return GainCurve(gcExponential, dbPerS=float(dbPerS))
+
+
+def heldGain(curve: GainCurve, untilMs: float, step: float = 1.02) -> GainCurve:
+    """heldGain (gain/curves.nim:162-212): the curve up to untilMs and its
+    gain at untilMs after it, as knots: an exponential is two knots from 0, a
+    power curve knots from minMs every `step` in time plus one at untilMs, a
+    knot curve its knots before untilMs plus the gain there."""
+    if not (_finite(untilMs) and untilMs > 0):
+        raise ValueError(f"untilMs must be finite and > 0, got {untilMs}")
+    if not (_finite(step) and step > 1):
+        raise ValueError(f"step must be finite and > 1, got {step}")
+    if curve.kind == gcKnots and not curve.knots:
+        raise ValueError("an unbuilt curve cannot be held")
+    out = []
+    if curve.kind == gcExponential:
+        out = [(0.0, curve.gainDbAt(0)), (float(untilMs), curve.gainDbAt(untilMs))]
+    elif curve.kind == gcPower:
+        start = curve.minMs
+        if untilMs <= start:
+            out.append((float(untilMs), curve.gainDbAt(untilMs)))
+        else:
+            count = int(math.log(untilMs / start) / math.log(step)) + 2
+            if count > 100_000:
+                raise ValueError(f"holding this power curve would take {count} knots; widen "
+                                 "step or shorten untilMs")
+            t = start
+            while t < untilMs:
+                out.append((t, curve.gainDbAt(t)))
+                nxt = t * step
+                if not nxt > t:
+                    break
+                t = nxt
+            if not out or out[-1][0] < untilMs:
+                out.append((float(untilMs), curve.gainDbAt(untilMs)))
+    else:
+        for k in curve.knots:
+            if k[0] < untilMs:
+                out.append(k)
+            else:
+                break
+        if not out or out[-1][0] < untilMs:
+            out.append((float(untilMs), curve.gainDbAt(untilMs)))
+    return knotGain(out)
+
+
 @dataclasses.dataclass
 class Op:
     """One operation as plain data (pipeline.nim:158-174).  Parameters are
@@ -282,6 +421,9 @@ class Op:
     percentile: float = 99.0
     mute: Optional[MuteCurve] = None  # muteTop, muteBottom
     taper_ms: float = 0.0
+    curve: Optional[GainCurve] = None  # gainCurve, tabulated over t_min_ms .. t_max_ms
+    t_min_ms: float = 0.0
+    t_max_ms: float = math.nan
 
     def to_c(self) -> _lib.OpC:
         o = _lib.OpC()
@@ -301,7 +443,25 @@ class Op:
             o.spec = spec_c(self.spec)
         if self.mute is not None:
             mute_fields(o, self.mute, self.taper_ms)
+        if self.curve is not None:
+            gain_fields(o, self.curve, self.t_min_ms, self.t_max_ms)
         return o
+
+
+def gain_fields(o, c: GainCurve, t_min_ms: float, t_max_ms: float):
+    """Fill the gain curve fields of a C op; the struct keeps the knot
+    arrays it points to alive."""
+    o.gain_kind = c.kind
+    o.gain_knots = len(c.knots)
+    if c.knots:
+        o.gain_knot_ms = (C.c_double * len(c.knots))(*[k[0] for k in c.knots])
+        o.gain_knot_db = (C.c_double * len(c.knots))(*[k[1] for k in c.knots])
+    o.gain_power = c.power
+    o.gain_ref_ms = c.refMs
+    o.gain_min_ms = c.minMs
+    o.gain_db_per_s = c.dbPerS
+    o.gain_tmin_ms = float(t_min_ms)
+    o.gain_tmax_ms = float(t_max_ms)
 
 
 def mute_fields(o, c: MuteCurve, taper_ms: float):
@@ -351,6 +511,12 @@ def muteTopOp(curve: MuteCurve, taperMs: float = 0.0) -> Op:
 def muteBottomOp(curve: MuteCurve, taperMs: float = 0.0) -> Op:
     """muteBottomOp (pipeline.nim:287-289): a bottom mute with a taper after it."""
     return Op(opMuteBottom, mute=curve, taper_ms=float(taperMs))
+
+
+def gainCurveOp(curve: GainCurve, tMinMs: float = 0.0, tMaxMs: float = NaN) -> Op:
+    """gainCurveOp (pipeline.nim:318-322): a time-variant gain, tabulated over
+    tMinMs .. tMaxMs (NaN: to the end of the record, ``recordSamples``)."""
+    return Op(opGainCurve, curve=curve, t_min_ms=float(tMinMs), t_max_ms=float(tMaxMs))
 
 
 def filterOp(spec: FilterSpec, trend: int = trLinear) -> Op:
