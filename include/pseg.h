@@ -211,11 +211,13 @@ pseg_status pseg_plan_apply(const pseg_plan *plan, const void *src, void *dst, i
                             int32_t cols, const int32_t *row_len, void *ws, size_t ws_bytes,
                             void *stream);
 
-/* The same with the rows' trace keys, which mutes read (muteCol,
- * core/mute.nim:189-194: the mute time's column (t - t0Ms) / dtMs -
- * firstSample): row_keys holds (t0Ms, offset) per row as float64 in device
- * memory (ChunkMeta.keys), or is NULL (t0Ms 0, offset NaN: a timeMute then
- * applies, an offset-dependent mute leaves the rows alone); first_sample is
+/* The same with the rows' trace keys, which mutes and gain curves read
+ * (muteCol, core/mute.nim:189-194: the mute time's column (t - t0Ms) / dtMs -
+ * firstSample; sampleTimeMs, core/types.nim:237-240: sample j of a row at
+ * t0Ms + (firstSample + j)·dtMs): row_keys holds (t0Ms, offset) per row as
+ * float64 in device memory (ChunkMeta.keys), or is NULL (t0Ms 0, offset NaN:
+ * a timeMute then applies, an offset-dependent mute leaves the rows alone); a
+ * NaN t0Ms leaves the row to mutes and gain curves; first_sample is
  * ChunkMeta.firstSample.  pseg_plan_apply is this with NULL and 0. */
 pseg_status pseg_plan_apply_keys(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
                                  int32_t cols, const int32_t *row_len, const double *row_keys,

@@ -1111,6 +1111,102 @@ __device__ __forceinline__ void mute_step(const DevMute &M, bool top, T (&v)[PER
   }
 }
 
+// pow as a call of its own: inlined into the row kernel, its polynomial
+// constants are hoisted out of the loop over the steps into VGPRs and spill.
+__device__ __attribute__((noinline)) double pow_call(double b, double e) { return pow(b, e); }
+
+// gainDbAt / factorAt (gain/curves.nim:134-160) in float64, without
+// contraction, as the curve's own arithmetic: linear in dB between knots
+// (bisection) and flat outside them, 10^(dB/20); (max(t, minMs)/refMs)^power
+// directly for a power curve.
+__device__ __forceinline__ double gain_factor_at(const DevGain &G, double t) {
+#pragma clang fp contract(off)
+  if (isnan(t)) return t;
+  const int kind = uni(G.kind);
+  if (kind == 1) return pow_call(fmax(t, uni(G.min_ms)) / uni(G.ref_ms), uni(G.power));
+  double db;
+  if (kind == 2) {
+    db = uni(G.db_per_s) * t / 1000;
+  } else {
+    gptr<double> kt = as_global<double>(uni_ptr(G.knot_ms));
+    gptr<double> kd = as_global<double>(uni_ptr(G.knot_db));
+    const int last = uni(G.knots) - 1;
+    if (t <= kt[0]) {
+      db = kd[0];
+    } else if (t >= kt[last]) {
+      db = kd[last];
+    } else {
+      int lo = 0, hi = last;  // kt[lo] < t < kt[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (kt[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      const double f = (t - kt[lo]) / (kt[hi] - kt[lo]);
+      db = kd[lo] + f * (kd[hi] - kd[lo]);
+    }
+  }
+  return pow_call(10.0, db / 20);
+}
+
+// applyGainCurve's workspace form on the row's live part (gain/curves.nim:
+// 340-355, 380-416), sample 0 at `start` ms: where the row lies on the table
+// (spanOf, computed once per row in SGPRs), its entries (two interpolated in T
+// off the grid); elsewhere T(factorAt(t)), a zero staying zero; a NaN start
+// leaves the row alone.  The row goes through LDS: each lane stores its
+// samples, scales them there in a loop that is not unrolled (one copy of the
+// curve's pow and bisection, and no register of the row live across it, where
+// it would spill), and loads them back; a lane touches only its own samples.
+template <typename T, int NTH, int PER>
+__device__ __forceinline__ void gain_step(const DevGain &G, T (&v)[PER], int n, double start,
+                                          char *lds) {
+#pragma clang fp contract(off)
+  if (isnan(start)) return;
+  const int tid_ = opaque_tid();
+  const double dt = uni(G.dt_ms);
+  const double u = (start - uni(G.t_min_ms)) / dt;  // the table position of sample 0
+  int64_t s = 0;
+  int j0 = n, j1 = n;
+  bool on_grid = false;
+  double f = 0;
+  if (isfinite(u) && fabs(u) <= 1e15) {
+    const double r = round(u);
+    on_grid = fabs(u - r) <= 1e-9 * fmax(1.0, fabs(u));
+    s = (int64_t)(on_grid ? r : floor(u));
+    f = u - (double)s;
+    const int64_t last = (int64_t)uni((int)G.len) - (on_grid ? 1 : 2);  // the last entry s + j may take
+    const int64_t a = min(max(-s, (int64_t)0), (int64_t)n);
+    const int64_t b = min(max(last - s + 1, a), (int64_t)n);
+    j0 = (int)a;
+    j1 = (int)b;
+  }
+  gptr<T> tab = as_global<T>(uni_ptr(G.table));
+  const T ft = (T)f;
+  T *x = (T *)lds;
+  __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int j = tid_ + NTH * k;
+    if (j < n) x[j] = v[k];
+  }
+#pragma unroll 1
+  for (int j = tid_; j < n; j += NTH) {
+    const T xv = x[j];
+    if (j >= j0 && j < j1) {
+      const T a = tab[s + j];
+      x[j] = xv * (on_grid ? a : a + ft * (tab[s + j + 1] - a));
+    } else if (xv != (T)0) {
+      x[j] = xv * (T)gain_factor_at(G, start + (double)j * dt);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int j = tid_ + NTH * k;
+    v[k] = j < n ? x[j] : (T)0;  // slots past the live end hold +0
+  }
+  __syncthreads();  // before the next step writes LDS
+}
+
 // Per-trace normalization (gain/gain.nim:198-219).
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds,

@@ -470,6 +470,36 @@ void patch_fft(DevFft &f, char *base) {
   patch(f.split, base);
 }
 
+// gainDbAt / factorAt (gain/curves.nim:134-160) of a gainCurve op, for its
+// table: linear in dB between knots (bisection), flat outside them,
+// 10^(dB/20); (max(t, minMs)/refMs)^power directly for a power curve.
+double gain_factor_host(const pseg_op &op, double t) {
+  if (std::isnan(t)) return t;
+  if (op.gain_kind == 1) return std::pow(std::fmax(t, op.gain_min_ms) / op.gain_ref_ms, op.gain_power);
+  double db;
+  if (op.gain_kind == 2) {
+    db = op.gain_db_per_s * t / 1000;
+  } else {
+    const double *kt = op.gain_knot_ms, *kd = op.gain_knot_db;
+    const int last = op.gain_knots - 1;
+    if (t <= kt[0]) {
+      db = kd[0];
+    } else if (t >= kt[last]) {
+      db = kd[last];
+    } else {
+      int lo = 0, hi = last;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (kt[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      const double f = (t - kt[lo]) / (kt[hi] - kt[lo]);
+      db = kd[lo] + f * (kd[hi] - kd[lo]);
+    }
+  }
+  return std::pow(10.0, db / 20);
+}
+
 const char *op_name(int kind) {
   static const char *names[] = {"opDemean", "opDetrend", "opScale", "opGainDb", "opClip",
                                 "opPolarity", "opMuteTop", "opMuteBottom", "opFilter",
@@ -659,8 +689,68 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       lds = 0;
       break;
     }
+    case PSEG_OP_GAIN_CURVE: {
+      // gainCurveOp (pipeline.nim:440-442): initGainCurvePlan (gain/curves.
+      // nim:214-249) tabulated to the end of the record, with the curve
+      // checks of knotGain / powerGain / exponentialGain (gain/curves.nim:
+      // 100-131)
+      DevGain &G = st.gain;
+      G.kind = op.gain_kind;
+      if (op.gain_kind == 1) {
+        if (!std::isfinite(op.gain_power)) invalid("a power must be finite, got %g", op.gain_power);
+        if (!finite_positive(op.gain_ref_ms)) invalid("refMs must be finite and positive, got %g", op.gain_ref_ms);
+        if (!finite_positive(op.gain_min_ms)) invalid("minMs must be finite and positive, got %g", op.gain_min_ms);
+      } else if (op.gain_kind == 2) {
+        if (!std::isfinite(op.gain_db_per_s)) invalid("dbPerS must be finite, got %g", op.gain_db_per_s);
+      } else if (op.gain_kind == 0) {
+        if (op.gain_knots < 1 || !op.gain_knot_ms || !op.gain_knot_db)
+          invalid("the curve has no knots: build it with knotGain, powerGain or exponentialGain");
+        if (op.gain_knots > PSEG_MAX_GAIN_KNOTS)
+          invalid("a knot gain of %d knots (at most %d); UNSUPPORTED", op.gain_knots, PSEG_MAX_GAIN_KNOTS);
+        for (int32_t k = 0; k < op.gain_knots; k++) {
+          if (!std::isfinite(op.gain_knot_ms[k]) || !std::isfinite(op.gain_knot_db[k]))
+            invalid("knot %d is not finite: (%g ms, %g dB)", k, op.gain_knot_ms[k], op.gain_knot_db[k]);
+          if (k > 0 && !(op.gain_knot_ms[k] > op.gain_knot_ms[k - 1]))
+            invalid("knot times must strictly increase: %g ms is followed by %g ms",
+                    op.gain_knot_ms[k - 1], op.gain_knot_ms[k]);
+        }
+      } else {
+        invalid("unknown gain curve %d", op.gain_kind);
+      }
+      const double lo = op.gain_tmin_ms;
+      const double hi = std::isnan(op.gain_tmax_ms) ? lo + (double)(P.record - 1) * dt : op.gain_tmax_ms;
+      if (!(std::isfinite(lo) && std::isfinite(hi)))
+        invalid("the table limits must be finite, got %g .. %g", lo, hi);
+      if (hi < lo) invalid("tMaxMs %g is before tMinMs %g", hi, lo);
+      const double span = std::floor((hi - lo) / dt + 1e-9);
+      if (span >= (double)(1 << 26)) invalid("a gain table of %g samples is too long", span);
+      G.len = (int64_t)span + 1;
+      G.power = op.gain_power;
+      G.ref_ms = op.gain_ref_ms;
+      G.min_ms = op.gain_min_ms;
+      G.db_per_s = op.gain_db_per_s;
+      G.t_min_ms = lo;
+      G.dt_ms = dt;
+      std::vector<T> tab((size_t)G.len);
+      for (int64_t k = 0; k < G.len; k++) {
+        const double t = lo + (double)k * dt;
+        const double g = gain_factor_host(op, t);
+        tab[k] = (T)g;
+        // 0 · Inf would turn muted zeros into NaN
+        if (std::isinf((double)tab[k]) || std::isnan(g))
+          invalid("the gain at %g ms (%g) overflows %s", t, g, sizeof(T) == 4 ? "float32" : "float64");
+      }
+      set_off(G.table, blob.add_vec(tab));
+      if (op.gain_kind == 0) {
+        G.knots = op.gain_knots;
+        set_off(G.knot_ms, blob.add_vec(std::vector<double>(op.gain_knot_ms, op.gain_knot_ms + op.gain_knots)));
+        set_off(G.knot_db, blob.add_vec(std::vector<double>(op.gain_knot_db, op.gain_knot_db + op.gain_knots)));
+      }
+      st.kind = kGainCurve;
+      lds = (size_t)samples * sizeof(T);  // the row's factors
+      break;
+    }
     case PSEG_OP_IIR:
-    case PSEG_OP_GAIN_CURVE:
     case PSEG_OP_CALIBRATE:
     case PSEG_OP_UNCALIBRATE:
       throw Invalid{"UNSUPPORTED"};
@@ -681,6 +771,10 @@ void patch_step(DevStep &st, char *base) {
   } else if (st.kind == kMuteTop || st.kind == kMuteBottom) {
     patch(st.mute.offsets, base);
     patch(st.mute.times, base);
+  } else if (st.kind == kGainCurve) {
+    patch(st.gain.table, base);
+    patch(st.gain.knot_ms, base);
+    patch(st.gain.knot_db, base);
   }
 }
 

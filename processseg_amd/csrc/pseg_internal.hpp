@@ -24,6 +24,7 @@ enum StepKind : int32_t {
   kNormPanel = 8,
   kMuteTop = 9,
   kMuteBottom = 10,
+  kGainCurve = 11,
 };
 
 // Divide by a runtime constant with a multiply and a shift (Granlund-
@@ -106,6 +107,19 @@ struct DevMute {
   const double *times;
 };
 
+// A GainCurvePlan (gain/curves.nim:86-93): the curve, and its factors as T
+// at t_min_ms + k dt_ms for k < len.
+struct DevGain {
+  int32_t kind;       // 0 knots, 1 power, 2 exponential
+  int32_t knots;
+  int64_t len;
+  double power, ref_ms, min_ms;
+  double db_per_s;
+  double t_min_ms, dt_ms;
+  const void *table;
+  const double *knot_ms, *knot_db;
+};
+
 struct DevStep {
   int32_t kind;
   int32_t stat;       // normalize
@@ -116,6 +130,7 @@ struct DevStep {
   DevFilter filter;
   DevAgc agc;
   DevMute mute;
+  DevGain gain;
 };
 
 struct DevPlan {

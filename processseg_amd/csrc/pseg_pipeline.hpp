@@ -157,6 +157,15 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
             mute_step<T, NTH, PER>(st.mute, uni(st.kind) == kMuteTop, v, n, col);
           }
           break;
+        case kGainCurve:
+          if constexpr (FULL) {
+            // sampleTimeMs(r, 0) = t0Ms + firstSample·dt (core/types.nim:237-240)
+            const double *keys = (const double *)uni_ptr(J.keys);
+            const double t0 = keys ? uni(keys[2 * r]) : 0.0;
+            const double start = t0 + uni((double)J.col0) * uni(st.gain.dt_ms);
+            gain_step<T, NTH, PER>(st.gain, v, n, start, lds);
+          }
+          break;
         default: break;
       }
     }

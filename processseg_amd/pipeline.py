@@ -711,7 +711,7 @@ class PipelinePlan:
                            pad=info.step_pad[i], margin=info.step_margin[i],
                            reflect=bool(info.step_reflect[i])) for i in range(n)]
         # mutes read the rows' trace keys (t0Ms, offset) from the meta
-        self.readsKeys = any(op.kind in (opMuteTop, opMuteBottom) for op in flow.ops)
+        self.readsKeys = any(op.kind in (opMuteTop, opMuteBottom, opGainCurve) for op in flow.ops)
         self._ws = {}
 
     def __len__(self):
@@ -739,7 +739,8 @@ class PipelinePlan:
         """apply(src, dst, meta) (src -> dst) or apply(samples, meta) (in
         place), pipeline.nim:736-772.  `row_len` (a device int32 tensor) may
         replace `meta`; then `keys` ([rows, 2] float64 device tensor of
-        t0Ms, offset) and `first_sample` give what mutes read from a meta.
+        t0Ms, offset) and `first_sample` give what mutes and gain curves read
+        from a meta.
         Asynchronous on `stream` (default: torch's current)."""
         import torch
         if isinstance(dst, ChunkMeta) and meta is None:
