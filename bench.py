@@ -246,6 +246,37 @@ def muted_flow_leg(torch, P, dev, steps, warmup):
             "ms_per_step": round(t * 1000, 5), "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
 
 
+def gain_flow_leg(torch, P, dev, steps, warmup):
+    """The headline panel with a time-variant gain in the flow (the kernel
+    form with every step): demean > band-pass > spherical divergence
+    (powerGain 2, (t / 1 s)²) > AGC, the rows' t0 from the ChunkMeta (every
+    fourth row delayed 33.3 ms, off the table's grid); one Plan.apply per
+    step, 4 buffer pairs."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    t0 = np.where(np.arange(ROWS) % 4 == 3, 33.3, 0.0)
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live, t0Ms=t0)
+    flow = P.initPipeline([
+        P.demeanOp(), P.bandPassOp(P.Corners(5, 10, 70, 90)),
+        P.gainCurveOp(P.powerGain(2.0)), P.agcOp(250)])
+    plan = flow.compile(np.float32, DT, COLS)
+    srcs = [torch.from_numpy(x).to(dev) for _ in range(4)]
+    dsts = [torch.empty_like(srcs[0]) for _ in range(4)]
+    for i in range(warmup):
+        plan.apply(srcs[i % 4], dsts[i % 4], meta)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        plan.apply(srcs[i % 4], dsts[i % 4], meta)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / steps
+    samples = ROWS * COLS
+    return {"workload": "demean > band-pass 5-10-70-90 Hz > powerGain 2 (t^2) > AGC 250 ms, "
+                        "2000x3001 f32, t0 from the meta (a quarter of the rows off the grid)",
+            "value": round(samples / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5), "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -507,6 +538,10 @@ def main():
         log("muted flow leg")
         line["secondary"]["muted_flow"] = muted_flow_leg(torch, P, dev,
                                                          max(10, args.steps // 4), 3)
+        torch.cuda.empty_cache()
+        log("gain curve flow leg")
+        line["secondary"]["gain_flow"] = gain_flow_leg(torch, P, dev,
+                                                       max(10, args.steps // 4), 3)
         torch.cuda.empty_cache()
         log("float64 panel leg")
         line["secondary"]["f64_panel"] = f64_panel_leg(torch, P, flow, dev,
