@@ -415,6 +415,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true",
                     help="only the headline panel (no correlation leg)")
+    ap.add_argument("--legs", default="all",
+                    help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
+                         "gain_flow, f64_panel, correlation, uneven_batch, serving), or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
     ap.add_argument("--copy-probe", action="store_true",
@@ -530,34 +533,44 @@ def main():
         line["cpu_baseline"] = cpu
         line["accuracy"] = acc
     if not args.no_secondary:
-        log("panels as one batch")
-        line["secondary"] = {"panels_as_batch": panels_batch_leg(
-            torch, P, plan, srcs, dsts, row_len, max(10, args.steps // 10), 3)}
+        legs = set(args.legs.split(",")) if args.legs != "all" else None
+
+        def want(name):
+            return legs is None or name in legs
+
+        sec = line["secondary"] = {}
+        if want("panels_as_batch"):
+            log("panels as one batch")
+            sec["panels_as_batch"] = panels_batch_leg(
+                torch, P, plan, srcs, dsts, row_len, max(10, args.steps // 10), 3)
         del srcs, dsts
         torch.cuda.empty_cache()
-        log("muted flow leg")
-        line["secondary"]["muted_flow"] = muted_flow_leg(torch, P, dev,
-                                                         max(10, args.steps // 4), 3)
-        torch.cuda.empty_cache()
-        log("gain curve flow leg")
-        line["secondary"]["gain_flow"] = gain_flow_leg(torch, P, dev,
-                                                       max(10, args.steps // 4), 3)
-        torch.cuda.empty_cache()
-        log("float64 panel leg")
-        line["secondary"]["f64_panel"] = f64_panel_leg(torch, P, flow, dev,
-                                                       max(10, args.steps // 4), 3)
-        torch.cuda.empty_cache()
-        log("correlation leg")
-        line["secondary"]["correlation"] = correlation_leg(
-            torch, dev, max(10, args.steps // 4), 3, 4, not args.no_cpu)
-        torch.cuda.empty_cache()
-        log("uneven batch leg")
-        line["secondary"]["uneven_batch"] = batch_leg(torch, dev, max(5, args.steps // 20), 2)
-        torch.cuda.empty_cache()
-        sv = serve_leg(torch, dev, args.serve_seconds, 200.0)
-        line["secondary"]["serving"] = sv
-        line["p50_ms"] = sv["batched"]["warm"]["p50_ms"]
-        line["p99_ms"] = sv["batched"]["warm"]["p99_ms"]
+        if want("muted_flow"):
+            log("muted flow leg")
+            sec["muted_flow"] = muted_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("gain_flow"):
+            log("gain curve flow leg")
+            sec["gain_flow"] = gain_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("f64_panel"):
+            log("float64 panel leg")
+            sec["f64_panel"] = f64_panel_leg(torch, P, flow, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("correlation"):
+            log("correlation leg")
+            sec["correlation"] = correlation_leg(
+                torch, dev, max(10, args.steps // 4), 3, 4, not args.no_cpu)
+            torch.cuda.empty_cache()
+        if want("uneven_batch"):
+            log("uneven batch leg")
+            sec["uneven_batch"] = batch_leg(torch, dev, max(5, args.steps // 20), 2)
+            torch.cuda.empty_cache()
+        if want("serving"):
+            sv = serve_leg(torch, dev, args.serve_seconds, 200.0)
+            sec["serving"] = sv
+            line["p50_ms"] = sv["batched"]["warm"]["p50_ms"]
+            line["p99_ms"] = sv["batched"]["warm"]["p99_ms"]
     print(json.dumps(line), flush=True)
 
 
