@@ -44,7 +44,19 @@ class OrOp(C.Structure):
                 ("gain_knot_ms", C.POINTER(C.c_double)), ("gain_knot_db", C.POINTER(C.c_double)),
                 ("gain_power", C.c_double), ("gain_ref_ms", C.c_double),
                 ("gain_min_ms", C.c_double), ("gain_db_per_s", C.c_double),
-                ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double)]
+                ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double),
+                ("iir_phase", C.c_int32), ("reserved2", C.c_int32)]
+
+
+class OrBiquad(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("b0", "b1", "b2", "a1", "a2", "g")]
+
+
+class OrIir(C.Structure):
+    _fields_ = [("n_sections", C.c_int32), ("phase", C.c_int32), ("trend", C.c_int32),
+                ("reflect", C.c_int32), ("pad", C.c_int64), ("lead", C.c_int64),
+                ("tail", C.c_int64), ("samples", C.c_int64), ("dt_ms", C.c_double),
+                ("dc_gain", C.c_double), ("dc_slope", C.c_double), ("sec", OrBiquad * 32)]
 
 
 class OrFilterInfo(C.Structure):
@@ -93,6 +105,12 @@ def lib():
         L.or_is_fast_len.argtypes = [C.c_int64]
         L.or_mute_time_ms.argtypes = [C.POINTER(OrOp), C.c_double]
         L.or_mute_time_ms.restype = C.c_double
+        L.or_iir_init.argtypes = [C.POINTER(OrIir), C.POINTER(OrSpec), C.c_double, C.c_int64,
+                                  C.c_int32, C.c_int32, C.c_char_p, C.c_int]
+        L.or_iir_magnitude_at.argtypes = [C.POINTER(OrIir), C.c_double]
+        L.or_iir_magnitude_at.restype = C.c_double
+        L.or_iir_order.argtypes = [C.POINTER(OrIir)]
+        L.or_iir_order.restype = C.c_int32
         for fn in ("or_gain_db_at", "or_gain_factor_at"):
             getattr(L, fn).argtypes = [C.POINTER(OrOp), C.c_double]
             getattr(L, fn).restype = C.c_double
@@ -118,6 +136,7 @@ def lib():
                 C.POINTER(OrOp), C.c_int32, C.c_double, C.c_int64, C.c_int64, p, p,
                 C.c_int32, C.c_int32, i32p, C.c_void_p, C.c_int64] + E
             getattr(L, f"or_mute_top_{t}").argtypes = [p, C.c_int64, C.c_double, C.c_double]
+            getattr(L, f"or_iir_apply_{t}").argtypes = [C.POINTER(OrIir), p, C.c_int64]
             getattr(L, f"or_gain_curve_apply_{t}").argtypes = [
                 C.POINTER(OrOp), C.c_double, C.c_int64, p, C.c_int64, C.c_double,
                 C.POINTER(C.c_int64), C.c_char_p, C.c_int]
@@ -220,6 +239,7 @@ def op_struct(op) -> OrOp:
         for i, (a, b) in enumerate(zip(mute.offsets, mute.timesMs)):
             o.mute_offsets[i] = a
             o.mute_times[i] = b
+    o.iir_phase = int(getattr(op, "phase", 0))
     curve = getattr(op, "curve", None)
     if curve is not None:
         gain_fields(o, curve, getattr(op, "t_min_ms", 0.0), getattr(op, "t_max_ms", math.nan))
@@ -410,6 +430,36 @@ def pipeline_apply(ops, src: np.ndarray, row_len, dt_ms: float, samples: int | N
 def mute_time_ms(op, offset: float) -> float:
     """muteTimeMs of a mute op (core/mute.nim:100-118)."""
     return float(lib().or_mute_time_ms(C.byref(op_struct(op)), offset))
+
+
+class IirPlan:
+    """initIirPlan (filter/iir.nim:193-239) through the C restatement:
+    order, phase, requiredMargin, magnitudeAt and applyFilter on one trace
+    (a copy).  ValueError as the plan raises."""
+
+    def __init__(self, spec, dt_ms: float, samples: int, phase: int = 0, trend: int = 2):
+        self.p = OrIir()
+        e = _err()
+        _check(lib().or_iir_init(C.byref(self.p), C.byref(spec_struct(spec)), dt_ms, samples,
+                                 phase, trend, e, 512), e)
+
+    @property
+    def order(self) -> int:
+        return int(lib().or_iir_order(C.byref(self.p)))
+
+    @property
+    def requiredMargin(self) -> int:
+        return int(self.p.pad)
+
+    def magnitudeAt(self, f: float) -> float:
+        return float(lib().or_iir_magnitude_at(C.byref(self.p), f))
+
+    def filtered(self, x: np.ndarray) -> np.ndarray:
+        y = np.ascontiguousarray(x).copy()
+        if y.shape[0] > self.p.samples:
+            raise ValueError("a trace longer than the plan's samples")
+        getattr(lib(), f"or_iir_apply_{_sfx(y.dtype)}")(C.byref(self.p), y, y.shape[0])
+        return y
 
 
 def gain_db_at(curve, t_ms: float) -> float:

@@ -749,6 +749,38 @@ void SFX(or_mute_bottom)(T *x, int64_t n, double end_col, double taper_cols) {
 
 /* ------------------------------------------------------------- pipeline -- */
 
+/* ---------------------------------------------------------------- IIR -- */
+
+/* applyFilter on one trace (filter/iir.nim:363-391, 418-433): the trend
+ * removed, the pads (odd reflections before, and after for zero phase, or
+ * zeros after), the passes in float64, the trend's steady-state response
+ * G(1)·(a + b·i) − b·G'(1) added back.  buf: lead + n + tail doubles. */
+static void SFX(iir_run_row)(const or_iir *p, T *x, int64_t n, double *buf) {
+  if (n <= 0) return;
+  double a, b;
+  SFX(trend_of)(p->trend, x, n, &a, &b);
+  const int64_t L = p->lead;
+  for (int64_t i = 0; i < n; i++) buf[L + i] = (double)x[i] - (a + b * (double)i);
+  if (p->reflect) {
+    const double d0 = buf[L];
+    for (int64_t k = 1; k <= L; k++) buf[L - k] = 2 * d0 - buf[L + (k < n - 1 ? k : n - 1)];
+    const double dn = buf[L + n - 1];
+    for (int64_t k = 1; k <= p->tail; k++)
+      buf[L + n - 1 + k] = 2 * dn - buf[L + n - 1 - (k < n - 1 ? k : n - 1)];
+  } else {
+    for (int64_t k = 1; k <= p->tail; k++) buf[L + n - 1 + k] = 0;
+  }
+  or_iir_run(p, buf, L + n + p->tail);
+  const double ga = a * p->dc_gain - b * p->dc_slope, gb = b * p->dc_gain;
+  for (int64_t i = 0; i < n; i++) x[i] = (T)(buf[L + i] + (ga + gb * (double)i));
+}
+
+void SFX(or_iir_apply)(const or_iir *p, T *x, int64_t n) {
+  double *buf = (double *)malloc(sizeof(double) * (size_t)(p->lead + n + p->tail + 1));
+  SFX(iir_run_row)(p, x, n, buf);
+  free(buf);
+}
+
 /* ---------------------------------------------------------- gain curve -- */
 
 /* GainCurvePlan (gain/curves.nim:214-249): T(factorAt(t)) at
@@ -842,6 +874,7 @@ typedef struct SFX(step) {
   double factor, lo, hi;
   or_op mute;          /* a mute step: its op (curve and taper) */
   SFX(gainplan) gain;
+  or_iir iir;
   SFX(filterplan) filter;
   SFX(agcplan) agc;
   int stat;
@@ -922,6 +955,11 @@ static int SFX(plan_compile)(SFX(plan) *P, const or_op *ops, int32_t n_ops, doub
         s->kind = op->scope == 1 ? ST_NORM_PANEL : ST_NORM_TRACE;
         s->stat = op->stat; s->target = op->target; s->percentile = op->percentile;
         break;
+      case 9:
+        /* iirOp (pipeline.nim:428-430) */
+        if (or_iir_init(&s->iir, &op->spec, dt, samples, op->iir_phase, op->trend, msg, (int)sizeof msg)) { bad = 1; break; }
+        s->kind = ST_IIR;
+        break;
       case 12:
         /* gainCurveOp: tabulated to the end of the record (pipeline.nim:440-442) */
         if (SFX(gain_init)(&s->gain, op, dt, P->record, msg, (int)sizeof msg)) { bad = 1; break; }
@@ -954,6 +992,11 @@ static int SFX(plan_compile)(SFX(plan) *P, const or_op *ops, int32_t n_ops, doub
       need = f > need ? f : need;
     }
     if (s->kind == ST_NORM_TRACE) need = samples > need ? samples : need;
+    if (s->kind == ST_IIR) {
+      P->margin += s->iir.pad;
+      int64_t f = s->iir.lead + samples + s->iir.tail;
+      need = f > need ? f : need;
+    }
     if (s->kind == ST_NORM_PANEL && i > start) {
       P->part_a[P->n_parts] = start; P->part_b[P->n_parts] = i; P->n_parts++;
       start = i;
@@ -984,6 +1027,7 @@ static void SFX(run_step)(const SFX(step) *s, T *x, int64_t n, void *scratch, co
        * (core/types.nim:237-240) */
       SFX(gain_apply)(&s->gain, x, n, (key ? key[0] : 0.0) + (double)first * dt);
       break;
+    case ST_IIR: SFX(iir_run_row)(&s->iir, x, n, (double *)scratch); break;
     case ST_DEMEAN: SFX(demean)(x, n); break;
     case ST_DETREND: SFX(detrend)(x, n); break;
     case ST_POLARITY: SFX(polarity)(x, n); break;

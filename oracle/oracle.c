@@ -25,7 +25,7 @@
 
 /* Step kinds of a compiled plan (pipeline.nim:180-183), the subset restated. */
 enum { ST_DEMEAN, ST_DETREND, ST_SCALE, ST_CLIP, ST_POLARITY, ST_FILTER, ST_AGC,
-       ST_NORM_TRACE, ST_NORM_PANEL, ST_MUTE_TOP, ST_MUTE_BOTTOM, ST_GAIN_CURVE };
+       ST_NORM_TRACE, ST_NORM_PANEL, ST_MUTE_TOP, ST_MUTE_BOTTOM, ST_GAIN_CURVE, ST_IIR };
 
 static int or_seterr(char *err, int errlen, const char *msg) {
   if (err && errlen > 0) {
@@ -281,6 +281,9 @@ static double or_order_stat(const uint32_t *h, int64_t j, double mx) {
   return mx;
 }
 
+/* runSingle (filter/iir.nim:393-396), below with the IIR design. */
+static void or_iir_run(const or_iir *p, double *buf, int64_t m);
+
 /* ------------------------------------------------- instantiate float64 -- */
 
 #define T double
@@ -378,6 +381,222 @@ static int or_validate(const or_spec *s, double dt_ms, char *err, int errlen) {
     if (s->notch_hz[i] >= nyq) return or_seterr(err, errlen, "a notch is not below Nyquist");
   }
   return 0;
+}
+
+/* ---------------------------------------------------------------- IIR -- */
+
+static or_biquad or_biquad_of(double b0, double b1, double b2, double a1, double a2) {
+  or_biquad q = {b0, b1, b2, a1, a2, (b0 + b1 + b2) / (1 + a1 + a2)};
+  return q;
+}
+
+/* butterSections (filter/iir.nim:113-148): `order` poles with the -3 dB
+ * corner at fc Hz by the prewarped bilinear transform; high: a high-pass
+ * (gain 1 at Nyquist), else a low-pass (gain 1 at 0 Hz).  Appends to sec. */
+static int or_butter_sections(double fc, double dt_ms, int order, int high, or_biquad *sec) {
+  const double fs = 1000 / dt_ms;
+  const double wc = 2 * fs * tan(OR_PI * fc / fs);
+  const double k2 = 2 * fs;
+  int n = 0;
+  for (int k = 0; k < order / 2; k++) {
+    const double th = OR_PI * (double)(2 * k + 1) / (double)(2 * order);
+    const double sr = -wc * sin(th), si = wc * cos(th);   /* analog pole s */
+    /* z = (k2 + s) / (k2 - s) */
+    const double nr = k2 + sr, ni = si, dr = k2 - sr, di = -si;
+    const double den = dr * dr + di * di;
+    const double zr = (nr * dr + ni * di) / den, zi = (ni * dr - nr * di) / den;
+    const double a1 = -2 * zr, a2 = zr * zr + zi * zi;
+    if (high) {
+      const double g = (1 - a1 + a2) / 4;
+      sec[n++] = or_biquad_of(g, -2 * g, g, a1, a2);
+    } else {
+      const double g = (1 + a1 + a2) / 4;
+      sec[n++] = or_biquad_of(g, 2 * g, g, a1, a2);
+    }
+  }
+  if (order % 2 == 1) {
+    const double z = (k2 - wc) / (k2 + wc);
+    const double a1 = -z;
+    if (high) {
+      const double g = (1 - a1) / 2;
+      sec[n++] = or_biquad_of(g, -g, 0, a1, 0);
+    } else {
+      const double g = (1 + a1) / 2;
+      sec[n++] = or_biquad_of(g, g, 0, a1, 0);
+    }
+  }
+  return n;
+}
+
+/* causalLength (filter/iir.nim:150-182): the samples after a spike that hold
+ * 99.99 % of the energy of the causal impulse response, or of its complement,
+ * whichever is longer. */
+static int64_t or_causal_length(const or_biquad *sec, int ns) {
+  int64_t n = 1024;
+  for (;;) {
+    double *h = (double *)calloc((size_t)n, sizeof(double));
+    h[0] = 1;
+    for (int c = 0; c < ns; c++) {
+      double s1 = 0, s2 = 0;
+      const or_biquad q = sec[c];
+      for (int64_t j = 0; j < n; j++) {
+        const double x = h[j];
+        const double y = q.b0 * x + s1;
+        s1 = q.b1 * x - q.a1 * y + s2;
+        s2 = q.b2 * x - q.a2 * y;
+        h[j] = y;
+      }
+    }
+    double rest = 0, tail = 0;
+    for (int64_t j = 1; j < n; j++) {
+      rest += h[j] * h[j];
+      if (j >= n - n / 4) tail += h[j] * h[j];
+    }
+    if (tail <= 1e-12 * fmax(rest, 1e-300) || n >= ((int64_t)1 << 24)) {
+      int64_t best = 0;
+      const double cs[2] = {h[0], 1 - h[0]};
+      for (int i = 0; i < 2; i++) {
+        const double total = cs[i] * cs[i] + rest;
+        double e = cs[i] * cs[i];
+        int64_t L = 0;
+        while (e < 0.9999 * total && L < n - 1) {
+          L++;
+          e += h[L] * h[L];
+        }
+        if (L > best) best = L;
+      }
+      free(h);
+      return best;
+    }
+    free(h);
+    n *= 2;
+  }
+}
+
+/* poles (filter/iir.nim:184-191). */
+static int or_iir_poles(double slope, double per_pole, const char *side, char *err, int errlen) {
+  const double n = slope / per_pole;
+  if (!(n >= 1 && n <= 32 && n == round(n))) {
+    char msg[200];
+    snprintf(msg, sizeof msg, "the %s slope of an IIR filter must be a multiple of %g dB/octave, "
+             "1 .. 32 poles; got %g", side, per_pole, slope);
+    or_seterr(err, errlen, msg);
+    return -1;
+  }
+  return (int)n;
+}
+
+/* initIirPlan (filter/iir.nim:193-239). */
+int or_iir_init(or_iir *p, const or_spec *spec, double dt_ms, int64_t samples, int32_t phase,
+                int32_t trend, char *err, int errlen) {
+  memset(p, 0, sizeof(*p));
+  if (samples < 1) return or_seterr(err, errlen, "an IIR plan needs samples >= 1");
+  if (or_validate(spec, dt_ms, err, errlen)) return -1;
+  if (spec->kind != 1)
+    return or_seterr(err, errlen, "an IIR filter needs a Butterworth spec (butterLowCut, butterHighCut, butterBandPass)");
+  if (spec->n_notch > 0 || spec->zero_dc)
+    return or_seterr(err, errlen, "an IIR filter takes no notches and no 0 Hz suppression");
+  const double per_pole = phase == 0 ? 12.0 : 6.0;
+  or_biquad low[OR_MAX_SECTIONS], high[OR_MAX_SECTIONS];
+  int nl = 0, nh = 0;
+  if (spec->low_hz > 0) {
+    int np = or_iir_poles(spec->low_slope, per_pole, "low-cut", err, errlen);
+    if (np < 0) return -1;
+    nl = or_butter_sections(spec->low_hz, dt_ms, np, 1, low);
+  }
+  if (spec->high_hz < INFINITY) {
+    int np = or_iir_poles(spec->high_slope, per_pole, "high-cut", err, errlen);
+    if (np < 0) return -1;
+    nh = or_butter_sections(spec->high_hz, dt_ms, np, 0, high);
+  }
+  for (int i = 0; i < nl; i++) p->sec[p->n_sections++] = low[i];
+  for (int i = 0; i < nh; i++) p->sec[p->n_sections++] = high[i];
+  p->phase = phase;
+  if (phase == 0) {
+    if (or_impulse_length(spec, dt_ms, &p->pad, err, errlen)) return -1;
+  } else {
+    int64_t a = or_causal_length(low, nl), b = or_causal_length(high, nh);
+    p->pad = a > b ? a : b;
+  }
+  p->samples = samples;
+  p->dt_ms = dt_ms;
+  /* G(1) and G'(1) of the cascade from each section's N(q)/D(q) */
+  double g = 1, dg = 0;
+  for (int i = 0; i < p->n_sections; i++) {
+    const or_biquad c = p->sec[i];
+    const double n1 = c.b0 + c.b1 + c.b2, d1 = 1 + c.a1 + c.a2;
+    const double gi = n1 / d1;
+    const double dgi = ((c.b1 + 2 * c.b2) * d1 - n1 * (c.a1 + 2 * c.a2)) / (d1 * d1);
+    dg = dg * gi + g * dgi;
+    g *= gi;
+  }
+  p->dc_gain = phase == 0 ? g * g : g;
+  p->dc_slope = phase == 0 ? 0.0 : dg;
+  p->trend = trend;
+  p->reflect = fabs(p->dc_gain) >= 0.5 || trend != 2;
+  p->lead = p->reflect ? p->pad : 0;
+  p->tail = phase == 0 ? p->pad : 0;
+  return 0;
+}
+
+double or_iir_magnitude_at(const or_iir *p, double f) {
+  const double w = 2 * OR_PI * f * p->dt_ms / 1000;
+  const double zr = cos(w), zi = -sin(w);        /* z^-1 */
+  const double z2r = zr * zr - zi * zi, z2i = 2 * zr * zi;
+  double hr = 1, hi = 0;
+  for (int i = 0; i < p->n_sections; i++) {
+    const or_biquad s = p->sec[i];
+    const double nr = s.b0 + s.b1 * zr + s.b2 * z2r, ni = s.b1 * zi + s.b2 * z2i;
+    const double dr = 1 + s.a1 * zr + s.a2 * z2r, di = s.a1 * zi + s.a2 * z2i;
+    const double den = dr * dr + di * di;
+    const double qr = (nr * dr + ni * di) / den, qi = (ni * dr - nr * di) / den;
+    const double tr = hr * qr - hi * qi;
+    hi = hr * qi + hi * qr;
+    hr = tr;
+  }
+  const double m = hypot(hr, hi);
+  return p->phase == 0 ? m * m : m;
+}
+
+int32_t or_iir_order(const or_iir *p) {
+  int32_t o = 0;
+  for (int i = 0; i < p->n_sections; i++) o += p->sec[i].a2 == 0 ? 1 : 2;
+  return o;
+}
+
+/* sosKernel (filter/iir.nim:319-351) on one trace of m samples in place: a
+ * steady-state start on the pass's first input or a start from rest, and
+ * states below 1e-250 set to 0 every 256 steps. */
+static void or_sos_pass(double *buf, int64_t m, const or_biquad *c, int steady, int backward) {
+  double s1 = 0, s2 = 0;
+  if (steady && m > 0) {
+    const double u = buf[backward ? m - 1 : 0];
+    const double y = c->g * u;
+    s2 = c->b2 * u - c->a2 * y;
+    s1 = y - c->b0 * u;
+  }
+  int64_t done = 0;
+  while (done < m) {
+    const int64_t run = m - done < 256 ? m - done : 256;
+    for (int64_t k = 0; k < run; k++) {
+      const int64_t j = backward ? m - 1 - done - k : done + k;
+      const double x = buf[j];
+      const double y = c->b0 * x + s1;
+      s1 = c->b1 * x - c->a1 * y + s2;
+      s2 = c->b2 * x - c->a2 * y;
+      buf[j] = y;
+    }
+    s1 = fabs(s1) < 1e-250 ? 0.0 : s1;
+    s2 = fabs(s2) < 1e-250 ? 0.0 : s2;
+    done += run;
+  }
+}
+
+/* runSingle (filter/iir.nim:393-396). */
+static void or_iir_run(const or_iir *p, double *buf, int64_t m) {
+  for (int i = 0; i < p->n_sections; i++) or_sos_pass(buf, m, &p->sec[i], p->reflect, 0);
+  if (p->phase == 0)
+    for (int i = 0; i < p->n_sections; i++) or_sos_pass(buf, m, &p->sec[i], p->reflect, 1);
 }
 
 /* energyLength (filter/design.nim:288-298). */

@@ -50,8 +50,8 @@ typedef struct or_spec {
 
 /* Op (pipeline.nim:158-174), as plain data.  kind follows OpKind
  * (pipeline.nim:92-108): 0 demean, 1 detrend, 2 scale, 3 gainDb, 4 clip,
- * 5 polarity, 6 muteTop, 7 muteBottom, 8 filter, 10 agc, 11 normalize,
- * 12 gainCurve.
+ * 5 polarity, 6 muteTop, 7 muteBottom, 8 filter, 9 iir (spec, trend and
+ * iir_phase), 10 agc, 11 normalize, 12 gainCurve.
  * Other kinds are refused. */
 typedef struct or_op {
   int32_t kind;
@@ -91,7 +91,39 @@ typedef struct or_op {
   double gain_power, gain_ref_ms, gain_min_ms;
   double gain_db_per_s;
   double gain_tmin_ms, gain_tmax_ms;
+  int32_t iir_phase;  /* iir: 0 ipZero, 1 ipMinimum (filter/iir.nim:77-80) */
+  int32_t reserved2;
 } or_op;
+
+/* ---------------------------------------------------------------- IIR -- */
+/* IirPlan (filter/iir.nim:82-101, 193-239): a Butterworth biquad cascade
+ * (transposed direct form II, a0 = 1), minimum phase (one causal pass) or
+ * zero phase (forward-backward). */
+#define OR_MAX_SECTIONS 32
+typedef struct or_biquad {
+  double b0, b1, b2, a1, a2;
+  double g;               /* gain at 0 Hz */
+} or_biquad;
+typedef struct or_iir {
+  int32_t n_sections;
+  int32_t phase;          /* 0 ipZero, 1 ipMinimum */
+  int32_t trend;          /* 0 trNone, 1 trMean, 2 trLinear */
+  int32_t reflect;        /* odd-reflection pads and steady-state starts */
+  int64_t pad, lead, tail, samples;
+  double dt_ms, dc_gain, dc_slope;
+  or_biquad sec[OR_MAX_SECTIONS];
+} or_iir;
+/* initIirPlan; 0 ok, -1 invalid (message in err). */
+int or_iir_init(or_iir *p, const or_spec *spec, double dt_ms, int64_t samples, int32_t phase,
+                int32_t trend, char *err, int errlen);
+/* magnitudeAt (filter/iir.nim:268-278); order (:260-262). */
+double or_iir_magnitude_at(const or_iir *p, double f);
+int32_t or_iir_order(const or_iir *p);
+/* applyFilter on one trace x[0 ..< n] (n <= samples), in place (filter/iir.nim:
+ * 418-433): trend removed, padded, the passes in float64, the trend's
+ * steady-state response added back. */
+void or_iir_apply_f32(const or_iir *p, float *x, int64_t n);
+void or_iir_apply_f64(const or_iir *p, double *x, int64_t n);
 
 /* ------------------------------------------------------------- lengths -- */
 int or_is_fast_len(int64_t n);

@@ -14,7 +14,7 @@ from .pipeline import (  # noqa: F401
     ChunkMeta, Corners, FilterSpec, Inf, MuteCurve, NaN, Op, Pipeline, PipelinePlan, agcMeanAbs,
     agcOp, lmoMute, mkLmo, mkTable, mkTime, muteBottomOp, muteTopOp, tableMute, timeMute,
     GainCurve, GainKnot, exponentialGain, gainCurveOp, gcExponential, gcKnots, gcPower,
-    heldGain, knotGain, powerGain,
+    heldGain, knotGain, powerGain, ipMinimum, ipZero,
     agcRms, bandPass, bandPassOp, butterBandPass, butterHighCut, butterLowCut, clipOp,
     decodeWindow, demeanOp, detrendOp, filterOp, float32, float64, gainDbOp, highCut, iirOp,
     initPipeline, lowCut, nextFastLen, normalizeOp, normMax, normPercentile, normRms, perPanel,

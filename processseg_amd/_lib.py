@@ -42,7 +42,8 @@ class OpC(C.Structure):
                 ("gain_knot_ms", C.POINTER(C.c_double)), ("gain_knot_db", C.POINTER(C.c_double)),
                 ("gain_power", C.c_double), ("gain_ref_ms", C.c_double),
                 ("gain_min_ms", C.c_double), ("gain_db_per_s", C.c_double),
-                ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double)]
+                ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double),
+                ("iir_phase", C.c_int32), ("reserved2", C.c_int32)]
 
 
 class PlanInfoC(C.Structure):

@@ -35,6 +35,8 @@ float64 = np.float64
 
 # TrendRemoval (filter/edges.nim:8-14)
 trNone, trMean, trLinear = 0, 1, 2
+# IirPhase (filter/iir.nim:77-80)
+ipZero, ipMinimum = 0, 1
 # AgcMode (gain/gain.nim:92-95)
 agcRms, agcMeanAbs = 0, 1
 # NormStat / NormScope (gain/gain.nim:72-81)
@@ -421,6 +423,7 @@ class Op:
     percentile: float = 99.0
     mute: Optional[MuteCurve] = None  # muteTop, muteBottom
     taper_ms: float = 0.0
+    phase: int = 0              # iir: ipZero, ipMinimum
     curve: Optional[GainCurve] = None  # gainCurve, tabulated over t_min_ms .. t_max_ms
     t_min_ms: float = 0.0
     t_max_ms: float = math.nan
@@ -439,6 +442,7 @@ class Op:
         o.target = self.target
         o.epsilon = self.epsilon
         o.percentile = self.percentile
+        o.iir_phase = self.phase
         if self.spec is not None:
             o.spec = spec_c(self.spec)
         if self.mute is not None:
@@ -539,8 +543,11 @@ def normalizeOp(stat: int, scope: int = perTrace, target: float = 1.0,
               percentile=float(percentile))
 
 
-def iirOp(*_a, **_k) -> Op:
-    return Op(opIir)
+def iirOp(spec: FilterSpec, phase: int = 0, trend: int = trLinear) -> Op:
+    """iirOp (pipeline.nim:300-303): a Butterworth biquad cascade (a
+    butterBandPass / butterLowCut / butterHighCut spec), zero phase
+    (ipZero, forward-backward) or minimum phase (ipMinimum)."""
+    return Op(opIir, spec=spec, trend=trend, phase=int(phase))
 
 
 def _ops_c(ops: Sequence[Op]):
