@@ -113,6 +113,12 @@ typedef struct pseg_op {
   double gain_power, gain_ref_ms, gain_min_ms;
   double gain_db_per_s;
   double gain_tmin_ms, gain_tmax_ms;
+  /* iirOp (pipeline.nim:300-303, filter/iir.nim:193-239): a Butterworth
+   * spec in `spec` (kind 1, no notches, no zero_dc), `trend`, and the phase:
+   * 0 ipZero (forward-backward, slopes multiples of 12 dB/octave), 1
+   * ipMinimum (one causal pass, multiples of 6). */
+  int32_t iir_phase;
+  int32_t reserved2;
 } pseg_op;
 
 typedef struct pseg_plan pseg_plan;

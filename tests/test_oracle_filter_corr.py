@@ -269,4 +269,4 @@ def test_compile_errors_name_the_op(oracle):
     with pytest.raises(ValueError):
         oracle.plan_info([P.agcOp(100)], np.float32, 2.0, 0)
     with pytest.raises(P.UnsupportedOp):
-        P.initPipeline([P.demeanOp(), P.iirOp()]).compile(np.float32, 2.0, 1000)
+        P.initPipeline([P.demeanOp(), P.Op(P.pipeline.opCalibrate)]).compile(np.float32, 2.0, 1000)
