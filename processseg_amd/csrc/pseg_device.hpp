@@ -1207,6 +1207,137 @@ __device__ __forceinline__ void gain_step(const DevGain &G, T (&v)[PER], int n, 
   __syncthreads();  // before the next step writes LDS
 }
 
+// One pass of one biquad section (sosKernel, filter/iir.nim:319-351) over
+// u[0 ..< m] in place, forward or backward, as a parallel scan: thread t
+// takes positions [t·C, t·C + C) of the pass (C odd, so that the lanes'
+// float64 accesses spread over the LDS banks).  The transposed direct form
+// II state s = (s1, s2) advances as s' = A·s + B·x with A = [[-a1, 1],
+// [-a2, 0]], so a chunk is the affine map s -> M·s + e, M = A^len and e the
+// chunk's end state from rest.  A steady-state start on the first input u0
+// is the same as a start from rest on x − u0 plus the constant output g·u0,
+// so the scan runs on x − u0 from rest (a constant pad then scans as exact
+// zeros, and the rounding scales with the signal, not with the level it
+// sits on).  (1) each thread runs its chunk from rest for e, and the two unit
+// states with no input for M; (2) an inclusive Hillis-Steele scan of the
+// maps over the threads in LDS (`sc`: six float64 per thread); (3) each
+// thread runs its chunk again from its start state (the maps of the chunks
+// before it applied to rest), writing the outputs.  The outputs equal the
+// serial recursion's up to float64 rounding.
+template <int NTH>
+__device__ __forceinline__ void sos_scan_pass(double *u, int m, const double *c6, bool steady,
+                                              bool backward, double *sc) {
+  const int tid_ = opaque_tid();
+  const double b0 = uni(c6[0]), b1 = uni(c6[1]), b2 = uni(c6[2]);
+  const double a1 = uni(c6[3]), a2 = uni(c6[4]), g = uni(c6[5]);
+  int C = (m + NTH - 1) / NTH;
+  C |= 1;
+  const int p0 = min(tid_ * C, m), p1 = min(p0 + C, m);
+  // the level the pass starts from (steady state) or 0 (rest)
+  const double u0 = steady ? u[backward ? m - 1 : 0] : 0.0;
+  // (1) the chunk from rest, then A^len on the unit states (two loops: fewer
+  // float64 values live at once)
+  double e1 = 0, e2 = 0;
+  for (int p = p0; p < p1; p++) {
+    const int j = backward ? m - 1 - p : p;
+    const double x = u[j] - u0;
+    const double y = b0 * x + e1;
+    const double t1 = b1 * x - a1 * y + e2;
+    e2 = b2 * x - a2 * y;
+    e1 = t1;
+  }
+  double m11 = 1, m21 = 0, m12 = 0, m22 = 1;  // columns: images of (1,0) and (0,1)
+  for (int p = p0; p < p1; p++) {
+    const double n11 = -a1 * m11 + m21, n21 = -a2 * m11;
+    const double n12 = -a1 * m12 + m22, n22 = -a2 * m12;
+    m11 = n11; m21 = n21; m12 = n12; m22 = n22;
+  }
+  // (2) the scan: after it, thread t holds the composition of chunks 0 .. t
+  double *S = sc + 6 * tid_;
+  for (int d = 1; d < NTH; d <<= 1) {
+    S[0] = m11; S[1] = m21; S[2] = m12; S[3] = m22; S[4] = e1; S[5] = e2;
+    __syncthreads();
+    if (tid_ >= d) {
+      const double *Q = sc + 6 * (tid_ - d);  // chunks before: apply first
+      const double q11 = Q[0], q21 = Q[1], q12 = Q[2], q22 = Q[3], f1 = Q[4], f2 = Q[5];
+      const double r11 = m11 * q11 + m12 * q21, r21 = m21 * q11 + m22 * q21;
+      const double r12 = m11 * q12 + m12 * q22, r22 = m21 * q12 + m22 * q22;
+      const double g1 = m11 * f1 + m12 * f2 + e1, g2 = m21 * f1 + m22 * f2 + e2;
+      m11 = r11; m21 = r21; m12 = r12; m22 = r22; e1 = g1; e2 = g2;
+    }
+    __syncthreads();
+  }
+  S[0] = m11; S[1] = m21; S[2] = m12; S[3] = m22; S[4] = e1; S[5] = e2;
+  __syncthreads();
+  // (3) the chunk again from its start state: the maps of chunks 0 .. t-1
+  // applied to rest
+  double s1 = 0, s2 = 0;
+  if (tid_ > 0) {
+    const double *Q = sc + 6 * (tid_ - 1);
+    s1 = Q[4];
+    s2 = Q[5];
+  }
+  const double y0 = g * u0;  // the steady output of the level
+  for (int p = p0; p < p1; p++) {
+    const int j = backward ? m - 1 - p : p;
+    const double x = u[j] - u0;
+    const double y = b0 * x + s1;
+    s1 = b1 * x - a1 * y + s2;
+    s2 = b2 * x - a2 * y;
+    u[j] = y + y0;
+  }
+  __syncthreads();  // the pass is done (and `sc` free) before the next reads
+}
+
+// An IIR step (applyFilter, filter/iir.nim:363-433) on the row in registers:
+// the trend removed (float64 block sums), the row with its pads (odd
+// reflections before, and after for zero phase, or zeros after) as float64
+// in LDS, each section's pass as a parallel scan (forward, then backward for
+// zero phase), the trend's steady-state response G(1)·(a + b·i) − b·G'(1)
+// added back on the way to the registers.
+template <typename T, int NTH, int PER>
+__device__ __forceinline__ void iir_step(const DevIir &I, T (&v)[PER], int n, char *lds,
+                                         RedSlots &red) {
+  const int tid_ = opaque_tid();
+  double a, b;
+  trend_of<T, NTH, PER>(v, n, uni(I.trend), red, a, b);
+  a = uni(a);  // the same on every lane: SGPRs across the passes
+  b = uni(b);
+  const int lead = uni(I.lead), tail = uni(I.tail);
+  const bool reflect = uni(I.reflect) != 0;
+  double *u = (double *)lds;
+  double *sc = u + uni(I.span);
+  __syncthreads();  // the previous step's LDS reads are done
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int j = tid_ + NTH * k;
+    if (j < n) u[lead + j] = (double)v[k] - (a + b * (double)j);
+  }
+  __syncthreads();
+  // the pads read only the data
+  if (reflect) {
+    const double d0 = u[lead], dn = u[lead + n - 1];
+    for (int k = tid_ + 1; k <= lead; k += NTH) u[lead - k] = 2 * d0 - u[lead + min(k, n - 1)];
+    for (int k = tid_ + 1; k <= tail; k += NTH)
+      u[lead + n - 1 + k] = 2 * dn - u[lead + n - 1 - min(k, n - 1)];
+  } else {
+    for (int k = tid_ + 1; k <= tail; k += NTH) u[lead + n - 1 + k] = 0;
+  }
+  __syncthreads();
+  const int m = lead + n + tail;
+  const int ns = uni(I.n_sections);
+  const double *sec = (const double *)uni_ptr(I.sec);
+  for (int i = 0; i < ns; i++) sos_scan_pass<NTH>(u, m, sec + 6 * i, reflect, false, sc);
+  if (uni(I.zero_phase))
+    for (int i = 0; i < ns; i++) sos_scan_pass<NTH>(u, m, sec + 6 * i, reflect, true, sc);
+  const double ga = a * uni(I.dc_gain) - b * uni(I.dc_slope), gb = b * uni(I.dc_gain);
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int j = tid_ + NTH * k;
+    v[k] = j < n ? (T)(u[lead + j] + (ga + gb * (double)j)) : (T)0;
+  }
+  __syncthreads();  // before the next step writes LDS
+}
+
 // Per-trace normalization (gain/gain.nim:198-219).
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds,

@@ -470,6 +470,123 @@ void patch_fft(DevFft &f, char *base) {
   patch(f.split, base);
 }
 
+// ---------------------------------------------------------------- IIR --
+
+struct Sos {
+  double b0, b1, b2, a1, a2, g;
+};
+
+Sos sos_of(double b0, double b1, double b2, double a1, double a2) {
+  return Sos{b0, b1, b2, a1, a2, (b0 + b1 + b2) / (1 + a1 + a2)};
+}
+
+// butterSections (filter/iir.nim:113-148): a digital Butterworth high-pass
+// (high) or low-pass of `order` poles, -3 dB at fc Hz, by the bilinear
+// transform prewarped to the corner; conjugate pole pairs as biquads, one
+// first-order section for an odd order.
+void butter_sections(double fc, double dt, int order, bool high, std::vector<Sos> &out) {
+  const double fs = 1000 / dt;
+  const double wc = 2 * fs * std::tan(kPi * fc / fs);
+  const double k2 = 2 * fs;
+  for (int k = 0; k < order / 2; k++) {
+    const double th = kPi * (double)(2 * k + 1) / (double)(2 * order);
+    const std::complex<double> sp(-wc * std::sin(th), wc * std::cos(th));
+    const std::complex<double> z = (std::complex<double>(k2, 0) + sp) / (std::complex<double>(k2, 0) - sp);
+    const double a1 = -2 * z.real(), a2 = z.real() * z.real() + z.imag() * z.imag();
+    const double g = high ? (1 - a1 + a2) / 4 : (1 + a1 + a2) / 4;
+    out.push_back(high ? sos_of(g, -2 * g, g, a1, a2) : sos_of(g, 2 * g, g, a1, a2));
+  }
+  if (order % 2 == 1) {
+    const double a1 = -(k2 - wc) / (k2 + wc);
+    const double g = high ? (1 - a1) / 2 : (1 + a1) / 2;
+    out.push_back(high ? sos_of(g, -g, 0, a1, 0) : sos_of(g, g, 0, a1, 0));
+  }
+}
+
+// causalLength (filter/iir.nim:150-182): samples after a spike holding
+// 99.99 % of the energy of the causal impulse response, or of its
+// complement, whichever is longer.
+int64_t causal_length(const std::vector<Sos> &secs) {
+  for (int64_t n = 1024;; n *= 2) {
+    std::vector<double> h(n, 0.0);
+    h[0] = 1;
+    for (const Sos &c : secs) {
+      double s1 = 0, s2 = 0;
+      for (int64_t j = 0; j < n; j++) {
+        const double x = h[j];
+        const double y = c.b0 * x + s1;
+        s1 = c.b1 * x - c.a1 * y + s2;
+        s2 = c.b2 * x - c.a2 * y;
+        h[j] = y;
+      }
+    }
+    double rest = 0, tail = 0;
+    for (int64_t j = 1; j < n; j++) {
+      rest += h[j] * h[j];
+      if (j >= n - n / 4) tail += h[j] * h[j];
+    }
+    if (tail <= 1e-12 * std::max(rest, 1e-300) || n >= ((int64_t)1 << 24)) {
+      int64_t best = 0;
+      for (double c0 : {h[0], 1 - h[0]}) {
+        const double total = c0 * c0 + rest;
+        double e = c0 * c0;
+        int64_t L = 0;
+        while (e < 0.9999 * total && L < n - 1) {
+          L++;
+          e += h[L] * h[L];
+        }
+        best = std::max(best, L);
+      }
+      return best;
+    }
+  }
+}
+
+int iir_poles(double slope, double per_pole, const char *side) {
+  const double n = slope / per_pole;
+  if (!(n >= 1 && n <= 32 && n == std::round(n)))
+    invalid("the %s slope of an IIR filter must be a multiple of %g dB/octave (%d per pole), "
+            "1 .. 32 poles; got %g", side, per_pole, (int)per_pole, slope);
+  return (int)n;
+}
+
+// initIirPlan (filter/iir.nim:193-239) up to the plan's fields: the
+// sections, the margin (the spec's impulse length for zero phase, the
+// causal length for minimum phase), G(1) and G'(1).
+struct IirDesign {
+  std::vector<Sos> secs;
+  int64_t pad;
+  double dc_gain, dc_slope;
+};
+
+IirDesign design_iir(const pseg_filter_spec &spec, double dt, int phase) {
+  if (phase != 0 && phase != 1) invalid("unknown IIR phase %d", phase);
+  validate(spec, dt);
+  if (spec.kind != 1)
+    invalid("an IIR filter needs a Butterworth spec (butterLowCut, butterHighCut, butterBandPass)");
+  if (spec.n_notch > 0 || spec.zero_dc) invalid("an IIR filter takes no notches and no 0 Hz suppression");
+  const double per_pole = phase == 0 ? 12.0 : 6.0;
+  std::vector<Sos> low, high;
+  if (spec.low_hz > 0) butter_sections(spec.low_hz, dt, iir_poles(spec.low_slope, per_pole, "low-cut"), true, low);
+  if (spec.high_hz < kInf)
+    butter_sections(spec.high_hz, dt, iir_poles(spec.high_slope, per_pole, "high-cut"), false, high);
+  IirDesign d;
+  d.secs = low;
+  d.secs.insert(d.secs.end(), high.begin(), high.end());
+  d.pad = phase == 0 ? impulse_length(spec, dt) : std::max(causal_length(low), causal_length(high));
+  double g = 1, dg = 0;
+  for (const Sos &c : d.secs) {
+    const double n1 = c.b0 + c.b1 + c.b2, d1 = 1 + c.a1 + c.a2;
+    const double gi = n1 / d1;
+    const double dgi = ((c.b1 + 2 * c.b2) * d1 - n1 * (c.a1 + 2 * c.a2)) / (d1 * d1);
+    dg = dg * gi + g * dgi;
+    g *= gi;
+  }
+  d.dc_gain = phase == 0 ? g * g : g;
+  d.dc_slope = phase == 0 ? 0.0 : dg;
+  return d;
+}
+
 // gainDbAt / factorAt (gain/curves.nim:134-160) of a gainCurve op, for its
 // table: linear in dB between knots (bisection), flat outside them,
 // 10^(dB/20); (max(t, minMs)/refMs)^power directly for a power curve.
@@ -750,7 +867,38 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       lds = (size_t)samples * sizeof(T);  // the row's factors
       break;
     }
-    case PSEG_OP_IIR:
+    case PSEG_OP_IIR: {
+      // iirOp (pipeline.nim:428-430): initIirPlan (filter/iir.nim:193-239)
+      if (op.trend < 0 || op.trend > 2) invalid("unknown trend removal %d", op.trend);
+      const IirDesign d = design_iir(op.spec, dt, op.iir_phase);
+      DevIir &I = st.iir;
+      const bool reflect = std::fabs(d.dc_gain) >= 0.5 || op.trend != 2;
+      const int64_t lead = reflect ? d.pad : 0;
+      const int64_t tail = op.iir_phase == 0 ? d.pad : 0;
+      const int64_t span = lead + samples + tail;
+      if (span > (1 << 20)) invalid("an IIR row of %lld padded samples is beyond this GPU path; UNSUPPORTED", (long long)span);
+      I.n_sections = (int32_t)d.secs.size();
+      I.zero_phase = op.iir_phase == 0;
+      I.reflect = reflect;
+      I.trend = op.trend;
+      I.lead = (int32_t)lead;
+      I.tail = (int32_t)tail;
+      I.span = (int32_t)span;
+      I.dc_gain = d.dc_gain;
+      I.dc_slope = d.dc_slope;
+      std::vector<double> sec;
+      for (const Sos &c : d.secs) sec.insert(sec.end(), {c.b0, c.b1, c.b2, c.a1, c.a2, c.g});
+      set_off(I.sec, blob.add_vec(sec));
+      st.kind = kIir;
+      P.step_margin[i] = (int32_t)d.pad;
+      P.step_pad[i] = (int32_t)lead;
+      P.step_reflect[i] = reflect;
+      P.margin += (int32_t)d.pad;
+      // the padded row in float64; the scan's six float64 per thread are
+      // added once the plan's threads per row are known
+      lds = (size_t)span * sizeof(double);
+      break;
+    }
     case PSEG_OP_CALIBRATE:
     case PSEG_OP_UNCALIBRATE:
       throw Invalid{"UNSUPPORTED"};
@@ -771,6 +919,8 @@ void patch_step(DevStep &st, char *base) {
   } else if (st.kind == kMuteTop || st.kind == kMuteBottom) {
     patch(st.mute.offsets, base);
     patch(st.mute.times, base);
+  } else if (st.kind == kIir) {
+    patch(st.iir.sec, base);
   } else if (st.kind == kGainCurve) {
     patch(st.gain.table, base);
     patch(st.gain.knot_ms, base);
@@ -836,6 +986,19 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
     delete P;
     return fail(PSEG_E_UNSUPPORTED, "the plan's FFTs are too long for the in-place transform "
                 "of one workgroup (at most 512 threads)");
+  }
+  // an IIR step's scan: one affine map (a 2 x 2 matrix and a vector, six
+  // float64) per thread after the padded row
+  for (int i = 0; i < n_ops; i++) {
+    if (H.steps[i].kind != kIir) continue;
+    P->step_lds[i] += (size_t)P->threads * 6 * sizeof(double);
+    if (P->step_lds[i] > kMaxRowLds) {
+      const size_t need = P->step_lds[i];
+      delete P;
+      return fail(PSEG_E_UNSUPPORTED, "op %d (%s): %zu bytes of LDS per row (the padded row in "
+                  "float64) do not fit one workgroup (at most %zu)", i, op_name(ops[i].kind), need,
+                  kMaxRowLds);
+    }
   }
   // parts: a per-panel normalization after the first op starts one
   // (pipeline.nim:515-522)
@@ -1061,7 +1224,7 @@ pseg_status pseg_required_margin(const pseg_op *ops, int32_t n_ops, double dt_ms
         if (h < 1) invalid("an AGC window of %g ms is under 3 samples at %g ms", ops[i].window_ms, dt_ms);
         m += (int64_t)h;
       } else if (ops[i].kind == PSEG_OP_IIR) {
-        return fail(PSEG_E_UNSUPPORTED, "op %d (%s): not part of the GPU path", i, op_name(ops[i].kind));
+        m += design_iir(ops[i].spec, dt_ms, ops[i].iir_phase).pad;
       }
     } catch (const Invalid &e) {
       return fail(PSEG_E_INVALID, "op %d (%s): %s", i, op_name(ops[i].kind), e.msg.c_str());

@@ -25,6 +25,7 @@ enum StepKind : int32_t {
   kMuteTop = 9,
   kMuteBottom = 10,
   kGainCurve = 11,
+  kIir = 12,
 };
 
 // Divide by a runtime constant with a multiply and a shift (Granlund-
@@ -120,6 +121,23 @@ struct DevGain {
   const double *knot_ms, *knot_db;
 };
 
+// An IirPlan (filter/iir.nim:86-101): the cascade's sections as (b0, b1, b2,
+// a1, a2, g) float64 sextets, the pads before (lead) and after (tail) the
+// data, steady-state starts with odd-reflection pads or starts from rest,
+// forward only (minimum phase) or forward then backward (zero phase), and
+// the trend's response G(1)·(a + b·i) − b·G'(1).
+struct DevIir {
+  int32_t n_sections;
+  int32_t zero_phase;
+  int32_t reflect;
+  int32_t trend;
+  int32_t lead, tail;
+  int32_t span;         // lead + samples + tail: the float64 row buffer in LDS
+  int32_t reserved;
+  double dc_gain, dc_slope;
+  const double *sec;
+};
+
 struct DevStep {
   int32_t kind;
   int32_t stat;       // normalize
@@ -131,6 +149,7 @@ struct DevStep {
   DevAgc agc;
   DevMute mute;
   DevGain gain;
+  DevIir iir;
 };
 
 struct DevPlan {

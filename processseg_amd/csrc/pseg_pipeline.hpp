@@ -157,6 +157,9 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
             mute_step<T, NTH, PER>(st.mute, uni(st.kind) == kMuteTop, v, n, col);
           }
           break;
+        case kIir:
+          if constexpr (FULL) iir_step<T, NTH, PER>(st.iir, v, n, lds, red);
+          break;
         case kGainCurve:
           if constexpr (FULL) {
             // sampleTimeMs(r, 0) = t0Ms + firstSample·dt (core/types.nim:237-240)
