@@ -277,6 +277,34 @@ def gain_flow_leg(torch, P, dev, steps, warmup):
             "ms_per_step": round(t * 1000, 5), "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
 
 
+def iir_flow_leg(torch, P, dev, steps, warmup):
+    """The headline panel through a time-domain Butterworth band-pass (the
+    kernel form with every step): demean > zero-phase IIR band-pass 5-70 Hz
+    24 dB/octave (two biquads, forward and backward: four scanned passes) >
+    AGC 250 ms; one Plan.apply per step, 4 buffer pairs."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    flow = P.initPipeline([P.demeanOp(), P.iirOp(P.butterBandPass(5, 70, 24)), P.agcOp(250)])
+    plan = flow.compile(np.float32, DT, COLS)
+    srcs = [torch.from_numpy(x).to(dev) for _ in range(4)]
+    dsts = [torch.empty_like(srcs[0]) for _ in range(4)]
+    for i in range(warmup):
+        plan.apply(srcs[i % 4], dsts[i % 4], meta)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        plan.apply(srcs[i % 4], dsts[i % 4], meta)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / steps
+    samples = ROWS * COLS
+    return {"workload": "demean > zero-phase IIR Butterworth band-pass 5-70 Hz 24 dB/oct > "
+                        "AGC 250 ms, 2000x3001 f32",
+            "value": round(samples / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5), "achieved_GBps": round(samples * 8 / t / 1e9, 1),
+            "margin": plan.requiredMargin}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -417,7 +445,8 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, f64_panel, correlation, uneven_batch, serving), or all")
+                         "gain_flow, iir_flow, f64_panel, correlation, uneven_batch, serving), "
+                         "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
     ap.add_argument("--copy-probe", action="store_true",
@@ -552,6 +581,10 @@ def main():
         if want("gain_flow"):
             log("gain curve flow leg")
             sec["gain_flow"] = gain_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("iir_flow"):
+            log("IIR flow leg")
+            sec["iir_flow"] = iir_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
             torch.cuda.empty_cache()
         if want("f64_panel"):
             log("float64 panel leg")

@@ -1326,9 +1326,11 @@ __device__ __forceinline__ void iir_step(const DevIir &I, T (&v)[PER], int n, ch
   const int m = lead + n + tail;
   const int ns = uni(I.n_sections);
   const double *sec = (const double *)uni_ptr(I.sec);
-  for (int i = 0; i < ns; i++) sos_scan_pass<NTH>(u, m, sec + 6 * i, reflect, false, sc);
-  if (uni(I.zero_phase))
-    for (int i = 0; i < ns; i++) sos_scan_pass<NTH>(u, m, sec + 6 * i, reflect, true, sc);
+  // the forward passes, then (zero phase) the backward ones: one loop, so
+  // that the pass is inlined once
+  const int passes = uni(I.zero_phase) ? 2 * ns : ns;
+  for (int i = 0; i < passes; i++)
+    sos_scan_pass<NTH>(u, m, sec + 6 * (i < ns ? i : i - ns), reflect, i >= ns, sc);
   const double ga = a * uni(I.dc_gain) - b * uni(I.dc_slope), gb = b * uni(I.dc_gain);
 #pragma unroll
   for (int k = 0; k < PER; k++) {
