@@ -1234,9 +1234,11 @@ __device__ __forceinline__ void sos_scan_pass(double *u, int m, const double *c6
   const int p0 = min(tid_ * C, m), p1 = min(p0 + C, m);
   // the level the pass starts from (steady state) or 0 (rest)
   const double u0 = steady ? u[backward ? m - 1 : 0] : 0.0;
-  // (1) the chunk from rest, then A^len on the unit states (two loops: fewer
-  // float64 values live at once)
+  // (1) the chunk from rest, and A^len on the unit states: three independent
+  // chains in one loop, so that each hides the others' latency
   double e1 = 0, e2 = 0;
+  double m11 = 1, m21 = 0, m12 = 0, m22 = 1;  // columns: images of (1,0) and (0,1)
+#pragma unroll 4
   for (int p = p0; p < p1; p++) {
     const int j = backward ? m - 1 - p : p;
     const double x = u[j] - u0;
@@ -1244,9 +1246,6 @@ __device__ __forceinline__ void sos_scan_pass(double *u, int m, const double *c6
     const double t1 = b1 * x - a1 * y + e2;
     e2 = b2 * x - a2 * y;
     e1 = t1;
-  }
-  double m11 = 1, m21 = 0, m12 = 0, m22 = 1;  // columns: images of (1,0) and (0,1)
-  for (int p = p0; p < p1; p++) {
     const double n11 = -a1 * m11 + m21, n21 = -a2 * m11;
     const double n12 = -a1 * m12 + m22, n22 = -a2 * m12;
     m11 = n11; m21 = n21; m12 = n12; m22 = n22;
@@ -1277,6 +1276,7 @@ __device__ __forceinline__ void sos_scan_pass(double *u, int m, const double *c6
     s2 = Q[5];
   }
   const double y0 = g * u0;  // the steady output of the level
+#pragma unroll 4
   for (int p = p0; p < p1; p++) {
     const int j = backward ? m - 1 - p : p;
     const double x = u[j] - u0;
