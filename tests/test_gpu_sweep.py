@@ -6,9 +6,13 @@ lengths one either side of a class edge), a sample interval, a flow of one
 to four steps from the ops the GPU compiles (filters of every kind and trend
 mode, AGC of both modes with windows shorter and longer than a thread's run,
 the three per-trace statistics), live lengths from 0 to the full record, and
-in-place or separate output.  The bar is the one of the single-step tests:
+in-place or separate output; a second sweep draws from every op the GPU
+compiles, mutes, gain curves and IIR cascades included, with the rows' keys
+and a window's firstSample.  The bar is the one of the single-step tests:
 float32 within 1e-5 relative L2 of the float64 oracle, float64 within 1e-12;
 samples past a live end and non-live rows unchanged, bit for bit."""
+import math
+
 import numpy as np
 import pytest
 
@@ -95,6 +99,89 @@ def test_sweep_case(gpu, oracle, case):
     tol = 1e-5 if dtype == np.float32 else 1e-12
     for r in range(rows):
         n = int(live[r])
+        if n < 0:
+            assert np.array_equal(got[r], x[r]), f"row {r}: not live, not copied"
+            continue
+        assert np.array_equal(got[r, n:], x[r, n:]), f"row {r}: changed past its live end"
+        if n == 0:
+            continue
+        e = rel_l2(got[r, :n], want[r, :n])
+        assert e <= tol, (f"case {i} ({samples} samples, dt {dt}, {[op.kind for op in ops]}, "
+                          f"{np.dtype(dtype).name}) row {r} ({n} live): rel L2 {e:.3e}")
+
+
+def draw_widened(u, dt):
+    """One op of the GPU's whole repertoire, the widened ops included: mutes,
+    gain curves and IIR cascades next to the core ones."""
+    kind = int(u[0] * 8)
+    if kind == 0:
+        curve = (P.timeMute(100 + u[1] * 1500), P.lmoMute(1500 + u[1] * 3000, padMs=-10),
+                 P.tableMute([0.0, 1000, 3000], [200.0, 900, 2500]))[int(u[2] * 3)]
+        op = P.muteTopOp if u[3] < 0.5 else P.muteBottomOp
+        return op(curve, taperMs=u[3] * 60)
+    if kind == 1:
+        curve = (P.powerGain(1 + u[1], minMs=5), P.exponentialGain(-6 + 12 * u[1]),
+                 P.knotGain([(0.0, 0.0), (500 + 1000 * u[1], 12.0), (4000.0, 6.0)]))[int(u[2] * 3)]
+        return P.gainCurveOp(curve, tMaxMs=(math.nan, 400.0)[int(u[3] * 2)])
+    if kind == 2:
+        phase = int(u[1] * 2)
+        step = 12 if phase == P.ipZero else 6
+        spec = (P.butterBandPass(3 + 4 * u[2], 60 + 20 * u[3], 2 * step, 2 * step),
+                P.butterLowCut(2 + 6 * u[2], 3 * step),
+                P.butterHighCut(50 + 40 * u[3], 4 * step))[int(u[2] * 3)]
+        return P.iirOp(spec, phase=phase, trend=int(u[3] * 3))
+    core = (0, 3, 4, 6, 7)[kind - 3]   # demean, band-pass, Butterworth FFT, AGC, normalize
+    return draw_op(np.array([(core + 0.5) / 9, u[1], u[2], u[3]]), dt)
+
+
+def widened_cases(count=48):
+    u = synth.uniforms(20261206, (count, 24), stream=13)
+    out = []
+    for i in range(count):
+        samples = LENGTHS[int(u[i, 0] * len(LENGTHS))]
+        dt = (1.0, 2.0, 4.0)[int(u[i, 1] * 3)]
+        nops = 1 + int(u[i, 2] * 4)
+        ops = [draw_widened(u[i, 4 + 4 * k:8 + 4 * k], dt) for k in range(nops)]
+        dtype = np.float32 if u[i, 3] < 0.7 else np.float64
+        rows = 2 + int(u[i, 20] * 8)
+        first = int(u[i, 21] * 3) * 100
+        out.append((i, samples, dt, ops, dtype, rows, first, int(u[i, 22] * 1000)))
+    return out
+
+
+@pytest.mark.parametrize("case", widened_cases(), ids=lambda c: f"wide{c[0]}-n{c[1]}")
+def test_sweep_widened(gpu, oracle, case):
+    """The same bar with mutes, gain curves and IIR cascades in the flows,
+    the rows' t0 and offset from a ChunkMeta, and windows from trace sample
+    0, 100 or 200."""
+    import torch
+    i, samples, dt, ops, dtype, rows, first, seed = case
+    flow = P.initPipeline(ops)
+    try:
+        plan = flow.compile(dtype, dt, samples)
+    except (ValueError, P.UnsupportedOp) as e:
+        pytest.skip(f"not compiled: {e}")
+    x, _ = synth.gather(rows, samples, dt, seed, dtype=dtype)
+    x = x + dtype(0.2)
+    live = synth.live_lengths(seed + 1, rows, 0, samples).astype(np.int64)
+    live[0] = samples
+    t0 = np.where(np.arange(rows) % 3 == 1, 37.5, 0.0)
+    offsets = np.linspace(0, 2500, rows)
+    status = np.zeros(rows, np.uint8)
+    if rows > 2:
+        status[1] = P.tsDead
+    meta = P.ChunkMeta(rows, dt, numSamples=live + first, status=status, t0Ms=t0,
+                       offset=offsets, firstSample=first)
+    rl = meta.row_len(samples)
+    src = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    dst = torch.empty_like(src)
+    plan.apply(src, dst, meta)
+    got = dst.cpu().numpy()
+    want = oracle.pipeline_apply(flow.ops, x.astype(np.float64), rl, dt, keys=meta.keys(),
+                                 first_sample=first)
+    tol = 1e-5 if dtype == np.float32 else 1e-12
+    for r in range(rows):
+        n = int(rl[r])
         if n < 0:
             assert np.array_equal(got[r], x[r]), f"row {r}: not live, not copied"
             continue
