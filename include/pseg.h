@@ -245,6 +245,9 @@ typedef struct pseg_batch_item {
   void *dst;               /* device, same shape; == src runs in place */
   int32_t rows, cols;      /* cols <= the plan's samples */
   const int32_t *row_len;  /* device, rows int32 from pseg_meta_flatten */
+  const double *row_keys;  /* device (t0Ms, offset) per row, or NULL (as in
+                              pseg_plan_apply_keys): mutes and gain curves */
+  int64_t first_sample;    /* ChunkMeta.firstSample, >= 0 */
 } pseg_batch_item;
 typedef struct pseg_batch pseg_batch;
 

@@ -66,7 +66,8 @@ class CorrInfoC(C.Structure):
 
 class BatchItemC(C.Structure):
     _fields_ = [("plan", C.c_void_p), ("src", C.c_void_p), ("dst", C.c_void_p),
-                ("rows", C.c_int32), ("cols", C.c_int32), ("row_len", C.c_void_p)]
+                ("rows", C.c_int32), ("cols", C.c_int32), ("row_len", C.c_void_p),
+                ("row_keys", C.c_void_p), ("first_sample", C.c_int64)]
 
 
 class BatchInfoC(C.Structure):

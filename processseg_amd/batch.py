@@ -24,12 +24,15 @@ from .pipeline import ChunkMeta, PipelinePlan, _stream_handle, sameInterval
 class BatchItem:
     """One record: plan.apply(src, dst, meta) (dst may be src: in place).
     `row_len` (device int32, from ChunkMeta.device_row_len) may replace
-    `meta`."""
+    `meta`; then `keys` ([rows, 2] float64 device tensor of t0Ms, offset) and
+    `first_sample` give what mutes and gain curves read from a meta."""
     plan: PipelinePlan
     src: object
     dst: object = None
     meta: Optional[ChunkMeta] = None
     row_len: object = None
+    keys: object = None
+    first_sample: int = 0
 
 
 class PipelineBatch:
@@ -50,6 +53,7 @@ class PipelineBatch:
                 raise ValueError(f"item {i}: src is {tuple(it.src.shape)}, dst {tuple(dst.shape)}")
             rows, cols = it.src.shape
             row_len = it.row_len
+            keys, first = it.keys, int(it.first_sample)
             if row_len is None:
                 if it.meta is None:
                     raise ValueError(f"item {i}: needs a ChunkMeta (or row_len)")
@@ -60,9 +64,18 @@ class PipelineBatch:
                     raise ValueError(f"item {i}: the plan is for dtMs = {it.plan.dtMs}, the rows "
                                      f"have {it.meta.dtMs}")
                 row_len = it.meta.device_row_len(cols, it.src.device)
+                if it.plan.readsKeys:
+                    keys = it.meta.device_keys(it.src.device)
+                first = it.meta.firstSample
+            if keys is not None and (not isinstance(keys, torch.Tensor) or not keys.is_cuda
+                                     or keys.dtype != torch.float64
+                                     or tuple(keys.shape) != (rows, 2)
+                                     or not keys.is_contiguous()):
+                raise ValueError(f"item {i}: keys must be a contiguous [rows, 2] float64 GPU tensor")
             arr[i] = _lib.BatchItemC(it.plan._h.value, it.src.data_ptr(), dst.data_ptr(), rows,
-                                     cols, row_len.data_ptr())
-            keep += [it.plan, it.src, dst, row_len]
+                                     cols, row_len.data_ptr(),
+                                     keys.data_ptr() if keys is not None else None, first)
+            keep += [it.plan, it.src, dst, row_len, keys]
         self._keep = keep  # the plans and buffers outlive the batch
         self._h = C.c_void_p()
         _lib.check(_lib.lib().pseg_batch_create(arr, len(items), C.byref(self._h)))

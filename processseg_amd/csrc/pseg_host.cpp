@@ -1343,6 +1343,9 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
     if (it.rows == 0) continue;
     if (!it.src || !it.dst || !it.row_len)
       return fail(PSEG_E_INVALID, "item %d: NULL panel or row_len", i);
+    if (it.first_sample < 0)
+      return fail(PSEG_E_INVALID, "item %d: firstSample must be >= 0, got %lld", i,
+                  (long long)it.first_sample);
     if (it.cols > P->samples)
       return fail(PSEG_E_INVALID, "item %d: the block has %d columns, the plan takes %d samples",
                   i, it.cols, P->samples);
@@ -1412,6 +1415,8 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
       J.src = it.src;
       J.dst = it.dst;
       J.num = it.row_len;
+      J.keys = it.row_keys;
+      J.col0 = it.first_sample;
       J.src_pitch = J.dst_pitch = it.cols;
       J.rows = it.rows;
       J.cols = it.cols;

@@ -232,6 +232,8 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 1) void pipeline_kernel(J
       J.dst = uni_ptr(G.dst);
       J.num = uni_ptr(G.num);
       J.factor = uni_ptr(G.factor);
+      J.keys = uni_ptr(G.keys);
+      J.col0 = ((int64_t)uni((int)(G.col0 >> 32)) << 32) | (uint32_t)uni((int)G.col0);
       J.src_pitch = ((int64_t)uni((int)(G.src_pitch >> 32)) << 32) | (uint32_t)uni((int)G.src_pitch);
       J.dst_pitch = ((int64_t)uni((int)(G.dst_pitch >> 32)) << 32) | (uint32_t)uni((int)G.dst_pitch);
       J.rows = uni(G.rows);

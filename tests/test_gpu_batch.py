@@ -125,3 +125,38 @@ def test_batch_refuses_overlapping_items(gpu):
     l = torch.full((8,), 1000, dtype=torch.int32, device="cuda")
     with pytest.raises(ValueError, match="overlaps"):
         P.PipelineBatch([P.BatchItem(plan, a, row_len=l), P.BatchItem(plan, a[:4], row_len=l)])
+
+
+def test_batch_of_flows_reading_trace_keys(gpu):
+    """Records whose flows read the rows' keys — a top LMO mute and a t²
+    gain curve, an IIR band-pass and a bottom table mute — with their own
+    t0Ms, offsets and windows (firstSample), in one batch: each record
+    equals plan.apply of that record with its ChunkMeta, bit for bit."""
+    import torch
+    flows = [
+        P.initPipeline([P.demeanOp(), P.muteTopOp(P.lmoMute(2000.0), taperMs=20),
+                        P.gainCurveOp(P.powerGain(2.0)), P.agcOp(300)]),
+        P.initPipeline([P.iirOp(P.butterBandPass(4, 60, 24)),
+                        P.muteBottomOp(P.tableMute([0.0, 2000], [1500.0, 2600]), taperMs=40)]),
+    ]
+    recs = make_records(12, seed=23)
+    items, metas, want = [], [], []
+    for i, (x, live, dt) in enumerate(recs):
+        rows, cols = x.shape
+        first = (i % 3) * 50
+        meta = P.ChunkMeta(rows, dt, numSamples=np.maximum(live, 0) + first,
+                           t0Ms=np.linspace(0, 40, rows), offset=np.linspace(-1500, 2500, rows),
+                           firstSample=first)
+        plan = flows[i % 2].compile(np.float32, dt, cols)
+        src = torch.from_numpy(x).cuda()
+        ref = torch.empty_like(src)
+        plan.apply(src, ref, meta)
+        want.append(ref)
+        dst = torch.empty_like(src)
+        items.append(P.BatchItem(plan, src, dst, meta=meta))
+        metas.append(meta)
+    batch = P.PipelineBatch(items)
+    batch.launch()
+    torch.cuda.synchronize()
+    for i, (it, ref) in enumerate(zip(items, want)):
+        assert torch.equal(it.dst, ref), f"record {i} differs from its own apply"
