@@ -305,6 +305,42 @@ def iir_flow_leg(torch, P, dev, steps, warmup):
             "margin": plan.requiredMargin}
 
 
+def decode_leg(torch, P, dev, steps, warmup):
+    """SEG-Y decode of the 2000 x 3001 panel's traces (240-byte headers, IBM
+    float32 samples, big-endian) to a float32 panel on the device, 8 rotating
+    buffer pairs (450 MB, beyond the Infinity Cache).  Algorithmic bytes:
+    4 read + 4 written per sample."""
+    from processseg_amd.workloads import big_panel
+    x, _ = big_panel()
+    rows, ns = x.shape
+    a = np.abs(x.astype(np.float64))
+    e = np.where(a > 0, np.floor(np.log2(np.maximum(a, 1e-300)) / 4).astype(np.int64) + 1, 0)
+    f = np.where(a > 0, np.floor(a / 16.0 ** e * 2 ** 24), 0).astype(np.uint64)
+    w = (((x < 0).astype(np.uint32) << 31) | (((e + 64) & 0x7F).astype(np.uint32) << 24)
+         | (f.astype(np.uint32) & 0xFFFFFF))
+    blob = np.zeros((rows, 240 + 4 * ns), np.uint8)
+    blob[:, 240:] = w.astype(">u4").view(np.uint8).reshape(rows, 4 * ns)
+    raws = [torch.from_numpy(blob.reshape(-1)).to(dev) for _ in range(8)]
+    outs = [torch.empty(rows, ns, device=dev) for _ in range(8)]
+    for i in range(warmup):
+        P.decodeSamples(raws[i % 8], P.fmtIbm32, ns, rows=rows, out=outs[i % 8])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        P.decodeSamples(raws[i % 8], P.fmtIbm32, ns, rows=rows, out=outs[i % 8])
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / steps
+    algo = rows * ns * 8
+    return {"workload": "SEG-Y IBM float32 traces (240-byte headers) of the 2000x3001 panel to "
+                        "float32 on the device",
+            "value": round(rows * ns / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5),
+            "roofline": {"bound": "hbm", "achieved": round(algo / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_launch": algo,
+                         "note": "wall clock over back-to-back launches, not kernel events"}}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -445,7 +481,8 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, iir_flow, f64_panel, correlation, uneven_batch, serving), "
+                         "gain_flow, iir_flow, decode, f64_panel, correlation, uneven_batch, "
+                         "serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
@@ -585,6 +622,10 @@ def main():
         if want("iir_flow"):
             log("IIR flow leg")
             sec["iir_flow"] = iir_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("decode"):
+            log("decode leg")
+            sec["decode"] = decode_leg(torch, P, dev, max(20, args.steps // 2), 5)
             torch.cuda.empty_cache()
         if want("f64_panel"):
             log("float64 panel leg")

@@ -145,6 +145,19 @@ typedef struct pseg_plan_info {
   int32_t reserved;
 } pseg_plan_info;
 
+/* ---------------------------------------------------------- sample decode -- */
+
+/* SEG-Y data sample formats (SEG-Y rev 2, binary header bytes 3225-3226),
+ * all big-endian: the decode parseSEG runs before the prefetcher's transform
+ * hands a chunk to the plan (batch.nim:267-309, docs/integration.md:294-298). */
+typedef enum pseg_sample_format {
+  PSEG_FMT_IBM32 = 1,   /* IBM System/360 float: (-1)^s * f/2^24 * 16^(e-64) */
+  PSEG_FMT_INT32 = 2,
+  PSEG_FMT_INT16 = 3,
+  PSEG_FMT_IEEE32 = 5,
+  PSEG_FMT_INT8 = 8
+} pseg_sample_format;
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,
@@ -229,6 +242,17 @@ pseg_status pseg_plan_apply_keys(const pseg_plan *plan, const void *src, void *d
                                  int32_t cols, const int32_t *row_len, const double *row_keys,
                                  int64_t first_sample, void *ws, size_t ws_bytes,
                                  void *stream);
+
+/* Decodes `rows` traces of big-endian samples on the device into a [rows,
+ * cols] panel of `dtype` (C-contiguous): trace r's sample j is the word at
+ * raw + r*pitch + offset + j*bytes (a SEG-Y file's traces: offset 240, pitch
+ * 240 + bytes * samples).  IBM, int16, int8 and IEEE words decode exactly;
+ * int32 to float32 rounds to nearest.  For 4-byte formats raw, pitch and
+ * offset must be multiples of 4.  Asynchronous on `stream`.  No counterpart
+ * in processSEG itself: parseSEG decodes on the host (its decode workers). */
+pseg_status pseg_decode_samples(int32_t format, const void *raw, int64_t pitch, int64_t offset,
+                                int32_t rows, int32_t cols, int32_t dtype, void *dst,
+                                void *stream);
 
 /* ---------------------------------------------------------------- batch -- */
 

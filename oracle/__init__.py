@@ -579,3 +579,37 @@ def rel_l2(a, b) -> float:
     d = math.sqrt(float(np.sum((a - b) ** 2)))
     r = math.sqrt(float(np.sum(b ** 2)))
     return d / r if r > 0 else d
+
+
+# ---------------------------------------------------------- sample decode --
+
+def decode_samples(raw: np.ndarray, fmt: int, samples: int, rows: int, offset: int = 240,
+                   pitch: int | None = None, dtype=np.float32) -> np.ndarray:
+    """SEG-Y sample decode (SEG-Y rev 2 data sample format codes, big-endian
+    words), restated from the standard's definitions in numpy: IBM float32
+    (-1)^s · f/2^24 · 16^(e−64), exact in float64 and rounded once to
+    float32; int32, int16, int8 two's complement; IEEE float32 as is."""
+    nb = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1}[fmt]
+    if pitch is None:
+        pitch = offset + nb * samples
+    raw = np.ascontiguousarray(raw, np.uint8)
+    out = np.empty((rows, samples), dtype)
+    for r in range(rows):
+        b = raw[r * pitch + offset: r * pitch + offset + nb * samples]
+        if fmt == 1:
+            w = b.view(">u4").astype(np.uint64)
+            frac = (w & 0xFFFFFF).astype(np.float64)
+            expo = ((w >> 24) & 0x7F).astype(np.int64)
+            mag = np.ldexp(frac, (4 * expo - 280).astype(np.int32))
+            with np.errstate(over="ignore"):
+                out[r] = np.where((w >> 31) == 1, -mag, mag).astype(dtype)
+        elif fmt == 2:
+            out[r] = b.view(">i4").astype(dtype)
+        elif fmt == 3:
+            out[r] = b.view(">i2").astype(dtype)
+        elif fmt == 5:
+            with np.errstate(invalid="ignore"):
+                out[r] = b.view(">f4").astype(dtype)
+        else:
+            out[r] = b.view("i1").astype(dtype)
+    return out

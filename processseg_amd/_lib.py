@@ -133,6 +133,7 @@ SYMBOLS = [
     "pseg_batch_launch", "pseg_batch_info_get", "pseg_batch_destroy", "pseg_server_create",
     "pseg_server_add_flow", "pseg_server_submit", "pseg_server_wait", "pseg_server_read",
     "pseg_server_release", "pseg_server_info_get", "pseg_server_destroy",
+    "pseg_decode_samples",
 ]
 
 
@@ -164,6 +165,7 @@ def lib():
     L.pseg_plan_workspace_size.argtypes = [vp, i32, i32, C.POINTER(C.c_size_t)]
     L.pseg_plan_apply.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_size_t, vp]
     L.pseg_plan_apply_keys.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, C.c_size_t, vp]
+    L.pseg_decode_samples.argtypes = [i32, vp, C.c_int64, C.c_int64, i32, i32, i32, vp, vp]
     L.pseg_corr_compile.argtypes = [i32, vp, i32, f64, f64, f64, i32, i32, i32, f64, i32,
                                     C.POINTER(vp)]
     L.pseg_corr_destroy.argtypes = [vp]

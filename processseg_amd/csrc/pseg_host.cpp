@@ -1122,6 +1122,32 @@ int32_t pseg_device_count(void) {
   return c;
 }
 
+pseg_status pseg_decode_samples(int32_t format, const void *raw, int64_t pitch, int64_t offset,
+                                int32_t rows, int32_t cols, int32_t dtype, void *dst,
+                                void *stream) {
+  if (format != PSEG_FMT_IBM32 && format != PSEG_FMT_INT32 && format != PSEG_FMT_INT16 &&
+      format != PSEG_FMT_IEEE32 && format != PSEG_FMT_INT8)
+    return fail(PSEG_E_INVALID, "unknown sample format %d (SEG-Y codes 1, 2, 3, 5 and 8)", format);
+  if (dtype != PSEG_F32 && dtype != PSEG_F64) return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (rows == 0 || cols == 0) return PSEG_OK;
+  if (!raw || !dst) return fail(PSEG_E_INVALID, "raw or dst is NULL");
+  const int64_t bytes = format == PSEG_FMT_INT16 ? 2 : format == PSEG_FMT_INT8 ? 1 : 4;
+  if (offset < 0 || pitch < offset + (int64_t)cols * bytes)
+    return fail(PSEG_E_INVALID, "%d samples of %lld bytes from byte %lld do not fit a trace of %lld "
+                "bytes", cols, (long long)bytes, (long long)offset, (long long)pitch);
+  if (bytes == 4 && ((((uintptr_t)raw) | (uintptr_t)pitch | (uintptr_t)offset) & 3))
+    return fail(PSEG_E_INVALID, "4-byte samples need raw, pitch and offset at multiples of 4");
+  const size_t in_bytes = (size_t)(rows - 1) * (size_t)pitch + (size_t)offset + (size_t)cols * bytes;
+  const size_t out_bytes = (size_t)rows * (size_t)cols * (dtype == PSEG_F32 ? 4 : 8);
+  const uintptr_t a = (uintptr_t)raw, b = (uintptr_t)dst;
+  if (a < b + out_bytes && b < a + in_bytes)
+    return fail(PSEG_E_INVALID, "dst overlaps the raw traces");
+  hipError_t e = launch_decode(dtype, format, raw, pitch, offset, rows, cols, dst, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "launch of the sample decode");
+  return PSEG_OK;
+}
+
 pseg_status pseg_meta_flatten(int32_t rows, int32_t cols, int64_t first_sample,
                               const int64_t *num_samples, const uint8_t *status,
                               const int64_t *failed, int32_t n_failed,

@@ -165,8 +165,9 @@ struct DevPlan {
 // [crop0, crop1) are stored to dst[r * dst_pitch + 0 ..).  When has_factor,
 // the live part is first scaled by *factor (a part after a per-panel
 // normalization).  Jobs of a batch are sorted by item0, the first work item
-// (row) of the job.  Mutes read the rows' trace keys (t0Ms, offset per row;
-// NULL: 0 and NaN) and col0, the trace sample of window column 0.
+// (row) of the job.  Mutes and gain curves read the rows' trace keys (t0Ms,
+// offset per row; NULL: 0 and NaN) and col0, the trace sample of window
+// column 0.
 struct Job {
   const DevPlan *plan;
   const void *src;
@@ -234,6 +235,11 @@ hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t c
                                double percentile, double *row_sq, uint32_t *row_max,
                                uint32_t *hist, double *factor, hipStream_t stream);
 // The correlation kernel with `threads` (512 or 1024) threads per record.
+// SEG-Y sample decode (format 1, 2, 3, 5 or 8, big-endian) of `rows` rows
+// `pitch` bytes apart, samples from byte `offset` of each, to a [rows, cols]
+// panel of dtype.
+hipError_t launch_decode(int dtype, int format, const void *raw, int64_t pitch, int64_t offset,
+                         int32_t rows, int32_t cols, void *dst, hipStream_t s);
 hipError_t launch_correlate(int dtype, int threads, const DevCorr *dev_plan, size_t lds_bytes,
                             const void *src, int32_t rows, int32_t src_cols,
                             const int32_t *row_len, void *dst, hipStream_t stream);
