@@ -1218,10 +1218,11 @@ __device__ __forceinline__ void gain_step(const DevGain &G, T (&v)[PER], int n, 
 // so the scan runs on x − u0 from rest (a constant pad then scans as exact
 // zeros, and the rounding scales with the signal, not with the level it
 // sits on).  (1) each thread runs its chunk from rest for e, and the two unit
-// states with no input for M; (2) an inclusive Hillis-Steele scan of the
-// maps over the threads in LDS (`sc`: six float64 per thread); (3) each
-// thread runs its chunk again from its start state (the maps of the chunks
-// before it applied to rest), writing the outputs.  The outputs equal the
+// states with no input for M; (2) an inclusive scan of the maps over the
+// threads: Hillis-Steele by shuffles inside each wave, then the wave totals
+// through LDS (`sc`: six float64 per wave); (3) each thread runs its chunk
+// again from its start state (the maps of the chunks before it applied to
+// rest), writing the outputs.  The outputs equal the
 // serial recursion's up to float64 rounding.
 template <int NTH>
 __device__ __forceinline__ void sos_scan_pass(double *u, int m, const double *c6, bool steady,
@@ -1250,31 +1251,42 @@ __device__ __forceinline__ void sos_scan_pass(double *u, int m, const double *c6
     const double n12 = -a1 * m12 + m22, n22 = -a2 * m12;
     m11 = n11; m21 = n21; m12 = n12; m22 = n22;
   }
-  // (2) the scan: after it, thread t holds the composition of chunks 0 .. t
-  double *S = sc + 6 * tid_;
-  for (int d = 1; d < NTH; d <<= 1) {
-    S[0] = m11; S[1] = m21; S[2] = m12; S[3] = m22; S[4] = e1; S[5] = e2;
-    __syncthreads();
-    if (tid_ >= d) {
-      const double *Q = sc + 6 * (tid_ - d);  // chunks before: apply first
-      const double q11 = Q[0], q21 = Q[1], q12 = Q[2], q22 = Q[3], f1 = Q[4], f2 = Q[5];
+  // (2) the scan: thread t composes the maps of chunks 0 .. t — inside each
+  // wave by shuffles (no barrier), then across waves through the wave totals
+  // in LDS (one barrier)
+  const int lane = tid_ & 63, wid = tid_ >> 6;
+  for (int d = 1; d < 64; d <<= 1) {
+    const double q11 = __shfl_up(m11, d), q21 = __shfl_up(m21, d);
+    const double q12 = __shfl_up(m12, d), q22 = __shfl_up(m22, d);
+    const double f1 = __shfl_up(e1, d), f2 = __shfl_up(e2, d);
+    if (lane >= d) {  // the chunks before apply first
       const double r11 = m11 * q11 + m12 * q21, r21 = m21 * q11 + m22 * q21;
       const double r12 = m11 * q12 + m12 * q22, r22 = m21 * q12 + m22 * q22;
       const double g1 = m11 * f1 + m12 * f2 + e1, g2 = m21 * f1 + m22 * f2 + e2;
       m11 = r11; m21 = r21; m12 = r12; m22 = r22; e1 = g1; e2 = g2;
     }
+  }
+  // the end state of the waves before this one, from rest
+  double w1 = 0, w2 = 0;
+  if constexpr (NTH > 64) {
+    if (lane == 63) {
+      double *S = sc + 6 * wid;
+      S[0] = m11; S[1] = m21; S[2] = m12; S[3] = m22; S[4] = e1; S[5] = e2;
+    }
     __syncthreads();
+    for (int w = 0; w < wid; w++) {  // whole waves, in order
+      const double *Q = sc + 6 * w;
+      const double t1 = Q[0] * w1 + Q[2] * w2 + Q[4], t2 = Q[1] * w1 + Q[3] * w2 + Q[5];
+      w1 = t1;
+      w2 = t2;
+    }
   }
-  S[0] = m11; S[1] = m21; S[2] = m12; S[3] = m22; S[4] = e1; S[5] = e2;
-  __syncthreads();
-  // (3) the chunk again from its start state: the maps of chunks 0 .. t-1
-  // applied to rest
-  double s1 = 0, s2 = 0;
-  if (tid_ > 0) {
-    const double *Q = sc + 6 * (tid_ - 1);
-    s1 = Q[4];
-    s2 = Q[5];
-  }
+  // (3) the chunk again from its start state: the end state of chunks
+  // 0 .. t-1 from rest — the lane below's (its composition applied to the
+  // waves before), or the waves before alone for a wave's first lane
+  const double x1 = __shfl_up(m11 * w1 + m12 * w2 + e1, 1);
+  const double x2 = __shfl_up(m21 * w1 + m22 * w2 + e2, 1);
+  double s1 = lane > 0 ? x1 : w1, s2 = lane > 0 ? x2 : w2;
   const double y0 = g * u0;  // the steady output of the level
 #pragma unroll 4
   for (int p = p0; p < p1; p++) {
