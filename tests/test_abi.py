@@ -131,3 +131,25 @@ def test_server_checks_arguments():
     t = C.c_uint64()
     assert L.pseg_server_submit(None, 0, 0, 1, 0, 1, C.byref(t)) == _lib.E_INVALID
     assert L.pseg_server_destroy(None) == _lib.OK
+
+
+def test_decode_checks_before_touching_memory():
+    """pseg_decode_samples refuses bad arguments before any device work (the
+    pointers below are never dereferenced)."""
+    L = _lib.lib()
+    fake_raw, fake_dst = 1 << 20, 1 << 30
+    bad = [
+        (4, fake_raw, 4240, 240, 2, 1000, 0, fake_dst),        # not a SEG-Y format code
+        (1, fake_raw, 4240, 240, 2, 1000, 7, fake_dst),        # unknown dtype
+        (1, fake_raw, 4240, 240, -1, 1000, 0, fake_dst),       # negative rows
+        (1, None, 4240, 240, 2, 1000, 0, fake_dst),            # NULL raw
+        (1, fake_raw, 4000, 240, 2, 1000, 0, fake_dst),        # samples past the pitch
+        (1, fake_raw, 4242, 240, 2, 1000, 0, fake_dst),        # 4-byte words misaligned
+        (1, fake_raw + 2, 4240, 240, 2, 1000, 0, fake_dst),    # raw misaligned
+        (1, fake_raw, 4240, 240, 2, 1000, 0, fake_raw + 64),   # dst over the traces
+    ]
+    for args in bad:
+        assert L.pseg_decode_samples(*args, None) == _lib.E_INVALID, args
+    # empty shapes are no-ops
+    assert L.pseg_decode_samples(1, fake_raw, 4240, 240, 0, 1000, 0, fake_dst, None) == _lib.OK
+    assert L.pseg_decode_samples(1, fake_raw, 4240, 240, 2, 0, 0, fake_dst, None) == _lib.OK
