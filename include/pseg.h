@@ -254,6 +254,19 @@ pseg_status pseg_decode_samples(int32_t format, const void *raw, int64_t pitch, 
                                 int32_t rows, int32_t cols, int32_t dtype, void *dst,
                                 void *stream);
 
+/* pseg_plan_apply_keys with the source as raw SEG-Y traces (as
+ * pseg_decode_samples reads them) instead of a panel: each row is decoded
+ * as it is loaded and the plan runs on it, so the traces cross HBM once,
+ * into dst ([rows, cols] of the plan's dtype; samples past a live end hold
+ * the decoded values).  Plans with a per-panel normalization are refused
+ * (PSEG_E_UNSUPPORTED: their statistics read a decoded panel).  No
+ * counterpart in processSEG: the prefetcher's transform (batch.nim:267-309)
+ * runs the plan on parseSEG's decoded chunk, a second pass. */
+pseg_status pseg_plan_apply_raw(const pseg_plan *plan, int32_t format, const void *raw,
+                                int64_t pitch, int64_t offset, void *dst, int32_t rows,
+                                int32_t cols, const int32_t *row_len, const double *row_keys,
+                                int64_t first_sample, void *stream);
+
 /* ---------------------------------------------------------------- batch -- */
 
 /* Many blocks, each with its own plan (different sample intervals, widths

@@ -4,40 +4,11 @@
 // (SEG-Y rev 2, data sample format code): 1 IBM float32, 2 int32, 3 int16,
 // 5 IEEE float32, 8 int8 — what parseSEG's decode produces before the
 // prefetcher's transform runs the plan (batch.nim:267-309).
-#include "pseg_internal.hpp"
+#include "pseg_device.hpp"
 
 namespace pseg {
 
 namespace {
-
-// IBM System/360 single precision: sign, 7-bit base-16 exponent in excess
-// 64, 24-bit fraction below the radix point: (-1)^s · f/2^24 · 16^(e-64).
-// The value is exact in float64 (24 significant bits, binary exponents
-// -280 .. 228), so float32 output is one correctly rounded conversion.
-__device__ __forceinline__ double ibm_to_double(uint32_t w) {
-  const double mag = ldexp((double)(w & 0xFFFFFFu), 4 * (int)((w >> 24) & 0x7Fu) - 280);
-  return (w >> 31) ? -mag : mag;
-}
-
-// The raw word of sample j (big-endian, as a file holds it), widened.
-template <int FMT>
-__device__ __forceinline__ uint32_t load_word(const unsigned char *p) {
-  if constexpr (FMT == 3) return (uint32_t)((p[0] << 8) | p[1]);
-  else if constexpr (FMT == 8) return (uint32_t)p[0];
-  else return __builtin_bswap32(*(const uint32_t *)p);
-}
-
-// The word's value as T: IBM through its exact float64 value; int32 straight
-// to T (one rounding to nearest for float32); IEEE float32 as it is (a NaN
-// keeps its payload in float32).
-template <int FMT, typename T>
-__device__ __forceinline__ T word_value(uint32_t w) {
-  if constexpr (FMT == 1) return (T)ibm_to_double(w);
-  else if constexpr (FMT == 2) return (T)(int32_t)w;
-  else if constexpr (FMT == 3) return (T)(int16_t)(uint16_t)w;
-  else if constexpr (FMT == 5) return (T)__uint_as_float(w);
-  else return (T)(int8_t)(uint8_t)w;
-}
 
 // One workgroup of 256 threads per row; each thread loads K words a round
 // before it converts and stores them (K loads in flight).  Measured on the

@@ -1352,6 +1352,52 @@ __device__ __forceinline__ void iir_step(const DevIir &I, T (&v)[PER], int n, ch
   __syncthreads();  // before the next step writes LDS
 }
 
+// ------------------------------------------------- SEG-Y sample words
+
+// IBM System/360 single precision: sign, 7-bit base-16 exponent in excess
+// 64, 24-bit fraction below the radix point: (-1)^s · f/2^24 · 16^(e-64).
+// The value is exact in float64 (24 significant bits, binary exponents
+// -280 .. 228), so float32 output is one correctly rounded conversion.
+__device__ __forceinline__ double ibm_to_double(uint32_t w) {
+  const double mag = ldexp((double)(w & 0xFFFFFFu), 4 * (int)((w >> 24) & 0x7Fu) - 280);
+  return (w >> 31) ? -mag : mag;
+}
+
+// The raw word of sample j (big-endian, as a file holds it), widened.
+template <int FMT>
+__device__ __forceinline__ uint32_t load_word(const unsigned char *p) {
+  if constexpr (FMT == 3) return (uint32_t)((p[0] << 8) | p[1]);
+  else if constexpr (FMT == 8) return (uint32_t)p[0];
+  else return __builtin_bswap32(*(const uint32_t *)p);
+}
+
+// The word's value as T: IBM through its exact float64 value; int32 straight
+// to T (one rounding to nearest for float32); IEEE float32 as it is (a NaN
+// keeps its payload in float32).
+template <int FMT, typename T>
+__device__ __forceinline__ T word_value(uint32_t w) {
+  if constexpr (FMT == 1) return (T)ibm_to_double(w);
+  else if constexpr (FMT == 2) return (T)(int32_t)w;
+  else if constexpr (FMT == 3) return (T)(int16_t)(uint16_t)w;
+  else if constexpr (FMT == 5) return (T)__uint_as_float(w);
+  else return (T)(int8_t)(uint8_t)w;
+}
+
+// A raw word of format `fmt` (1, 2, 3, 5 or 8) at p as T, the format chosen
+// at run time (one copy of each conversion, for loops that are not unrolled).
+template <typename T>
+__device__ __forceinline__ T raw_sample(int fmt, const unsigned char *p) {
+  switch (fmt) {
+    case 1: return word_value<1, T>(load_word<1>(p));
+    case 2: return word_value<2, T>(load_word<2>(p));
+    case 3: return word_value<3, T>(load_word<3>(p));
+    case 5: return word_value<5, T>(load_word<5>(p));
+    default: return word_value<8, T>(load_word<8>(p));
+  }
+}
+
+__device__ __forceinline__ int raw_sample_bytes(int fmt) { return fmt == 3 ? 2 : fmt == 8 ? 1 : 4; }
+
 // Per-trace normalization (gain/gain.nim:198-219).
 template <typename T, int NTH, int PER>
 __device__ __forceinline__ void norm_trace_step(const DevStep &st, T (&v)[PER], int n, char *lds,

@@ -1352,6 +1352,65 @@ pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst,
   return PSEG_OK;
 }
 
+pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *raw,
+                                int64_t pitch, int64_t offset, void *dst, int32_t rows,
+                                int32_t cols, const int32_t *row_len, const double *row_keys,
+                                int64_t first_sample, void *stream) {
+  if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
+  if (format != PSEG_FMT_IBM32 && format != PSEG_FMT_INT32 && format != PSEG_FMT_INT16 &&
+      format != PSEG_FMT_IEEE32 && format != PSEG_FMT_INT8)
+    return fail(PSEG_E_INVALID, "unknown sample format %d (SEG-Y codes 1, 2, 3, 5 and 8)", format);
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
+  for (int i = 0; i < P->n_steps; i++)
+    if (P->host.steps[i].kind == kNormPanel)
+      return fail(PSEG_E_UNSUPPORTED, "a plan with a per-panel normalization reads its panel "
+                  "statistics from samples: decode first (pseg_decode_samples), then apply");
+  if (rows == 0 || cols == 0) return PSEG_OK;
+  if (!raw || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL traces, panel or row_len");
+  if (cols > P->samples)
+    return fail(PSEG_E_INVALID, "the block has %d columns, the plan takes %d samples", cols, P->samples);
+  const int64_t bytes = format == PSEG_FMT_INT16 ? 2 : format == PSEG_FMT_INT8 ? 1 : 4;
+  if (offset < 0 || pitch < offset + (int64_t)cols * bytes)
+    return fail(PSEG_E_INVALID, "%d samples of %lld bytes from byte %lld do not fit a trace of %lld "
+                "bytes", cols, (long long)bytes, (long long)offset, (long long)pitch);
+  if (bytes == 4 && ((((uintptr_t)raw) | (uintptr_t)pitch | (uintptr_t)offset) & 3))
+    return fail(PSEG_E_INVALID, "4-byte samples need raw, pitch and offset at multiples of 4");
+  const size_t esz = P->dtype == PSEG_F32 ? 4 : 8;
+  const size_t in_bytes = (size_t)(rows - 1) * (size_t)pitch + (size_t)offset + (size_t)cols * bytes;
+  const uintptr_t a = (uintptr_t)raw, b = (uintptr_t)dst;
+  if (a < b + (size_t)rows * cols * esz && b < a + in_bytes)
+    return fail(PSEG_E_INVALID, "dst overlaps the raw traces");
+  JobLaunch L;
+  memset(&L, 0, sizeof L);
+  Job &J = L.single;
+  J.plan = P->dev;
+  J.src = raw;
+  J.src_fmt = format;
+  J.src_off = offset;
+  J.src_pitch = pitch;
+  J.dst = dst;
+  J.num = row_len;
+  J.keys = row_keys;
+  J.col0 = first_sample;
+  J.dst_pitch = cols;
+  J.rows = rows;
+  J.cols = cols;
+  J.crop1 = cols;
+  J.step_first = 0;
+  J.step_last = P->n_steps;
+  // the decoded row passes through LDS; the kernel form with every step
+  size_t lds = std::max<size_t>(1024, (size_t)cols * esz);
+  for (int i = 0; i < P->n_steps; i++) lds = std::max(lds, P->step_lds[i]);
+  L.lds_bytes = lds;
+  L.total = rows;
+  L.max_cols = cols;
+  L.full = 1;
+  hipError_t e = launch_pipeline(P->dtype, P->threads, L, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "pipeline launch");
+  return PSEG_OK;
+}
+
 // ------------------------------------------------------------------ batch
 
 pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batch **out) {

@@ -177,8 +177,11 @@ struct Job {
   const double *keys;
   int64_t src_pitch, dst_pitch;
   int64_t col0;
+  int64_t src_off;    // raw traces: the byte of each trace's sample 0
   int32_t rows, cols, first, crop0, crop1, item0;
-  int32_t step_first, step_last, has_factor, reserved;
+  int32_t step_first, step_last, has_factor;
+  int32_t src_fmt;    // 0: src holds T; else raw SEG-Y traces of this format,
+                      // src_pitch bytes apart (the kernel form with every step)
 };
 
 // One launch: a single job by value (n_jobs == 0) or n_jobs jobs in device

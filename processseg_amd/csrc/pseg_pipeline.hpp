@@ -38,10 +38,29 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   const int n_row = num < 0 ? -1 : min(max(num - J.first, 0), cols);
   const int n = n_row;
   T v[PER];
+  bool raw = false;
+  if constexpr (FULL) raw = uni(J.src_fmt) != 0;
+  if (raw) {
+    // raw SEG-Y traces: each lane decodes its own samples into LDS (one copy
+    // of the conversions, in a loop that is not unrolled), then loads them
+    const int fmt = uni(J.src_fmt);
+    const int bytes = raw_sample_bytes(fmt);
+    const unsigned char *rb = (const unsigned char *)J.src + r * J.src_pitch + J.src_off;
+    T *st = (T *)lds;
+#pragma unroll 1
+    for (int j = tid_; j < cols; j += NTH) st[j] = raw_sample<T>(fmt, rb + (int64_t)j * bytes);
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NTH * k;
-    v[k] = j < cols ? s[j] : (T)0;
+    for (int k = 0; k < PER; k++) {
+      int j = tid_ + NTH * k;
+      v[k] = j < cols ? st[j] : (T)0;
+    }
+    __syncthreads();  // before the steps write LDS
+  } else {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      int j = tid_ + NTH * k;
+      v[k] = j < cols ? s[j] : (T)0;
+    }
   }
   // Samples past the live end leave now, unchanged: every step changes live
   // samples only.  Their slots then hold +0 for the steps (sums over every
@@ -234,6 +253,8 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 1) void pipeline_kernel(J
       J.factor = uni_ptr(G.factor);
       J.keys = uni_ptr(G.keys);
       J.col0 = ((int64_t)uni((int)(G.col0 >> 32)) << 32) | (uint32_t)uni((int)G.col0);
+      J.src_off = ((int64_t)uni((int)(G.src_off >> 32)) << 32) | (uint32_t)uni((int)G.src_off);
+      J.src_fmt = uni(G.src_fmt);
       J.src_pitch = ((int64_t)uni((int)(G.src_pitch >> 32)) << 32) | (uint32_t)uni((int)G.src_pitch);
       J.dst_pitch = ((int64_t)uni((int)(G.dst_pitch >> 32)) << 32) | (uint32_t)uni((int)G.dst_pitch);
       J.rows = uni(G.rows);

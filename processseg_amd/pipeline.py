@@ -791,6 +791,48 @@ class PipelinePlan:
             ws.data_ptr() if ws is not None else None, need, _stream_handle(stream)))
         return dst
 
+    def applyRaw(self, raw, fmt: int, dst, meta: Optional[ChunkMeta] = None, *, row_len=None,
+                 keys=None, first_sample: int = 0, offset: int = 240,
+                 pitch: Optional[int] = None, stream=None):
+        """apply with the source as raw SEG-Y traces on the device (a uint8
+        tensor: a header of `offset` bytes, then big-endian samples of format
+        `fmt`, traces `pitch` bytes apart): each row is decoded as the kernel
+        loads it, so the traces cross HBM once into `dst` ([rows, cols] of the
+        plan's dtype).  As decodeSamples then apply, in one pass."""
+        import torch
+        nb = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1}.get(fmt)
+        if nb is None:
+            raise ValueError(f"unknown sample format {fmt} (SEG-Y codes 1, 2, 3, 5 and 8)")
+        if not isinstance(raw, torch.Tensor) or not raw.is_cuda or raw.dtype != torch.uint8 \
+                or not raw.is_contiguous():
+            raise ValueError("raw must be a contiguous uint8 GPU tensor")
+        if not isinstance(dst, torch.Tensor) or not dst.is_cuda or dst.dim() != 2 \
+                or not dst.is_contiguous():
+            raise ValueError("dst must be a C-contiguous [rows, cols] GPU tensor")
+        if np.dtype(str(dst.dtype).replace("torch.", "")) != self.T:
+            raise ValueError(f"dst holds {dst.dtype}, the plan {self.T}")
+        rows, cols = dst.shape
+        if pitch is None:
+            pitch = offset + nb * cols
+        if rows > 0 and (rows - 1) * pitch + offset + nb * cols > raw.numel():
+            raise ValueError(f"{rows} traces of pitch {pitch} do not fit {raw.numel()} bytes")
+        if row_len is None:
+            if meta is None:
+                raise ValueError("applyRaw needs a ChunkMeta (or row_len)")
+            if len(meta) != rows:
+                raise ValueError(f"dst has {rows} rows, the meta {len(meta)}")
+            if not sameInterval(meta.dtMs, self.dtMs):
+                raise ValueError(f"the plan is for dtMs = {self.dtMs}, the rows have {meta.dtMs}")
+            row_len = meta.device_row_len(cols, dst.device)
+            if self.readsKeys:
+                keys = meta.device_keys(dst.device)
+            first_sample = meta.firstSample
+        _lib.check(_lib.lib().pseg_plan_apply_raw(
+            self._h, fmt, raw.data_ptr(), pitch, offset, dst.data_ptr(), rows, cols,
+            row_len.data_ptr(), keys.data_ptr() if keys is not None else None, int(first_sample),
+            _stream_handle(stream)))
+        return dst
+
     def __del__(self):
         try:
             if self._h:

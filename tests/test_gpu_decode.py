@@ -103,3 +103,62 @@ def test_decode_then_flow(gpu, oracle):
     for r in range(rows):
         n = int(live[r])
         assert rel_l2(got[r, :n], want[r, :n]) <= 1e-5
+
+
+def ibm_words(x):
+    """x (float64) as IBM float32 words, the fraction truncated to 24 bits."""
+    s = (x < 0).astype(np.uint32)
+    a = np.abs(x)
+    e = np.where(a > 0, np.floor(np.log2(np.maximum(a, 1e-300)) / 4).astype(np.int64) + 1, 0)
+    f = np.where(a > 0, np.floor(a / 16.0 ** e * 2 ** 24), 0).astype(np.uint64)
+    return (s << 31) | (((e + 64) & 0x7F).astype(np.uint32) << 24) | (f.astype(np.uint32) & 0xFFFFFF)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("fmt", [P.fmtIbm32, P.fmtInt16, P.fmtIeee32])
+def test_apply_raw_equals_decode_then_apply(gpu, oracle, dtype, fmt):
+    """Plan.applyRaw on raw traces equals decodeSamples then Plan.apply:
+    bit for bit, for a core flow and for a flow with a mute and a gain curve
+    that read the rows' keys in a window; and the oracle within the bar."""
+    import torch
+    from tests.common import rel_l2
+    rows, ns, first = 48, 2001, 100
+    x, live = W.big_panel(rows, ns, seed=5)
+    xd = x.astype(np.float64) * 1000
+    if fmt == P.fmtIbm32:
+        w, b = ibm_words(xd), ">u4"
+    elif fmt == P.fmtInt16:
+        w, b = np.clip(np.round(xd), -32768, 32767).astype(np.int64), ">i2"
+    else:
+        w, b = xd.astype(np.float32), ">f4"
+    nb = np.dtype(b).itemsize
+    blob = np.zeros((rows, 240 + nb * ns), np.uint8)
+    blob[:, 240:] = np.asarray(w).astype(b).view(np.uint8).reshape(rows, nb * ns)
+    raw = torch.from_numpy(blob.reshape(-1)).cuda()
+    meta = P.ChunkMeta(rows, 2.0, numSamples=live + first, firstSample=first,
+                       t0Ms=np.linspace(0, 30, rows), offset=np.linspace(100, 2500, rows))
+    flows = [W.big_flow(),
+             P.initPipeline([P.demeanOp(), P.bandPassOp(P.Corners(4, 8, 60, 80)),
+                             P.muteTopOp(P.lmoMute(2500.0), taperMs=20),
+                             P.gainCurveOp(P.powerGain(2.0)), P.agcOp(300)])]
+    dec = P.decodeSamples(raw, fmt, ns, rows=rows, dtype=dtype)
+    for flow in flows:
+        plan = flow.compile(dtype, 2.0, ns)
+        two = torch.empty_like(dec)
+        plan.apply(dec, two, meta)
+        one = torch.empty_like(dec)
+        plan.applyRaw(raw, fmt, one, meta)
+        torch.cuda.synchronize()
+        assert torch.equal(one, two), "one pass differs from decode then apply"
+        rl = meta.row_len(ns)
+        want = oracle.pipeline_apply(flow.ops, dec.cpu().numpy().astype(np.float64), rl, 2.0,
+                                     keys=meta.keys(), first_sample=first)
+        got = one.cpu().numpy()
+        tol = 1e-5 if dtype == np.float32 else 1e-12
+        for r in range(rows):
+            n = int(rl[r])
+            if n > 0:
+                assert rel_l2(got[r, :n], want[r, :n]) <= tol, r
+    with pytest.raises(P.UnsupportedOp):
+        panel = P.initPipeline([P.normalizeOp(P.normRms, P.perPanel)]).compile(dtype, 2.0, ns)
+        panel.applyRaw(raw, fmt, torch.empty_like(dec), meta)
