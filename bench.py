@@ -341,6 +341,50 @@ def decode_leg(torch, P, dev, steps, warmup):
                          "note": "wall clock over back-to-back launches, not kernel events"}}
 
 
+def raw_flow_leg(torch, P, dev, steps, warmup):
+    """The headline flow from raw SEG-Y IBM traces of the 2000 x 3001 panel:
+    Plan.applyRaw (each row decoded as the kernel loads it) against
+    decodeSamples then Plan.apply (two passes), 4 buffer sets each."""
+    from processseg_amd.workloads import big_flow, big_panel
+    x, live = big_panel()
+    rows, ns = x.shape
+    a = np.abs(x.astype(np.float64))
+    e = np.where(a > 0, np.floor(np.log2(np.maximum(a, 1e-300)) / 4).astype(np.int64) + 1, 0)
+    f = np.where(a > 0, np.floor(a / 16.0 ** e * 2 ** 24), 0).astype(np.uint64)
+    w = (((x < 0).astype(np.uint32) << 31) | (((e + 64) & 0x7F).astype(np.uint32) << 24)
+         | (f.astype(np.uint32) & 0xFFFFFF))
+    blob = np.zeros((rows, 240 + 4 * ns), np.uint8)
+    blob[:, 240:] = w.astype(">u4").view(np.uint8).reshape(rows, 4 * ns)
+    meta = P.ChunkMeta(rows, DT, numSamples=live)
+    plan = big_flow().compile(np.float32, DT, ns)
+    raws = [torch.from_numpy(blob.reshape(-1)).to(dev) for _ in range(4)]
+    mids = [torch.empty(rows, ns, device=dev) for _ in range(4)]
+    outs = [torch.empty(rows, ns, device=dev) for _ in range(4)]
+
+    def one(i):
+        plan.applyRaw(raws[i % 4], P.fmtIbm32, outs[i % 4], meta)
+
+    def two(i):
+        P.decodeSamples(raws[i % 4], P.fmtIbm32, ns, rows=rows, out=mids[i % 4])
+        plan.apply(mids[i % 4], outs[i % 4], meta)
+
+    res = {}
+    for name, fn in (("one_pass", one), ("decode_then_apply", two)):
+        for i in range(warmup):
+            fn(i)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(steps):
+            fn(i)
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t1) / steps
+    return {"workload": "the headline flow from SEG-Y IBM float32 traces (240-byte headers) of "
+                        "the 2000x3001 panel",
+            "value": round(rows * ns / res["one_pass"] / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(res["one_pass"] * 1000, 5),
+            "ms_decode_then_apply": round(res["decode_then_apply"] * 1000, 5)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -481,8 +525,8 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, iir_flow, decode, f64_panel, correlation, uneven_batch, "
-                         "serving), "
+                         "gain_flow, iir_flow, decode, raw_flow, f64_panel, correlation, "
+                         "uneven_batch, serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
@@ -626,6 +670,10 @@ def main():
         if want("decode"):
             log("decode leg")
             sec["decode"] = decode_leg(torch, P, dev, max(20, args.steps // 2), 5)
+            torch.cuda.empty_cache()
+        if want("raw_flow"):
+            log("raw flow leg")
+            sec["raw_flow"] = raw_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
             torch.cuda.empty_cache()
         if want("f64_panel"):
             log("float64 panel leg")
