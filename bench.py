@@ -83,11 +83,11 @@ def kernel_name(cols, threads, dtype="float"):
     """The instantiation pseg_plan_apply launches for rows of `cols` samples
     with the plan's `threads` per row: the row lives in registers,
     ceil(cols / threads) samples per thread, rounded up to a compiled width;
-    the form with demean / filter / AGC only (`false`)."""
+    the form with demean / filter / AGC only (form 0)."""
     per = -(-cols // threads)
     widths = (8, 16, 24) if threads == 512 else (4, 8, 12, 16, 24)
     per = min(w for w in widths if w >= per)
-    return f"pseg::pipeline_kernel<{dtype}, {threads}, {per}, false>"
+    return f"pseg::pipeline_kernel<{dtype}, {threads}, {per}, 0>"
 
 
 def copy_probe(torch, dev, args):

@@ -972,7 +972,7 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   }
   H.n_steps = n_ops;
   P->n_steps = n_ops;
-  for (int i = 0; i < n_ops; i++) P->full |= !core_step(H.steps[i].kind);
+  for (int i = 0; i < n_ops; i++) P->full = std::max(P->full, step_form(H.steps[i].kind));
   // the fewest threads per row that hold the row (24 samples a thread) and
   // run every transform of the plan in place
   for (int t = pipeline_threads_for(samples); t != 0 && P->threads == 0; t *= 2) {
@@ -1345,7 +1345,8 @@ pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst,
     L.lds_bytes = lds;
     L.total = rows;
     L.max_cols = cols;
-    for (int i = J.step_first; i < J.step_last; i++) L.full |= !core_step(P->host.steps[i].kind);
+    for (int i = J.step_first; i < J.step_last; i++)
+      L.full = std::max(L.full, step_form(P->host.steps[i].kind));
     hipError_t e = launch_pipeline(P->dtype, P->threads, L, s);
     if (e != hipSuccess) return hip_fail(e, "pipeline launch");
   }
@@ -1405,7 +1406,7 @@ pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *
   L.lds_bytes = lds;
   L.total = rows;
   L.max_cols = cols;
-  L.full = 1;
+  L.full = std::max(1, P->full);
   hipError_t e = launch_pipeline(P->dtype, P->threads, L, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "pipeline launch");
   return PSEG_OK;

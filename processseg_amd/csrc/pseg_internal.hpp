@@ -187,9 +187,9 @@ struct Job {
 // One launch: a single job by value (n_jobs == 0) or n_jobs jobs in device
 // memory; `total` work items; direct (one block per item) or persistent
 // (counter != NULL, `grid` blocks take items from *counter, cleared first).
-// `full`: some job runs a step outside demean / filter / AGC, so the launch
-// takes the kernel compiled with every step (the other, smaller one holds
-// only those three, for the instruction cache).
+// `full`: the kernel form (step_form): 0 when every step is demean / filter /
+// AGC (the smallest kernel, for the instruction cache), 1 for the form with
+// every step but the IIR cascades, 2 for the form with every step.
 struct JobLaunch {
   Job single;
   const Job *jobs;
@@ -230,6 +230,9 @@ constexpr int kMaxBatchClasses = 12;
 constexpr bool core_step(int kind) {
   return kind == kDemean || kind == kFilter || kind == kAgc || kind == kNormPanel;
 }
+// The kernel form a step needs (JobLaunch::full): 0 core, 1 every step but
+// IIR, 2 every step.
+constexpr int step_form(int kind) { return core_step(kind) ? 0 : kind == kIir ? 2 : 1; }
 // Per-panel normalization factor (into *factor, NaN: leave the panel alone)
 // of the live rows of x; scratch: rows doubles, 2 rows uint32 and, for a
 // percentile, 65536 uint32 of histogram (cleared here).

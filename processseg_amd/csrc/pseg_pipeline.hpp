@@ -1,7 +1,9 @@
 // The fused row kernel of the panel pipeline, templated on the row width:
-// NTH threads (NTH / 64 wave64) per row, PER samples per thread; FULL: every
-// step kind, else demean / filter / AGC only (less code for the instruction
-// cache, the form most flows take).
+// NTH threads (NTH / 64 wave64) per row, PER samples per thread; FORM 0:
+// demean / filter / AGC only (less code for the instruction cache, the form
+// most flows take), 1: every step kind but the IIR cascades, 2: every step
+// kind (the IIR pass's float64 state costs the other steps a spill, so only
+// flows with an IIR step take it).
 //
 // One workgroup per row.  The row crosses HBM once: it is loaded into
 // registers (sample j on thread j mod NTH, slot j / NTH, so every load and
@@ -27,7 +29,7 @@ namespace {
 // One row of one job: load the row's window into registers, run the job's
 // steps on its live part, store the job's crop.  Every thread of the block
 // runs it with the same (job, row).
-template <typename T, int NTH, int PER, bool FULL>
+template <typename T, int NTH, int PER, int FORM>
 __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedSlots &red) {
   const int tid_ = opaque_tid();
   const DevPlan *plan = J.plan;
@@ -39,7 +41,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   const int n = n_row;
   T v[PER];
   bool raw = false;
-  if constexpr (FULL) raw = uni(J.src_fmt) != 0;
+  if constexpr (FORM >= 1) raw = uni(J.src_fmt) != 0;
   if (raw) {
     // raw SEG-Y traces: each lane decodes its own samples into LDS (one copy
     // of the conversions, in a loop that is not unrolled), then loads them
@@ -111,7 +113,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
         case kFilter: filter_step<T, NTH, PER>(st.filter, v, n, lds, red); break;
         case kAgc: agc_step<T, NTH, PER>(st.agc, v, n, lds, red); break;
         case kDetrend:
-          if constexpr (FULL) {
+          if constexpr (FORM >= 1) {
             const int tid_ = opaque_tid();
             // detrend (core/basic.nim:61-81)
             double a, b;
@@ -125,7 +127,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
           }
           break;
         case kScale:
-          if constexpr (FULL) {
+          if constexpr (FORM >= 1) {
             const int tid_ = opaque_tid();
             const T kf = (T)uni(st.factor);
 #pragma unroll
@@ -136,7 +138,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
           }
           break;
         case kClip:
-          if constexpr (FULL) {
+          if constexpr (FORM >= 1) {
             const int tid_ = opaque_tid();
             const T lo = (T)uni(st.lo), hi = (T)uni(st.hi);
 #pragma unroll
@@ -150,7 +152,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
           }
           break;
         case kPolarity:
-          if constexpr (FULL) {
+          if constexpr (FORM >= 1) {
             const int tid_ = opaque_tid();
 #pragma unroll
             for (int k = 0; k < PER; k++) {
@@ -160,11 +162,11 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
           }
           break;
         case kNormTrace:
-          if constexpr (FULL) norm_trace_step<T, NTH, PER>(st, v, n, lds, red);
+          if constexpr (FORM >= 1) norm_trace_step<T, NTH, PER>(st, v, n, lds, red);
           break;
         case kMuteTop:
         case kMuteBottom:
-          if constexpr (FULL) {
+          if constexpr (FORM >= 1) {
             // muteCol (core/mute.nim:189-194) from the row's keys (the
             // job's fields through readfirstlane: SGPRs, whatever the
             // compiler hoists)
@@ -177,10 +179,10 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
           }
           break;
         case kIir:
-          if constexpr (FULL) iir_step<T, NTH, PER>(st.iir, v, n, lds, red);
+          if constexpr (FORM == 2) iir_step<T, NTH, PER>(st.iir, v, n, lds, red);
           break;
         case kGainCurve:
-          if constexpr (FULL) {
+          if constexpr (FORM >= 1) {
             // sampleTimeMs(r, 0) = t0Ms + firstSample·dt (core/types.nim:237-240)
             const double *keys = (const double *)uni_ptr(J.keys);
             const double t0 = keys ? uni(keys[2 * r]) : 0.0;
@@ -222,7 +224,7 @@ __device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int n_jobs
 // from the counter until they run out, so rows of very different cost
 // balance across the chip; each block exits once the counter passes `total`.
 // n_jobs == 0: the single job passed by value.
-template <typename T, int NTH, int PER, bool FULL>
+template <typename T, int NTH, int PER, int FORM>
 __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
                                                       int32_t n_jobs, int32_t total,
                                                       uint32_t *__restrict__ counter) {
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 1) void pipeline_kernel(J
     }
     (void)jp;
     RedSlots red{red_lds, 2 * (NTH / 64), 0};
-    run_row<T, NTH, PER, FULL>(J, row, lds, red);
+    run_row<T, NTH, PER, FORM>(J, row, lds, red);
     if (counter == nullptr) break;
   }
 }
@@ -278,20 +280,20 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 1) void pipeline_kernel(J
 namespace {
 
 // Lets the kernel take up to kMaxRowLds bytes of dynamic LDS (once).
-template <typename T, int NTH, int PER, bool FULL>
+template <typename T, int NTH, int PER, int FORM>
 hipError_t prepare_kernel() {
   static const hipError_t e = hipFuncSetAttribute(
-      (const void *)pipeline_kernel<T, NTH, PER, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (const void *)pipeline_kernel<T, NTH, PER, FORM>, hipFuncAttributeMaxDynamicSharedMemorySize,
       (int)kMaxRowLds);
   return e;
 }
 
-// One launch of pipeline_kernel<T, NTH, PER, FULL>: clears the work counter of a
+// One launch of pipeline_kernel<T, NTH, PER, FORM>: clears the work counter of a
 // persistent launch first (a kernel, not a memset, so the launch can be
 // captured in a graph).
-template <typename T, int NTH, int PER, bool FULL>
+template <typename T, int NTH, int PER, int FORM>
 hipError_t launch_one(const JobLaunch &a, hipStream_t stream) {
-  hipError_t e = prepare_kernel<T, NTH, PER, FULL>();
+  hipError_t e = prepare_kernel<T, NTH, PER, FORM>();
   if (e != hipSuccess) return e;
   if (a.counter) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, a.counter, 1);
@@ -299,20 +301,20 @@ hipError_t launch_one(const JobLaunch &a, hipStream_t stream) {
     if (e != hipSuccess) return e;
   }
   const int grid = a.counter ? a.grid : a.total;
-  hipLaunchKernelGGL((pipeline_kernel<T, NTH, PER, FULL>), dim3(grid), dim3(NTH), a.lds_bytes, stream,
+  hipLaunchKernelGGL((pipeline_kernel<T, NTH, PER, FORM>), dim3(grid), dim3(NTH), a.lds_bytes, stream,
                      a.single, a.jobs, a.n_jobs, a.total, a.counter);
   return hipGetLastError();
 }
 
-// Workgroups of pipeline_kernel<T, NTH, PER, FULL> resident on the whole device at
+// Workgroups of pipeline_kernel<T, NTH, PER, FORM> resident on the whole device at
 // once with `lds` bytes of dynamic LDS each (the grid of a persistent launch).
-template <typename T, int NTH, int PER, bool FULL>
+template <typename T, int NTH, int PER, int FORM>
 hipError_t resident_one(size_t lds, int *blocks) {
-  hipError_t e = prepare_kernel<T, NTH, PER, FULL>();
+  hipError_t e = prepare_kernel<T, NTH, PER, FORM>();
   if (e != hipSuccess) return e;
   int per_cu = 0, dev = 0, cus = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void *)pipeline_kernel<T, NTH, PER, FULL>, NTH, lds);
+      &per_cu, (const void *)pipeline_kernel<T, NTH, PER, FORM>, NTH, lds);
   if (e != hipSuccess) return e;
   e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;

@@ -12,7 +12,7 @@ struct pseg_plan {
 
   int dtype;
   int threads;           // workgroup size of the fused kernel (pipeline_threads_for)
-  int full;              // a step outside demean / filter / AGC (JobLaunch::full)
+  int full;              // the kernel form its steps need (step_form, JobLaunch::full)
   double dt_ms;
   int32_t samples, record, margin;
   int n_steps;
