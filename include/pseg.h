@@ -335,6 +335,9 @@ typedef struct pseg_server_config {
   size_t arena_bytes;          /* device memory for results */
   int32_t max_batch_rows;      /* rows per launch (0: no limit) */
   int32_t max_inflight;        /* launches in flight, 1 or 2 */
+  const double *trace_keys;    /* device, traces x (t0Ms, offset) float64, or
+                                  NULL (t0Ms 0, offset NaN): what mutes and
+                                  gain curves read */
 } pseg_server_config;
 
 typedef struct pseg_serve_result {

@@ -79,7 +79,8 @@ class ServerConfigC(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("traces", C.c_int32), ("samples", C.c_int32),
                 ("max_batch_requests", C.c_int32), ("pitch", C.c_int64), ("survey", C.c_void_p),
                 ("trace_len", C.c_void_p), ("dt_ms", C.c_double), ("arena_bytes", C.c_size_t),
-                ("max_batch_rows", C.c_int32), ("max_inflight", C.c_int32)]
+                ("max_batch_rows", C.c_int32), ("max_inflight", C.c_int32),
+                ("trace_keys", C.c_void_p)]
 
 
 class ServeResultC(C.Structure):

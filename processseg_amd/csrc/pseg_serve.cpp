@@ -303,7 +303,8 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
       J.rows = r.ntraces;
       J.cols = r.window;
       J.first = r.first;
-      J.col0 = r.first;  // the survey's traces start at t0 0 (no keys)
+      J.keys = cfg.trace_keys ? cfg.trace_keys + 2 * (size_t)r.trace0 : nullptr;
+      J.col0 = r.first;  // the window's column 0 is trace sample `first`
       J.crop0 = r.shown_first - r.first;
       J.crop1 = J.crop0 + (r.shown_last - r.shown_first + 1);
       J.item0 = item0;

@@ -40,10 +40,12 @@ class ServeResult:
 class Server:
     def __init__(self, survey, trace_len, dtMs: float, *, arena_bytes: int = 1 << 30,
                  max_batch_requests: int = 256, max_batch_rows: int = 0,
-                 max_inflight: int = 2):
+                 max_inflight: int = 2, trace_keys=None):
         """`survey`: a [traces, samples] GPU tensor (C-contiguous rows);
         `trace_len`: int32 GPU tensor, each trace's numSamples (-1: not
-        live).  Both must outlive the server."""
+        live); `trace_keys`: optional [traces, 2] float64 GPU tensor of each
+        trace's (t0Ms, offset), which mutes and gain curves read (None: t0Ms
+        0, offset NaN).  All must outlive the server."""
         import torch
         if not isinstance(survey, torch.Tensor) or not survey.is_cuda or survey.dim() != 2 \
                 or survey.stride(1) != 1:
@@ -52,7 +54,12 @@ class Server:
                 or trace_len.numel() != survey.shape[0] or not trace_len.is_cuda:
             raise ValueError("trace_len must be an int32 GPU tensor with one value per trace")
         self.T = np.dtype(str(survey.dtype).replace("torch.", ""))
-        self._keep = (survey, trace_len)
+        if trace_keys is not None and (
+                not isinstance(trace_keys, torch.Tensor) or not trace_keys.is_cuda
+                or trace_keys.dtype != torch.float64 or not trace_keys.is_contiguous()
+                or tuple(trace_keys.shape) != (survey.shape[0], 2)):
+            raise ValueError("trace_keys must be a contiguous [traces, 2] float64 GPU tensor")
+        self._keep = (survey, trace_len, trace_keys)
         cfg = _lib.ServerConfigC()
         cfg.dtype = _dtype_code(self.T)
         cfg.traces, cfg.samples = survey.shape
@@ -64,6 +71,7 @@ class Server:
         cfg.arena_bytes = arena_bytes
         cfg.max_batch_rows = max_batch_rows
         cfg.max_inflight = max_inflight
+        cfg.trace_keys = trace_keys.data_ptr() if trace_keys is not None else None
         self.traces, self.samples, self.dtMs = int(survey.shape[0]), int(survey.shape[1]), dtMs
         self._h = C.c_void_p()
         _lib.check(_lib.lib().pseg_server_create(C.byref(cfg), C.byref(self._h)))

@@ -170,3 +170,37 @@ def test_server_argument_errors(gpu, survey):
     with pytest.raises(P.UnsupportedOp, match="per-panel"):
         srv.add_flow(P.initPipeline([P.normalizeOp(P.normRms, scope=P.perPanel)]))
     srv.close()
+
+
+def test_served_flows_read_trace_keys(gpu, survey):
+    """A flow with a top LMO mute and a t² gain curve, served with the
+    survey's per-trace keys (t0Ms, offset), equals Plan.apply on the same
+    decoded window with those keys and the window's firstSample, bit for
+    bit."""
+    import torch
+    x, tl, xs, tls = survey
+    keys = np.stack([np.where(np.arange(TRACES) % 3 == 0, 20.0, 0.0),
+                     np.linspace(-3000, 3000, TRACES)], axis=1)
+    kd = torch.from_numpy(keys).cuda()
+    flow = P.initPipeline([P.bandPassOp(P.Corners(4, 8, 60, 80)),
+                           P.muteTopOp(P.lmoMute(2500.0), taperMs=30),
+                           P.gainCurveOp(P.powerGain(2.0)), P.agcOp(400)])
+    srv = P.Server(xs, tls, DT, arena_bytes=128 << 20, trace_keys=kd)
+    fid = srv.add_flow(flow)
+    for t0, n, a, b in ((5, 40, 0, 999), (300, 64, 1500, 3200), (590, 10, 3000, 4000)):
+        t = srv.submit(fid, t0, n, a, b)
+        srv.wait(t)
+        got = srv.read(t).cpu().numpy()
+        srv.release(t)
+        m = flow.requiredMargin(DT)
+        w = P.decodeWindow(range(a, b + 1), m, SAMPLES)
+        first, cols = w.start, len(w)
+        win = torch.from_numpy(np.ascontiguousarray(x[t0:t0 + n, first:first + cols])).cuda()
+        tls_w = tl[t0:t0 + n]
+        row_len = np.where(tls_w < 0, -1, np.clip(tls_w - first, 0, cols)).astype(np.int32)
+        plan = flow.compile(np.float32, DT, cols, recordSamples=SAMPLES)
+        dst = torch.empty_like(win)
+        plan.apply(win, dst, row_len=torch.from_numpy(row_len).cuda(),
+                   keys=kd[t0:t0 + n].contiguous(), first_sample=first)
+        want = dst.cpu().numpy()[:, a - first:b - first + 1]
+        assert np.array_equal(got, want), (t0, n, a, b)
