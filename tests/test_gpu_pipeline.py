@@ -270,7 +270,7 @@ def test_config2_full_panel(gpu, oracle, dtype):
         assert e32 <= F32_TOL
 
 
-@pytest.mark.parametrize("panel_stat", [None, P.normPercentile])
+@pytest.mark.parametrize("panel_stat", [None, P.normPercentile, "widened"])
 def test_apply_graph_replay(gpu, panel_stat):
     """Plan.apply captured in a HIP graph (one kernel; with per-panel
     normalizations, the statistics kernels between the parts, whose
@@ -279,7 +279,11 @@ def test_apply_graph_replay(gpu, panel_stat):
     import torch
     x, live = config2_panel(rows=200, cols=1500)
     ops = [P.demeanOp(), P.bandPassOp(P.Corners(5, 10, 70, 90)), P.agcOp(250)]
-    if panel_stat is not None:
+    if panel_stat == "widened":   # the kernel form with every step
+        ops = [P.demeanOp(), P.iirOp(P.butterBandPass(5, 70, 24)),
+               P.muteTopOp(P.timeMute(300.0), taperMs=20), P.gainCurveOp(P.powerGain(2.0)),
+               P.agcOp(250)]
+    elif panel_stat is not None:
         ops += [P.normalizeOp(panel_stat, scope=P.perPanel)]
     flow = P.initPipeline(ops)
     plan = flow.compile(np.float32, 2.0, 1500)
