@@ -613,3 +613,87 @@ def decode_samples(raw: np.ndarray, fmt: int, samples: int, rows: int, offset: i
         else:
             out[r] = b.view("i1").astype(dtype)
     return out
+
+
+# --------------------------------------------------------------- spectrum --
+
+def _round_half_away(x: float) -> int:
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+def spectrum_plan(dt_ms: float, window_len: int, taper_fraction: float = 0.1, nfft: int = 0,
+                  demean: bool = True) -> dict:
+    """initSpectrumPlan (fft/spectrum.nim:77-101): the Tukey taper (sin² ramps
+    of round(fraction · window) samples at each end), its sum, nfft (the
+    next power of two by default; a 2·3·5-smooth length ≥ the window)."""
+    if window_len < 1:
+        raise ValueError(f"a spectrum window needs >= 1 sample, got {window_len}")
+    if not (dt_ms > 0 and dt_ms < math.inf):
+        raise ValueError(f"dtMs must be finite and > 0, got {dt_ms}")
+    if not (taper_fraction >= 0 and taper_fraction <= 0.5):
+        raise ValueError(f"taperFraction must be in 0 .. 0.5, got {taper_fraction}")
+    n = nfft if nfft else 1 << (window_len - 1).bit_length()
+    if n < window_len:
+        raise ValueError(f"nfft {n} is shorter than the window ({window_len})")
+    if not is_fast_len(n):
+        raise ValueError(f"nfft {n} is not 2·3·5-smooth")
+    ramp = _round_half_away(taper_fraction * window_len)
+    d = np.minimum(np.arange(window_len), window_len - 1 - np.arange(window_len))
+    w = np.where(d >= ramp, 1.0, np.sin(np.pi / 2 * (d + 1) / (ramp + 1)) ** 2)
+    return {"taper": w, "taper_sum": float(np.sum(w)), "nfft": n, "bins": n // 2 + 1,
+            "dt_ms": dt_ms, "demean": demean, "window": window_len}
+
+
+def _spectrum_window(p: dict, x: np.ndarray):
+    """windowInto (fft/spectrum.nim:271-293): demeaned (a finite sum only),
+    tapered — a shorter window takes min(taper[j], taper[n-1-j]) and its own
+    Σw — and zero-padded to nfft; Σw is NaN for a window holding NaN or Inf."""
+    W = p["window"]
+    n = min(len(x), W)
+    xs = np.asarray(x[:n], np.float64)
+    s = float(np.sum(xs))
+    finite = math.isfinite(s)
+    m = s / n if (p["demean"] and n > 0 and finite) else 0.0
+    t = p["taper"]
+    w = t[:n] if n == W else np.minimum(t[:n], t[:n][::-1])
+    y = np.zeros(p["nfft"])
+    y[:n] = (xs - m) * w
+    sw = p["taper_sum"] if n == W else float(np.sum(w))
+    return y, (sw if finite else math.nan)
+
+
+def _sides(p: dict) -> np.ndarray:
+    s = np.full(p["bins"], 2.0)
+    s[0] = 1.0
+    if p["nfft"] % 2 == 0:
+        s[-1] = 1.0
+    return s
+
+
+def spectrum_amplitude(p: dict, x: np.ndarray):
+    """amplitudeInto (fft/spectrum.nim:295-312): |X[k]|·sides/Σw and the phase."""
+    y, sw = _spectrum_window(p, x)
+    X = np.fft.rfft(y)
+    return np.abs(X) * _sides(p) / sw, np.angle(X)
+
+
+def spectrum_average(p: dict, x: np.ndarray, row_len, first_col: int):
+    """averageAmplitudeInto (fft/spectrum.nim:334-433): over live rows with
+    live samples past first_col and a finite window, sqrt(mean(|X|²/Σw²))·sides;
+    returns (amp, rows used)."""
+    if first_col < 0:
+        raise ValueError(f"firstCol must be >= 0, got {first_col}")
+    acc = np.zeros(p["bins"])
+    used = 0
+    for r in range(x.shape[0]):
+        live = int(row_len[r])
+        if live <= first_col:        # not live (-1), or no live sample in the window
+            continue
+        last = min(live, first_col + p["window"])
+        y, sw = _spectrum_window(p, x[r, first_col:last])
+        if sw > 0:
+            X = np.fft.rfft(y)
+            acc += (X.real ** 2 + X.imag ** 2) / (sw * sw)
+            used += 1
+    amp = np.sqrt(acc / used) * _sides(p) if used else np.zeros(p["bins"])
+    return amp, used
