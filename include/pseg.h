@@ -158,6 +158,30 @@ typedef enum pseg_sample_format {
   PSEG_FMT_INT8 = 8
 } pseg_sample_format;
 
+/* --------------------------------------------------------------- spectra -- */
+
+/* A SpectrumPlan (fft/spectrum.nim:77-101): windows of `window` samples,
+ * demeaned (demean != 0), Tukey-tapered (sin² ramps of round(taper_fraction *
+ * window) samples at each end), zero-padded to nfft (0: the next power of
+ * two; else a 2·3·5-smooth length >= the window; odd lengths give
+ * PSEG_E_UNSUPPORTED here). */
+typedef struct pseg_spectrum_plan pseg_spectrum_plan;
+pseg_status pseg_spectrum_compile(int32_t dtype, double dt_ms, int32_t window,
+                                  double taper_fraction, int32_t nfft, int32_t demean,
+                                  pseg_spectrum_plan **plan);
+pseg_status pseg_spectrum_destroy(pseg_spectrum_plan *plan);
+pseg_status pseg_spectrum_info_get(const pseg_spectrum_plan *plan, int32_t *nfft, int32_t *bins);
+/* averageAmplitudeInto (fft/spectrum.nim:334-433): the RMS amplitude
+ * spectrum, sqrt(mean over rows of |X[k]|² / Σw²) · sides(k), of the windows
+ * from column first_col of the live rows of src ([rows, cols], C-contiguous)
+ * with a live sample there and no NaN or Inf in the window; row_len from
+ * pseg_meta_flatten.  acc: device, bins + 1 float64 of scratch (cleared
+ * here; acc[bins] is left holding the rows used); amp: device, bins float64.
+ * The rows add their powers in any order (float64 atomics). */
+pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
+                                  int32_t cols, const int32_t *row_len, int32_t first_col,
+                                  double *acc, double *amp, void *stream);
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,

@@ -24,6 +24,7 @@ from .correlate import (  # noqa: F401
     CorrelationPlan, Pilot, SweepSpec, SweepTaper, cmCorrelate, cmDeconvolve, generatePilot,
     initCorrelationPlan, initSweepTaper, linearSweep, samplesInLength, sweepSamples)
 from .batch import BatchItem, PipelineBatch  # noqa: F401
+from .spectrum import SpectrumPlan, initSpectrumPlan  # noqa: F401
 from .decode import (  # noqa: F401
     decodeSamples, fmtIbm32, fmtIeee32, fmtInt8, fmtInt16, fmtInt32)
 from .serve import ServeResult, Server  # noqa: F401

@@ -134,7 +134,8 @@ SYMBOLS = [
     "pseg_batch_launch", "pseg_batch_info_get", "pseg_batch_destroy", "pseg_server_create",
     "pseg_server_add_flow", "pseg_server_submit", "pseg_server_wait", "pseg_server_read",
     "pseg_server_release", "pseg_server_info_get", "pseg_server_destroy",
-    "pseg_decode_samples", "pseg_plan_apply_raw",
+    "pseg_decode_samples", "pseg_plan_apply_raw", "pseg_spectrum_compile",
+    "pseg_spectrum_destroy", "pseg_spectrum_info_get", "pseg_spectrum_average",
 ]
 
 
@@ -167,6 +168,10 @@ def lib():
     L.pseg_plan_apply.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_size_t, vp]
     L.pseg_plan_apply_keys.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, C.c_size_t, vp]
     L.pseg_decode_samples.argtypes = [i32, vp, C.c_int64, C.c_int64, i32, i32, i32, vp, vp]
+    L.pseg_spectrum_compile.argtypes = [i32, f64, i32, f64, i32, i32, C.POINTER(vp)]
+    L.pseg_spectrum_destroy.argtypes = [vp]
+    L.pseg_spectrum_info_get.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    L.pseg_spectrum_average.argtypes = [vp, vp, i32, i32, vp, i32, vp, vp, vp]
     L.pseg_plan_apply_raw.argtypes = [vp, i32, vp, C.c_int64, C.c_int64, vp, i32, i32, vp, vp,
                                       C.c_int64, vp]
     L.pseg_corr_compile.argtypes = [i32, vp, i32, f64, f64, f64, i32, i32, i32, f64, i32,

@@ -1412,6 +1412,136 @@ pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *
   return PSEG_OK;
 }
 
+// --------------------------------------------------------------- spectrum
+
+struct pseg_spectrum_plan {
+  int dtype;
+  int32_t window, nfft, bins, threads;
+  size_t lds;
+  DevSpec *dev;
+  char *base;
+};
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+pseg_status spectrum_compile(double dt, int32_t window, double taper_fraction, int32_t nfft,
+                             int32_t demean, pseg_spectrum_plan **out) {
+  // initSpectrumPlan (fft/spectrum.nim:77-101)
+  if (window < 1) return fail(PSEG_E_INVALID, "a spectrum window needs >= 1 sample, got %d", window);
+  if (!(dt > 0 && dt < kInf)) return fail(PSEG_E_INVALID, "dtMs must be finite and > 0, got %g", dt);
+  if (!(taper_fraction >= 0 && taper_fraction <= 0.5))
+    return fail(PSEG_E_INVALID, "taperFraction must be in 0 .. 0.5, got %g", taper_fraction);
+  int64_t n = nfft;
+  if (n == 0) {
+    n = 1;
+    while (n < window) n *= 2;  // nextPowerOfTwo
+  }
+  if (n < window) return fail(PSEG_E_INVALID, "nfft %lld is shorter than the window (%d)", (long long)n, window);
+  if (next_fast(n, false) != n) return fail(PSEG_E_INVALID, "nfft %lld is not 2·3·5-smooth", (long long)n);
+  if (n % 2 != 0 || n < 2)
+    return fail(PSEG_E_UNSUPPORTED, "an odd nfft (%lld) is beyond this GPU path", (long long)n);
+  if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
+  int threads = 0;
+  for (int t = 256; t <= 1024 && threads == 0; t *= 2)
+    if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
+  const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
+  if (threads == 0 || lds > kMaxRowLds)
+    return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
+  Blob blob;
+  blob.bytes.resize(sizeof(DevSpec));
+  DevSpec H;
+  memset(&H, 0, sizeof H);
+  build_fft<T>(n, H.fft, blob, kFuseCorr);
+  const double r = taper_fraction * (double)window;
+  const int64_t ramp = (int64_t)std::floor(r + 0.5);  // round half away from zero (r >= 0)
+  std::vector<T> taper(window);
+  double sum = 0;
+  for (int32_t i = 0; i < window; i++) {
+    const int64_t d = std::min<int64_t>(i, window - 1 - i);
+    double w = 1.0;
+    if (d < ramp) {
+      const double sn = std::sin(kPi / 2 * (double)(d + 1) / (double)(ramp + 1));
+      w = sn * sn;
+    }
+    taper[i] = (T)w;
+    sum += (double)taper[i];
+  }
+  set_off(H.taper, blob.add_vec(taper));
+  H.taper_sum = sum;
+  H.window = window;
+  H.demean = demean != 0;
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc of a spectrum plan");
+  patch_fft(H.fft, base);
+  patch(H.taper, base);
+  memcpy(blob.bytes.data(), &H, sizeof H);
+  e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    return hip_fail(e, "hipMemcpy of a spectrum plan");
+  }
+  pseg_spectrum_plan *P = new pseg_spectrum_plan;
+  P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
+  P->window = window;
+  P->nfft = (int32_t)n;
+  P->bins = (int32_t)(n / 2 + 1);
+  P->threads = threads;
+  P->lds = lds;
+  P->dev = (DevSpec *)base;
+  P->base = base;
+  *out = P;
+  return PSEG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pseg_status pseg_spectrum_compile(int32_t dtype, double dt_ms, int32_t window,
+                                  double taper_fraction, int32_t nfft, int32_t demean,
+                                  pseg_spectrum_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  return guarded([&] {
+    if (dtype == PSEG_F32) return spectrum_compile<float>(dt_ms, window, taper_fraction, nfft, demean, out);
+    if (dtype == PSEG_F64) return spectrum_compile<double>(dt_ms, window, taper_fraction, nfft, demean, out);
+    return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
+  });
+}
+
+pseg_status pseg_spectrum_destroy(pseg_spectrum_plan *plan) {
+  if (!plan) return PSEG_OK;
+  (void)hipFree(plan->base);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_spectrum_info_get(const pseg_spectrum_plan *plan, int32_t *nfft, int32_t *bins) {
+  if (!plan || !nfft || !bins) return fail(PSEG_E_INVALID, "NULL plan or output");
+  *nfft = plan->nfft;
+  *bins = plan->bins;
+  return PSEG_OK;
+}
+
+pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
+                                  int32_t cols, const int32_t *row_len, int32_t first_col,
+                                  double *acc, double *amp, void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrum_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_col < 0) return fail(PSEG_E_INVALID, "firstCol must be >= 0, got %d", first_col);
+  if (!acc || !amp) return fail(PSEG_E_INVALID, "NULL acc or amp");
+  if (rows > 0 && cols > 0 && (!src || !row_len)) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  hipError_t e = launch_spectrum(plan->dtype, plan->threads, plan->dev, plan->lds, src, rows, cols,
+                                 row_len, first_col, acc, amp, plan->bins, plan->nfft,
+                                 (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "spectrum launch");
+  return PSEG_OK;
+}
+
 // ------------------------------------------------------------------ batch
 
 pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batch **out) {

@@ -241,6 +241,22 @@ hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t c
                                double percentile, double *row_sq, uint32_t *row_max,
                                uint32_t *hist, double *factor, hipStream_t stream);
 // The correlation kernel with `threads` (512 or 1024) threads per record.
+// A SpectrumPlan (fft/spectrum.nim:61-101): the real FFT of nfft points, the
+// window's Tukey taper (T, `window` values) and its float64 sum.
+struct DevSpec {
+  DevFft fft;
+  const void *taper;
+  double taper_sum;
+  int32_t window;
+  int32_t demean;
+};
+// averageAmplitudeInto (fft/spectrum.nim:334-433) of rows [rows, cols] from
+// column first_col: acc (bins + 1 float64: the power sums, then the rows
+// used) is cleared, filled, and turned into amp (bins float64).
+hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t lds, const void *src,
+                           int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
+                           double *acc, double *amp, int32_t bins, int32_t nfft, hipStream_t s);
+
 // SEG-Y sample decode (format 1, 2, 3, 5 or 8, big-endian) of `rows` rows
 // `pitch` bytes apart, samples from byte `offset` of each, to a [rows, cols]
 // panel of dtype.

@@ -243,6 +243,90 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
   }
 }
 
+// ---------------------------------------------------------------- spectrum
+
+// |X[k]|² / Σw² of one row into acc[k] (float64 atomics: rows add in any
+// order).  combine hands over 2·X[k] except at 0 and Nyquist.
+template <typename T> struct PowerAcc {
+  double *acc;
+  double inv_sw2;
+  int m;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    const double re = (double)x.x, im = (double)x.y;
+    double p = re * re + im * im;
+    if (k != 0 && k != m) p *= 0.25;
+    atomicAdd(acc + k, p * inv_sw2);
+    return x;
+  }
+};
+
+// One row's window (windowInto, fft/spectrum.nim:271-293) through the real
+// FFT into the power sums: demeaned when the plan says so and the window's
+// float64 sum is finite (a NaN or Inf leaves the row out), tapered (a window
+// cut by the live end takes min(taper[j], taper[n-1-j]) and its own Σw),
+// zero-padded to nfft.
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict__ plan,
+                                                       const T *__restrict__ src, int32_t cols,
+                                                       const int32_t *__restrict__ row_len,
+                                                       int32_t first_col, double *__restrict__ acc) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red_lds[4 * (NTC / 64)];
+  const DevSpec &P = *plan;
+  const int64_t row = blockIdx.x;
+  const int live = row_len[row];
+  if (live <= first_col) return;  // not live, or nothing live in the window
+  const int W = P.window;
+  const int n = min(live, first_col + W) - first_col;
+  const T *s = src + row * (int64_t)cols + first_col;
+  RedSlots red{red_lds, 2 * (NTC / 64), 0};
+  double sum = 0;
+  for (int j = threadIdx.x; j < n; j += NTC) sum += (double)s[j];
+  sum = block_sum<NTC>(sum, red.next());
+  if (!(sum - sum == 0)) return;  // a NaN or Inf in the window
+  const T mean = P.demean ? (T)(sum / (double)n) : (T)0;
+  const DevFft &f = P.fft;
+  const int nf = f.n;
+  T *buf = (T *)lds;
+  gptr<T> tp = as_global<T>(P.taper);
+  const bool full = n == W;
+  double swl = 0;
+  for (int j = threadIdx.x; j < nf; j += NTC) {
+    T y = (T)0;
+    if (j < n) {
+      T w = tp[j];
+      if (!full) {
+        const T w2 = tp[n - 1 - j];
+        w = w2 < w ? w2 : w;
+      }
+      y = (s[j] - mean) * w;
+      swl += (double)w;
+    }
+    buf[j] = y;
+  }
+  const double sw = full ? P.taper_sum : block_sum<NTC>(swl, red.next());
+  __syncthreads();
+  V2<T> *A = (V2<T> *)lds;
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  combine<T, NTC>(f, A, A, PowerAcc<T>{acc, 1.0 / (sw * sw), f.m});
+  if (threadIdx.x == 0) atomicAdd(acc + f.m + 1, 1.0);
+}
+
+__global__ void fill_f64_kernel(double *p, int32_t n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// amp[k] = sqrt(acc[k] / used) · sides(k) (1 at 0 Hz and Nyquist), 0 when no
+// row was used.
+__global__ void spectrum_finish_kernel(const double *acc, int32_t bins, int32_t nfft,
+                                       double *amp) {
+  const double used = acc[bins];
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < bins; k += gridDim.x * blockDim.x) {
+    const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
+    amp[k] = used > 0 ? sqrt(acc[k] / used) * sides : 0.0;
+  }
+}
+
 // -------------------------------------------------------------- launchers
 
 namespace {
@@ -266,6 +350,19 @@ hipError_t panel_typed(const T *x, int32_t rows, int32_t cols, const int32_t *ro
   }
   hipLaunchKernelGGL((panel_factor_kernel<T>), dim3(1), dim3(1024), 0, stream, rows, row_len,
                      row_sq, row_max, hist, stat, target, percentile, factor);
+  return hipGetLastError();
+}
+
+template <typename T, int NTC>
+hipError_t spectrum_typed(const DevSpec *dev, size_t lds, const T *src, int32_t rows,
+                          int32_t cols, const int32_t *row_len, int32_t first_col, double *acc,
+                          hipStream_t stream) {
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)spectrum_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((spectrum_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src,
+                     cols, row_len, first_col, acc);
   return hipGetLastError();
 }
 
@@ -333,6 +430,36 @@ hipError_t launch_correlate(int dtype, int threads, const DevCorr *dev_plan, siz
                                    (float *)dst, stream);
   return corr_typed<double, 1024>(dev_plan, lds_bytes, sd, rows, src_cols, row_len,
                                   (double *)dst, stream);
+}
+
+}  // namespace pseg
+
+namespace pseg {
+
+hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t lds, const void *src,
+                           int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
+                           double *acc, double *amp, int32_t bins, int32_t nfft, hipStream_t s) {
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(std::max(1, std::min(64, (bins + 256) / 256))), dim3(256),
+                     0, s, acc, bins + 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (rows > 0 && cols > 0) {
+    if (dtype == 0) {
+      const float *x = (const float *)src;
+      e = threads == 256 ? spectrum_typed<float, 256>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
+        : threads == 512 ? spectrum_typed<float, 512>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
+                         : spectrum_typed<float, 1024>(dev, lds, x, rows, cols, row_len, first_col, acc, s);
+    } else {
+      const double *x = (const double *)src;
+      e = threads == 256 ? spectrum_typed<double, 256>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
+        : threads == 512 ? spectrum_typed<double, 512>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
+                         : spectrum_typed<double, 1024>(dev, lds, x, rows, cols, row_len, first_col, acc, s);
+    }
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(spectrum_finish_kernel, dim3(std::max(1, std::min(64, (bins + 255) / 256))),
+                     dim3(256), 0, s, acc, bins, nfft, amp);
+  return hipGetLastError();
 }
 
 }  // namespace pseg
