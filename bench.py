@@ -385,6 +385,31 @@ def raw_flow_leg(torch, P, dev, steps, warmup):
             "ms_decode_then_apply": round(res["decode_then_apply"] * 1000, 5)}
 
 
+def spectrum_leg(torch, P, dev, steps, warmup):
+    """The RMS amplitude spectrum of the 2000 x 3001 panel's windows of 2001
+    samples from column 500 (10 % Tukey taper, nfft 2048, 1025 bins): one
+    averageAmplitudeInto per step.  Algorithmic bytes: the windows read."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    sp = P.initSpectrumPlan(np.float32, DT, 2001)
+    src = torch.from_numpy(x).to(dev)
+    rl = meta.device_row_len(COLS, dev)
+    amp = torch.empty(sp.bins, dtype=torch.float64, device=dev)
+    for _ in range(warmup):
+        sp.averageAmplitudeInto(src, firstCol=500, amp=amp, row_len=rl, sync=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        sp.averageAmplitudeInto(src, firstCol=500, amp=amp, row_len=rl, sync=False)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / steps
+    return {"workload": "RMS amplitude spectrum of 2000 windows of 2001 samples (nfft 2048) of "
+                        "the 2000x3001 f32 panel",
+            "value": round(ROWS * 2001 / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -525,8 +550,8 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, iir_flow, decode, raw_flow, f64_panel, correlation, "
-                         "uneven_batch, serving), "
+                         "gain_flow, iir_flow, decode, raw_flow, spectrum, f64_panel, "
+                         "correlation, uneven_batch, serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
@@ -674,6 +699,10 @@ def main():
         if want("raw_flow"):
             log("raw flow leg")
             sec["raw_flow"] = raw_flow_leg(torch, P, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("spectrum"):
+            log("spectrum leg")
+            sec["spectrum"] = spectrum_leg(torch, P, dev, max(10, args.steps // 4), 3)
             torch.cuda.empty_cache()
         if want("f64_panel"):
             log("float64 panel leg")

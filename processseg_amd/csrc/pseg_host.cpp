@@ -1448,7 +1448,8 @@ pseg_status spectrum_compile(double dt, int32_t window, double taper_fraction, i
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
     if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
-  if (threads == 0 || lds > kMaxRowLds)
+  // the FFT buffer and the workgroup's power sums (n / 2 + 1 float64)
+  if (threads == 0 || (lds + 15) / 16 * 16 + (size_t)(n / 2 + 1) * 8 > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
   Blob blob;
   blob.bytes.resize(sizeof(DevSpec));

@@ -245,8 +245,9 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
 
 // ---------------------------------------------------------------- spectrum
 
-// |X[k]|² / Σw² of one row into acc[k] (float64 atomics: rows add in any
-// order).  combine hands over 2·X[k] except at 0 and Nyquist.
+// |X[k]|² / Σw² of one row added to the workgroup's sums in LDS (each bin
+// has one owner thread in combine, so plain adds).  combine hands over
+// 2·X[k] except at 0 and Nyquist.
 template <typename T> struct PowerAcc {
   double *acc;
   double inv_sw2;
@@ -255,61 +256,72 @@ template <typename T> struct PowerAcc {
     const double re = (double)x.x, im = (double)x.y;
     double p = re * re + im * im;
     if (k != 0 && k != m) p *= 0.25;
-    atomicAdd(acc + k, p * inv_sw2);
+    acc[k] += p * inv_sw2;
     return x;
   }
 };
 
-// One row's window (windowInto, fft/spectrum.nim:271-293) through the real
-// FFT into the power sums: demeaned when the plan says so and the window's
-// float64 sum is finite (a NaN or Inf leaves the row out), tapered (a window
-// cut by the live end takes min(taper[j], taper[n-1-j]) and its own Σw),
-// zero-padded to nfft.
+// Rows blockIdx.x, + gridDim.x, ...: each row's window (windowInto,
+// fft/spectrum.nim:271-293) through the real FFT into the workgroup's power
+// sums, which go to acc with one float64 atomic per bin at the end.  A
+// window is demeaned when the plan says so and its float64 sum is finite (a
+// NaN or Inf leaves the row out), tapered (a window cut by the live end
+// takes min(taper[j], taper[n-1-j]) and its own Σw), zero-padded to nfft.
 template <typename T, int NTC>
 __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict__ plan,
-                                                       const T *__restrict__ src, int32_t cols,
+                                                       const T *__restrict__ src, int32_t rows,
+                                                       int32_t cols,
                                                        const int32_t *__restrict__ row_len,
-                                                       int32_t first_col, double *__restrict__ acc) {
+                                                       int32_t first_col, size_t acc_off,
+                                                       double *__restrict__ acc) {
   extern __shared__ __align__(16) char lds[];
   __shared__ double red_lds[4 * (NTC / 64)];
   const DevSpec &P = *plan;
-  const int64_t row = blockIdx.x;
-  const int live = row_len[row];
-  if (live <= first_col) return;  // not live, or nothing live in the window
-  const int W = P.window;
-  const int n = min(live, first_col + W) - first_col;
-  const T *s = src + row * (int64_t)cols + first_col;
-  RedSlots red{red_lds, 2 * (NTC / 64), 0};
-  double sum = 0;
-  for (int j = threadIdx.x; j < n; j += NTC) sum += (double)s[j];
-  sum = block_sum<NTC>(sum, red.next());
-  if (!(sum - sum == 0)) return;  // a NaN or Inf in the window
-  const T mean = P.demean ? (T)(sum / (double)n) : (T)0;
   const DevFft &f = P.fft;
-  const int nf = f.n;
-  T *buf = (T *)lds;
-  gptr<T> tp = as_global<T>(P.taper);
-  const bool full = n == W;
-  double swl = 0;
-  for (int j = threadIdx.x; j < nf; j += NTC) {
-    T y = (T)0;
-    if (j < n) {
-      T w = tp[j];
-      if (!full) {
-        const T w2 = tp[n - 1 - j];
-        w = w2 < w ? w2 : w;
-      }
-      y = (s[j] - mean) * w;
-      swl += (double)w;
-    }
-    buf[j] = y;
-  }
-  const double sw = full ? P.taper_sum : block_sum<NTC>(swl, red.next());
+  const int nf = f.n, bins = f.m + 1;
+  double *lacc = (double *)(lds + acc_off);
+  for (int k = threadIdx.x; k < bins; k += NTC) lacc[k] = 0;
   __syncthreads();
-  V2<T> *A = (V2<T> *)lds;
-  complex_forward<T, NTC, kFuseCorr>(f, A);
-  combine<T, NTC>(f, A, A, PowerAcc<T>{acc, 1.0 / (sw * sw), f.m});
-  if (threadIdx.x == 0) atomicAdd(acc + f.m + 1, 1.0);
+  RedSlots red{red_lds, 2 * (NTC / 64), 0};
+  gptr<T> tp = as_global<T>(P.taper);
+  const int W = P.window;
+  double used = 0;
+  for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int live = row_len[row];
+    if (live <= first_col) continue;  // not live, or nothing live in the window
+    const int n = min(live, first_col + W) - first_col;
+    const T *s = src + row * (int64_t)cols + first_col;
+    double sum = 0;
+    for (int j = threadIdx.x; j < n; j += NTC) sum += (double)s[j];
+    sum = block_sum<NTC>(sum, red.next());
+    if (!(sum - sum == 0)) continue;  // a NaN or Inf in the window
+    const T mean = P.demean ? (T)(sum / (double)n) : (T)0;
+    T *buf = (T *)lds;
+    const bool full = n == W;
+    double swl = 0;
+    for (int j = threadIdx.x; j < nf; j += NTC) {
+      T y = (T)0;
+      if (j < n) {
+        T w = tp[j];
+        if (!full) {
+          const T w2 = tp[n - 1 - j];
+          w = w2 < w ? w2 : w;
+        }
+        y = (s[j] - mean) * w;
+        swl += (double)w;
+      }
+      buf[j] = y;
+    }
+    const double sw = full ? P.taper_sum : block_sum<NTC>(swl, red.next());
+    __syncthreads();
+    V2<T> *A = (V2<T> *)lds;
+    complex_forward<T, NTC, kFuseCorr>(f, A);
+    combine<T, NTC>(f, A, A, PowerAcc<T>{lacc, 1.0 / (sw * sw), f.m});
+    __syncthreads();  // the buffer is free for the next row
+    used += 1;
+  }
+  for (int k = threadIdx.x; k < bins; k += NTC) atomicAdd(acc + k, lacc[k]);
+  if (threadIdx.x == 0 && used > 0) atomicAdd(acc + bins, used);
 }
 
 __global__ void fill_f64_kernel(double *p, int32_t n) {
@@ -353,16 +365,21 @@ hipError_t panel_typed(const T *x, int32_t rows, int32_t cols, const int32_t *ro
   return hipGetLastError();
 }
 
+// The FFT buffer, then the workgroup's bins float64 power sums (16-byte
+// aligned); one workgroup per CU-slot pass over the rows.
 template <typename T, int NTC>
-hipError_t spectrum_typed(const DevSpec *dev, size_t lds, const T *src, int32_t rows,
-                          int32_t cols, const int32_t *row_len, int32_t first_col, double *acc,
-                          hipStream_t stream) {
+hipError_t spectrum_typed(const DevSpec *dev, size_t lds, int32_t bins, const T *src,
+                          int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
+                          double *acc, hipStream_t stream) {
   static const hipError_t attr = hipFuncSetAttribute(
       (const void *)spectrum_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
       (int)kMaxRowLds);
   if (attr != hipSuccess) return attr;
-  hipLaunchKernelGGL((spectrum_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src,
-                     cols, row_len, first_col, acc);
+  const size_t acc_off = (lds + 15) / 16 * 16;
+  const size_t total = acc_off + (size_t)bins * sizeof(double);
+  const int grid = std::min(rows, 1024);
+  hipLaunchKernelGGL((spectrum_kernel<T, NTC>), dim3(grid), dim3(NTC), total, stream, dev, src,
+                     rows, cols, row_len, first_col, acc_off, acc);
   return hipGetLastError();
 }
 
@@ -446,14 +463,14 @@ hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t ld
   if (rows > 0 && cols > 0) {
     if (dtype == 0) {
       const float *x = (const float *)src;
-      e = threads == 256 ? spectrum_typed<float, 256>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
-        : threads == 512 ? spectrum_typed<float, 512>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
-                         : spectrum_typed<float, 1024>(dev, lds, x, rows, cols, row_len, first_col, acc, s);
+      e = threads == 256 ? spectrum_typed<float, 256>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
+        : threads == 512 ? spectrum_typed<float, 512>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
+                         : spectrum_typed<float, 1024>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s);
     } else {
       const double *x = (const double *)src;
-      e = threads == 256 ? spectrum_typed<double, 256>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
-        : threads == 512 ? spectrum_typed<double, 512>(dev, lds, x, rows, cols, row_len, first_col, acc, s)
-                         : spectrum_typed<double, 1024>(dev, lds, x, rows, cols, row_len, first_col, acc, s);
+      e = threads == 256 ? spectrum_typed<double, 256>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
+        : threads == 512 ? spectrum_typed<double, 512>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
+                         : spectrum_typed<double, 1024>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s);
     }
     if (e != hipSuccess) return e;
   }
