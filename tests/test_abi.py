@@ -153,3 +153,17 @@ def test_decode_checks_before_touching_memory():
     # empty shapes are no-ops
     assert L.pseg_decode_samples(1, fake_raw, 4240, 240, 0, 1000, 0, fake_dst, None) == _lib.OK
     assert L.pseg_decode_samples(1, fake_raw, 4240, 240, 2, 0, 0, fake_dst, None) == _lib.OK
+
+
+def test_spectrum_compile_checks_before_device_work():
+    """pseg_spectrum_compile refuses bad plans (fft/spectrum.nim:77-101) before
+    it touches the device; an odd 2·3·5-smooth nfft is beyond the GPU path."""
+    import ctypes as C
+    L = _lib.lib()
+    h = C.c_void_p()
+    for args in [(0, 2.0, 0, 0.1, 0, 1), (0, 0.0, 10, 0.1, 0, 1), (0, float("nan"), 10, 0.1, 0, 1),
+                 (0, 2.0, 10, 0.6, 0, 1), (0, 2.0, 10, -0.1, 0, 1), (0, 2.0, 100, 0.1, 64, 1),
+                 (0, 2.0, 100, 0.1, 101, 1), (7, 2.0, 100, 0.1, 0, 1)]:
+        assert L.pseg_spectrum_compile(*args, C.byref(h)) == _lib.E_INVALID, args
+    assert L.pseg_spectrum_compile(0, 2.0, 100, 0.1, 125, 1, C.byref(h)) == _lib.E_UNSUPPORTED
+    assert L.pseg_spectrum_average(None, None, 1, 1, None, 0, None, None, None) == _lib.E_INVALID
