@@ -182,6 +182,15 @@ pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *sr
                                   int32_t cols, const int32_t *row_len, int32_t first_col,
                                   double *acc, double *amp, void *stream);
 
+/* amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
+ * first_col: amp = |X[k]|·sides(k)/Σw and phase = atan2(Im, Re) (radians,
+ * from the window's first sample), [rows, bins] of the plan's dtype on the
+ * device.  A window holding NaN or Inf gives NaN in every bin; rows with no
+ * live sample in the window are left as they are. */
+pseg_status pseg_spectrum_amplitude(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
+                                    int32_t cols, const int32_t *row_len, int32_t first_col,
+                                    void *amp, void *phase, void *stream);
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,

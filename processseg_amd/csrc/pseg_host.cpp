@@ -1543,6 +1543,20 @@ pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *sr
   return PSEG_OK;
 }
 
+pseg_status pseg_spectrum_amplitude(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
+                                    int32_t cols, const int32_t *row_len, int32_t first_col,
+                                    void *amp, void *phase, void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrum_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_col < 0) return fail(PSEG_E_INVALID, "firstCol must be >= 0, got %d", first_col);
+  if (rows > 0 && cols > 0 && (!src || !row_len || !amp || !phase))
+    return fail(PSEG_E_INVALID, "NULL panel, row_len, amp or phase");
+  hipError_t e = launch_spectrum_rows(plan->dtype, plan->threads, plan->dev, plan->lds, src, rows,
+                                      cols, row_len, first_col, amp, phase, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "spectrum launch");
+  return PSEG_OK;
+}
+
 // ------------------------------------------------------------------ batch
 
 pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batch **out) {

@@ -257,6 +257,13 @@ hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t ld
                            int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
                            double *acc, double *amp, int32_t bins, int32_t nfft, hipStream_t s);
 
+// amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
+// first_col: amp and phase, [rows, bins] of dtype.
+hipError_t launch_spectrum_rows(int dtype, int threads, const DevSpec *dev, size_t lds,
+                                const void *src, int32_t rows, int32_t cols,
+                                const int32_t *row_len, int32_t first_col, void *amp, void *phase,
+                                hipStream_t s);
+
 // SEG-Y sample decode (format 1, 2, 3, 5 or 8, big-endian) of `rows` rows
 // `pitch` bytes apart, samples from byte `offset` of each, to a [rows, cols]
 // panel of dtype.

@@ -324,6 +324,76 @@ __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict
   if (threadIdx.x == 0 && used > 0) atomicAdd(acc + bins, used);
 }
 
+// amplitudeInto (fft/spectrum.nim:295-312) per row: |X[k]|·sides(k)/Σw and
+// atan2(Im, Re) as T into the row's bins.  combine hands over 2·X[k] except
+// at 0 and Nyquist (the phase does not care).
+template <typename T> struct AmpPhaseOut {
+  T *amp, *phase;
+  double inv_sw;
+  int m, nfft;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    const double re = (double)x.x, im = (double)x.y;
+    const double half = (k != 0 && k != m) ? 0.5 : 1.0;
+    const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
+    amp[k] = (T)(hypot(re, im) * half * sides * inv_sw);
+    phase[k] = (T)atan2(im, re);
+    return x;
+  }
+};
+
+// One workgroup per row: the row's window from first_col (as averaged
+// above, but a NaN or Inf stays: the window's sum and Σw are NaN, and so is
+// every bin), through the real FFT, into amp and phase [rows, bins].  Rows
+// with no live sample in the window are left as they are.
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void spectrum_rows_kernel(const DevSpec *__restrict__ plan,
+                                                            const T *__restrict__ src, int32_t cols,
+                                                            const int32_t *__restrict__ row_len,
+                                                            int32_t first_col, T *__restrict__ amp,
+                                                            T *__restrict__ phase) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red_lds[4 * (NTC / 64)];
+  const DevSpec &P = *plan;
+  const DevFft &f = P.fft;
+  const int nf = f.n, bins = f.m + 1;
+  const int64_t row = blockIdx.x;
+  const int live = row_len[row];
+  if (live <= first_col) return;
+  const int W = P.window;
+  const int n = min(live, first_col + W) - first_col;
+  const T *s = src + row * (int64_t)cols + first_col;
+  RedSlots red{red_lds, 2 * (NTC / 64), 0};
+  double sum = 0;
+  for (int j = threadIdx.x; j < n; j += NTC) sum += (double)s[j];
+  sum = block_sum<NTC>(sum, red.next());
+  const bool finite = sum - sum == 0;
+  const T mean = (P.demean && finite) ? (T)(sum / (double)n) : (T)0;
+  gptr<T> tp = as_global<T>(P.taper);
+  T *buf = (T *)lds;
+  const bool full = n == W;
+  double swl = 0;
+  for (int j = threadIdx.x; j < nf; j += NTC) {
+    T y = (T)0;
+    if (j < n) {
+      T w = tp[j];
+      if (!full) {
+        const T w2 = tp[n - 1 - j];
+        w = w2 < w ? w2 : w;
+      }
+      y = (s[j] - mean) * w;
+      swl += (double)w;
+    }
+    buf[j] = y;
+  }
+  double sw = full ? P.taper_sum : block_sum<NTC>(swl, red.next());
+  if (!finite) sw = NAN;
+  __syncthreads();
+  V2<T> *A = (V2<T> *)lds;
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  combine<T, NTC>(f, A, A, AmpPhaseOut<T>{amp + row * (int64_t)bins, phase + row * (int64_t)bins,
+                                          1.0 / sw, f.m, nf});
+}
+
 __global__ void fill_f64_kernel(double *p, int32_t n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -362,6 +432,19 @@ hipError_t panel_typed(const T *x, int32_t rows, int32_t cols, const int32_t *ro
   }
   hipLaunchKernelGGL((panel_factor_kernel<T>), dim3(1), dim3(1024), 0, stream, rows, row_len,
                      row_sq, row_max, hist, stat, target, percentile, factor);
+  return hipGetLastError();
+}
+
+template <typename T, int NTC>
+hipError_t spectrum_rows_typed(const DevSpec *dev, size_t lds, const T *src, int32_t rows,
+                               int32_t cols, const int32_t *row_len, int32_t first_col, T *amp,
+                               T *phase, hipStream_t stream) {
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)spectrum_rows_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((spectrum_rows_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src,
+                     cols, row_len, first_col, amp, phase);
   return hipGetLastError();
 }
 
@@ -477,6 +560,29 @@ hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t ld
   hipLaunchKernelGGL(spectrum_finish_kernel, dim3(std::max(1, std::min(64, (bins + 255) / 256))),
                      dim3(256), 0, s, acc, bins, nfft, amp);
   return hipGetLastError();
+}
+
+}  // namespace pseg
+
+namespace pseg {
+
+hipError_t launch_spectrum_rows(int dtype, int threads, const DevSpec *dev, size_t lds,
+                                const void *src, int32_t rows, int32_t cols,
+                                const int32_t *row_len, int32_t first_col, void *amp, void *phase,
+                                hipStream_t s) {
+  if (rows == 0 || cols == 0) return hipSuccess;
+  if (dtype == 0) {
+    const float *x = (const float *)src;
+    float *a = (float *)amp, *p = (float *)phase;
+    return threads == 256 ? spectrum_rows_typed<float, 256>(dev, lds, x, rows, cols, row_len, first_col, a, p, s)
+         : threads == 512 ? spectrum_rows_typed<float, 512>(dev, lds, x, rows, cols, row_len, first_col, a, p, s)
+                          : spectrum_rows_typed<float, 1024>(dev, lds, x, rows, cols, row_len, first_col, a, p, s);
+  }
+  const double *x = (const double *)src;
+  double *a = (double *)amp, *p = (double *)phase;
+  return threads == 256 ? spectrum_rows_typed<double, 256>(dev, lds, x, rows, cols, row_len, first_col, a, p, s)
+       : threads == 512 ? spectrum_rows_typed<double, 512>(dev, lds, x, rows, cols, row_len, first_col, a, p, s)
+                        : spectrum_rows_typed<double, 1024>(dev, lds, x, rows, cols, row_len, first_col, a, p, s);
 }
 
 }  // namespace pseg

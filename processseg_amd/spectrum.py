@@ -69,6 +69,38 @@ class SpectrumPlan:
         used = int(acc[self.bins].item()) if sync else None
         return amp, used
 
+    def amplitudeInto(self, samples, meta: ChunkMeta | None = None, firstCol: int = 0,
+                      amp=None, phase=None, *, row_len=None, stream=None):
+        """amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
+        firstCol: (amp, phase), [rows, bins] GPU tensors of the plan's dtype.
+        A window holding NaN or Inf gives NaN in every bin; rows with no live
+        sample in the window are left as they are (amp and phase are created
+        zeroed when not given)."""
+        import torch
+        if not isinstance(samples, torch.Tensor) or not samples.is_cuda or samples.dim() != 2 \
+                or not samples.is_contiguous():
+            raise ValueError("samples must be a C-contiguous [rows, cols] GPU tensor")
+        if np.dtype(str(samples.dtype).replace("torch.", "")) != self.T:
+            raise ValueError(f"samples hold {samples.dtype}, the plan {self.T}")
+        rows, cols = samples.shape
+        if row_len is None:
+            if meta is None:
+                raise ValueError("amplitudeInto needs a ChunkMeta (or row_len)")
+            row_len = meta.device_row_len(cols, samples.device)
+        for t in (amp, phase):
+            if t is not None and (t.dtype != samples.dtype or tuple(t.shape) != (rows, self.bins)
+                                  or not t.is_contiguous() or not t.is_cuda):
+                raise ValueError(f"amp and phase must be contiguous [{rows}, {self.bins}] GPU "
+                                 f"tensors of {samples.dtype}")
+        if amp is None:
+            amp = torch.zeros((rows, self.bins), dtype=samples.dtype, device=samples.device)
+        if phase is None:
+            phase = torch.zeros((rows, self.bins), dtype=samples.dtype, device=samples.device)
+        _lib.check(_lib.lib().pseg_spectrum_amplitude(
+            self._h, samples.data_ptr(), rows, cols, row_len.data_ptr(), int(firstCol),
+            amp.data_ptr(), phase.data_ptr(), _stream_handle(stream)))
+        return amp, phase
+
     def __del__(self):
         try:
             if self._h:

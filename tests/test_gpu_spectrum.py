@@ -69,3 +69,40 @@ def test_spectrum_refusals(gpu):
         P.initSpectrumPlan(np.float32, 2.0, 10, taperFraction=0.6)
     with pytest.raises(P.UnsupportedOp):
         P.initSpectrumPlan(np.float32, 2.0, 100, nfft=125)   # odd smooth length
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("win,nfft,taper,first", [(501, 0, 0.1, 250), (400, 400, 0.2, 0),
+                                                  (1000, 1000, 0.0, 37)])
+def test_amplitude_per_row(gpu, oracle, dtype, win, nfft, taper, first):
+    """amplitudeInto row by row: the amplitude within the bar of the oracle,
+    the phase where the amplitude is significant, NaN rows NaN, rows with
+    nothing live in the window untouched."""
+    import torch
+    rows, cols = 16, 2001
+    x, _ = synth.gather(rows, cols, 2.0, win, dtype=dtype)
+    x = x + dtype(0.3)
+    rl = synth.live_lengths(win + 1, rows, first + 1, cols).astype(np.int32)
+    rl[2] = -1
+    rl[4] = first
+    x[6, first + 5] = np.nan
+    rl[6] = cols
+    sp = P.initSpectrumPlan(dtype, 2.0, win, taper, nfft)
+    amp, ph = sp.amplitudeInto(torch.from_numpy(np.ascontiguousarray(x)).cuda(), firstCol=first,
+                               row_len=torch.from_numpy(rl).cuda())
+    amp, ph = amp.cpu().numpy(), ph.cpu().numpy()
+    p = oracle.spectrum_plan(2.0, win, taper, nfft)
+    tol = 1e-5 if dtype == np.float32 else 1e-12
+    for r in range(rows):
+        if rl[r] <= first:
+            assert np.all(amp[r] == 0) and np.all(ph[r] == 0), f"row {r} untouched"
+            continue
+        last = min(int(rl[r]), first + win)
+        wa, wp = oracle.spectrum_amplitude(p, x[r, first:last].astype(np.float64))
+        if r == 6:
+            assert np.all(np.isnan(amp[r])) and np.all(np.isnan(wa))
+            continue
+        assert rel_l2(amp[r], wa) <= tol, (r, rel_l2(amp[r], wa))
+        big = wa > 1e-3 * wa.max()
+        d = np.angle(np.exp(1j * (ph[r][big].astype(np.float64) - wp[big])))
+        assert np.max(np.abs(d)) <= (1e-3 if dtype == np.float32 else 1e-9), r
