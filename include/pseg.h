@@ -191,6 +191,23 @@ pseg_status pseg_spectrum_amplitude(const pseg_spectrum_plan *plan, const void *
                                     int32_t cols, const int32_t *row_len, int32_t first_col,
                                     void *amp, void *phase, void *stream);
 
+/* A HilbertPlan (fft/hilbert.nim:44-53): traces of up to `samples` samples,
+ * zero-padded to nfft >= 2 * samples (0: nextFastLen(2 * samples)) so the
+ * transform does not wrap. */
+typedef struct pseg_hilbert_plan pseg_hilbert_plan;
+pseg_status pseg_hilbert_compile(int32_t dtype, int32_t samples, int32_t nfft,
+                                 pseg_hilbert_plan **plan);
+pseg_status pseg_hilbert_destroy(pseg_hilbert_plan *plan);
+pseg_status pseg_hilbert_nfft(const pseg_hilbert_plan *plan, int32_t *nfft);
+/* hilbertInto (envelope = 0) or envelopeInto (envelope != 0,
+ * sqrt(x² + H[x]²)) (fft/hilbert.nim:127-150) of each row's live samples
+ * (row_len from pseg_meta_flatten; cols <= samples) into dst ([rows, cols],
+ * the same block as src or disjoint).  Rows not live and samples past a
+ * live end are left as they are in dst. */
+pseg_status pseg_hilbert_apply(const pseg_hilbert_plan *plan, const void *src, void *dst,
+                               int32_t rows, int32_t cols, const int32_t *row_len,
+                               int32_t envelope, void *stream);
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,

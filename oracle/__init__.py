@@ -697,3 +697,29 @@ def spectrum_average(p: dict, x: np.ndarray, row_len, first_col: int):
             used += 1
     amp = np.sqrt(acc / used) * _sides(p) if used else np.zeros(p["bins"])
     return amp, used
+
+
+# ---------------------------------------------------------------- Hilbert --
+
+def hilbert(x: np.ndarray, samples: int, nfft: int = 0, envelope: bool = False) -> np.ndarray:
+    """hilbertInto / envelopeInto (fft/hilbert.nim:44-150) in float64: x
+    zero-padded to nfft (nextFastLen(2·samples) by default; >= 2·samples),
+    -i·sign(f) on the spectrum with 0 Hz and Nyquist set to zero, inverse."""
+    if samples < 1:
+        raise ValueError(f"a Hilbert plan needs samples >= 1, got {samples}")
+    n = nfft if nfft else next_fast_len(2 * samples)
+    if n < 2 * samples:
+        raise ValueError(f"nfft {n} is below twice the samples ({2 * samples})")
+    if not is_fast_len(n):
+        raise ValueError(f"nfft {n} is not 2·3·5-smooth")
+    xs = np.asarray(x, np.float64)
+    buf = np.zeros(n)
+    buf[:xs.size] = xs
+    X = np.fft.rfft(buf)
+    X[0] = 0
+    last = X.size - 1 if n % 2 == 0 else X.size
+    X[1:last] = -1j * X[1:last]
+    if last < X.size:
+        X[last] = 0
+    h = np.fft.irfft(X, n)[:xs.size]
+    return np.sqrt(xs * xs + h * h) if envelope else h

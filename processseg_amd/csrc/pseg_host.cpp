@@ -1543,6 +1543,108 @@ pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *sr
   return PSEG_OK;
 }
 
+struct pseg_hilbert_plan {
+  int dtype;
+  int32_t samples, nfft, threads;
+  size_t lds;
+  DevFft *dev;
+  char *base;
+};
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+pseg_status hilbert_compile(int32_t samples, int32_t nfft, pseg_hilbert_plan **out) {
+  // initHilbertPlan (fft/hilbert.nim:44-53)
+  if (samples < 1) return fail(PSEG_E_INVALID, "a Hilbert plan needs samples >= 1, got %d", samples);
+  const int64_t n = nfft == 0 ? next_fast(2 * (int64_t)samples, true) : nfft;
+  if (n < 2 * (int64_t)samples)
+    return fail(PSEG_E_INVALID, "nfft %lld is below twice the samples (%lld): the transform would wrap",
+                (long long)n, 2 * (long long)samples);
+  if (next_fast(n, false) != n) return fail(PSEG_E_INVALID, "nfft %lld is not 2·3·5-smooth", (long long)n);
+  if (n % 2 != 0) return fail(PSEG_E_UNSUPPORTED, "an odd nfft (%lld) is beyond this GPU path", (long long)n);
+  if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
+  int threads = 0;
+  for (int t = 256; t <= 1024 && threads == 0; t *= 2)
+    if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
+  const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
+  if (threads == 0 || lds > kMaxRowLds)
+    return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
+  Blob blob;
+  blob.bytes.resize(sizeof(DevFft));
+  DevFft H;
+  memset(&H, 0, sizeof H);
+  build_fft<T>(n, H, blob, kFuseCorr);
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc of a Hilbert plan");
+  patch_fft(H, base);
+  memcpy(blob.bytes.data(), &H, sizeof H);
+  e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    return hip_fail(e, "hipMemcpy of a Hilbert plan");
+  }
+  pseg_hilbert_plan *P = new pseg_hilbert_plan;
+  P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
+  P->samples = samples;
+  P->nfft = (int32_t)n;
+  P->threads = threads;
+  P->lds = lds;
+  P->dev = (DevFft *)base;
+  P->base = base;
+  *out = P;
+  return PSEG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pseg_status pseg_hilbert_compile(int32_t dtype, int32_t samples, int32_t nfft,
+                                 pseg_hilbert_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  return guarded([&] {
+    if (dtype == PSEG_F32) return hilbert_compile<float>(samples, nfft, out);
+    if (dtype == PSEG_F64) return hilbert_compile<double>(samples, nfft, out);
+    return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
+  });
+}
+
+pseg_status pseg_hilbert_destroy(pseg_hilbert_plan *plan) {
+  if (!plan) return PSEG_OK;
+  (void)hipFree(plan->base);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_hilbert_nfft(const pseg_hilbert_plan *plan, int32_t *nfft) {
+  if (!plan || !nfft) return fail(PSEG_E_INVALID, "NULL plan or output");
+  *nfft = plan->nfft;
+  return PSEG_OK;
+}
+
+pseg_status pseg_hilbert_apply(const pseg_hilbert_plan *plan, const void *src, void *dst,
+                               int32_t rows, int32_t cols, const int32_t *row_len,
+                               int32_t envelope, void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_hilbert_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (cols > plan->samples)
+    return fail(PSEG_E_INVALID, "traces of %d columns, the plan takes %d samples", cols, plan->samples);
+  if (rows > 0 && cols > 0 && (!src || !dst || !row_len))
+    return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  const size_t esz = plan->dtype == PSEG_F32 ? 4 : 8;
+  if (rows > 0 && cols > 0 && overlap(src, dst, (size_t)rows * (size_t)cols * esz))
+    return fail(PSEG_E_INVALID, "src and dst overlap without being the same block");
+  hipError_t e = launch_hilbert(plan->dtype, plan->threads, plan->dev, plan->lds, src, rows, cols,
+                                row_len, envelope, dst, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "Hilbert launch");
+  return PSEG_OK;
+}
+
 pseg_status pseg_spectrum_amplitude(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
                                     int32_t cols, const int32_t *row_len, int32_t first_col,
                                     void *amp, void *phase, void *stream) {

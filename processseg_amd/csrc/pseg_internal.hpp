@@ -264,6 +264,12 @@ hipError_t launch_spectrum_rows(int dtype, int threads, const DevSpec *dev, size
                                 const int32_t *row_len, int32_t first_col, void *amp, void *phase,
                                 hipStream_t s);
 
+// hilbertInto / envelopeInto (fft/hilbert.nim:127-150) of each row's live
+// samples (src and dst [rows, cols] of dtype; dst may be src).
+hipError_t launch_hilbert(int dtype, int threads, const DevFft *dev, size_t lds, const void *src,
+                          int32_t rows, int32_t cols, const int32_t *row_len, int32_t envelope,
+                          void *dst, hipStream_t s);
+
 // SEG-Y sample decode (format 1, 2, 3, 5 or 8, big-endian) of `rows` rows
 // `pitch` bytes apart, samples from byte `offset` of each, to a [rows, cols]
 // panel of dtype.

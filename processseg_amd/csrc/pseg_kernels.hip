@@ -394,6 +394,55 @@ __global__ __launch_bounds__(NTC) void spectrum_rows_kernel(const DevSpec *__res
                                           1.0 / sw, f.m, nf});
 }
 
+// The Hilbert transform's multiplier (fft/hilbert.nim:107-125): -i·sign(f)
+// on the bins between 0 Hz and Nyquist, 0 on those two, with the split's 1/2
+// and the inverse's 1/n folded in (combine's convention).
+template <typename T> struct HilbertMul {
+  T fold;
+  int m;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    if (k == 0 || k == m) return mk<T>((T)0, (T)0);
+    return mk<T>(x.y * fold, -x.x * fold);
+  }
+};
+
+// One workgroup per row: the row's live samples zero-padded to nfft
+// (>= twice the plan's samples: no wrap), forward, the multiplier, inverse;
+// H[x] (or the envelope sqrt(x² + H²), envelope != 0) into dst over the live
+// length.  Rows not live and samples past the live end are left as they are.
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void hilbert_kernel(const DevFft *__restrict__ fp,
+                                                      const T *__restrict__ src, int32_t cols,
+                                                      const int32_t *__restrict__ row_len,
+                                                      int32_t envelope, T *__restrict__ dst) {
+  extern __shared__ __align__(16) char lds[];
+  const DevFft &f = *fp;
+  const int64_t row = blockIdx.x;
+  const int live = min(row_len[row], cols);
+  if (live <= 0) return;
+  const int n = f.n;
+  const T *s = src + row * (int64_t)cols;
+  T *buf = (T *)lds;
+  for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
+  __syncthreads();
+  V2<T> *A = (V2<T> *)lds;
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  combine<T, NTC>(f, A, A, HilbertMul<T>{(T)(0.5 / (double)n), f.m});
+  __syncthreads();
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  const T *out = (const T *)A;
+  T *d = dst + row * (int64_t)cols;
+  for (int j = threadIdx.x; j < live; j += NTC) {
+    const T h = out[j ^ 1];
+    if (envelope) {
+      const T x = s[j];
+      d[j] = sqrt(x * x + h * h);
+    } else {
+      d[j] = h;
+    }
+  }
+}
+
 __global__ void fill_f64_kernel(double *p, int32_t n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -445,6 +494,18 @@ hipError_t spectrum_rows_typed(const DevSpec *dev, size_t lds, const T *src, int
   if (attr != hipSuccess) return attr;
   hipLaunchKernelGGL((spectrum_rows_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src,
                      cols, row_len, first_col, amp, phase);
+  return hipGetLastError();
+}
+
+template <typename T, int NTC>
+hipError_t hilbert_typed(const DevFft *dev, size_t lds, const T *src, int32_t rows, int32_t cols,
+                         const int32_t *row_len, int32_t envelope, T *dst, hipStream_t stream) {
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)hilbert_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((hilbert_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src, cols,
+                     row_len, envelope, dst);
   return hipGetLastError();
 }
 
@@ -583,6 +644,28 @@ hipError_t launch_spectrum_rows(int dtype, int threads, const DevSpec *dev, size
   return threads == 256 ? spectrum_rows_typed<double, 256>(dev, lds, x, rows, cols, row_len, first_col, a, p, s)
        : threads == 512 ? spectrum_rows_typed<double, 512>(dev, lds, x, rows, cols, row_len, first_col, a, p, s)
                         : spectrum_rows_typed<double, 1024>(dev, lds, x, rows, cols, row_len, first_col, a, p, s);
+}
+
+}  // namespace pseg
+
+namespace pseg {
+
+hipError_t launch_hilbert(int dtype, int threads, const DevFft *dev, size_t lds, const void *src,
+                          int32_t rows, int32_t cols, const int32_t *row_len, int32_t envelope,
+                          void *dst, hipStream_t s) {
+  if (rows == 0 || cols == 0) return hipSuccess;
+  if (dtype == 0) {
+    const float *x = (const float *)src;
+    float *d = (float *)dst;
+    return threads == 256 ? hilbert_typed<float, 256>(dev, lds, x, rows, cols, row_len, envelope, d, s)
+         : threads == 512 ? hilbert_typed<float, 512>(dev, lds, x, rows, cols, row_len, envelope, d, s)
+                          : hilbert_typed<float, 1024>(dev, lds, x, rows, cols, row_len, envelope, d, s);
+  }
+  const double *x = (const double *)src;
+  double *d = (double *)dst;
+  return threads == 256 ? hilbert_typed<double, 256>(dev, lds, x, rows, cols, row_len, envelope, d, s)
+       : threads == 512 ? hilbert_typed<double, 512>(dev, lds, x, rows, cols, row_len, envelope, d, s)
+                        : hilbert_typed<double, 1024>(dev, lds, x, rows, cols, row_len, envelope, d, s);
 }
 
 }  // namespace pseg

@@ -1,0 +1,75 @@
+"""The Hilbert transform and the envelope of traces on the GPU (fft/hilbert.nim),
+through the same real FFT as the filters: each row's live samples
+zero-padded to nfft >= 2·samples, -i·sign(f) on the spectrum (0 Hz and
+Nyquist set to zero), inverse.
+
+```python
+hp = initHilbertPlan(np.float32, samples=4001)
+h = hp.hilbertInto(panel, row_len=rl)        # H[cos] = sin
+env = hp.envelopeInto(panel, row_len=rl)     # |x + i·H[x]|
+```
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .pipeline import ChunkMeta, _stream_handle
+
+
+class HilbertPlan:
+    """initHilbertPlan (fft/hilbert.nim:44-53)."""
+
+    def __init__(self, T, samples: int, nfft: int = 0):
+        self.T = np.dtype(T).type
+        self.samples = int(samples)
+        self._h = C.c_void_p()
+        _lib.check(_lib.lib().pseg_hilbert_compile(0 if self.T == np.float32 else 1,
+                                                    self.samples, int(nfft), C.byref(self._h)))
+        n = C.c_int32()
+        _lib.check(_lib.lib().pseg_hilbert_nfft(self._h, C.byref(n)))
+        self.nfft = int(n.value)
+
+    def _run(self, samples, dst, meta, row_len, envelope, stream):
+        import torch
+        if not isinstance(samples, torch.Tensor) or not samples.is_cuda or samples.dim() != 2 \
+                or not samples.is_contiguous():
+            raise ValueError("samples must be a C-contiguous [rows, cols] GPU tensor")
+        if np.dtype(str(samples.dtype).replace("torch.", "")) != self.T:
+            raise ValueError(f"samples hold {samples.dtype}, the plan {self.T}")
+        rows, cols = samples.shape
+        if row_len is None:
+            if meta is None:
+                raise ValueError("needs a ChunkMeta (or row_len)")
+            row_len = meta.device_row_len(cols, samples.device)
+        if dst is None:
+            dst = samples.clone()
+        _lib.check(_lib.lib().pseg_hilbert_apply(
+            self._h, samples.data_ptr(), dst.data_ptr(), rows, cols, row_len.data_ptr(),
+            1 if envelope else 0, _stream_handle(stream)))
+        return dst
+
+    def hilbertInto(self, samples, dst=None, meta: ChunkMeta | None = None, *, row_len=None,
+                    stream=None):
+        """hilbertInto (fft/hilbert.nim:127-132) of every live row; dst (default:
+        a copy of samples) keeps its values past the live ends."""
+        return self._run(samples, dst, meta, row_len, False, stream)
+
+    def envelopeInto(self, samples, dst=None, meta: ChunkMeta | None = None, *, row_len=None,
+                     stream=None):
+        """envelopeInto (fft/hilbert.nim:134-140): sqrt(x² + H[x]²)."""
+        return self._run(samples, dst, meta, row_len, True, stream)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().pseg_hilbert_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def initHilbertPlan(T, samples: int, nfft: int = 0) -> HilbertPlan:
+    return HilbertPlan(T, samples, nfft)
