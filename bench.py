@@ -410,6 +410,29 @@ def spectrum_leg(torch, P, dev, steps, warmup):
             "ms_per_step": round(t * 1000, 5)}
 
 
+def hilbert_leg(torch, P, dev, steps, warmup):
+    """The envelope of every live row of the 2000 x 3001 panel (nfft 6144,
+    no wrap): one envelopeInto per step into rotating outputs."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    hp = P.initHilbertPlan(np.float32, COLS)
+    src = torch.from_numpy(x).to(dev)
+    rl = meta.device_row_len(COLS, dev)
+    outs = [torch.empty_like(src) for _ in range(4)]
+    for i in range(warmup):
+        hp.envelopeInto(src, outs[i % 4], row_len=rl)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        hp.envelopeInto(src, outs[i % 4], row_len=rl)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / steps
+    return {"workload": "envelope (Hilbert transform, nfft 6144) of the 2000x3001 f32 panel",
+            "value": round(ROWS * COLS / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5), "nfft": hp.nfft}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -550,8 +573,8 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, iir_flow, decode, raw_flow, spectrum, f64_panel, "
-                         "correlation, uneven_batch, serving), "
+                         "gain_flow, iir_flow, decode, raw_flow, spectrum, hilbert, "
+                         "f64_panel, correlation, uneven_batch, serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
@@ -703,6 +726,10 @@ def main():
         if want("spectrum"):
             log("spectrum leg")
             sec["spectrum"] = spectrum_leg(torch, P, dev, max(10, args.steps // 4), 3)
+            torch.cuda.empty_cache()
+        if want("hilbert"):
+            log("Hilbert leg")
+            sec["hilbert"] = hilbert_leg(torch, P, dev, max(10, args.steps // 4), 3)
             torch.cuda.empty_cache()
         if want("f64_panel"):
             log("float64 panel leg")
