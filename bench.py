@@ -433,6 +433,27 @@ def hilbert_leg(torch, P, dev, steps, warmup):
             "ms_per_step": round(t * 1000, 5), "nfft": hp.nfft}
 
 
+def spectrogram_leg(torch, P, dev, steps, warmup):
+    """The spectrogram of one 8001-sample trace at 1 ms: 257-sample Hann
+    frames every 10 samples (801 frames, nfft 512), one spectrogramInto per
+    step into rotating outputs."""
+    sg = P.initSpectrogramPlan(np.float32, 257, 10, 1.0, 8001)
+    x = torch.from_numpy(np.random.default_rng(5).standard_normal(8001).astype(np.float32)).to(dev)
+    outs = [torch.empty((sg.frames, sg.bins), dtype=torch.float64, device=dev) for _ in range(4)]
+    for i in range(warmup):
+        sg.spectrogramInto(x, outs[i % 4])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        sg.spectrogramInto(x, outs[i % 4])
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t1) / steps
+    return {"workload": "spectrogram of one 8001-sample f32 trace (257-sample Hann frames every "
+                        "10 samples, 801 frames, nfft 512)",
+            "value": round(sg.frames / t / 1e6, 3), "unit": "Mframes/s",
+            "ms_per_step": round(t * 1000, 5)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -573,7 +594,7 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, iir_flow, decode, raw_flow, spectrum, hilbert, "
+                         "gain_flow, iir_flow, decode, raw_flow, spectrum, hilbert, spectrogram, "
                          "f64_panel, correlation, uneven_batch, serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
@@ -731,6 +752,9 @@ def main():
             log("Hilbert leg")
             sec["hilbert"] = hilbert_leg(torch, P, dev, max(10, args.steps // 4), 3)
             torch.cuda.empty_cache()
+        if want("spectrogram"):
+            log("spectrogram leg")
+            sec["spectrogram"] = spectrogram_leg(torch, P, dev, max(20, args.steps), 5)
         if want("f64_panel"):
             log("float64 panel leg")
             sec["f64_panel"] = f64_panel_leg(torch, P, flow, dev, max(10, args.steps // 4), 3)

@@ -191,6 +191,22 @@ pseg_status pseg_spectrum_amplitude(const pseg_spectrum_plan *plan, const void *
                                     int32_t cols, const int32_t *row_len, int32_t first_col,
                                     void *amp, void *phase, void *stream);
 
+/* A SpectrogramPlan (fft/spectrogram.nim): frames i = 0 .. (samples-1)/hop
+ * centred on column i*hop, windowLen samples weighted by a Hann window
+ * sin²(pi (j+1)/(windowLen+1)), zero-padded to nfft (0: the next power of
+ * two); samples outside the trace count as zero. */
+typedef struct pseg_spectrogram_plan pseg_spectrogram_plan;
+pseg_status pseg_spectrogram_compile(int32_t dtype, int32_t window, int32_t hop, double dt_ms,
+                                     int32_t samples, int32_t nfft, pseg_spectrogram_plan **plan);
+pseg_status pseg_spectrogram_destroy(pseg_spectrogram_plan *plan);
+pseg_status pseg_spectrogram_info_get(const pseg_spectrogram_plan *plan, int32_t *frames,
+                                      int32_t *bins, int32_t *nfft);
+/* spectrogramInto of one device trace x[0 ..< len] (len <= samples): amp
+ * (device, frames * bins float64) = |X[k]|·sides(k)/Σw per frame, Σw the
+ * weights of the frame's samples inside the trace; a frame with none is 0. */
+pseg_status pseg_spectrogram_apply(const pseg_spectrogram_plan *plan, const void *x, int32_t len,
+                                   double *amp, void *stream);
+
 /* A HilbertPlan (fft/hilbert.nim:44-53): traces of up to `samples` samples,
  * zero-padded to nfft >= 2 * samples (0: nextFastLen(2 * samples)) so the
  * transform does not wrap. */

@@ -723,3 +723,48 @@ def hilbert(x: np.ndarray, samples: int, nfft: int = 0, envelope: bool = False) 
         X[last] = 0
     h = np.fft.irfft(X, n)[:xs.size]
     return np.sqrt(xs * xs + h * h) if envelope else h
+
+
+# ------------------------------------------------------------ spectrogram --
+
+def spectrogram(x: np.ndarray, window_len: int, hop: int, dt_ms: float, samples: int,
+                nfft: int = 0) -> np.ndarray:
+    """initSpectrogramPlan + spectrogramInto (fft/spectrogram.nim:63-85,
+    263-346) in float64: frame i covers columns i·hop − window_len//2 + j,
+    weighted by sin²(π(j+1)/(window_len+1)) (outside the trace: zero),
+    zero-padded to nfft (the next power of two of the window by default);
+    amp[i, k] = |X[k]|·2/Σw (1/Σw at 0 Hz and at Nyquist), Σw the weights
+    of the frame's samples inside the trace; a frame with none reads 0."""
+    if window_len < 2:
+        raise ValueError(f"windowLen must be >= 2, got {window_len}")
+    if hop < 1:
+        raise ValueError(f"hop must be >= 1, got {hop}")
+    if samples < 1:
+        raise ValueError(f"samples must be >= 1, got {samples}")
+    if not (dt_ms > 0 and dt_ms < math.inf):
+        raise ValueError(f"dtMs must be finite and > 0, got {dt_ms}")
+    n = nfft if nfft else 1 << (window_len - 1).bit_length()
+    if n < window_len:
+        raise ValueError(f"nfft {n} is shorter than the window ({window_len})")
+    if not is_fast_len(n):
+        raise ValueError(f"nfft {n} is not 2·3·5-smooth")
+    xs = np.asarray(x, np.float64)
+    if xs.size > samples:
+        raise ValueError("a trace longer than the plan's samples")
+    frames = (samples - 1) // hop + 1
+    w = np.sin(np.pi * np.arange(1, window_len + 1) / (window_len + 1)) ** 2
+    cols = np.arange(frames)[:, None] * hop - window_len // 2 + np.arange(window_len)[None, :]
+    inside = (cols >= 0) & (cols < xs.size)
+    buf = np.zeros((frames, n))
+    buf[:, :window_len] = np.where(inside, xs[np.clip(cols, 0, max(xs.size - 1, 0))]
+                                   if xs.size else 0.0, 0.0) * w
+    sw = np.sum(np.where(inside, w, 0.0), axis=1)
+    X = np.fft.rfft(buf, axis=1)
+    sides = np.full(n // 2 + 1, 2.0)
+    sides[0] = 1.0
+    if n % 2 == 0:
+        sides[-1] = 1.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.hypot(X.real, X.imag) * sides[None, :] / sw[:, None]
+    amp[sw <= 0] = 0.0
+    return amp

@@ -1543,6 +1543,123 @@ pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *sr
   return PSEG_OK;
 }
 
+struct pseg_spectrogram_plan {
+  int dtype;
+  int32_t window, hop, samples, frames, nfft, bins, threads;
+  size_t lds;
+  DevSpec *dev;
+  char *base;
+};
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+pseg_status spectrogram_compile(int32_t window, int32_t hop, double dt, int32_t samples,
+                                int32_t nfft, pseg_spectrogram_plan **out) {
+  // initSpectrogramPlan (fft/spectrogram.nim)
+  if (window < 2) return fail(PSEG_E_INVALID, "windowLen must be >= 2, got %d", window);
+  if (hop < 1) return fail(PSEG_E_INVALID, "hop must be >= 1, got %d", hop);
+  if (samples < 1) return fail(PSEG_E_INVALID, "samples must be >= 1, got %d", samples);
+  if (!(dt > 0 && dt < kInf)) return fail(PSEG_E_INVALID, "dtMs must be finite and positive, got %g", dt);
+  int64_t n = nfft;
+  if (n == 0) {
+    n = 1;
+    while (n < window) n *= 2;
+  }
+  if (n < window) return fail(PSEG_E_INVALID, "nfft %lld is shorter than the window (%d)", (long long)n, window);
+  if (next_fast(n, false) != n) return fail(PSEG_E_INVALID, "nfft %lld is not 2·3·5-smooth", (long long)n);
+  if (n % 2 != 0) return fail(PSEG_E_UNSUPPORTED, "an odd nfft (%lld) is beyond this GPU path", (long long)n);
+  if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
+  int threads = 0;
+  for (int t = 256; t <= 1024 && threads == 0; t *= 2)
+    if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
+  const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
+  if (threads == 0 || lds > kMaxRowLds)
+    return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
+  Blob blob;
+  blob.bytes.resize(sizeof(DevSpec));
+  DevSpec H;
+  memset(&H, 0, sizeof H);
+  build_fft<T>(n, H.fft, blob, kFuseCorr);
+  std::vector<T> w(window);
+  for (int32_t j = 0; j < window; j++) {
+    const double sn = std::sin(kPi * (double)(j + 1) / (double)(window + 1));
+    w[j] = (T)(sn * sn);
+  }
+  set_off(H.taper, blob.add_vec(w));
+  H.window = window;
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc of a spectrogram plan");
+  patch_fft(H.fft, base);
+  patch(H.taper, base);
+  memcpy(blob.bytes.data(), &H, sizeof H);
+  e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    return hip_fail(e, "hipMemcpy of a spectrogram plan");
+  }
+  pseg_spectrogram_plan *P = new pseg_spectrogram_plan;
+  P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
+  P->window = window;
+  P->hop = hop;
+  P->samples = samples;
+  P->frames = (samples - 1) / hop + 1;
+  P->nfft = (int32_t)n;
+  P->bins = (int32_t)(n / 2 + 1);
+  P->threads = threads;
+  P->lds = lds;
+  P->dev = (DevSpec *)base;
+  P->base = base;
+  *out = P;
+  return PSEG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pseg_status pseg_spectrogram_compile(int32_t dtype, int32_t window, int32_t hop, double dt_ms,
+                                     int32_t samples, int32_t nfft, pseg_spectrogram_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  return guarded([&] {
+    if (dtype == PSEG_F32) return spectrogram_compile<float>(window, hop, dt_ms, samples, nfft, out);
+    if (dtype == PSEG_F64) return spectrogram_compile<double>(window, hop, dt_ms, samples, nfft, out);
+    return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
+  });
+}
+
+pseg_status pseg_spectrogram_destroy(pseg_spectrogram_plan *plan) {
+  if (!plan) return PSEG_OK;
+  (void)hipFree(plan->base);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_spectrogram_info_get(const pseg_spectrogram_plan *plan, int32_t *frames,
+                                      int32_t *bins, int32_t *nfft) {
+  if (!plan || !frames || !bins || !nfft) return fail(PSEG_E_INVALID, "NULL plan or output");
+  *frames = plan->frames;
+  *bins = plan->bins;
+  *nfft = plan->nfft;
+  return PSEG_OK;
+}
+
+pseg_status pseg_spectrogram_apply(const pseg_spectrogram_plan *plan, const void *x, int32_t len,
+                                   double *amp, void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrogram_compile");
+  if (len < 0 || len > plan->samples)
+    return fail(PSEG_E_INVALID, "a trace of %d samples, the plan takes 0 .. %d", len, plan->samples);
+  if (!amp || (len > 0 && !x)) return fail(PSEG_E_INVALID, "NULL trace or amp");
+  hipError_t e = launch_spectrogram(plan->dtype, plan->threads, plan->dev, plan->lds, x, len,
+                                    plan->hop, plan->frames, amp, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "spectrogram launch");
+  return PSEG_OK;
+}
+
 struct pseg_hilbert_plan {
   int dtype;
   int32_t samples, nfft, threads;

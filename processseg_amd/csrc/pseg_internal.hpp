@@ -270,6 +270,13 @@ hipError_t launch_hilbert(int dtype, int threads, const DevFft *dev, size_t lds,
                           int32_t rows, int32_t cols, const int32_t *row_len, int32_t envelope,
                           void *dst, hipStream_t s);
 
+// spectrogramInto (fft/spectrogram.nim) of one trace of `len` samples:
+// frames x bins float64 amplitudes (the DevSpec's taper holds the Hann
+// weights).
+hipError_t launch_spectrogram(int dtype, int threads, const DevSpec *dev, size_t lds,
+                              const void *x, int32_t len, int32_t hop, int32_t frames,
+                              double *amp, hipStream_t s);
+
 // SEG-Y sample decode (format 1, 2, 3, 5 or 8, big-endian) of `rows` rows
 // `pitch` bytes apart, samples from byte `offset` of each, to a [rows, cols]
 // panel of dtype.

@@ -443,6 +443,63 @@ __global__ __launch_bounds__(NTC) void hilbert_kernel(const DevFft *__restrict__
   }
 }
 
+// |X[k]|·sides(k)/Σw of a frame, float64, into the frame's bins (combine
+// hands over 2·X[k] except at 0 and Nyquist).
+template <typename T> struct FrameAmp {
+  double *amp;
+  double inv_sw;
+  int m, nfft;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    const double re = (double)x.x, im = (double)x.y;
+    const double half = (k != 0 && k != m) ? 0.5 : 1.0;
+    const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
+    amp[k] = hypot(re, im) * half * sides * inv_sw;
+    return x;
+  }
+};
+
+// spectrogramInto (fft/spectrogram.nim): one workgroup per frame i, the
+// window's samples x[i·hop − W/2 + j] (outside the trace: zero) times the
+// Hann weights, zero-padded to nfft, through the real FFT; Σw counts the
+// weights of the samples inside the trace; a frame with none reads 0.
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void spectrogram_kernel(const DevSpec *__restrict__ plan,
+                                                          const T *__restrict__ x, int32_t len,
+                                                          int32_t hop, double *__restrict__ amp) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red_lds[4 * (NTC / 64)];
+  const DevSpec &P = *plan;
+  const DevFft &f = P.fft;
+  const int nf = f.n, bins = f.m + 1, W = P.window;
+  const int64_t frame = blockIdx.x;
+  double *out = amp + frame * (int64_t)bins;
+  const int64_t start = frame * (int64_t)hop - W / 2;
+  const int a = (int)max((int64_t)0, -start);
+  const int b = (int)min((int64_t)W, (int64_t)len - start);
+  if (b <= a) {
+    for (int k = threadIdx.x; k < bins; k += NTC) out[k] = 0;
+    return;
+  }
+  gptr<T> wt = as_global<T>(P.taper);
+  T *buf = (T *)lds;
+  RedSlots red{red_lds, 2 * (NTC / 64), 0};
+  double swl = 0;
+  for (int j = threadIdx.x; j < nf; j += NTC) {
+    T y = (T)0;
+    if (j >= a && j < b) {
+      const T w = wt[j];
+      y = x[start + j] * w;
+      swl += (double)w;
+    }
+    buf[j] = y;
+  }
+  const double sw = block_sum<NTC>(swl, red.next());
+  __syncthreads();
+  V2<T> *A = (V2<T> *)lds;
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  combine<T, NTC>(f, A, A, FrameAmp<T>{out, 1.0 / sw, f.m, nf});
+}
+
 __global__ void fill_f64_kernel(double *p, int32_t n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -506,6 +563,18 @@ hipError_t hilbert_typed(const DevFft *dev, size_t lds, const T *src, int32_t ro
   if (attr != hipSuccess) return attr;
   hipLaunchKernelGGL((hilbert_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src, cols,
                      row_len, envelope, dst);
+  return hipGetLastError();
+}
+
+template <typename T, int NTC>
+hipError_t spectrogram_typed(const DevSpec *dev, size_t lds, const T *x, int32_t len, int32_t hop,
+                             int32_t frames, double *amp, hipStream_t stream) {
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)spectrogram_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((spectrogram_kernel<T, NTC>), dim3(frames), dim3(NTC), lds, stream, dev, x,
+                     len, hop, amp);
   return hipGetLastError();
 }
 
@@ -666,6 +735,26 @@ hipError_t launch_hilbert(int dtype, int threads, const DevFft *dev, size_t lds,
   return threads == 256 ? hilbert_typed<double, 256>(dev, lds, x, rows, cols, row_len, envelope, d, s)
        : threads == 512 ? hilbert_typed<double, 512>(dev, lds, x, rows, cols, row_len, envelope, d, s)
                         : hilbert_typed<double, 1024>(dev, lds, x, rows, cols, row_len, envelope, d, s);
+}
+
+}  // namespace pseg
+
+namespace pseg {
+
+hipError_t launch_spectrogram(int dtype, int threads, const DevSpec *dev, size_t lds,
+                              const void *x, int32_t len, int32_t hop, int32_t frames,
+                              double *amp, hipStream_t s) {
+  if (frames == 0) return hipSuccess;
+  if (dtype == 0) {
+    const float *v = (const float *)x;
+    return threads == 256 ? spectrogram_typed<float, 256>(dev, lds, v, len, hop, frames, amp, s)
+         : threads == 512 ? spectrogram_typed<float, 512>(dev, lds, v, len, hop, frames, amp, s)
+                          : spectrogram_typed<float, 1024>(dev, lds, v, len, hop, frames, amp, s);
+  }
+  const double *v = (const double *)x;
+  return threads == 256 ? spectrogram_typed<double, 256>(dev, lds, v, len, hop, frames, amp, s)
+       : threads == 512 ? spectrogram_typed<double, 512>(dev, lds, v, len, hop, frames, amp, s)
+                        : spectrogram_typed<double, 1024>(dev, lds, v, len, hop, frames, amp, s);
 }
 
 }  // namespace pseg

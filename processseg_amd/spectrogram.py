@@ -1,0 +1,73 @@
+"""Spectrograms of traces on the GPU (fft/spectrogram.nim): one workgroup per
+frame, Hann-weighted and zero-padded to nfft, through the same real FFT as
+the filters; amplitudes |X[k]|·2/Σw (1/Σw at 0 Hz and Nyquist) in float64.
+
+```python
+sg = initSpectrogramPlan(np.float32, windowLen=257, hop=10, dtMs=1, samples=8001)
+amp = sg.spectrogramInto(trace)          # [frames, bins] float64 on the device
+```
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .pipeline import _stream_handle
+
+
+class SpectrogramPlan:
+    """initSpectrogramPlan (fft/spectrogram.nim:63-85)."""
+
+    def __init__(self, T, windowLen: int, hop: int, dtMs: float, samples: int, nfft: int = 0):
+        self.T = np.dtype(T).type
+        self.windowLen, self.hop, self.dtMs = int(windowLen), int(hop), float(dtMs)
+        self.samples = int(samples)
+        self._h = C.c_void_p()
+        _lib.check(_lib.lib().pseg_spectrogram_compile(
+            0 if self.T == np.float32 else 1, self.windowLen, self.hop, self.dtMs,
+            self.samples, int(nfft), C.byref(self._h)))
+        fr, bi, nf = C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(_lib.lib().pseg_spectrogram_info_get(self._h, C.byref(fr), C.byref(bi),
+                                                        C.byref(nf)))
+        self.frames, self.bins, self.nfft = int(fr.value), int(bi.value), int(nf.value)
+
+    def frequency(self, k: int) -> float:
+        """The frequency of bin k, Hz (fft/spectrogram.nim:119-121)."""
+        return k * 1000.0 / (self.nfft * self.dtMs)
+
+    def centre(self, frame: int) -> int:
+        """The trace column frame `frame` is centred on (fft/spectrogram.nim:115-117)."""
+        return frame * self.hop
+
+    def spectrogramInto(self, trace, amp=None, *, stream=None):
+        """spectrogramInto (fft/spectrogram.nim:263-346) of one GPU trace of at
+        most `samples` values: amp is [frames, bins] float64 on the device."""
+        import torch
+        if not isinstance(trace, torch.Tensor) or not trace.is_cuda or trace.dim() != 1 \
+                or not trace.is_contiguous():
+            raise ValueError("trace must be a contiguous 1-D GPU tensor")
+        if np.dtype(str(trace.dtype).replace("torch.", "")) != self.T:
+            raise ValueError(f"trace holds {trace.dtype}, the plan {self.T}")
+        if amp is None:
+            amp = torch.empty((self.frames, self.bins), dtype=torch.float64, device=trace.device)
+        elif amp.dtype != torch.float64 or amp.numel() != self.frames * self.bins \
+                or not amp.is_contiguous():
+            raise ValueError("amp needs frames·bins contiguous float64 values")
+        _lib.check(_lib.lib().pseg_spectrogram_apply(self._h, trace.data_ptr(), trace.numel(),
+                                                     amp.data_ptr(), _stream_handle(stream)))
+        return amp
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.lib().pseg_spectrogram_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def initSpectrogramPlan(T, windowLen: int, hop: int, dtMs: float, samples: int,
+                        nfft: int = 0) -> SpectrogramPlan:
+    return SpectrogramPlan(T, windowLen, hop, dtMs, samples, nfft)

@@ -167,3 +167,17 @@ def test_spectrum_compile_checks_before_device_work():
         assert L.pseg_spectrum_compile(*args, C.byref(h)) == _lib.E_INVALID, args
     assert L.pseg_spectrum_compile(0, 2.0, 100, 0.1, 125, 1, C.byref(h)) == _lib.E_UNSUPPORTED
     assert L.pseg_spectrum_average(None, None, 1, 1, None, 0, None, None, None) == _lib.E_INVALID
+
+
+def test_spectrogram_compile_checks_before_device_work():
+    """pseg_spectrogram_compile refuses bad plans (fft/spectrogram.nim:63-85)
+    before it touches the device; an odd nfft is beyond the GPU path."""
+    import ctypes as C
+    L = _lib.lib()
+    h = C.c_void_p()
+    for args in [(0, 1, 10, 1.0, 100, 0), (0, 64, 0, 1.0, 100, 0), (0, 64, 10, 0.0, 100, 0),
+                 (0, 64, 10, 1.0, 0, 0), (0, 64, 10, 1.0, 100, 32), (0, 64, 10, 1.0, 100, 77),
+                 (3, 64, 10, 1.0, 100, 0)]:
+        assert L.pseg_spectrogram_compile(*args, C.byref(h)) == _lib.E_INVALID, args
+    assert L.pseg_spectrogram_compile(0, 64, 10, 1.0, 100, 75, C.byref(h)) == _lib.E_UNSUPPORTED
+    assert L.pseg_spectrogram_apply(None, None, 0, None, None) == _lib.E_INVALID
