@@ -171,16 +171,25 @@ pseg_status pseg_spectrum_compile(int32_t dtype, double dt_ms, int32_t window,
                                   pseg_spectrum_plan **plan);
 pseg_status pseg_spectrum_destroy(pseg_spectrum_plan *plan);
 pseg_status pseg_spectrum_info_get(const pseg_spectrum_plan *plan, int32_t *nfft, int32_t *bins);
-/* averageAmplitudeInto (fft/spectrum.nim:334-433): the RMS amplitude
- * spectrum, sqrt(mean over rows of |X[k]|² / Σw²) · sides(k), of the windows
- * from column first_col of the live rows of src ([rows, cols], C-contiguous)
- * with a live sample there and no NaN or Inf in the window; row_len from
- * pseg_meta_flatten.  acc: device, bins + 1 float64 of scratch (cleared
- * here; acc[bins] is left holding the rows used); amp: device, bins float64.
- * The rows add their powers in any order (float64 atomics). */
+/* The workspace pseg_spectrum_average needs for `rows` rows, in bytes (the
+ * partial sums of its workgroups; 0 for no rows).  Replaces the pool's
+ * partial buffer, ensureAvgScratch (core/pool.nim:448-460). */
+pseg_status pseg_spectrum_workspace_size(const pseg_spectrum_plan *plan, int32_t rows,
+                                         size_t *bytes);
+/* averageAmplitudeInto (fft/spectrum.nim:334-433; the pool form
+ * core/pool.nim:776-800): the RMS amplitude spectrum, sqrt(mean over rows
+ * of |X[k]|² / Σw²) · sides(k), of the windows from column first_col of the
+ * live rows of src ([rows, cols], C-contiguous) with a live sample there and
+ * no NaN or Inf in the window; row_len from pseg_meta_flatten.  ws: device,
+ * at least pseg_spectrum_workspace_size bytes (PSEG_E_WORKSPACE otherwise),
+ * nothing to clear; amp: device, bins float64; used: device, one float64 (the
+ * rows averaged), or NULL.  The rows' powers are summed in an order fixed by
+ * the row count and the plan, so repeated calls give the same bits.  Calls on
+ * different streams need their own ws. */
 pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
                                   int32_t cols, const int32_t *row_len, int32_t first_col,
-                                  double *acc, double *amp, void *stream);
+                                  void *ws, size_t ws_bytes, double *amp, double *used,
+                                  void *stream);
 
 /* amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
  * first_col: amp = |X[k]|·sides(k)/Σw and phase = atan2(Im, Re) (radians,

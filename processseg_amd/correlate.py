@@ -16,7 +16,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .pipeline import ChunkMeta, _dtype_code, _stream_handle, sameInterval
+from .pipeline import ChunkMeta, _check_panel, _dtype_code, _rows_of, _stream_handle
 
 cmCorrelate, cmDeconvolve = 0, 1
 
@@ -122,25 +122,18 @@ class CorrelationPlan:
 
     def correlateInto(self, src, dst, meta: ChunkMeta = None, *, row_len=None, stream=None):
         """correlateInto (vibroseis/correlate.nim:421-464)."""
-        import torch
-        for t, name in ((src, "src"), (dst, "dst")):
-            if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dim() != 2 \
-                    or not t.is_contiguous():
-                raise ValueError(f"{name} must be a C-contiguous [rows, cols] GPU tensor")
-        if src.shape[0] != dst.shape[0]:
-            raise ValueError(f"src has {src.shape[0]} rows, dst {dst.shape[0]}")
+        _check_panel(src, "src", self.T)
+        _check_panel(dst, "dst", self.T, rows=src.shape[0], cols=self.outSamples,
+                     device=src.device)
         rows, cols = src.shape
-        if row_len is None:
-            if meta is None:
-                raise ValueError("correlateInto needs a ChunkMeta (or row_len)")
-            if len(meta) != rows:
-                raise ValueError(f"samples have {rows} rows, the meta {len(meta)}")
-            if not sameInterval(meta.dtMs, self.dtMs):
-                raise ValueError(f"the records are at {meta.dtMs} ms, the plan at {self.dtMs} ms")
-            if meta.firstSample != 0:
-                raise ValueError("correlation needs whole records from sample 0, got firstSample "
-                                 f"{meta.firstSample}")
-            row_len = meta.device_row_len(cols, src.device, for_correlation=True)
+        if cols > self.recordSamples:
+            raise ValueError(f"the records have {cols} samples, the plan takes at most "
+                             f"{self.recordSamples}")
+        if row_len is None and meta is not None and meta.firstSample != 0:
+            raise ValueError("correlation needs whole records from sample 0, got firstSample "
+                             f"{meta.firstSample}")
+        row_len = _rows_of(meta, row_len, rows, cols, src.device, self.dtMs, "correlateInto",
+                           for_correlation=True)
         _lib.check(_lib.lib().pseg_corr_apply(self._h, src.data_ptr(), rows, cols,
                                               row_len.data_ptr(), dst.data_ptr(), dst.shape[1],
                                               _stream_handle(stream)))

@@ -1528,17 +1528,35 @@ pseg_status pseg_spectrum_info_get(const pseg_spectrum_plan *plan, int32_t *nfft
   return PSEG_OK;
 }
 
+pseg_status pseg_spectrum_workspace_size(const pseg_spectrum_plan *plan, int32_t rows,
+                                         size_t *bytes) {
+  if (!plan || !bytes) return fail(PSEG_E_INVALID, "NULL plan or output");
+  if (rows < 0) return fail(PSEG_E_INVALID, "rows must be >= 0");
+  *bytes = rows == 0 ? 0
+                     : (size_t)spectrum_groups(rows, plan->bins) * ((size_t)plan->bins + 1) *
+                           sizeof(double);
+  return PSEG_OK;
+}
+
 pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
                                   int32_t cols, const int32_t *row_len, int32_t first_col,
-                                  double *acc, double *amp, void *stream) {
+                                  void *ws, size_t ws_bytes, double *amp, double *used,
+                                  void *stream) {
   if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrum_compile");
   if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
   if (first_col < 0) return fail(PSEG_E_INVALID, "firstCol must be >= 0, got %d", first_col);
-  if (!acc || !amp) return fail(PSEG_E_INVALID, "NULL acc or amp");
+  if (!amp) return fail(PSEG_E_INVALID, "NULL amp");
   if (rows > 0 && cols > 0 && (!src || !row_len)) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  size_t need = 0;
+  pseg_spectrum_workspace_size(plan, rows, &need);
+  if (rows > 0 && cols > 0 && (!ws || ws_bytes < need))
+    return fail(PSEG_E_WORKSPACE, "the workspace holds %zu bytes, the average of %d rows needs %zu",
+                ws ? ws_bytes : (size_t)0, rows, need);
+  if (((uintptr_t)ws | (uintptr_t)amp | (uintptr_t)used) % sizeof(double) != 0)
+    return fail(PSEG_E_INVALID, "ws, amp and used must be 8-byte aligned");
   hipError_t e = launch_spectrum(plan->dtype, plan->threads, plan->dev, plan->lds, src, rows, cols,
-                                 row_len, first_col, acc, amp, plan->bins, plan->nfft,
-                                 (hipStream_t)stream);
+                                 row_len, first_col, (double *)ws, amp, used, plan->bins,
+                                 plan->nfft, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "spectrum launch");
   return PSEG_OK;
 }

@@ -255,7 +255,8 @@ struct DevSpec {
 // used) is cleared, filled, and turned into amp (bins float64).
 hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t lds, const void *src,
                            int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
-                           double *acc, double *amp, int32_t bins, int32_t nfft, hipStream_t s);
+                           double *parts, double *amp, double *used, int32_t bins, int32_t nfft,
+                           hipStream_t stream);
 
 // amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
 // first_col: amp and phase, [rows, bins] of dtype.
@@ -288,6 +289,19 @@ hipError_t launch_correlate(int dtype, int threads, const DevCorr *dev_plan, siz
 // Dynamic LDS one row of the pipeline kernel may hold (the workgroup's
 // 160 KiB less room for the kernel's static LDS).
 constexpr size_t kMaxRowLds = 160 * 1024 - 2048;
+
+// An averaged spectrum's partial sums: one per workgroup, bins + 1 float64
+// each (the power sums, then the rows used).  Workgroups: at most 1024, one
+// per row below that, and at most kSpecPartials / (bins + 1) so the partials
+// stay within kSpecPartials float64 (16 MiB).  A function of the row count
+// and the plan only, so the sum's order (and its bits) is too.
+constexpr int64_t kSpecPartials = (int64_t)1 << 21;
+inline int32_t spectrum_groups(int32_t rows, int32_t bins) {
+  int64_t g = rows < 1024 ? rows : 1024;
+  const int64_t cap = kSpecPartials / ((int64_t)bins + 1);
+  if (g > cap) g = cap;
+  return (int32_t)(g < 1 ? 1 : g);
+}
 // The widest row each thread count serves (samples per thread x threads).
 constexpr int pipeline_max_cols(int threads) {
   return threads == 128 ? 128 * 24 : threads == 256 ? 256 * 24 : threads == 512 ? 512 * 24 : 0;

@@ -261,19 +261,24 @@ template <typename T> struct PowerAcc {
   }
 };
 
-// Rows blockIdx.x, + gridDim.x, ...: each row's window (windowInto,
+// Rows blockIdx.x, + gridDim.x, ... (a fixed assignment: the grid depends
+// on the row count and the plan only): each row's window (windowInto,
 // fft/spectrum.nim:271-293) through the real FFT into the workgroup's power
-// sums, which go to acc with one float64 atomic per bin at the end.  A
-// window is demeaned when the plan says so and its float64 sum is finite (a
-// NaN or Inf leaves the row out), tapered (a window cut by the live end
-// takes min(taper[j], taper[n-1-j]) and its own Σw), zero-padded to nfft.
+// sums, which it writes as its partial, parts[blockIdx.x] = (bins power
+// sums, rows used); spectrum_finish_kernel adds the partials in a fixed
+// order, so the result's bits do not depend on the order workgroups run
+// (the reference's pool form does the same with fixed batches of rows,
+// core/pool.nim:776-800).  A window is demeaned when the plan says so and
+// its float64 sum is finite (a NaN or Inf leaves the row out), tapered (a
+// window cut by the live end takes min(taper[j], taper[n-1-j]) and its own
+// Σw), zero-padded to nfft.
 template <typename T, int NTC>
 __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict__ plan,
                                                        const T *__restrict__ src, int32_t rows,
                                                        int32_t cols,
                                                        const int32_t *__restrict__ row_len,
                                                        int32_t first_col, size_t acc_off,
-                                                       double *__restrict__ acc) {
+                                                       double *__restrict__ parts) {
   extern __shared__ __align__(16) char lds[];
   __shared__ double red_lds[4 * (NTC / 64)];
   const DevSpec &P = *plan;
@@ -287,7 +292,7 @@ __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict
   const int W = P.window;
   double used = 0;
   for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
-    const int live = row_len[row];
+    const int live = min(row_len[row], cols);
     if (live <= first_col) continue;  // not live, or nothing live in the window
     const int n = min(live, first_col + W) - first_col;
     const T *s = src + row * (int64_t)cols + first_col;
@@ -320,8 +325,9 @@ __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict
     __syncthreads();  // the buffer is free for the next row
     used += 1;
   }
-  for (int k = threadIdx.x; k < bins; k += NTC) atomicAdd(acc + k, lacc[k]);
-  if (threadIdx.x == 0 && used > 0) atomicAdd(acc + bins, used);
+  double *part = parts + (size_t)blockIdx.x * ((size_t)bins + 1);
+  for (int k = threadIdx.x; k < bins; k += NTC) part[k] = lacc[k];
+  if (threadIdx.x == 0) part[bins] = used;
 }
 
 // amplitudeInto (fft/spectrum.nim:295-312) per row: |X[k]|·sides(k)/Σw and
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(NTC) void spectrum_rows_kernel(const DevSpec *__res
   const DevFft &f = P.fft;
   const int nf = f.n, bins = f.m + 1;
   const int64_t row = blockIdx.x;
-  const int live = row_len[row];
+  const int live = min(row_len[row], cols);
   if (live <= first_col) return;
   const int W = P.window;
   const int n = min(live, first_col + W) - first_col;
@@ -500,18 +506,44 @@ __global__ __launch_bounds__(NTC) void spectrogram_kernel(const DevSpec *__restr
   combine<T, NTC>(f, A, A, FrameAmp<T>{out, 1.0 / sw, f.m, nf});
 }
 
-__global__ void fill_f64_kernel(double *p, int32_t n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
-
-// amp[k] = sqrt(acc[k] / used) · sides(k) (1 at 0 Hz and Nyquist), 0 when no
-// row was used.
-__global__ void spectrum_finish_kernel(const double *acc, int32_t bins, int32_t nfft,
-                                       double *amp) {
-  const double used = acc[bins];
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < bins; k += gridDim.x * blockDim.x) {
-    const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
-    amp[k] = used > 0 ? sqrt(acc[k] / used) * sides : 0.0;
+// The partials of `groups` workgroups, parts[g] = (bins power sums, rows
+// used), summed in a fixed order: wave w of a block takes the groups w,
+// w + 16, w + 32, ... for the block's 64 bins (one per lane), then the 16
+// waves' sums are added in wave order.  amp[k] = sqrt(Σ / used) · sides(k)
+// (1 at 0 Hz and Nyquist), 0 when no row was used; *used_out (when not
+// NULL) = the rows used.
+__global__ __launch_bounds__(1024) void spectrum_finish_kernel(const double *__restrict__ parts,
+                                                               int32_t groups, int32_t bins,
+                                                               int32_t nfft,
+                                                               double *__restrict__ amp,
+                                                               double *__restrict__ used_out) {
+  __shared__ double s_sum[16][64];
+  __shared__ double s_used[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + lane;
+  const size_t pitch = (size_t)bins + 1;
+  const int kk = k < bins ? k : bins;  // lanes past the last bin read the used column
+  double s = 0, u = 0;
+#pragma unroll 8
+  for (int g = wave; g < groups; g += 16) {
+    const double *p = parts + (size_t)g * pitch;
+    s += p[kk];
+    u += p[bins];
+  }
+  s_sum[wave][lane] = s;
+  if (lane == 0) s_used[wave] = u;
+  __syncthreads();
+  if (wave == 0) {
+    double t = 0, used = 0;
+    for (int w = 0; w < 16; w++) {
+      t += s_sum[w][lane];
+      used += s_used[w];
+    }
+    if (k < bins) {
+      const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
+      amp[k] = used > 0 ? sqrt(t / used) * sides : 0.0;
+    }
+    if (blockIdx.x == 0 && lane == 0 && used_out != nullptr) *used_out = used;
   }
 }
 
@@ -583,16 +615,16 @@ hipError_t spectrogram_typed(const DevSpec *dev, size_t lds, const T *x, int32_t
 template <typename T, int NTC>
 hipError_t spectrum_typed(const DevSpec *dev, size_t lds, int32_t bins, const T *src,
                           int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
-                          double *acc, hipStream_t stream) {
+                          double *parts, hipStream_t stream) {
   static const hipError_t attr = hipFuncSetAttribute(
       (const void *)spectrum_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
       (int)kMaxRowLds);
   if (attr != hipSuccess) return attr;
   const size_t acc_off = (lds + 15) / 16 * 16;
   const size_t total = acc_off + (size_t)bins * sizeof(double);
-  const int grid = std::min(rows, 1024);
+  const int grid = spectrum_groups(rows, bins);
   hipLaunchKernelGGL((spectrum_kernel<T, NTC>), dim3(grid), dim3(NTC), total, stream, dev, src,
-                     rows, cols, row_len, first_col, acc_off, acc);
+                     rows, cols, row_len, first_col, acc_off, parts);
   return hipGetLastError();
 }
 
@@ -668,27 +700,26 @@ namespace pseg {
 
 hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t lds, const void *src,
                            int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
-                           double *acc, double *amp, int32_t bins, int32_t nfft, hipStream_t s) {
-  hipLaunchKernelGGL(fill_f64_kernel, dim3(std::max(1, std::min(64, (bins + 256) / 256))), dim3(256),
-                     0, s, acc, bins + 1);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (rows > 0 && cols > 0) {
+                           double *parts, double *amp, double *used, int32_t bins, int32_t nfft,
+                           hipStream_t s) {
+  hipError_t e = hipSuccess;
+  const int32_t groups = rows > 0 && cols > 0 ? spectrum_groups(rows, bins) : 0;
+  if (groups > 0) {
     if (dtype == 0) {
       const float *x = (const float *)src;
-      e = threads == 256 ? spectrum_typed<float, 256>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
-        : threads == 512 ? spectrum_typed<float, 512>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
-                         : spectrum_typed<float, 1024>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s);
+      e = threads == 256 ? spectrum_typed<float, 256>(dev, lds, bins, x, rows, cols, row_len, first_col, parts, s)
+        : threads == 512 ? spectrum_typed<float, 512>(dev, lds, bins, x, rows, cols, row_len, first_col, parts, s)
+                         : spectrum_typed<float, 1024>(dev, lds, bins, x, rows, cols, row_len, first_col, parts, s);
     } else {
       const double *x = (const double *)src;
-      e = threads == 256 ? spectrum_typed<double, 256>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
-        : threads == 512 ? spectrum_typed<double, 512>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s)
-                         : spectrum_typed<double, 1024>(dev, lds, bins, x, rows, cols, row_len, first_col, acc, s);
+      e = threads == 256 ? spectrum_typed<double, 256>(dev, lds, bins, x, rows, cols, row_len, first_col, parts, s)
+        : threads == 512 ? spectrum_typed<double, 512>(dev, lds, bins, x, rows, cols, row_len, first_col, parts, s)
+                         : spectrum_typed<double, 1024>(dev, lds, bins, x, rows, cols, row_len, first_col, parts, s);
     }
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(spectrum_finish_kernel, dim3(std::max(1, std::min(64, (bins + 255) / 256))),
-                     dim3(256), 0, s, acc, bins, nfft, amp);
+  hipLaunchKernelGGL(spectrum_finish_kernel, dim3((bins + 63) / 64), dim3(1024), 0, s, parts,
+                     groups, bins, nfft, amp, used);
   return hipGetLastError();
 }
 

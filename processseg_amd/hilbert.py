@@ -16,7 +16,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .pipeline import ChunkMeta, _stream_handle
+from .pipeline import ChunkMeta, _check_panel, _rows_of, _stream_handle
 
 
 class HilbertPlan:
@@ -33,19 +33,16 @@ class HilbertPlan:
         self.nfft = int(n.value)
 
     def _run(self, samples, dst, meta, row_len, envelope, stream):
-        import torch
-        if not isinstance(samples, torch.Tensor) or not samples.is_cuda or samples.dim() != 2 \
-                or not samples.is_contiguous():
-            raise ValueError("samples must be a C-contiguous [rows, cols] GPU tensor")
-        if np.dtype(str(samples.dtype).replace("torch.", "")) != self.T:
-            raise ValueError(f"samples hold {samples.dtype}, the plan {self.T}")
+        _check_panel(samples, "samples", self.T)
         rows, cols = samples.shape
-        if row_len is None:
-            if meta is None:
-                raise ValueError("needs a ChunkMeta (or row_len)")
-            row_len = meta.device_row_len(cols, samples.device)
+        if cols > self.samples:
+            raise ValueError(f"the rows have {cols} samples, the plan takes at most {self.samples}")
+        row_len = _rows_of(meta, row_len, rows, cols, samples.device, None,
+                           "hilbertInto / envelopeInto")
         if dst is None:
             dst = samples.clone()
+        else:
+            _check_panel(dst, "dst", self.T, rows=rows, cols=cols, device=samples.device)
         _lib.check(_lib.lib().pseg_hilbert_apply(
             self._h, samples.data_ptr(), dst.data_ptr(), rows, cols, row_len.data_ptr(),
             1 if envelope else 0, _stream_handle(stream)))

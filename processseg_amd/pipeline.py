@@ -693,6 +693,59 @@ def _stream_handle(stream) -> Optional[int]:
     return int(s.cuda_stream)
 
 
+def _check_panel(t, name: str, T=None, rows: Optional[int] = None, cols: Optional[int] = None,
+                 device=None):
+    """ValueError unless `t` is a C-contiguous [rows, cols] GPU tensor (of the
+    numpy dtype T, on `device`, of the given shape where those are given):
+    what the kernels index without bounds checks is checked here, before any
+    launch (the reference raises ValueError for a block that does not match)."""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
+    if t.dim() != 2:
+        raise ValueError(f"{name} must be a [rows, cols] tensor, got rank {t.dim()}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be C-contiguous (row-major)")
+    if T is not None and np.dtype(str(t.dtype).replace("torch.", "")) != np.dtype(T):
+        raise ValueError(f"{name} holds {t.dtype}, the plan {np.dtype(T)}")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name} is on {t.device}, the samples on {device}")
+    if rows is not None and t.shape[0] != rows:
+        raise ValueError(f"{name} has {t.shape[0]} rows, expected {rows}")
+    if cols is not None and t.shape[1] != cols:
+        raise ValueError(f"{name} has {t.shape[1]} columns, expected {cols}")
+
+
+def _check_row_len(row_len, rows: int, device):
+    """ValueError unless `row_len` is a contiguous int32 GPU tensor of `rows`
+    values on `device` (one live length per row, -1 for a row left alone)."""
+    import torch
+    if (not isinstance(row_len, torch.Tensor) or not row_len.is_cuda
+            or row_len.dtype != torch.int32 or not row_len.is_contiguous()):
+        raise ValueError("row_len must be a contiguous int32 GPU tensor")
+    if row_len.numel() != rows:
+        raise ValueError(f"row_len has {row_len.numel()} values, the samples {rows} rows")
+    if row_len.device != device:
+        raise ValueError(f"row_len is on {row_len.device}, the samples on {device}")
+
+
+def _rows_of(meta, row_len, rows: int, cols: int, device, plan_dt: Optional[float], what: str,
+             for_correlation: bool = False):
+    """The device live lengths of a call: from `meta` (checked against the
+    block's rows and the plan's interval, as the reference's checkShape and
+    sameInterval do) or a caller's `row_len` (checked as _check_row_len)."""
+    if row_len is None:
+        if meta is None:
+            raise ValueError(f"{what} needs a ChunkMeta (or row_len)")
+        if len(meta) != rows:
+            raise ValueError(f"samples have {rows} rows, the meta {len(meta)}")
+        if plan_dt is not None and not sameInterval(meta.dtMs, plan_dt):
+            raise ValueError(f"the rows' interval {meta.dtMs} ms is not the plan's {plan_dt} ms")
+        return meta.device_row_len(cols, device, for_correlation)
+    _check_row_len(row_len, rows, device)
+    return row_len
+
+
 class PipelinePlan:
     """A compiled pipeline on the GPU (pipeline.nim:200-211).  Immutable; any
     number of streams may apply it at once."""
@@ -754,15 +807,8 @@ class PipelinePlan:
             meta, dst = dst, None
         if dst is None:
             dst = src
-        for t, name in ((src, "src"), (dst, "dst")):
-            if not isinstance(t, torch.Tensor) or not t.is_cuda:
-                raise ValueError(f"{name} must be a GPU tensor")
-            if t.dim() != 2:
-                raise ValueError(f"{name} must be a [rows, cols] tensor, got rank {t.dim()}")
-            if not t.is_contiguous():
-                raise ValueError(f"{name} must be C-contiguous (row-major)")
-            if np.dtype(str(t.dtype).replace("torch.", "")) != self.T:
-                raise ValueError(f"{name} holds {t.dtype}, the plan {self.T}")
+        _check_panel(src, "src", self.T)
+        _check_panel(dst, "dst", self.T, device=src.device)
         if src.shape != dst.shape:
             raise ValueError(f"src is {src.shape[0]} × {src.shape[1]}, dst {dst.shape[0]} × {dst.shape[1]}")
         rows, cols = src.shape
@@ -777,6 +823,8 @@ class PipelinePlan:
             if self.readsKeys:
                 keys = meta.device_keys(src.device)
             first_sample = meta.firstSample
+        else:
+            _check_row_len(row_len, rows, src.device)
         if keys is not None:
             if (not isinstance(keys, torch.Tensor) or not keys.is_cuda or keys.dtype != torch.float64
                     or tuple(keys.shape) != (rows, 2) or not keys.is_contiguous()):
@@ -812,6 +860,8 @@ class PipelinePlan:
         if np.dtype(str(dst.dtype).replace("torch.", "")) != self.T:
             raise ValueError(f"dst holds {dst.dtype}, the plan {self.T}")
         rows, cols = dst.shape
+        if raw.device != dst.device:
+            raise ValueError(f"raw is on {raw.device}, dst on {dst.device}")
         if pitch is None:
             pitch = offset + nb * cols
         if rows > 0 and (rows - 1) * pitch + offset + nb * cols > raw.numel():
@@ -827,6 +877,12 @@ class PipelinePlan:
             if self.readsKeys:
                 keys = meta.device_keys(dst.device)
             first_sample = meta.firstSample
+        else:
+            _check_row_len(row_len, rows, dst.device)
+        if keys is not None:
+            if (not isinstance(keys, torch.Tensor) or not keys.is_cuda or keys.dtype != torch.float64
+                    or tuple(keys.shape) != (rows, 2) or not keys.is_contiguous()):
+                raise ValueError("keys must be a contiguous [rows, 2] float64 GPU tensor")
         _lib.check(_lib.lib().pseg_plan_apply_raw(
             self._h, fmt, raw.data_ptr(), pitch, offset, dst.data_ptr(), rows, cols,
             row_len.data_ptr(), keys.data_ptr() if keys is not None else None, int(first_sample),

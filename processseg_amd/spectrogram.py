@@ -52,9 +52,13 @@ class SpectrogramPlan:
             raise ValueError(f"trace holds {trace.dtype}, the plan {self.T}")
         if amp is None:
             amp = torch.empty((self.frames, self.bins), dtype=torch.float64, device=trace.device)
-        elif amp.dtype != torch.float64 or amp.numel() != self.frames * self.bins \
-                or not amp.is_contiguous():
-            raise ValueError("amp needs frames·bins contiguous float64 values")
+        elif (not isinstance(amp, torch.Tensor) or not amp.is_cuda or amp.device != trace.device
+              or amp.dtype != torch.float64 or amp.numel() != self.frames * self.bins
+              or not amp.is_contiguous()):
+            raise ValueError("amp needs frames·bins contiguous float64 values on the trace's GPU")
+        if trace.numel() > self.samples:
+            raise ValueError(f"the trace has {trace.numel()} samples, the plan takes at most "
+                             f"{self.samples}")
         _lib.check(_lib.lib().pseg_spectrogram_apply(self._h, trace.data_ptr(), trace.numel(),
                                                      amp.data_ptr(), _stream_handle(stream)))
         return amp

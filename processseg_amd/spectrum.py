@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .pipeline import ChunkMeta, _stream_handle, sameInterval
+from .pipeline import ChunkMeta, _check_panel, _rows_of, _stream_handle
 
 
 class SpectrumPlan:
@@ -33,41 +33,57 @@ class SpectrumPlan:
         n, b = C.c_int32(), C.c_int32()
         _lib.check(_lib.lib().pseg_spectrum_info_get(self._h, C.byref(n), C.byref(b)))
         self.nfft, self.bins = int(n.value), int(b.value)
-        self._acc = {}
 
     def binHz(self, k: int) -> float:
         """binHz (fft/spectrum.nim:121-123)."""
         return k * 1000 / (self.nfft * self.dtMs)
 
+    def workspaceSize(self, rows: int) -> int:
+        """Bytes of device workspace an average over `rows` rows needs (the
+        workgroups' partial sums)."""
+        b = C.c_size_t()
+        _lib.check(_lib.lib().pseg_spectrum_workspace_size(self._h, int(rows), C.byref(b)))
+        return int(b.value)
+
     def averageAmplitudeInto(self, samples, meta: ChunkMeta | None = None, firstCol: int = 0,
-                             amp=None, *, row_len=None, stream=None, sync: bool = True):
+                             amp=None, *, row_len=None, ws=None, stream=None, sync: bool = True):
         """averageAmplitudeInto (fft/spectrum.nim:410-433) on a [rows, cols]
         GPU tensor: returns (amp, used) — amp a float64 [bins] GPU tensor,
-        used the rows averaged (read back when `sync`; else None)."""
+        used the rows averaged (read back after `stream` has finished when
+        `sync`; else a one-value float64 GPU tensor written on `stream`).
+        `ws`: a uint8 GPU tensor of at least workspaceSize(rows) bytes, or
+        None for one allocated for this call.  The same input gives the same
+        bits on every call."""
         import torch
-        if not isinstance(samples, torch.Tensor) or not samples.is_cuda or samples.dim() != 2 \
-                or not samples.is_contiguous():
-            raise ValueError("samples must be a C-contiguous [rows, cols] GPU tensor")
-        if np.dtype(str(samples.dtype).replace("torch.", "")) != self.T:
-            raise ValueError(f"samples hold {samples.dtype}, the plan {self.T}")
+        _check_panel(samples, "samples", self.T)
         rows, cols = samples.shape
-        if row_len is None:
-            if meta is None:
-                raise ValueError("averageAmplitudeInto needs a ChunkMeta (or row_len)")
-            if not sameInterval(meta.dtMs, self.dtMs):
-                raise ValueError(f"the rows' interval {meta.dtMs} ms is not the plan's {self.dtMs} ms")
-            row_len = meta.device_row_len(cols, samples.device)
+        if int(firstCol) < 0:
+            raise ValueError(f"firstCol must be ≥ 0, got {firstCol}")
+        row_len = _rows_of(meta, row_len, rows, cols, samples.device, self.dtMs,
+                           "averageAmplitudeInto")
         dev = samples.device
-        acc = self._acc.get(dev)
-        if acc is None:
-            acc = self._acc[dev] = torch.empty(self.bins + 1, dtype=torch.float64, device=dev)
-        if amp is None:
-            amp = torch.empty(self.bins, dtype=torch.float64, device=dev)
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        need = self.workspaceSize(rows)
+        with torch.cuda.stream(s):
+            if ws is None:
+                ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+            elif (not isinstance(ws, torch.Tensor) or not ws.is_cuda or ws.device != dev
+                  or not ws.is_contiguous() or ws.numel() * ws.element_size() < need):
+                raise ValueError(f"ws must be a contiguous GPU tensor of at least {need} bytes")
+            if amp is None:
+                amp = torch.empty(self.bins, dtype=torch.float64, device=dev)
+            elif (not isinstance(amp, torch.Tensor) or amp.dtype != torch.float64
+                  or amp.numel() != self.bins or not amp.is_contiguous() or amp.device != dev):
+                raise ValueError(f"amp must be {self.bins} contiguous float64 values on {dev}")
+            used = torch.empty(1, dtype=torch.float64, device=dev)
         _lib.check(_lib.lib().pseg_spectrum_average(
             self._h, samples.data_ptr(), rows, cols, row_len.data_ptr(), int(firstCol),
-            acc.data_ptr(), amp.data_ptr(), _stream_handle(stream)))
-        used = int(acc[self.bins].item()) if sync else None
-        return amp, used
+            ws.data_ptr(), ws.numel() * ws.element_size(), amp.data_ptr(), used.data_ptr(),
+            _stream_handle(s)))
+        if not sync:
+            return amp, used
+        s.synchronize()
+        return amp, int(used.item())
 
     def amplitudeInto(self, samples, meta: ChunkMeta | None = None, firstCol: int = 0,
                       amp=None, phase=None, *, row_len=None, stream=None):
@@ -77,21 +93,14 @@ class SpectrumPlan:
         sample in the window are left as they are (amp and phase are created
         zeroed when not given)."""
         import torch
-        if not isinstance(samples, torch.Tensor) or not samples.is_cuda or samples.dim() != 2 \
-                or not samples.is_contiguous():
-            raise ValueError("samples must be a C-contiguous [rows, cols] GPU tensor")
-        if np.dtype(str(samples.dtype).replace("torch.", "")) != self.T:
-            raise ValueError(f"samples hold {samples.dtype}, the plan {self.T}")
+        _check_panel(samples, "samples", self.T)
         rows, cols = samples.shape
-        if row_len is None:
-            if meta is None:
-                raise ValueError("amplitudeInto needs a ChunkMeta (or row_len)")
-            row_len = meta.device_row_len(cols, samples.device)
-        for t in (amp, phase):
-            if t is not None and (t.dtype != samples.dtype or tuple(t.shape) != (rows, self.bins)
-                                  or not t.is_contiguous() or not t.is_cuda):
-                raise ValueError(f"amp and phase must be contiguous [{rows}, {self.bins}] GPU "
-                                 f"tensors of {samples.dtype}")
+        if int(firstCol) < 0:
+            raise ValueError(f"firstCol must be ≥ 0, got {firstCol}")
+        row_len = _rows_of(meta, row_len, rows, cols, samples.device, self.dtMs, "amplitudeInto")
+        for t, name in ((amp, "amp"), (phase, "phase")):
+            if t is not None:
+                _check_panel(t, name, self.T, rows=rows, cols=self.bins, device=samples.device)
         if amp is None:
             amp = torch.zeros((rows, self.bins), dtype=samples.dtype, device=samples.device)
         if phase is None:

@@ -166,7 +166,8 @@ def test_spectrum_compile_checks_before_device_work():
                  (0, 2.0, 100, 0.1, 101, 1), (7, 2.0, 100, 0.1, 0, 1)]:
         assert L.pseg_spectrum_compile(*args, C.byref(h)) == _lib.E_INVALID, args
     assert L.pseg_spectrum_compile(0, 2.0, 100, 0.1, 125, 1, C.byref(h)) == _lib.E_UNSUPPORTED
-    assert L.pseg_spectrum_average(None, None, 1, 1, None, 0, None, None, None) == _lib.E_INVALID
+    assert L.pseg_spectrum_average(None, None, 1, 1, None, 0, None, 0, None, None, None) == _lib.E_INVALID
+    assert L.pseg_spectrum_workspace_size(None, 1, None) == _lib.E_INVALID
 
 
 def test_spectrogram_compile_checks_before_device_work():

@@ -106,3 +106,97 @@ def test_amplitude_per_row(gpu, oracle, dtype, win, nfft, taper, first):
         big = wa > 1e-3 * wa.max()
         d = np.angle(np.exp(1j * (ph[r][big].astype(np.float64) - wp[big])))
         assert np.max(np.abs(d)) <= (1e-3 if dtype == np.float32 else 1e-9), r
+
+
+def test_average_is_bit_reproducible(gpu, oracle):
+    """The rows' powers are summed in an order fixed by the row count and the
+    plan (the pool form's fixed batches, core/pool.nim:776-800): two calls
+    on the 2000 x 3001 panel, and a call with a caller workspace, give the
+    same bits."""
+    import torch
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel(2000, 3001, np.float32)
+    src = torch.from_numpy(x).cuda()
+    rl = torch.from_numpy(live.astype(np.int32)).cuda()
+    sp = P.initSpectrumPlan(np.float32, 2.0, 2001, 0.1)
+    a, ua = sp.averageAmplitudeInto(src, firstCol=500, row_len=rl)
+    b, ub = sp.averageAmplitudeInto(src, firstCol=500, row_len=rl)
+    ws = torch.empty(sp.workspaceSize(2000), dtype=torch.uint8, device="cuda")
+    c, uc = sp.averageAmplitudeInto(src, firstCol=500, row_len=rl, ws=ws)
+    assert ua == ub == uc == 2000
+    a, b, c = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert np.array_equal(a.view(np.uint64), c.view(np.uint64))
+    p = oracle.spectrum_plan(2.0, 2001, 0.1, 0)
+    rows = np.arange(0, 2000, 50)
+    want, _ = oracle.spectrum_average(p, x[rows].astype(np.float64), live[rows].astype(np.int32),
+                                      500)
+    sub, _ = sp.averageAmplitudeInto(src[rows].contiguous(), firstCol=500, row_len=rl[rows].contiguous())
+    assert rel_l2(sub.cpu().numpy(), want) <= 1e-5
+
+
+def test_average_on_two_side_streams(gpu, oracle):
+    """Two plans' averages queued on two side streams at once (each call its
+    own workspace, `used` read after its stream): each equals the oracle."""
+    import torch
+    rows, cols = 300, 2001
+    x, _ = synth.gather(rows, cols, 2.0, 17, dtype=np.float32)
+    rl = np.full(rows, cols, np.int32)
+    src = torch.from_numpy(x).cuda()
+    rlt = torch.from_numpy(rl).cuda()
+    plans = [P.initSpectrumPlan(np.float32, 2.0, 1001, 0.1), P.initSpectrumPlan(np.float32, 2.0, 500, 0.2)]
+    firsts = [100, 900]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    outs = []
+    for _ in range(3):
+        for sp, f, s in zip(plans, firsts, streams):
+            outs.append((sp, f, s, sp.averageAmplitudeInto(src, firstCol=f, row_len=rlt, stream=s,
+                                                           sync=False)))
+    for sp, f, s, (amp, used) in outs:
+        s.synchronize()
+        p = oracle.spectrum_plan(2.0, sp.windowLen, 0.1 if sp.windowLen == 1001 else 0.2, 0)
+        want, uw = oracle.spectrum_average(p, x.astype(np.float64), rl, f)
+        assert int(used.item()) == uw == rows
+        assert rel_l2(amp.cpu().numpy(), want) <= 1e-5
+
+
+def test_wrappers_refuse_before_launch(gpu):
+    """The Python wrappers check what the kernels index without bounds checks
+    (the reference's ValueError for a block that does not match its meta):
+    a meta or row_len of another row count, a row_len of another dtype, a dst
+    of another shape or dtype, an amp off the device."""
+    import torch
+    rows, cols = 8, 300
+    x = torch.zeros(rows, cols, device="cuda")
+    good = torch.full((rows,), cols, dtype=torch.int32, device="cuda")
+    bad_rl = [torch.full((rows - 1,), cols, dtype=torch.int32, device="cuda"),
+              torch.full((rows,), cols, dtype=torch.int64, device="cuda"),
+              torch.full((rows,), cols, dtype=torch.int32)]
+    short_meta = P.ChunkMeta(rows - 1, 2.0)
+    sp = P.initSpectrumPlan(np.float32, 2.0, 128, 0.1)
+    hp = P.initHilbertPlan(np.float32, cols)
+    calls = []
+    for rl in bad_rl:
+        calls += [lambda rl=rl: sp.averageAmplitudeInto(x, row_len=rl),
+                  lambda rl=rl: sp.amplitudeInto(x, row_len=rl),
+                  lambda rl=rl: hp.hilbertInto(x, row_len=rl),
+                  lambda rl=rl: hp.envelopeInto(x, row_len=rl)]
+    calls += [lambda: sp.averageAmplitudeInto(x, short_meta),
+              lambda: sp.amplitudeInto(x, short_meta),
+              lambda: sp.amplitudeInto(x, P.ChunkMeta(rows, 4.0)),
+              lambda: hp.hilbertInto(x, short_meta),
+              lambda: hp.hilbertInto(x, torch.zeros(rows, cols - 1, device="cuda"), row_len=good),
+              lambda: hp.hilbertInto(x, torch.zeros(rows, cols, dtype=torch.float64, device="cuda"),
+                                     row_len=good),
+              lambda: hp.envelopeInto(x, torch.zeros(rows, cols), row_len=good),
+              lambda: sp.amplitudeInto(x, row_len=good, amp=torch.zeros(rows, sp.bins)),
+              lambda: sp.averageAmplitudeInto(x, row_len=good, firstCol=-1)]
+    for i, c in enumerate(calls):
+        with pytest.raises(ValueError):
+            c()
+    sg = P.initSpectrogramPlan(np.float32, 64, 10, 1.0, 200)
+    with pytest.raises(ValueError):
+        sg.spectrogramInto(torch.zeros(200, device="cuda"), amp=torch.zeros(sg.frames, sg.bins,
+                                                                            dtype=torch.float64))
+    torch.cuda.synchronize()
