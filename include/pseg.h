@@ -191,6 +191,20 @@ pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *sr
                                   void *ws, size_t ws_bytes, double *amp, double *used,
                                   void *stream);
 
+/* pseg_spectrum_average without a workspace and without allocating: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums go to a static device array of 2^21
+ * float64 (16 MiB) per GPU, held until the process ends; every row count and
+ * plan fits (the workgroups are capped to it), a larger need would be
+ * PSEG_E_INVALID.  The same bits as pseg_spectrum_average on the same input.
+ * All calls on one GPU share the array: calls that overlap on one GPU (two
+ * streams, two threads) race and give wrong averages; serialize them. */
+pseg_status pseg_spectrum_average_noalloc(const pseg_spectrum_plan *plan, const void *src,
+                                          int32_t rows, int32_t cols, const int32_t *row_len,
+                                          int32_t first_col, double *amp, double *used,
+                                          void *stream);
+
 /* amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
  * first_col: amp = |X[k]|·sides(k)/Σw and phase = atan2(Im, Re) (radians,
  * from the window's first sample), [rows, bins] of the plan's dtype on the

@@ -136,7 +136,7 @@ SYMBOLS = [
     "pseg_server_release", "pseg_server_info_get", "pseg_server_destroy",
     "pseg_decode_samples", "pseg_plan_apply_raw", "pseg_spectrum_compile",
     "pseg_spectrum_destroy", "pseg_spectrum_info_get", "pseg_spectrum_average",
-    "pseg_spectrum_workspace_size",
+    "pseg_spectrum_workspace_size", "pseg_spectrum_average_noalloc",
     "pseg_spectrum_amplitude", "pseg_hilbert_compile", "pseg_hilbert_destroy",
     "pseg_hilbert_nfft", "pseg_hilbert_apply", "pseg_spectrogram_compile",
     "pseg_spectrogram_destroy", "pseg_spectrogram_info_get", "pseg_spectrogram_apply",
@@ -177,6 +177,7 @@ def lib():
     L.pseg_spectrum_info_get.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     L.pseg_spectrum_average.argtypes = [vp, vp, i32, i32, vp, i32, vp, C.c_size_t, vp, vp, vp]
     L.pseg_spectrum_workspace_size.argtypes = [vp, i32, C.POINTER(C.c_size_t)]
+    L.pseg_spectrum_average_noalloc.argtypes = [vp, vp, i32, i32, vp, i32, vp, vp, vp]
     L.pseg_spectrum_amplitude.argtypes = [vp, vp, i32, i32, vp, i32, vp, vp, vp]
     L.pseg_hilbert_compile.argtypes = [i32, i32, i32, C.POINTER(vp)]
     L.pseg_hilbert_destroy.argtypes = [vp]

@@ -1561,6 +1561,30 @@ pseg_status pseg_spectrum_average(const pseg_spectrum_plan *plan, const void *sr
   return PSEG_OK;
 }
 
+pseg_status pseg_spectrum_average_noalloc(const pseg_spectrum_plan *plan, const void *src,
+                                          int32_t rows, int32_t cols, const int32_t *row_len,
+                                          int32_t first_col, double *amp, double *used,
+                                          void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrum_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_col < 0) return fail(PSEG_E_INVALID, "firstCol must be >= 0, got %d", first_col);
+  if (!amp) return fail(PSEG_E_INVALID, "NULL amp");
+  if (rows > 0 && cols > 0 && (!src || !row_len)) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  if (((uintptr_t)amp | (uintptr_t)used) % sizeof(double) != 0)
+    return fail(PSEG_E_INVALID, "amp and used must be 8-byte aligned");
+  size_t need = 0;
+  pseg_spectrum_workspace_size(plan, rows, &need);
+  if (need > (size_t)kSpecPartials * sizeof(double))
+    return fail(PSEG_E_INVALID,
+                "the average of %d rows needs %zu bytes of partial sums, beyond the %zu of the "
+                "allocation-free scratch", rows, need, (size_t)kSpecPartials * sizeof(double));
+  hipError_t e = launch_spectrum_noalloc(plan->dtype, plan->threads, plan->dev, plan->lds, src,
+                                         rows, cols, row_len, first_col, amp, used, plan->bins,
+                                         plan->nfft, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "spectrum launch");
+  return PSEG_OK;
+}
+
 struct pseg_spectrogram_plan {
   int dtype;
   int32_t window, hop, samples, frames, nfft, bins, threads;

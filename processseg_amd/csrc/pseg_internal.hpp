@@ -257,6 +257,10 @@ hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t ld
                            int32_t rows, int32_t cols, const int32_t *row_len, int32_t first_col,
                            double *parts, double *amp, double *used, int32_t bins, int32_t nfft,
                            hipStream_t stream);
+hipError_t launch_spectrum_noalloc(int dtype, int threads, const DevSpec *dev, size_t lds,
+                                   const void *src, int32_t rows, int32_t cols,
+                                   const int32_t *row_len, int32_t first_col, double *amp,
+                                   double *used, int32_t bins, int32_t nfft, hipStream_t stream);
 
 // amplitudeInto (fft/spectrum.nim:295-312) of each row's window from
 // first_col: amp and phase, [rows, bins] of dtype.

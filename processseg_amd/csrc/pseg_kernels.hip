@@ -11,6 +11,8 @@
 // gain/gain.nim:221-240).
 #include "pseg_device.hpp"
 
+#include <mutex>
+
 namespace pseg {
 
 // Per-row sum of squares (float64) and maximum |x| bits (as float32 for the
@@ -786,6 +788,58 @@ hipError_t launch_spectrogram(int dtype, int threads, const DevSpec *dev, size_t
   return threads == 256 ? spectrogram_typed<double, 256>(dev, lds, v, len, hop, frames, amp, s)
        : threads == 512 ? spectrogram_typed<double, 512>(dev, lds, v, len, hop, frames, amp, s)
                         : spectrogram_typed<double, 1024>(dev, lds, v, len, hop, frames, amp, s);
+}
+
+}  // namespace pseg
+
+// ------------------------------------------- the allocation-free spectrum
+
+namespace {
+
+// The partial sums of pseg_spectrum_average_noalloc: kSpecPartials float64
+// (16 MiB) of static device memory, one copy per GPU the library runs on,
+// held until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is a stand-in, not a device address): it uses the
+// address hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) double scratch[pseg::kSpecPartials];
+
+constexpr int kMaxScratchDevices = 64;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+hipError_t scratch_address(double **out) {
+  static std::mutex mu;
+  static double *addr[kMaxScratchDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = (double *)p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
+}
+
+}  // namespace
+
+namespace pseg {
+
+hipError_t launch_spectrum_noalloc(int dtype, int threads, const DevSpec *dev, size_t lds,
+                                   const void *src, int32_t rows, int32_t cols,
+                                   const int32_t *row_len, int32_t first_col, double *amp,
+                                   double *used, int32_t bins, int32_t nfft, hipStream_t s) {
+  double *parts = nullptr;
+  hipError_t e = scratch_address(&parts);
+  if (e != hipSuccess) return e;
+  // every workgroup writes its whole partial before the finish kernel reads
+  // it: nothing in the scratch needs clearing
+  return launch_spectrum(dtype, threads, dev, lds, src, rows, cols, row_len, first_col, parts, amp,
+                         used, bins, nfft, s);
 }
 
 }  // namespace pseg
