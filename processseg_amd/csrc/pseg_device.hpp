@@ -302,18 +302,24 @@ template <typename T> struct Bfly<T, 5> {
 // i + i / s); pad_out (s == 1): y is written padded (one spare element after
 // every R1 R2, so the lanes of a store hit different banks).  Both only
 // change uniform strides and per-butterfly bases.
-template <typename T, int R1, int R2, int NTH>
+//
+// LAST: the pass holds the transform's last stage.  Every stage but the last
+// multiplies by twiddles, so the pass's twiddle multiplies are known at
+// compile time (no branch around them: the compiler issues the twiddle
+// loads of an iteration together instead of waiting on each group in turn).
+template <typename T, int R1, int R2, int NTH, bool LAST, int BUDGET>
 __device__ __forceinline__ void pass_inplace(V2<T> *__restrict__ x, CTab<T> tw1, CTab<T> tw2,
                                              int m, int s, int sub1, FastDiv sd, int pad_in,
                                              int pad_out) {
   constexpr int R = R1 * R2;
-  constexpr int MAXIT = pass_iters_max(R);
+  constexpr int MAXIT = pass_iters_max(R, BUDGET);
   // the strides are uniform: keep them (and their multiples) in SGPRs
   s = uni(s);
   const int sub2 = uni(sub1 / R2);
   const int nb = uni(m / R);
-  const bool twiddle1 = sub1 > 1;
-  const bool twiddle2 = R2 > 1 && sub2 > 1;
+  // sub1 > 1 and sub2 > 1 exactly when the stage is not the last
+  constexpr bool twiddle1 = !LAST || R2 > 1;
+  constexpr bool twiddle2 = R2 > 1 && !LAST;
   // input strides of j and j2 (in the padded layout when pad_in)
   const int in1 = uni(s * sub1 + pad_in * sub1), in2 = uni(s * sub2 + pad_in * sub2);
   const int out2 = uni(s * R1);  // output stride of k2
@@ -349,7 +355,7 @@ __device__ __forceinline__ void pass_inplace(V2<T> *__restrict__ x, CTab<T> tw1,
 #pragma unroll
       for (int j2 = 0; j2 < R2; j2++) {
         Bfly<T, R1>::run(v[it][j2]);
-        if (twiddle1) {
+        if constexpr (twiddle1) {
 #pragma unroll
           for (int k = 1; k < R1; k++)
             v[it][j2][k] = cmul<T>(v[it][j2][k], t1[j2 * t1_step + k]);
@@ -366,7 +372,7 @@ __device__ __forceinline__ void pass_inplace(V2<T> *__restrict__ x, CTab<T> tw1,
 #pragma unroll
           for (int j2 = 0; j2 < R2; j2++) u[j2] = v[it][j2][k];
           Bfly<T, R2>::run(u);
-          if (twiddle2) {
+          if constexpr (twiddle2) {
 #pragma unroll
             for (int k2 = 1; k2 < R2; k2++) u[k2] = cmul<T>(u[k2], t2[k2]);
           }
@@ -383,21 +389,43 @@ __device__ __forceinline__ void pass_inplace(V2<T> *__restrict__ x, CTab<T> tw1,
 // shared with the host); the registers of larger ones would cost more
 // occupancy than the LDS round trip they save.  The host checks that no
 // pass needs more than pass_iters_max butterflies per thread.
-template <typename T, int R1, int NTH, int FUSE>
+template <typename T, int R1, int NTH, int FUSE, bool LAST, int BUDGET>
 __device__ __forceinline__ void pass_second(int r2, V2<T> *x, CTab<T> t1, CTab<T> t2, int m,
                                             int s, int sub1, FastDiv sd, int pi, int po) {
   switch (r2) {
-    case 4: if constexpr (fusable(R1, 4, FUSE) && may_follow(R1, 4)) pass_inplace<T, R1, 4, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
-    case 2: if constexpr (fusable(R1, 2, FUSE) && may_follow(R1, 2)) pass_inplace<T, R1, 2, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
-    case 3: if constexpr (fusable(R1, 3, FUSE) && may_follow(R1, 3)) pass_inplace<T, R1, 3, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
-    default: if constexpr (fusable(R1, 5, FUSE) && may_follow(R1, 5)) pass_inplace<T, R1, 5, NTH>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 4: if constexpr (fusable(R1, 4, FUSE) && may_follow(R1, 4)) pass_inplace<T, R1, 4, NTH, LAST, BUDGET>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 2: if constexpr (fusable(R1, 2, FUSE) && may_follow(R1, 2)) pass_inplace<T, R1, 2, NTH, LAST, BUDGET>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    case 3: if constexpr (fusable(R1, 3, FUSE) && may_follow(R1, 3)) pass_inplace<T, R1, 3, NTH, LAST, BUDGET>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+    default: if constexpr (fusable(R1, 5, FUSE) && may_follow(R1, 5)) pass_inplace<T, R1, 5, NTH, LAST, BUDGET>(x, t1, t2, m, s, sub1, sd, pi, po); break;
+  }
+}
+
+template <typename T, int NTH, int FUSE, bool LAST, int BUDGET>
+__device__ __forceinline__ void pass_pair(int r, int r2, V2<T> *a, CTab<T> t, CTab<T> t2, int m,
+                                          int s, int sub, FastDiv sd, int pi, int po) {
+  switch (r) {
+    case 4: pass_second<T, 4, NTH, FUSE, LAST, BUDGET>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+    case 2: pass_second<T, 2, NTH, FUSE, LAST, BUDGET>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+    case 3: pass_second<T, 3, NTH, FUSE, LAST, BUDGET>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+    default: pass_second<T, 5, NTH, FUSE, LAST, BUDGET>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
+  }
+}
+
+template <typename T, int NTH, bool LAST, int BUDGET>
+__device__ __forceinline__ void pass_single(int r, V2<T> *a, CTab<T> t, int m, int s, int sub,
+                                            FastDiv sd, int pi, int po) {
+  switch (r) {
+    case 4: pass_inplace<T, 4, 1, NTH, LAST, BUDGET>(a, t, t, m, s, sub, sd, pi, po); break;
+    case 2: pass_inplace<T, 2, 1, NTH, LAST, BUDGET>(a, t, t, m, s, sub, sd, pi, po); break;
+    case 3: pass_inplace<T, 3, 1, NTH, LAST, BUDGET>(a, t, t, m, s, sub, sd, pi, po); break;
+    default: pass_inplace<T, 5, 1, NTH, LAST, BUDGET>(a, t, t, m, s, sub, sd, pi, po); break;
   }
 }
 
 // The complex forward FFT of the m points at a, in place (a holds f.stride
 // complex values): the plan's stages in passes, stage pairs of at most FUSE
 // points fused (the host built f's padded layout with the same FUSE).
-template <typename T, int NTH, int FUSE>
+template <typename T, int NTH, int FUSE, int BUDGET = kBudgetCorr>
 __device__ __forceinline__ void complex_forward(const DevFft &f, V2<T> *a) {
   CTab<T> tw = ctab<T>(uni_ptr(f.tw));
   int s = 1;
@@ -412,21 +440,13 @@ __device__ __forceinline__ void complex_forward(const DevFft &f, V2<T> *a) {
     const int po = padded && i == 0, pi = padded && s != 1 && s == uni(f.pad);
     if (i + 1 < nrad && fusable(r, r2, FUSE)) {
       CTab<T> t2 = tw + uni(f.tw_off[i + 1]);
-      switch (r) {
-        case 4: pass_second<T, 4, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-        case 2: pass_second<T, 2, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-        case 3: pass_second<T, 3, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-        default: pass_second<T, 5, NTH, FUSE>(r2, a, t, t2, m, s, sub, sd, pi, po); break;
-      }
+      if (i + 2 >= nrad) pass_pair<T, NTH, FUSE, true, BUDGET>(r, r2, a, t, t2, m, s, sub, sd, pi, po);
+      else pass_pair<T, NTH, FUSE, false, BUDGET>(r, r2, a, t, t2, m, s, sub, sd, pi, po);
       s *= r * r2;
       i += 2;
     } else {
-      switch (r) {
-        case 4: pass_inplace<T, 4, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
-        case 2: pass_inplace<T, 2, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
-        case 3: pass_inplace<T, 3, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
-        default: pass_inplace<T, 5, 1, NTH>(a, t, t, m, s, sub, sd, pi, po); break;
-      }
+      if (i + 1 >= nrad) pass_single<T, NTH, true, BUDGET>(r, a, t, m, s, sub, sd, pi, po);
+      else pass_single<T, NTH, false, BUDGET>(r, a, t, m, s, sub, sd, pi, po);
       s *= r;
       i += 1;
     }
@@ -597,7 +617,7 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   // copy of the FFT code serves both directions
 #pragma nounroll
   for (int dir = 0; dir < 2; dir++) {
-    complex_forward<T, NTH, kFusePipeline>(f, A);
+    complex_forward<T, NTH, kFusePipeline, kBudgetPipeline>(f, A);
     if (dir == 0) {
       combine<T, NTH>(f, A, A, GainMul<T>{as_global<T>(uni_ptr(F.gain))});
       __syncthreads();
@@ -757,9 +777,14 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
     if (j < n) s[j] = v[k];
   }
   __syncthreads();
-  const int c = (n + NTH - 1) / NTH;  // <= PER
+  // Runs of c contiguous samples, c odd: thread t's run starts at t·c, so the
+  // lanes of a wave touch LDS banks t·c mod 32, all different (an even c
+  // such as 24 puts 8 lanes on each bank); an even ceil(n / NTH) rounds up
+  // by one, so c <= PER + 1.
+  const int c = ((n + NTH - 1) / NTH) | 1;
   const int c0 = min((int)tid_ * c, n), c1 = min(c0 + c, n);
-  double tv[PER];
+  constexpr int RUN = PER + 1;
+  double tv[RUN];
   Seg pre, suf;
   pre.v = 0;
   pre.f = 0;
@@ -776,7 +801,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
     const int ks = r0 == 0 ? 0 : w - r0;
     double lo = 0, hi = 0;
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
+    for (int k = 0; k < RUN; k++) {
       const int e = c0 + k;
       double t = 0;
       if (e < c1) {
@@ -796,7 +821,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   } else {
     bool past_start = false;  // a block start seen after c0 (suffix)
 #pragma unroll
-    for (int k = 0; k < PER; k++) {
+    for (int k = 0; k < RUN; k++) {
       const int e = c0 + k;
       double t = 0;
       if (e < c1) {
@@ -830,7 +855,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   const int zk = ks == 0 ? w : ks;
   if (c <= w) {
 #pragma unroll
-    for (int k = PER - 1; k >= 0; k--) {
+    for (int k = RUN - 1; k >= 0; k--) {
       if (k < len) {
         run += tv[k];
         s[c0 + k] = (T)run;
@@ -840,7 +865,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   } else {
     r = len > 0 ? (r0 + len - 1) % w : 0;
 #pragma unroll
-    for (int k = PER - 1; k >= 0; k--) {
+    for (int k = RUN - 1; k >= 0; k--) {
       if (k < len) {
         run += tv[k];
         s[c0 + k] = (T)run;
@@ -859,7 +884,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   T plast = (T)0;
   const int jh = c0 - h;  // j - h - 1 at k = 0
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
+  for (int k = 0; k < RUN; k++) {
     if (k < len) {
       T pj;
       if (c <= w) {

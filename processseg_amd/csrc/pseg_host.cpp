@@ -418,10 +418,10 @@ size_t fft_lds_bytes(int64_t n, size_t esz, int fuse) {
 
 // Whether `threads` threads run every pass of an m-point transform in place
 // (no thread holds more than pass_iters_max butterflies of a pass).
-bool fft_fits(int64_t m, int threads, int fuse) {
+bool fft_fits(int64_t m, int threads, int fuse, int budget) {
   for (int r : fft_passes(m, fuse)) {
     const int64_t butterflies = m / r;
-    if ((butterflies + threads - 1) / threads > pass_iters_max(r)) return false;
+    if ((butterflies + threads - 1) / threads > pass_iters_max(r, budget)) return false;
   }
   return true;
 }
@@ -979,7 +979,7 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
     if (t > 512) break;
     bool fits = true;
     for (int i = 0; i < n_ops; i++)
-      if (P->step_nfft[i] > 0 && !fft_fits(P->step_nfft[i] / 2, t, kFusePipeline)) fits = false;
+      if (P->step_nfft[i] > 0 && !fft_fits(P->step_nfft[i] / 2, t, kFusePipeline, kBudgetPipeline)) fits = false;
     if (fits) P->threads = t;
   }
   if (P->threads == 0) {
@@ -1446,7 +1446,7 @@ pseg_status spectrum_compile(double dt, int32_t window, double taper_fraction, i
   if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
   int threads = 0;
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
-    if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
+    if (fft_fits(n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
   // the FFT buffer and the workgroup's power sums (n / 2 + 1 float64)
   if (threads == 0 || (lds + 15) / 16 * 16 + (size_t)(n / 2 + 1) * 8 > kMaxRowLds)
@@ -1616,7 +1616,7 @@ pseg_status spectrogram_compile(int32_t window, int32_t hop, double dt, int32_t 
   if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
   int threads = 0;
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
-    if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
+    if (fft_fits(n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
   if (threads == 0 || lds > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
@@ -1727,7 +1727,7 @@ pseg_status hilbert_compile(int32_t samples, int32_t nfft, pseg_hilbert_plan **o
   if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
   int threads = 0;
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
-    if (fft_fits(n / 2, t, kFuseCorr)) threads = t;
+    if (fft_fits(n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
   if (threads == 0 || lds > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
@@ -2035,8 +2035,8 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   size_t lds = fft_lds_bytes(n, esz, kFuseCorr);
   // 512 threads per record when every pass fits them (two records then
   // share a CU when their LDS allows), else 1024.
-  const int threads = fft_fits(n / 2, 512, kFuseCorr) ? 512 : 1024;
-  if (!fft_fits(n / 2, threads, kFuseCorr))
+  const int threads = fft_fits(n / 2, 512, kFuseCorr, kBudgetCorr) ? 512 : 1024;
+  if (!fft_fits(n / 2, threads, kFuseCorr, kBudgetCorr))
     return fail(PSEG_E_UNSUPPORTED, "a correlation FFT of %lld points is too long for the "
                 "in-place transform of one workgroup", (long long)n);
   if (lds > kMaxRowLds)

@@ -71,8 +71,15 @@ __host__ __device__ constexpr bool may_follow(int r1, int r2) {
 }
 
 // A pass of butterflies of r points runs in place with each thread holding
-// the inputs of at most this many butterflies (32 complex values).
-__host__ __device__ constexpr int pass_iters_max(int r) { return r >= 32 ? 1 : 32 / r; }
+// the inputs of at most this many butterflies: `budget` complex values.  The
+// fused row kernel holds 24 (its passes' registers beside the rest of the
+// row's state decide how many rows share a CU); the one-transform kernels
+// (correlation, spectra, Hilbert) hold 32.
+constexpr int kBudgetPipeline = 24;
+constexpr int kBudgetCorr = 32;
+__host__ __device__ constexpr int pass_iters_max(int r, int budget) {
+  return r >= budget ? 1 : budget / r;
+}
 
 struct DevFilter {
   DevFft fft;

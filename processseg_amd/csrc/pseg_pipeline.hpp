@@ -18,6 +18,10 @@
 #pragma once
 #include "pseg_device.hpp"
 
+#ifndef PSEG_F32_WAVES
+#define PSEG_F32_WAVES 4
+#endif
+
 #include <algorithm>
 
 namespace pseg {
@@ -225,7 +229,7 @@ __device__ __forceinline__ int find_job(const Job *__restrict__ jobs, int n_jobs
 // balance across the chip; each block exits once the counter passes `total`.
 // n_jobs == 0: the single job passed by value.
 template <typename T, int NTH, int PER, int FORM>
-__global__ __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
+__global__ __launch_bounds__(NTH, sizeof(T) == 4 ? PSEG_F32_WAVES : 1) void pipeline_kernel(Job single, const Job *__restrict__ jobs,
                                                       int32_t n_jobs, int32_t total,
                                                       uint32_t *__restrict__ counter) {
   extern __shared__ __align__(16) char lds[];
