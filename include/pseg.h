@@ -247,6 +247,21 @@ pseg_status pseg_hilbert_apply(const pseg_hilbert_plan *plan, const void *src, v
                                int32_t rows, int32_t cols, const int32_t *row_len,
                                int32_t envelope, void *stream);
 
+/* analyticInto (fft/hilbert.nim:160-215) of every live row of src ([rows,
+ * cols], C-contiguous, cols <= the plan's samples; row_len from
+ * pseg_meta_flatten), over its live length: env = |x + i·H[x]|, phase =
+ * arg(x + i·H[x]) in degrees in (-180, 180] (NaN where env is 0), hz = the
+ * circular mean of the two wrapped one-sample phase steps about each sample
+ * (NaN at the row's first and last live samples and where the sample or a
+ * neighbour is 0 or below peak·10^(floor_db/20), peak the row's largest
+ * envelope).  env, phase, hz: device, [rows, cols] float64; rows not live and
+ * columns past the live end are left as they are.  dt_ms finite and
+ * positive, floor_db not NaN (PSEG_E_INVALID). */
+pseg_status pseg_hilbert_analytic(const pseg_hilbert_plan *plan, const void *src, int32_t rows,
+                                  int32_t cols, const int32_t *row_len, double dt_ms,
+                                  double floor_db, double *env, double *phase, double *hz,
+                                  void *stream);
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,

@@ -725,6 +725,48 @@ def hilbert(x: np.ndarray, samples: int, nfft: int = 0, envelope: bool = False) 
     return np.sqrt(xs * xs + h * h) if envelope else h
 
 
+def _wrap_pi(d):
+    """wrapPi (fft/hilbert.nim:152-158): d brought into (-pi, pi]."""
+    two = 2 * math.pi
+    r = d - two * np.floor((d + math.pi) / two)
+    r = np.where(r <= -math.pi, r + two, r)
+    return np.where(r > math.pi, r - two, r)
+
+
+def analytic(x: np.ndarray, samples: int, dt_ms: float, floor_db: float = -60.0,
+             nfft: int = 0, h: np.ndarray | None = None):
+    """analyticInto (fft/hilbert.nim:160-215) in float64: the envelope
+    |x + i·H[x]|, the instantaneous phase in degrees in (-180, 180] (NaN
+    where the envelope is 0) and the instantaneous frequency in Hz, the
+    circular mean of the two wrapped one-sample phase steps about each
+    sample (NaN at both ends and where a sample or a neighbour is 0 or
+    below peak·10^(floor_db/20)).  `h`: the Hilbert transform to use (default:
+    this oracle's)."""
+    xs = np.asarray(x, np.float64)
+    n = xs.size
+    if h is None:
+        h = hilbert(xs, samples, nfft)
+    h = np.asarray(h, np.float64)
+    env = np.hypot(xs, h)
+    with np.errstate(invalid="ignore"):
+        deg = np.arctan2(h, xs) * (180 / math.pi)
+    deg = np.where(deg <= -180, deg + 360, deg)
+    phase = np.where(env > 0, deg, np.nan)
+    peak = float(np.max(env[env > 0])) if np.any(env > 0) else 0.0
+    thr = peak * math.pow(10.0, floor_db / 20.0)
+    scale = 1000 / (4 * math.pi * dt_ms)
+    hz = np.full(n, np.nan)
+    if n > 2:
+        e0, e1, e2 = env[:-2], env[1:-1], env[2:]
+        bad = (e0 <= 0) | (e1 <= 0) | (e2 <= 0) | (e0 < thr) | (e1 < thr) | (e2 < thr)
+        with np.errstate(invalid="ignore"):
+            back = _wrap_pi((phase[1:-1] - phase[:-2]) * math.pi / 180)
+            fwd = _wrap_pi((phase[2:] - phase[1:-1]) * math.pi / 180)
+            val = _wrap_pi(back + 0.5 * _wrap_pi(fwd - back)) * (2 * scale)
+        hz[1:-1] = np.where(bad, np.nan, val)
+    return env, phase, hz
+
+
 # ------------------------------------------------------------ spectrogram --
 
 def spectrogram(x: np.ndarray, window_len: int, hop: int, dt_ms: float, samples: int,

@@ -1804,6 +1804,25 @@ pseg_status pseg_hilbert_apply(const pseg_hilbert_plan *plan, const void *src, v
   return PSEG_OK;
 }
 
+pseg_status pseg_hilbert_analytic(const pseg_hilbert_plan *plan, const void *src, int32_t rows,
+                                  int32_t cols, const int32_t *row_len, double dt_ms,
+                                  double floor_db, double *env, double *phase, double *hz,
+                                  void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_hilbert_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (cols > plan->samples)
+    return fail(PSEG_E_INVALID, "traces of %d columns, the plan takes %d samples", cols, plan->samples);
+  if (!(dt_ms > 0 && dt_ms < kInf)) return fail(PSEG_E_INVALID, "dtMs must be finite and > 0, got %g", dt_ms);
+  if (std::isnan(floor_db)) return fail(PSEG_E_INVALID, "floorDb must not be NaN");
+  if (rows > 0 && cols > 0 && (!src || !row_len || !env || !phase || !hz))
+    return fail(PSEG_E_INVALID, "NULL panel, row_len, env, phase or hz");
+  const double thr_scale = std::pow(10.0, floor_db / 20.0);
+  hipError_t e = launch_analytic(plan->dtype, plan->threads, plan->dev, plan->lds, src, rows, cols,
+                                 row_len, dt_ms, thr_scale, env, phase, hz, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "analytic launch");
+  return PSEG_OK;
+}
+
 pseg_status pseg_spectrum_amplitude(const pseg_spectrum_plan *plan, const void *src, int32_t rows,
                                     int32_t cols, const int32_t *row_len, int32_t first_col,
                                     void *amp, void *phase, void *stream) {

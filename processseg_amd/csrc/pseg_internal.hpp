@@ -281,6 +281,10 @@ hipError_t launch_spectrum_rows(int dtype, int threads, const DevSpec *dev, size
 hipError_t launch_hilbert(int dtype, int threads, const DevFft *dev, size_t lds, const void *src,
                           int32_t rows, int32_t cols, const int32_t *row_len, int32_t envelope,
                           void *dst, hipStream_t s);
+hipError_t launch_analytic(int dtype, int threads, const DevFft *dev, size_t lds, const void *src,
+                           int32_t rows, int32_t cols, const int32_t *row_len, double dt_ms,
+                           double thr_scale, double *env, double *phase, double *hz,
+                           hipStream_t stream);
 
 // spectrogramInto (fft/spectrogram.nim) of one trace of `len` samples:
 // frames x bins float64 amplitudes (the DevSpec's taper holds the Hann

@@ -451,6 +451,104 @@ __global__ __launch_bounds__(NTC) void hilbert_kernel(const DevFft *__restrict__
   }
 }
 
+constexpr double kPi = 3.141592653589793;
+
+// The instantaneous phase of x + i·h in degrees, in (-180, 180]
+// (fft/hilbert.nim:190-194).
+__device__ __forceinline__ double phase_deg(double x, double h) {
+  double deg = atan2(h, x) * (180 / kPi);
+  if (deg <= -180) deg += 360;
+  return deg;
+}
+
+// wrapPi (fft/hilbert.nim:152-158): d brought into (-pi, pi].
+__device__ __forceinline__ double wrap_pi(double d) {
+  const double two = 2 * kPi;
+  double r = d - two * floor((d + kPi) / two);
+  if (r <= -kPi) r += two;
+  else if (r > kPi) r -= two;
+  return r;
+}
+
+// analyticInto (fft/hilbert.nim:160-215), one workgroup per row: the row's
+// live samples through the Hilbert transform (as hilbert_kernel), then per
+// sample in float64 the envelope hypot(x, h), the phase in degrees (NaN where
+// the envelope is 0) and the frequency, the circular mean of the two wrapped
+// one-sample phase steps about the sample, NaN at the row's ends and where
+// the sample or a neighbour is 0 or below peak·thr_scale (peak: the row's
+// largest envelope; thr_scale = 10^(floorDb/20)).  A neighbour's envelope and
+// phase are recomputed from x and H[x] (in LDS), not read back from the
+// outputs.  Rows not live and columns past the live end are left as they
+// are.
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void analytic_kernel(const DevFft *__restrict__ fp,
+                                                       const T *__restrict__ src, int32_t cols,
+                                                       const int32_t *__restrict__ row_len,
+                                                       double dt_ms, double thr_scale,
+                                                       double *__restrict__ env,
+                                                       double *__restrict__ phase,
+                                                       double *__restrict__ hz) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red[NTC / 64];
+  const DevFft &f = *fp;
+  const int64_t row = blockIdx.x;
+  const int live = min(row_len[row], cols);
+  if (live <= 0) return;
+  const int n = f.n;
+  const T *s = src + row * (int64_t)cols;
+  T *buf = (T *)lds;
+  for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
+  __syncthreads();
+  V2<T> *A = (V2<T> *)lds;
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  combine<T, NTC>(f, A, A, HilbertMul<T>{(T)(0.5 / (double)n), f.m});
+  __syncthreads();
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  const T *h = (const T *)A;  // H[x] at j is element j ^ 1
+  double *e_row = env + row * (int64_t)cols;
+  double *p_row = phase + row * (int64_t)cols;
+  double *f_row = hz + row * (int64_t)cols;
+  double mx = 0;
+  for (int j = threadIdx.x; j < live; j += NTC) {
+    const double xj = (double)s[j], hj = (double)h[j ^ 1];
+    const double e = hypot(xj, hj);
+    e_row[j] = e;
+    if (e > mx) mx = e;  // a NaN never raises the peak
+    p_row[j] = e > 0 ? phase_deg(xj, hj) : (double)NAN;
+  }
+  // the row's peak envelope (lane maxima, then the waves')
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double other = __shfl_xor(mx, o, 64);
+    if (other > mx) mx = other;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  double peak = 0;
+  for (int w = 0; w < NTC / 64; w++)
+    if (red[w] > peak) peak = red[w];
+  const double thr = peak * thr_scale;
+  const double scale = 1000 / (4 * kPi * dt_ms);
+  for (int j = threadIdx.x; j < live; j += NTC) {
+    double out = NAN;
+    if (j > 0 && j < live - 1) {
+      const double x0 = (double)s[j - 1], h0 = (double)h[(j - 1) ^ 1];
+      const double x1 = (double)s[j], h1 = (double)h[j ^ 1];
+      const double x2 = (double)s[j + 1], h2 = (double)h[(j + 1) ^ 1];
+      const double e0 = hypot(x0, h0), e1 = hypot(x1, h1), e2 = hypot(x2, h2);
+      if (!(e0 <= 0 || e1 <= 0 || e2 <= 0 || e0 < thr || e1 < thr || e2 < thr)) {
+        const double p0 = e0 > 0 ? phase_deg(x0, h0) : (double)NAN;
+        const double p1 = e1 > 0 ? phase_deg(x1, h1) : (double)NAN;
+        const double p2 = e2 > 0 ? phase_deg(x2, h2) : (double)NAN;
+        const double back = wrap_pi((p1 - p0) * kPi / 180);
+        const double fwd = wrap_pi((p2 - p1) * kPi / 180);
+        out = wrap_pi(back + 0.5 * wrap_pi(fwd - back)) * (2 * scale);
+      }
+    }
+    f_row[j] = out;
+  }
+}
+
 // |X[k]|·sides(k)/Σw of a frame, float64, into the frame's bins (combine
 // hands over 2·X[k] except at 0 and Nyquist).
 template <typename T> struct FrameAmp {
@@ -597,6 +695,19 @@ hipError_t hilbert_typed(const DevFft *dev, size_t lds, const T *src, int32_t ro
   if (attr != hipSuccess) return attr;
   hipLaunchKernelGGL((hilbert_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src, cols,
                      row_len, envelope, dst);
+  return hipGetLastError();
+}
+
+template <typename T, int NTC>
+hipError_t analytic_typed(const DevFft *dev, size_t lds, const T *src, int32_t rows, int32_t cols,
+                          const int32_t *row_len, double dt_ms, double thr_scale, double *env,
+                          double *phase, double *hz, hipStream_t stream) {
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)analytic_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((analytic_kernel<T, NTC>), dim3(rows), dim3(NTC), lds, stream, dev, src, cols,
+                     row_len, dt_ms, thr_scale, env, phase, hz);
   return hipGetLastError();
 }
 
@@ -768,6 +879,23 @@ hipError_t launch_hilbert(int dtype, int threads, const DevFft *dev, size_t lds,
   return threads == 256 ? hilbert_typed<double, 256>(dev, lds, x, rows, cols, row_len, envelope, d, s)
        : threads == 512 ? hilbert_typed<double, 512>(dev, lds, x, rows, cols, row_len, envelope, d, s)
                         : hilbert_typed<double, 1024>(dev, lds, x, rows, cols, row_len, envelope, d, s);
+}
+
+hipError_t launch_analytic(int dtype, int threads, const DevFft *dev, size_t lds, const void *src,
+                           int32_t rows, int32_t cols, const int32_t *row_len, double dt_ms,
+                           double thr_scale, double *env, double *phase, double *hz,
+                           hipStream_t s) {
+  if (rows == 0 || cols == 0) return hipSuccess;
+  if (dtype == 0) {
+    const float *x = (const float *)src;
+    return threads == 256 ? analytic_typed<float, 256>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s)
+         : threads == 512 ? analytic_typed<float, 512>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s)
+                          : analytic_typed<float, 1024>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s);
+  }
+  const double *x = (const double *)src;
+  return threads == 256 ? analytic_typed<double, 256>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s)
+       : threads == 512 ? analytic_typed<double, 512>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s)
+                        : analytic_typed<double, 1024>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s);
 }
 
 }  // namespace pseg

@@ -12,6 +12,7 @@ env = hp.envelopeInto(panel, row_len=rl)     # |x + i·H[x]|
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import numpy as np
 
@@ -58,6 +59,39 @@ class HilbertPlan:
                      stream=None):
         """envelopeInto (fft/hilbert.nim:134-140): sqrt(x² + H[x]²)."""
         return self._run(samples, dst, meta, row_len, True, stream)
+
+    def analyticInto(self, samples, dtMs: float, env=None, phaseDeg=None, hz=None,
+                     meta: ChunkMeta | None = None, *, row_len=None, floorDb: float = -60.0,
+                     stream=None):
+        """analyticInto (fft/hilbert.nim:160-215) of every live row over its
+        live length: (env, phaseDeg, hz), float64 [rows, cols] GPU tensors
+        (created as NaN when not given; rows not live and columns past the
+        live end keep what they hold).  phaseDeg in (-180, 180], NaN where
+        env is 0; hz the circular mean of the two one-sample phase steps
+        about a sample, NaN at the ends and below peak·10^(floorDb/20)."""
+        import torch
+        _check_panel(samples, "samples", self.T)
+        rows, cols = samples.shape
+        if cols > self.samples:
+            raise ValueError(f"the rows have {cols} samples, the plan takes at most {self.samples}")
+        if not (math.isfinite(dtMs) and dtMs > 0):
+            raise ValueError(f"dtMs must be finite and > 0, got {dtMs}")
+        if math.isnan(floorDb):
+            raise ValueError("floorDb must not be NaN")
+        row_len = _rows_of(meta, row_len, rows, cols, samples.device, None, "analyticInto")
+        outs = []
+        for t, name in ((env, "env"), (phaseDeg, "phaseDeg"), (hz, "hz")):
+            if t is None:
+                t = torch.full((rows, cols), float("nan"), dtype=torch.float64,
+                               device=samples.device)
+            else:
+                _check_panel(t, name, np.float64, rows=rows, cols=cols, device=samples.device)
+            outs.append(t)
+        _lib.check(_lib.lib().pseg_hilbert_analytic(
+            self._h, samples.data_ptr(), rows, cols, row_len.data_ptr(), float(dtMs),
+            float(floorDb), outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr(),
+            _stream_handle(stream)))
+        return tuple(outs)
 
     def __del__(self):
         try:
