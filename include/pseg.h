@@ -229,6 +229,26 @@ pseg_status pseg_spectrogram_info_get(const pseg_spectrogram_plan *plan, int32_t
  * weights of the frame's samples inside the trace; a frame with none is 0. */
 pseg_status pseg_spectrogram_apply(const pseg_spectrogram_plan *plan, const void *x, int32_t len,
                                    double *amp, void *stream);
+/* The workspace pseg_spectrogram_average needs for `rows` rows, in bytes. */
+pseg_status pseg_spectrogram_average_workspace_size(const pseg_spectrogram_plan *plan,
+                                                    int32_t rows, size_t *bytes);
+/* averageSpectrogramInto (fft/spectrogram.nim:370-430; the pool form
+ * core/pool.nim:569-631): the power mean sqrt(Σ amp² / used) of the
+ * spectrograms of the live rows of src ([rows, cols], cols <= the plan's
+ * samples; row_len from pseg_meta_flatten), each over its live samples,
+ * into amp (device, frames·bins float64).  Left out: rows not live or with
+ * no live sample, float64 rows whose live samples leave the safe range (NaN,
+ * infinity, subnormal, a nonzero magnitude below 1e-100 or above 1e140),
+ * rows whose spectrogram holds a NaN or an infinity.  used: device, one
+ * float64 (the rows averaged) or NULL; no row: amp all zero.  ws: device, at
+ * least pseg_spectrogram_average_workspace_size bytes (PSEG_E_WORKSPACE
+ * otherwise), one per stream.  One launch over rows × frames; the rows'
+ * powers are summed in an order fixed by the counts, so repeated calls give
+ * the same bits. */
+pseg_status pseg_spectrogram_average(const pseg_spectrogram_plan *plan, const void *src,
+                                     int32_t rows, int32_t cols, const int32_t *row_len,
+                                     void *ws, size_t ws_bytes, double *amp, double *used,
+                                     void *stream);
 
 /* A HilbertPlan (fft/hilbert.nim:44-53): traces of up to `samples` samples,
  * zero-padded to nfft >= 2 * samples (0: nextFastLen(2 * samples)) so the

@@ -810,3 +810,42 @@ def spectrogram(x: np.ndarray, window_len: int, hop: int, dt_ms: float, samples:
         amp = np.hypot(X.real, X.imag) * sides[None, :] / sw[:, None]
     amp[sw <= 0] = 0.0
     return amp
+
+
+def _leaves_float64_range(x: np.ndarray) -> bool:
+    """leavesFloat64Range (fft/spectrogram.nim:137-156): a NaN, an infinity,
+    a subnormal, or a nonzero magnitude below 1e-100 or above 1e140."""
+    a = np.abs(np.asarray(x, np.float64))
+    with np.errstate(invalid="ignore"):
+        bad = ~np.isfinite(a) | (a > 1e140) | ((a != 0) & (a < 1e-100))
+    return bool(np.any(bad))
+
+
+def spectrogram_average(x: np.ndarray, row_len, window_len: int, hop: int, dt_ms: float,
+                        samples: int, nfft: int = 0, dtype=np.float64):
+    """averageSpectrogramInto (fft/spectrogram.nim:370-430): the power mean
+    sqrt(Σ amp² / used) of the rows' spectrograms over their live samples
+    (row_len[r] < 0: not live; 0: no sample).  A float64 row whose live
+    samples leave the safe range is left out before the transform; any row
+    whose spectrogram holds a NaN or an infinity is left out.  Returns
+    (amp [frames, bins], used); with no row used amp is all zero."""
+    frames = (samples - 1) // hop + 1
+    acc = None
+    used = 0
+    for r in range(x.shape[0]):
+        n = int(row_len[r])
+        if n <= 0:
+            continue
+        xs = np.asarray(x[r, :n])
+        if np.dtype(dtype) == np.float64 and _leaves_float64_range(xs):
+            continue
+        with np.errstate(over="ignore", invalid="ignore"):
+            a = spectrogram(xs.astype(np.float64), window_len, hop, dt_ms, samples, nfft)
+        if not np.all(np.isfinite(a)):
+            continue
+        acc = a * a if acc is None else acc + a * a
+        used += 1
+    if used == 0:
+        n = nfft if nfft else 1 << (window_len - 1).bit_length()
+        return np.zeros((frames, n // 2 + 1)), 0
+    return np.sqrt(acc / used), used

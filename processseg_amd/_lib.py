@@ -137,7 +137,7 @@ SYMBOLS = [
     "pseg_decode_samples", "pseg_plan_apply_raw", "pseg_spectrum_compile",
     "pseg_spectrum_destroy", "pseg_spectrum_info_get", "pseg_spectrum_average",
     "pseg_spectrum_workspace_size", "pseg_spectrum_average_noalloc",
-    "pseg_spectrum_amplitude", "pseg_hilbert_analytic", "pseg_hilbert_compile", "pseg_hilbert_destroy",
+    "pseg_spectrum_amplitude", "pseg_spectrogram_average", "pseg_spectrogram_average_workspace_size", "pseg_hilbert_analytic", "pseg_hilbert_compile", "pseg_hilbert_destroy",
     "pseg_hilbert_nfft", "pseg_hilbert_apply", "pseg_spectrogram_compile",
     "pseg_spectrogram_destroy", "pseg_spectrogram_info_get", "pseg_spectrogram_apply",
 ]
@@ -182,6 +182,8 @@ def lib():
     L.pseg_hilbert_compile.argtypes = [i32, i32, i32, C.POINTER(vp)]
     L.pseg_hilbert_destroy.argtypes = [vp]
     L.pseg_hilbert_nfft.argtypes = [vp, C.POINTER(i32)]
+    L.pseg_spectrogram_average.argtypes = [vp, vp, i32, i32, vp, vp, C.c_size_t, vp, vp, vp]
+    L.pseg_spectrogram_average_workspace_size.argtypes = [vp, i32, C.POINTER(C.c_size_t)]
     L.pseg_hilbert_analytic.argtypes = [vp, vp, i32, i32, vp, f64, f64, vp, vp, vp, vp]
     L.pseg_hilbert_apply.argtypes = [vp, vp, vp, i32, i32, vp, i32, vp]
     L.pseg_spectrogram_compile.argtypes = [i32, i32, i32, f64, i32, i32, C.POINTER(vp)]

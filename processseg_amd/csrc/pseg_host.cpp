@@ -1690,6 +1690,66 @@ pseg_status pseg_spectrogram_info_get(const pseg_spectrogram_plan *plan, int32_t
   return PSEG_OK;
 }
 
+namespace {
+// The averaged spectrogram's workspace: status (rows int32), then the
+// partials (groups·frames·bins float64) and the groups' row counts.
+struct SgramWs {
+  size_t status_off, parts_off, used_off, bytes;
+  int32_t groups;
+};
+SgramWs sgram_ws(const pseg_spectrogram_plan *plan, int32_t rows) {
+  SgramWs w;
+  w.groups = sgram_groups(rows, plan->frames);
+  w.status_off = 0;
+  w.parts_off = ((size_t)rows * sizeof(int32_t) + 255) / 256 * 256;
+  w.used_off = w.parts_off + (size_t)w.groups * (size_t)plan->frames * (size_t)plan->bins * sizeof(double);
+  w.bytes = rows == 0 ? 0 : w.used_off + (size_t)w.groups * sizeof(double);
+  return w;
+}
+}  // namespace
+
+pseg_status pseg_spectrogram_average_workspace_size(const pseg_spectrogram_plan *plan,
+                                                    int32_t rows, size_t *bytes) {
+  if (!plan || !bytes) return fail(PSEG_E_INVALID, "NULL plan or output");
+  if (rows < 0) return fail(PSEG_E_INVALID, "rows must be >= 0");
+  *bytes = sgram_ws(plan, rows).bytes;
+  return PSEG_OK;
+}
+
+pseg_status pseg_spectrogram_average(const pseg_spectrogram_plan *plan, const void *src,
+                                     int32_t rows, int32_t cols, const int32_t *row_len,
+                                     void *ws, size_t ws_bytes, double *amp, double *used,
+                                     void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrogram_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (cols > plan->samples)
+    return fail(PSEG_E_INVALID, "rows of %d columns, the plan takes %d samples", cols, plan->samples);
+  if (!amp) return fail(PSEG_E_INVALID, "NULL amp");
+  if (((uintptr_t)ws | (uintptr_t)amp | (uintptr_t)used) % sizeof(double) != 0)
+    return fail(PSEG_E_INVALID, "ws, amp and used must be 8-byte aligned");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = (int64_t)plan->frames * plan->bins;
+  if (rows == 0 || cols == 0) {
+    // nothing averaged: amp all zero, used 0
+    hipError_t e = launch_zero_f64(amp, n, s);
+    if (e == hipSuccess && used) e = launch_zero_f64(used, 1, s);
+    if (e != hipSuccess) return hip_fail(e, "spectrogram launch");
+    return PSEG_OK;
+  }
+  if (!src || !row_len) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  const SgramWs w = sgram_ws(plan, rows);
+  if (!ws || ws_bytes < w.bytes)
+    return fail(PSEG_E_WORKSPACE, "the workspace holds %zu bytes, the average of %d rows needs %zu",
+                ws ? ws_bytes : (size_t)0, rows, w.bytes);
+  char *base = (char *)ws;
+  hipError_t e = launch_spectrogram_average(
+      plan->dtype, plan->threads, plan->dev, plan->lds, plan->bins, src, rows, cols, row_len,
+      plan->hop, plan->frames, w.groups, (int32_t *)(base + w.status_off),
+      (double *)(base + w.parts_off), (double *)(base + w.used_off), amp, used, s);
+  if (e != hipSuccess) return hip_fail(e, "spectrogram launch");
+  return PSEG_OK;
+}
+
 pseg_status pseg_spectrogram_apply(const pseg_spectrogram_plan *plan, const void *x, int32_t len,
                                    double *amp, void *stream) {
   if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_spectrogram_compile");

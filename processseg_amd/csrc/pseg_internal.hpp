@@ -292,6 +292,26 @@ hipError_t launch_analytic(int dtype, int threads, const DevFft *dev, size_t lds
 hipError_t launch_spectrogram(int dtype, int threads, const DevSpec *dev, size_t lds,
                               const void *x, int32_t len, int32_t hop, int32_t frames,
                               double *amp, hipStream_t s);
+// n float64 at p set to 0 by a kernel (graph-capturable, unlike a memset).
+hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t stream);
+// averageSpectrogramInto (fft/spectrogram.nim:370-430) over `rows` rows
+// [rows, cols]: status (rows int32), parts (groups·frames·bins float64) and
+// used (groups float64) are the workspace; amp: frames·bins float64.
+hipError_t launch_spectrogram_average(int dtype, int threads, const DevSpec *dev, size_t lds,
+                                      int32_t bins, const void *src, int32_t rows, int32_t cols,
+                                      const int32_t *row_len, int32_t hop, int32_t frames,
+                                      int32_t groups, int32_t *status, double *parts,
+                                      double *used, double *amp, double *used_out,
+                                      hipStream_t stream);
+// Workgroup rows of an averaged spectrogram: about 2048 workgroups over the
+// frames, at least one row each; a function of the counts only (the sum's
+// order is fixed by it).
+inline int32_t sgram_groups(int32_t rows, int32_t frames) {
+  int64_t g = (2048 + (int64_t)frames - 1) / (frames > 0 ? frames : 1);
+  if (g > rows) g = rows;
+  if (g > 65535) g = 65535;
+  return (int32_t)(g < 1 ? 1 : g);
+}
 
 // SEG-Y sample decode (format 1, 2, 3, 5 or 8, big-endian) of `rows` rows
 // `pitch` bytes apart, samples from byte `offset` of each, to a [rows, cols]

@@ -564,31 +564,24 @@ template <typename T> struct FrameAmp {
   }
 };
 
-// spectrogramInto (fft/spectrogram.nim): one workgroup per frame i, the
-// window's samples x[i·hop − W/2 + j] (outside the trace: zero) times the
-// Hann weights, zero-padded to nfft, through the real FFT; Σw counts the
-// weights of the samples inside the trace; a frame with none reads 0.
-template <typename T, int NTC>
-__global__ __launch_bounds__(NTC) void spectrogram_kernel(const DevSpec *__restrict__ plan,
-                                                          const T *__restrict__ x, int32_t len,
-                                                          int32_t hop, double *__restrict__ amp) {
-  extern __shared__ __align__(16) char lds[];
-  __shared__ double red_lds[4 * (NTC / 64)];
-  const DevSpec &P = *plan;
+// Frame `frame` of the trace x (len samples) through the real FFT: the
+// window's samples x[frame·hop − W/2 + j] (outside the trace: zero) times the
+// Hann weights, zero-padded to nfft; `hook` gets every bin (combine's
+// convention, see FrameAmp) with 1/Σw, Σw the weights of the samples inside
+// the trace.  Returns false (and calls nothing) for a frame with no sample
+// inside the trace.
+template <typename T, int NTC, typename Hook>
+__device__ __forceinline__ bool spectrogram_frame(const DevSpec &P, const T *__restrict__ x,
+                                                  int32_t len, int32_t hop, int64_t frame,
+                                                  char *lds, RedSlots &red, Hook hook) {
   const DevFft &f = P.fft;
-  const int nf = f.n, bins = f.m + 1, W = P.window;
-  const int64_t frame = blockIdx.x;
-  double *out = amp + frame * (int64_t)bins;
+  const int nf = f.n, W = P.window;
   const int64_t start = frame * (int64_t)hop - W / 2;
   const int a = (int)max((int64_t)0, -start);
   const int b = (int)min((int64_t)W, (int64_t)len - start);
-  if (b <= a) {
-    for (int k = threadIdx.x; k < bins; k += NTC) out[k] = 0;
-    return;
-  }
+  if (b <= a) return false;
   gptr<T> wt = as_global<T>(P.taper);
   T *buf = (T *)lds;
-  RedSlots red{red_lds, 2 * (NTC / 64), 0};
   double swl = 0;
   for (int j = threadIdx.x; j < nf; j += NTC) {
     T y = (T)0;
@@ -603,7 +596,173 @@ __global__ __launch_bounds__(NTC) void spectrogram_kernel(const DevSpec *__restr
   __syncthreads();
   V2<T> *A = (V2<T> *)lds;
   complex_forward<T, NTC, kFuseCorr>(f, A);
-  combine<T, NTC>(f, A, A, FrameAmp<T>{out, 1.0 / sw, f.m, nf});
+  hook.inv_sw = 1.0 / sw;
+  combine<T, NTC>(f, A, A, hook);
+  __syncthreads();  // the buffer is free for the next frame
+  return true;
+}
+
+// spectrogramInto (fft/spectrogram.nim): one workgroup per frame i; a
+// frame with no sample inside the trace reads 0.
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void spectrogram_kernel(const DevSpec *__restrict__ plan,
+                                                          const T *__restrict__ x, int32_t len,
+                                                          int32_t hop, double *__restrict__ amp) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red_lds[4 * (NTC / 64)];
+  const DevSpec &P = *plan;
+  const int bins = P.fft.m + 1;
+  const int64_t frame = blockIdx.x;
+  double *out = amp + frame * (int64_t)bins;
+  RedSlots red{red_lds, 2 * (NTC / 64), 0};
+  if (!spectrogram_frame<T, NTC>(P, x, len, hop, frame, lds, red,
+                                 FrameAmp<T>{out, 0.0, P.fft.m, P.fft.n}))
+    for (int k = threadIdx.x; k < bins; k += NTC) out[k] = 0;
+}
+
+// A frame's amplitudes (as FrameAmp) squared into the workgroup's float64
+// sums in LDS, and any NaN or infinity flagged (each bin has one owner
+// thread in combine, so plain adds).
+template <typename T> struct FrameAmpSq {
+  double *acc;
+  double inv_sw;
+  int m, nfft;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    const double re = (double)x.x, im = (double)x.y;
+    const double half = (k != 0 && k != m) ? 0.5 : 1.0;
+    const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
+    const double a = hypot(re, im) * half * sides * inv_sw;
+    acc[k] += a * a;
+    return x;
+  }
+};
+
+// Whether a frame's amplitudes are finite: hypot(re, im)·half·sides/Σw in
+// float64 is finite exactly when re and im are (as T) finite, Σw > 0 for a
+// frame with a sample; any other bin sets *bad.
+template <typename T> struct FrameFinite {
+  int *bad;
+  double inv_sw;
+  __device__ V2<T> operator()(V2<T> x, int) const {
+    if (!(x.x - x.x == (T)0 && x.y - x.y == (T)0)) *bad = 1;
+    return x;
+  }
+};
+
+// Which rows averageSpectrogramInto (fft/spectrogram.nim:370-430) averages,
+// one workgroup per row: status[r] = 0 for a row left out, 1 for a row
+// averaged, 2 for a row whose spectrogram sgram_avg_kernel checks first.
+// Out: rows not live or with no live sample, and a float64 row whose live
+// samples leave the safe range (a NaN, an infinity, a subnormal, a nonzero
+// magnitude below 1e-100 or above 1e140: averagedRow, leavesFloat64Range).
+// A float32 row of finite samples within ±1e30 is averaged: no sum of a
+// frame's weighted samples reaches FLT_MAX, so its spectrogram is finite;
+// any other float32 row is checked (the reference's own test: transform,
+// then amplitudesFinite).
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void sgram_status_kernel(const T *__restrict__ src,
+                                                           int32_t cols,
+                                                           const int32_t *__restrict__ row_len,
+                                                           int32_t *__restrict__ status) {
+  __shared__ int s_flag;
+  const int64_t row = blockIdx.x;
+  const int live = min(row_len[row], cols);
+  if (live <= 0) {
+    if (threadIdx.x == 0) status[row] = 0;
+    return;
+  }
+  const T *x = src + row * (int64_t)cols;
+  if (threadIdx.x == 0) s_flag = 0;
+  __syncthreads();
+  int flag = 0;
+  for (int j = threadIdx.x; j < live; j += NTC) {
+    const double v = fabs((double)x[j]);
+    if (sizeof(T) == 8) {
+      if (!(v <= 1e140) || (v != 0 && v < 1e-100)) flag = 1;
+    } else {
+      if (!(v <= 1e30)) flag = 2;
+    }
+  }
+  if (flag) atomicOr(&s_flag, flag);
+  __syncthreads();
+  if (threadIdx.x == 0) status[row] = (s_flag & 1) ? 0 : (s_flag & 2) ? 2 : 1;
+}
+
+// Frame blockIdx.x of the rows g, g + G, ... (G = gridDim.y, fixed by the
+// row and frame counts).  check != 0: the rows of status 2 only, each set to
+// 0 when this frame's amplitudes hold a NaN or an infinity (every frame of a
+// row is checked by one workgroup or another before any is summed: the
+// check is a launch of its own).  check == 0: the squared amplitudes of the
+// rows of status 1 or 2 summed in row order, written as the partial
+// parts[g][frame][bins]; the (frame 0) workgroups count their rows into
+// used[g].
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void sgram_avg_kernel(const DevSpec *__restrict__ plan,
+                                                        const T *__restrict__ src,
+                                                        int32_t rows, int32_t cols,
+                                                        const int32_t *__restrict__ row_len,
+                                                        int32_t *__restrict__ status,
+                                                        int32_t hop, int32_t frames,
+                                                        int32_t check, size_t acc_off,
+                                                        double *__restrict__ parts,
+                                                        double *__restrict__ used) {
+  extern __shared__ __align__(16) char lds[];
+  __shared__ double red_lds[4 * (NTC / 64)];
+  __shared__ int s_bad;
+  const DevSpec &P = *plan;
+  const int bins = P.fft.m + 1;
+  const int64_t frame = blockIdx.x;
+  const int g = blockIdx.y, G = gridDim.y;
+  double *acc = (double *)(lds + acc_off);
+  if (!check)
+    for (int k = threadIdx.x; k < bins; k += NTC) acc[k] = 0;
+  __syncthreads();
+  RedSlots red{red_lds, 2 * (NTC / 64), 0};
+  double n_used = 0;
+  for (int64_t r = g; r < rows; r += G) {
+    const int st = __builtin_amdgcn_readfirstlane(status[r]);
+    if (check ? st != 2 : st == 0) continue;
+    const int live = min(row_len[r], cols);
+    if (check) {
+      if (threadIdx.x == 0) s_bad = 0;
+      __syncthreads();
+      spectrogram_frame<T, NTC>(P, src + r * (int64_t)cols, live, hop, frame, lds, red,
+                                FrameFinite<T>{&s_bad, 0.0});
+      if (threadIdx.x == 0 && s_bad) atomicExch(status + r, 0);
+      __syncthreads();
+    } else {
+      spectrogram_frame<T, NTC>(P, src + r * (int64_t)cols, live, hop, frame, lds, red,
+                                FrameAmpSq<T>{acc, 0.0, P.fft.m, P.fft.n});
+      n_used += 1;
+    }
+  }
+  if (check) return;
+  __syncthreads();
+  double *part = parts + ((size_t)g * (size_t)frames + (size_t)frame) * (size_t)bins;
+  for (int k = threadIdx.x; k < bins; k += NTC) part[k] = acc[k];
+  if (frame == 0 && threadIdx.x == 0) used[g] = n_used;
+}
+
+__global__ void zero_f64_kernel(double *p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// amp[i] = sqrt(Σ_g parts[g][i] / used) over the G partials in order (i =
+// frame·bins + k), 0 when no row was used; *used_out = the rows used.
+__global__ void sgram_finish_kernel(const double *__restrict__ parts,
+                                    const double *__restrict__ used, int32_t groups, int64_t n,
+                                    double *__restrict__ amp, double *__restrict__ used_out) {
+  double u = 0;
+  for (int g = 0; g < groups; g++) u += used[g];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double t = 0;
+    for (int g = 0; g < groups; g++) t += parts[(size_t)g * (size_t)n + (size_t)i];
+    amp[i] = u > 0 ? sqrt(t / u) : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && used_out != nullptr) *used_out = u;
 }
 
 // The partials of `groups` workgroups, parts[g] = (bins power sums, rows
@@ -896,6 +1055,60 @@ hipError_t launch_analytic(int dtype, int threads, const DevFft *dev, size_t lds
   return threads == 256 ? analytic_typed<double, 256>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s)
        : threads == 512 ? analytic_typed<double, 512>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s)
                         : analytic_typed<double, 1024>(dev, lds, x, rows, cols, row_len, dt_ms, thr_scale, env, phase, hz, s);
+}
+
+template <typename T, int NTC>
+hipError_t sgram_avg_typed(const DevSpec *dev, size_t lds, int32_t bins, const T *src,
+                           int32_t rows, int32_t cols, const int32_t *row_len, int32_t hop,
+                           int32_t frames, int32_t groups, int32_t *status, double *parts,
+                           double *used, double *amp, double *used_out, hipStream_t stream) {
+  static const hipError_t a2 = hipFuncSetAttribute(
+      (const void *)sgram_avg_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (a2 != hipSuccess) return a2;
+  hipLaunchKernelGGL((sgram_status_kernel<T, NTC>), dim3(rows), dim3(NTC), 0, stream, src, cols,
+                     row_len, status);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t acc_off = (lds + 15) / 16 * 16;
+  const size_t total = acc_off + (size_t)bins * sizeof(double);
+  for (int check = 1; check >= 0; check--) {
+    hipLaunchKernelGGL((sgram_avg_kernel<T, NTC>), dim3(frames, groups), dim3(NTC), total, stream,
+                       dev, src, rows, cols, row_len, status, hop, frames, check, acc_off, parts,
+                       used);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int64_t n = (int64_t)frames * bins;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256));
+  hipLaunchKernelGGL(sgram_finish_kernel, dim3(grid), dim3(256), 0, stream, parts, used, groups,
+                     n, amp, used_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 255) / 256));
+  hipLaunchKernelGGL(zero_f64_kernel, dim3(grid), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrogram_average(int dtype, int threads, const DevSpec *dev, size_t lds,
+                                      int32_t bins, const void *src, int32_t rows, int32_t cols,
+                                      const int32_t *row_len, int32_t hop, int32_t frames,
+                                      int32_t groups, int32_t *status, double *parts,
+                                      double *used, double *amp, double *used_out,
+                                      hipStream_t s) {
+  if (dtype == 0) {
+    const float *x = (const float *)src;
+    return threads == 256 ? sgram_avg_typed<float, 256>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+         : threads == 512 ? sgram_avg_typed<float, 512>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+                          : sgram_avg_typed<float, 1024>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s);
+  }
+  const double *x = (const double *)src;
+  return threads == 256 ? sgram_avg_typed<double, 256>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+       : threads == 512 ? sgram_avg_typed<double, 512>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+                        : sgram_avg_typed<double, 1024>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s);
 }
 
 }  // namespace pseg

@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .pipeline import _stream_handle
+from .pipeline import ChunkMeta, _check_panel, _rows_of, _stream_handle
 
 
 class SpectrogramPlan:
@@ -62,6 +62,53 @@ class SpectrogramPlan:
         _lib.check(_lib.lib().pseg_spectrogram_apply(self._h, trace.data_ptr(), trace.numel(),
                                                      amp.data_ptr(), _stream_handle(stream)))
         return amp
+
+    def averageWorkspace(self, rows: int) -> int:
+        """Bytes of device workspace averageSpectrogramInto needs for `rows` rows."""
+        b = C.c_size_t()
+        _lib.check(_lib.lib().pseg_spectrogram_average_workspace_size(self._h, int(rows),
+                                                                       C.byref(b)))
+        return int(b.value)
+
+    def averageSpectrogramInto(self, samples, meta: ChunkMeta | None = None, amp=None, *,
+                               row_len=None, ws=None, stream=None, sync: bool = True):
+        """averageSpectrogramInto (fft/spectrogram.nim:370-430) of the live
+        rows of a [rows, cols] GPU tensor (cols <= samples), one launch over
+        rows × frames: returns (amp, used) — amp a float64 [frames, bins] GPU
+        tensor, used the rows averaged (read after `stream` when `sync`; else
+        a one-value float64 GPU tensor).  The same input gives the same bits
+        on every call."""
+        import torch
+        _check_panel(samples, "samples", self.T)
+        rows, cols = samples.shape
+        if cols > self.samples:
+            raise ValueError(f"the rows have {cols} samples, the plan takes at most {self.samples}")
+        row_len = _rows_of(meta, row_len, rows, cols, samples.device, self.dtMs,
+                           "averageSpectrogramInto")
+        dev = samples.device
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        need = self.averageWorkspace(rows)
+        with torch.cuda.stream(s):
+            if ws is None:
+                ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+            elif (not isinstance(ws, torch.Tensor) or not ws.is_cuda or ws.device != dev
+                  or not ws.is_contiguous() or ws.numel() * ws.element_size() < need):
+                raise ValueError(f"ws must be a contiguous GPU tensor of at least {need} bytes")
+            if amp is None:
+                amp = torch.empty((self.frames, self.bins), dtype=torch.float64, device=dev)
+            elif (not isinstance(amp, torch.Tensor) or amp.dtype != torch.float64
+                  or amp.numel() != self.frames * self.bins or not amp.is_contiguous()
+                  or amp.device != dev):
+                raise ValueError("amp needs frames·bins contiguous float64 values on the "
+                                 "samples' GPU")
+            used = torch.empty(1, dtype=torch.float64, device=dev)
+        _lib.check(_lib.lib().pseg_spectrogram_average(
+            self._h, samples.data_ptr(), rows, cols, row_len.data_ptr(), ws.data_ptr(),
+            ws.numel() * ws.element_size(), amp.data_ptr(), used.data_ptr(), _stream_handle(s)))
+        if not sync:
+            return amp, used
+        s.synchronize()
+        return amp, int(used.item())
 
     def __del__(self):
         try:

@@ -69,3 +69,75 @@ def test_refusals():
             O.spectrogram(np.zeros(10), *args, **kw)
     with pytest.raises(ValueError):
         O.spectrogram(np.zeros(101), 64, 10, 1, 100)
+
+
+# ------------------------------------------------- averaged spectrogram
+# tspectrogram.nim:157-234, 410-446 restated on the oracle.
+
+def _tone_rows(rows, f, amps, n=128, dt=2.0):
+    j = np.arange(n)
+    return np.stack([np.cos(2 * np.pi * f * j * dt / 1000) * a for a in amps[:rows]])
+
+
+def test_average_drops_a_nan_row():
+    """tspectrogram.nim:157-177: a NaN frame drops its trace (2 of 3 used);
+    the tone of amplitude 2 on bin 4 reads 2 in the middle frame."""
+    f = 4 * 1000 / (32 * 2.0)
+    x = _tone_rows(3, f, [2, 2, 2])
+    x[2, 10] = np.nan
+    amp, used = O.spectrogram_average(x, np.full(3, 128), 32, 16, 2.0, 128, 32)
+    assert used == 2
+    mid = amp.shape[0] // 2
+    assert abs(amp[mid, 4] - 2.0) < 1e-3
+
+
+@pytest.mark.parametrize("bad", [np.inf, -np.inf])
+def test_average_nonfinite_rows(bad):
+    """tspectrogram.nim:179-234: an infinity in a covered sample drops its
+    row; a float64 row with a NaN or an infinity between frames, or made of
+    subnormals, is left out before the transform (all-zero result)."""
+    f = 4 * 1000 / (32 * 2.0)
+    x = _tone_rows(2, f, [2, 2])
+    x[1, 20] = bad
+    amp, used = O.spectrogram_average(x, np.full(2, 128), 32, 16, 2.0, 128, 32)
+    assert used == 1 and abs(amp[amp.shape[0] // 2, 4] - 2.0) < 1e-3
+    x32 = _tone_rows(2, f, [2, 2]).astype(np.float32)
+    x32[0, 20] = np.inf
+    _, used32 = O.spectrogram_average(x32, np.full(2, 128), 32, 16, 2.0, 128, 32,
+                                      dtype=np.float32)
+    assert used32 == 1
+    gap = np.full((1, 40), 0.25)
+    for hole in (np.nan, np.inf, -np.inf):
+        g = gap.copy()
+        g[0, 5] = hole
+        amp, used = O.spectrogram_average(g, [40], 2, 10, 1.0, 40, 4)
+        assert used == 0 and np.all(amp == 0)
+        g = gap.copy()
+        g[0, 0] = hole
+        assert O.spectrogram_average(g, [40], 2, 10, 1.0, 40, 4)[1] == 0
+    amp, used = O.spectrogram_average(np.full((1, 40), 1e-320), [40], 2, 10, 1.0, 40, 4)
+    assert used == 0 and np.all(amp == 0)
+
+
+def test_average_is_the_power_mean():
+    """tspectrogram.nim:410-446: float32 rows average as the power mean of
+    their own spectrograms (within 1e-12 of the peak); an infinite row is left
+    out."""
+    f = 4 * 1000 / (32 * 2.0)
+    x = _tone_rows(3, f, [1, 2, 3]).astype(np.float32)
+
+    def power_mean(rows):
+        acc = sum(O.spectrogram(x[r].astype(np.float64), 32, 16, 2.0, 128, 32) ** 2 for r in rows)
+        return np.sqrt(acc / len(rows))
+
+    amp, used = O.spectrogram_average(x, np.full(3, 128), 32, 16, 2.0, 128, 32, dtype=np.float32)
+    want = power_mean([0, 1, 2])
+    assert used == 3 and np.max(np.abs(amp - want)) <= 1e-12 * want.max()
+    x[1, 20] = np.inf
+    amp, used = O.spectrogram_average(x, np.full(3, 128), 32, 16, 2.0, 128, 32, dtype=np.float32)
+    want = power_mean([0, 2])
+    assert used == 2 and np.max(np.abs(amp - want)) <= 1e-12 * want.max()
+    # rows not live or with no sample are not counted
+    amp, used = O.spectrogram_average(x, np.array([-1, 0, 128]), 32, 16, 2.0, 128, 32,
+                                      dtype=np.float32)
+    assert used == 1
