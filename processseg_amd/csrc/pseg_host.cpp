@@ -411,7 +411,7 @@ int32_t first_pass_pad(int64_t m, int fuse) {
 
 // LDS bytes of the in-place buffer of an n-point real transform in T.
 size_t fft_lds_bytes(int64_t n, size_t esz, int fuse) {
-  const int64_t m = n / 2;
+  const int64_t m = n % 2 ? n : n / 2;
   const int32_t pad = first_pass_pad(m, fuse);
   return (size_t)(m + (pad ? m / pad : 0)) * 2 * esz;
 }
@@ -430,7 +430,8 @@ template <typename T>
 void build_fft(int64_t n, DevFft &f, Blob &blob, int fuse) {
   memset(&f, 0, sizeof f);
   f.n = (int32_t)n;
-  f.m = (int32_t)(n / 2);
+  // an odd length runs as the n-point complex transform of (x, 0)
+  f.m = (int32_t)(n % 2 ? n : n / 2);
   std::vector<int> rad = radices_of(f.m);
   if ((int)rad.size() > kMaxRadices) invalid("an FFT of %lld points has too many stages", (long long)n);
   f.nrad = (int32_t)rad.size();
@@ -1441,12 +1442,10 @@ pseg_status spectrum_compile(double dt, int32_t window, double taper_fraction, i
   }
   if (n < window) return fail(PSEG_E_INVALID, "nfft %lld is shorter than the window (%d)", (long long)n, window);
   if (next_fast(n, false) != n) return fail(PSEG_E_INVALID, "nfft %lld is not 2·3·5-smooth", (long long)n);
-  if (n % 2 != 0 || n < 2)
-    return fail(PSEG_E_UNSUPPORTED, "an odd nfft (%lld) is beyond this GPU path", (long long)n);
   if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
   int threads = 0;
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
-    if (fft_fits(n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
+    if (fft_fits(n % 2 ? n : n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
   // the FFT buffer and the workgroup's power sums (n / 2 + 1 float64)
   if (threads == 0 || (lds + 15) / 16 * 16 + (size_t)(n / 2 + 1) * 8 > kMaxRowLds)
@@ -1612,11 +1611,10 @@ pseg_status spectrogram_compile(int32_t window, int32_t hop, double dt, int32_t 
   }
   if (n < window) return fail(PSEG_E_INVALID, "nfft %lld is shorter than the window (%d)", (long long)n, window);
   if (next_fast(n, false) != n) return fail(PSEG_E_INVALID, "nfft %lld is not 2·3·5-smooth", (long long)n);
-  if (n % 2 != 0) return fail(PSEG_E_UNSUPPORTED, "an odd nfft (%lld) is beyond this GPU path", (long long)n);
   if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
   int threads = 0;
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
-    if (fft_fits(n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
+    if (fft_fits(n % 2 ? n : n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
   if (threads == 0 || lds > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);
@@ -1783,11 +1781,10 @@ pseg_status hilbert_compile(int32_t samples, int32_t nfft, pseg_hilbert_plan **o
     return fail(PSEG_E_INVALID, "nfft %lld is below twice the samples (%lld): the transform would wrap",
                 (long long)n, 2 * (long long)samples);
   if (next_fast(n, false) != n) return fail(PSEG_E_INVALID, "nfft %lld is not 2·3·5-smooth", (long long)n);
-  if (n % 2 != 0) return fail(PSEG_E_UNSUPPORTED, "an odd nfft (%lld) is beyond this GPU path", (long long)n);
   if (n > (1 << 20)) return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points is beyond this GPU path", (long long)n);
   int threads = 0;
   for (int t = 256; t <= 1024 && threads == 0; t *= 2)
-    if (fft_fits(n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
+    if (fft_fits(n % 2 ? n : n / 2, t, kFuseCorr, kBudgetCorr)) threads = t;
   const size_t lds = std::max(fft_lds_bytes(n, sizeof(T), kFuseCorr), (size_t)n * sizeof(T));
   if (threads == 0 || lds > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "an FFT of %lld points does not fit one workgroup", (long long)n);

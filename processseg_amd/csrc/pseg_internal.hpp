@@ -38,8 +38,9 @@ struct FastDiv {
 // stride s = product of the earlier radices, sub = m / (s r).  Twiddles
 // w_cur^{pk} of stage i start at tw_off[i] (complex, interleaved re/im).
 struct DevFft {
-  int32_t n;          // real length (even)
-  int32_t m;          // complex length n / 2
+  int32_t n;          // real length
+  int32_t m;          // complex length: n / 2 for an even n; n for an odd n (the
+                      // n-point complex transform of (x, 0), no split)
   int32_t nrad;
   int32_t rad[kMaxRadices];
   int32_t sub[kMaxRadices];

@@ -205,6 +205,77 @@ __global__ __launch_bounds__(1024) void panel_factor_kernel(
   }
 }
 
+// Real transforms of any 2·3·5-smooth length n.  Even n: the samples sit
+// in the buffer as T values, the transform is the half-length complex one
+// plus the split (combine).  Odd n (a plan's explicit nfft): the samples sit
+// as complex (x, 0) and the transform is the n-point complex one (the plan's
+// DevFft then holds m = n).  The bins of either go to the same hooks in
+// combine's convention: 2·X[k], X[k] at 0 Hz (and at Nyquist for even n).
+__device__ __forceinline__ bool fft_odd(const DevFft &f) { return (f.n & 1) != 0; }
+
+// Sample j of the transform's input.
+template <typename T>
+__device__ __forceinline__ void put_sample(T *buf, int j, T y, bool odd) {
+  if (odd) {
+    buf[2 * j] = y;
+    buf[2 * j + 1] = (T)0;
+  } else {
+    buf[j] = y;
+  }
+}
+
+// The forward real transform of the samples in A (put by put_sample, the
+// block synchronized), every bin k = 0 .. n/2 handed to hook.
+template <typename T, int NTC, typename Hook>
+__device__ __forceinline__ void real_forward(const DevFft &f, V2<T> *A, Hook hook) {
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  if (!fft_odd(f)) {
+    combine<T, NTC>(f, A, A, hook);
+    return;
+  }
+  const int half = f.n / 2;
+  for (int k = threadIdx.x; k <= half; k += NTC) {
+    V2<T> x = A[k];
+    if (k != 0) x = mk<T>(x.x * (T)2, x.y * (T)2);
+    hook(x, k);
+  }
+}
+
+// The Hilbert transform of the real samples in A (put by put_sample, the
+// block synchronized): -i·sign(f) on the spectrum, 0 Hz (and Nyquist) set to
+// zero, inverse.  Afterwards H[x] at j is hilbert_at<T>(A, j, n).  Even n: the
+// half-length transform, the split with HilbertMul and the unsplit, the
+// inverse as the forward one on swapped pairs.  Odd n: the n-point complex
+// transform, conj(-i·sign(f)·X) in place, the forward transform again (the
+// inverse of Y is conj(FFT(conj Y)) / n; its real part is H[x]).
+template <typename T> struct HilbertMul;
+template <typename T, int NTC>
+__device__ __forceinline__ void hilbert_transform(const DevFft &f, V2<T> *A) {
+  const int n = f.n;
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+  if (!fft_odd(f)) {
+    combine<T, NTC>(f, A, A, HilbertMul<T>{(T)(0.5 / (double)n), f.m});
+  } else {
+    for (int k = threadIdx.x; k <= n / 2; k += NTC) {
+      if (k == 0) {
+        A[0] = mk<T>((T)0, (T)0);
+      } else {
+        const V2<T> a = A[k], b = A[n - k];
+        A[k] = mk<T>(a.y, a.x);       // conj(-i·a)
+        A[n - k] = mk<T>(-b.y, -b.x); // conj(+i·b)
+      }
+    }
+  }
+  __syncthreads();
+  complex_forward<T, NTC, kFuseCorr>(f, A);
+}
+
+template <typename T>
+__device__ __forceinline__ T hilbert_at(const V2<T> *A, int j, int n) {
+  if (n & 1) return A[j].x * (T)(1.0 / (double)n);
+  return ((const T *)A)[j ^ 1];
+}
+
 // Vibroseis correlation of one record per workgroup.
 template <typename T, int NTC>
 __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ plan,
@@ -285,7 +356,8 @@ __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict
   __shared__ double red_lds[4 * (NTC / 64)];
   const DevSpec &P = *plan;
   const DevFft &f = P.fft;
-  const int nf = f.n, bins = f.m + 1;
+  const int nf = f.n, bins = f.n / 2 + 1;
+  const bool odd = fft_odd(f);
   double *lacc = (double *)(lds + acc_off);
   for (int k = threadIdx.x; k < bins; k += NTC) lacc[k] = 0;
   __syncthreads();
@@ -317,13 +389,12 @@ __global__ __launch_bounds__(NTC) void spectrum_kernel(const DevSpec *__restrict
         y = (s[j] - mean) * w;
         swl += (double)w;
       }
-      buf[j] = y;
+      put_sample(buf, j, y, odd);
     }
     const double sw = full ? P.taper_sum : block_sum<NTC>(swl, red.next());
     __syncthreads();
     V2<T> *A = (V2<T> *)lds;
-    complex_forward<T, NTC, kFuseCorr>(f, A);
-    combine<T, NTC>(f, A, A, PowerAcc<T>{lacc, 1.0 / (sw * sw), f.m});
+    real_forward<T, NTC>(f, A, PowerAcc<T>{lacc, 1.0 / (sw * sw), f.m});
     __syncthreads();  // the buffer is free for the next row
     used += 1;
   }
@@ -363,7 +434,8 @@ __global__ __launch_bounds__(NTC) void spectrum_rows_kernel(const DevSpec *__res
   __shared__ double red_lds[4 * (NTC / 64)];
   const DevSpec &P = *plan;
   const DevFft &f = P.fft;
-  const int nf = f.n, bins = f.m + 1;
+  const int nf = f.n, bins = f.n / 2 + 1;
+  const bool odd = fft_odd(f);
   const int64_t row = blockIdx.x;
   const int live = min(row_len[row], cols);
   if (live <= first_col) return;
@@ -391,14 +463,13 @@ __global__ __launch_bounds__(NTC) void spectrum_rows_kernel(const DevSpec *__res
       y = (s[j] - mean) * w;
       swl += (double)w;
     }
-    buf[j] = y;
+    put_sample(buf, j, y, odd);
   }
   double sw = full ? P.taper_sum : block_sum<NTC>(swl, red.next());
   if (!finite) sw = NAN;
   __syncthreads();
   V2<T> *A = (V2<T> *)lds;
-  complex_forward<T, NTC, kFuseCorr>(f, A);
-  combine<T, NTC>(f, A, A, AmpPhaseOut<T>{amp + row * (int64_t)bins, phase + row * (int64_t)bins,
+  real_forward<T, NTC>(f, A, AmpPhaseOut<T>{amp + row * (int64_t)bins, phase + row * (int64_t)bins,
                                           1.0 / sw, f.m, nf});
 }
 
@@ -431,17 +502,14 @@ __global__ __launch_bounds__(NTC) void hilbert_kernel(const DevFft *__restrict__
   const int n = f.n;
   const T *s = src + row * (int64_t)cols;
   T *buf = (T *)lds;
-  for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
+  const bool odd = fft_odd(f);
+  for (int j = threadIdx.x; j < n; j += NTC) put_sample(buf, j, j < live ? s[j] : (T)0, odd);
   __syncthreads();
   V2<T> *A = (V2<T> *)lds;
-  complex_forward<T, NTC, kFuseCorr>(f, A);
-  combine<T, NTC>(f, A, A, HilbertMul<T>{(T)(0.5 / (double)n), f.m});
-  __syncthreads();
-  complex_forward<T, NTC, kFuseCorr>(f, A);
-  const T *out = (const T *)A;
+  hilbert_transform<T, NTC>(f, A);
   T *d = dst + row * (int64_t)cols;
   for (int j = threadIdx.x; j < live; j += NTC) {
-    const T h = out[j ^ 1];
+    const T h = hilbert_at<T>(A, j, n);
     if (envelope) {
       const T x = s[j];
       d[j] = sqrt(x * x + h * h);
@@ -497,20 +565,17 @@ __global__ __launch_bounds__(NTC) void analytic_kernel(const DevFft *__restrict_
   const int n = f.n;
   const T *s = src + row * (int64_t)cols;
   T *buf = (T *)lds;
-  for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
+  const bool odd = fft_odd(f);
+  for (int j = threadIdx.x; j < n; j += NTC) put_sample(buf, j, j < live ? s[j] : (T)0, odd);
   __syncthreads();
   V2<T> *A = (V2<T> *)lds;
-  complex_forward<T, NTC, kFuseCorr>(f, A);
-  combine<T, NTC>(f, A, A, HilbertMul<T>{(T)(0.5 / (double)n), f.m});
-  __syncthreads();
-  complex_forward<T, NTC, kFuseCorr>(f, A);
-  const T *h = (const T *)A;  // H[x] at j is element j ^ 1
+  hilbert_transform<T, NTC>(f, A);
   double *e_row = env + row * (int64_t)cols;
   double *p_row = phase + row * (int64_t)cols;
   double *f_row = hz + row * (int64_t)cols;
   double mx = 0;
   for (int j = threadIdx.x; j < live; j += NTC) {
-    const double xj = (double)s[j], hj = (double)h[j ^ 1];
+    const double xj = (double)s[j], hj = (double)hilbert_at<T>(A, j, n);
     const double e = hypot(xj, hj);
     e_row[j] = e;
     if (e > mx) mx = e;  // a NaN never raises the peak
@@ -532,9 +597,9 @@ __global__ __launch_bounds__(NTC) void analytic_kernel(const DevFft *__restrict_
   for (int j = threadIdx.x; j < live; j += NTC) {
     double out = NAN;
     if (j > 0 && j < live - 1) {
-      const double x0 = (double)s[j - 1], h0 = (double)h[(j - 1) ^ 1];
-      const double x1 = (double)s[j], h1 = (double)h[j ^ 1];
-      const double x2 = (double)s[j + 1], h2 = (double)h[(j + 1) ^ 1];
+      const double x0 = (double)s[j - 1], h0 = (double)hilbert_at<T>(A, j - 1, n);
+      const double x1 = (double)s[j], h1 = (double)hilbert_at<T>(A, j, n);
+      const double x2 = (double)s[j + 1], h2 = (double)hilbert_at<T>(A, j + 1, n);
       const double e0 = hypot(x0, h0), e1 = hypot(x1, h1), e2 = hypot(x2, h2);
       if (!(e0 <= 0 || e1 <= 0 || e2 <= 0 || e0 < thr || e1 < thr || e2 < thr)) {
         const double p0 = e0 > 0 ? phase_deg(x0, h0) : (double)NAN;
@@ -582,6 +647,7 @@ __device__ __forceinline__ bool spectrogram_frame(const DevSpec &P, const T *__r
   if (b <= a) return false;
   gptr<T> wt = as_global<T>(P.taper);
   T *buf = (T *)lds;
+  const bool odd = fft_odd(f);
   double swl = 0;
   for (int j = threadIdx.x; j < nf; j += NTC) {
     T y = (T)0;
@@ -590,14 +656,13 @@ __device__ __forceinline__ bool spectrogram_frame(const DevSpec &P, const T *__r
       y = x[start + j] * w;
       swl += (double)w;
     }
-    buf[j] = y;
+    put_sample(buf, j, y, odd);
   }
   const double sw = block_sum<NTC>(swl, red.next());
   __syncthreads();
   V2<T> *A = (V2<T> *)lds;
-  complex_forward<T, NTC, kFuseCorr>(f, A);
   hook.inv_sw = 1.0 / sw;
-  combine<T, NTC>(f, A, A, hook);
+  real_forward<T, NTC>(f, A, hook);
   __syncthreads();  // the buffer is free for the next frame
   return true;
 }
@@ -611,7 +676,7 @@ __global__ __launch_bounds__(NTC) void spectrogram_kernel(const DevSpec *__restr
   extern __shared__ __align__(16) char lds[];
   __shared__ double red_lds[4 * (NTC / 64)];
   const DevSpec &P = *plan;
-  const int bins = P.fft.m + 1;
+  const int bins = P.fft.n / 2 + 1;
   const int64_t frame = blockIdx.x;
   double *out = amp + frame * (int64_t)bins;
   RedSlots red{red_lds, 2 * (NTC / 64), 0};
@@ -710,7 +775,7 @@ __global__ __launch_bounds__(NTC) void sgram_avg_kernel(const DevSpec *__restric
   __shared__ double red_lds[4 * (NTC / 64)];
   __shared__ int s_bad;
   const DevSpec &P = *plan;
-  const int bins = P.fft.m + 1;
+  const int bins = P.fft.n / 2 + 1;
   const int64_t frame = blockIdx.x;
   const int g = blockIdx.y, G = gridDim.y;
   double *acc = (double *)(lds + acc_off);

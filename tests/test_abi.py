@@ -157,7 +157,7 @@ def test_decode_checks_before_touching_memory():
 
 def test_spectrum_compile_checks_before_device_work():
     """pseg_spectrum_compile refuses bad plans (fft/spectrum.nim:77-101) before
-    it touches the device; an odd 2·3·5-smooth nfft is beyond the GPU path."""
+    it touches the device."""
     import ctypes as C
     L = _lib.lib()
     h = C.c_void_p()
@@ -165,14 +165,13 @@ def test_spectrum_compile_checks_before_device_work():
                  (0, 2.0, 10, 0.6, 0, 1), (0, 2.0, 10, -0.1, 0, 1), (0, 2.0, 100, 0.1, 64, 1),
                  (0, 2.0, 100, 0.1, 101, 1), (7, 2.0, 100, 0.1, 0, 1)]:
         assert L.pseg_spectrum_compile(*args, C.byref(h)) == _lib.E_INVALID, args
-    assert L.pseg_spectrum_compile(0, 2.0, 100, 0.1, 125, 1, C.byref(h)) == _lib.E_UNSUPPORTED
     assert L.pseg_spectrum_average(None, None, 1, 1, None, 0, None, 0, None, None, None) == _lib.E_INVALID
     assert L.pseg_spectrum_workspace_size(None, 1, None) == _lib.E_INVALID
 
 
 def test_spectrogram_compile_checks_before_device_work():
     """pseg_spectrogram_compile refuses bad plans (fft/spectrogram.nim:63-85)
-    before it touches the device; an odd nfft is beyond the GPU path."""
+    before it touches the device."""
     import ctypes as C
     L = _lib.lib()
     h = C.c_void_p()
@@ -180,5 +179,4 @@ def test_spectrogram_compile_checks_before_device_work():
                  (0, 64, 10, 1.0, 0, 0), (0, 64, 10, 1.0, 100, 32), (0, 64, 10, 1.0, 100, 77),
                  (3, 64, 10, 1.0, 100, 0)]:
         assert L.pseg_spectrogram_compile(*args, C.byref(h)) == _lib.E_INVALID, args
-    assert L.pseg_spectrogram_compile(0, 64, 10, 1.0, 100, 75, C.byref(h)) == _lib.E_UNSUPPORTED
     assert L.pseg_spectrogram_apply(None, None, 0, None, None) == _lib.E_INVALID

@@ -118,3 +118,34 @@ def test_analytic(gpu, oracle, dtype, cols):
             continue
         dh = np.abs(hz[r, :n] - wh)[both]
         assert dh.size == 0 or np.max(dh) <= (1e-3 * fs if dtype == np.float32 else 1e-8 * fs), r
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("cols,nfft", [(500, 1125), (1000, 2025), (61, 135)])
+def test_hilbert_odd_nfft(gpu, oracle, dtype, cols, nfft):
+    """An odd 2·3·5-smooth nfft (fft/hilbert.nim:44-53 takes any smooth
+    length >= 2·samples): the transform, the envelope and the analytic
+    signal against the oracle over each row's live samples."""
+    import torch
+    rows = 6
+    x, _ = synth.gather(rows, cols, 2.0, nfft, dtype=dtype)
+    rl = np.full(rows, cols, np.int32)
+    rl[2] = cols // 2
+    rl[4] = -1
+    hp = P.initHilbertPlan(dtype, cols, nfft)
+    assert hp.nfft == nfft
+    src = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    rlt = torch.from_numpy(rl).cuda()
+    h = hp.hilbertInto(src, row_len=rlt).cpu().numpy()
+    env, ph, hz = hp.analyticInto(src, 2.0, row_len=rlt)
+    env = env.cpu().numpy()
+    tol = 1e-5 if dtype == np.float32 else 1e-12
+    for r in range(rows):
+        n = int(rl[r])
+        if n <= 0:
+            assert np.array_equal(h[r], x[r])
+            continue
+        want = oracle.hilbert(x[r, :n].astype(np.float64), cols, nfft)
+        assert rel_l2(h[r, :n], want) <= tol, (r, rel_l2(h[r, :n], want))
+        we, _, _ = oracle.analytic(x[r, :n].astype(np.float64), cols, 2.0, nfft=nfft)
+        assert rel_l2(env[r, :n], we) <= tol

@@ -25,7 +25,9 @@ def run(sp, x, rl, first):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("win,nfft,taper,first", [(501, 0, 0.1, 250), (400, 400, 0.5, 0),
                                                   (1000, 1000, 0.0, 37), (2001, 0, 0.25, 100),
-                                                  (128, 0, 0.1, 1800), (4000, 0, 0.2, 0)])
+                                                  (128, 0, 0.1, 1800), (4000, 0, 0.2, 0),
+                                                  (100, 125, 0.1, 9), (1001, 1125, 0.2, 50),
+                                                  (243, 243, 0.0, 3)])
 def test_average_spectrum(gpu, oracle, dtype, win, nfft, taper, first):
     rows, cols = 40, 4001
     x, _ = synth.gather(rows, cols, 2.0, win + first, dtype=dtype)
@@ -67,13 +69,13 @@ def test_spectrum_refusals(gpu):
         P.initSpectrumPlan(np.float32, 2.0, 100, nfft=101)
     with pytest.raises(ValueError):
         P.initSpectrumPlan(np.float32, 2.0, 10, taperFraction=0.6)
-    with pytest.raises(P.UnsupportedOp):
-        P.initSpectrumPlan(np.float32, 2.0, 100, nfft=125)   # odd smooth length
+    assert P.initSpectrumPlan(np.float32, 2.0, 100, nfft=125).bins == 63   # odd smooth length
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("win,nfft,taper,first", [(501, 0, 0.1, 250), (400, 400, 0.2, 0),
-                                                  (1000, 1000, 0.0, 37)])
+                                                  (1000, 1000, 0.0, 37), (100, 135, 0.1, 7),
+                                                  (1001, 1125, 0.0, 20)])
 def test_amplitude_per_row(gpu, oracle, dtype, win, nfft, taper, first):
     """amplitudeInto row by row: the amplitude within the bar of the oracle,
     the phase where the amplitude is significant, NaN rows NaN, rows with
