@@ -282,6 +282,48 @@ pseg_status pseg_hilbert_analytic(const pseg_hilbert_plan *plan, const void *src
                                   double floor_db, double *env, double *phase, double *hz,
                                   void *stream);
 
+/* --------------------------------------------------------- first breaks -- */
+
+/* FirstBreakParams (qc/firstbreak.nim:146-170): method 0 pmStaLta,
+ * 1 pmEnergyRatio, 2 pmCoppens; NaN search bounds and guide velocity for
+ * none; high_cut_hz = Inf and low_cut_hz = 0 for no band-pass. */
+typedef struct pseg_fb_params {
+  int32_t method;
+  int32_t reserved;
+  double window_ms, lta_ms, stabilization;
+  double low_cut_hz, high_cut_hz, slope_db_per_octave;
+  double search_start_ms, search_end_ms;
+  double guide_velocity, guide_t0_ms, before_ms, after_ms;
+  double snr_window_ms, low_db, high_db;
+} pseg_fb_params;
+
+typedef struct pseg_firstbreak_plan pseg_firstbreak_plan;
+
+/* initFirstBreakPlan (qc/firstbreak.nim:195-251) for traces of up to
+ * `samples` samples at dt_ms; PSEG_E_INVALID with the reference's
+ * ValueError text for bad parameters. */
+pseg_status pseg_firstbreak_compile(int32_t dtype, const pseg_fb_params *params, double dt_ms,
+                                    int64_t samples, pseg_firstbreak_plan **out);
+pseg_status pseg_firstbreak_destroy(pseg_firstbreak_plan *plan);
+/* requiredMargin (qc/firstbreak.nim:269-281): the samples a pick reads
+ * before its search window (the whole trace when it reads all of it). */
+pseg_status pseg_firstbreak_required_margin(const pseg_firstbreak_plan *plan, int64_t *out);
+/* workspaceSize (qc/firstbreak.nim:291-300) for a [rows, cols] block: the
+ * bytes of ws pseg_firstbreak_pick takes. */
+pseg_status pseg_firstbreak_workspace_size(const pseg_firstbreak_plan *plan, int32_t rows,
+                                           int32_t cols, size_t *out);
+/* pickInto (qc/firstbreak.nim:492-541) of every row of src ([rows, cols],
+ * C-contiguous, cols <= samples; row_len from pseg_meta_flatten, -1 for
+ * rows not live), each to its live length: keys NULL or [rows, 2] float64
+ * (t0Ms, offset) on the device, first_sample the meta's.  ws: device,
+ * 256-byte aligned, at least pseg_firstbreak_workspace_size bytes.  Per row:
+ * time_ms (NaN: no pick), col (-1: no pick), snr_db (NaN: none), conf
+ * (0 .. 1), all device arrays of rows. */
+pseg_status pseg_firstbreak_pick(const pseg_firstbreak_plan *plan, const void *src, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *keys,
+                                 int64_t first_sample, void *ws, size_t ws_bytes, double *time_ms,
+                                 int64_t *col, double *snr_db, double *conf, void *stream);
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,

@@ -75,6 +75,21 @@ class OrCorrInfo(C.Structure):
                 ("scale", C.c_double)]
 
 
+class OrFbParams(C.Structure):
+    _fields_ = [("method", C.c_int32), ("reserved", C.c_int32)] + [
+        (k, C.c_double) for k in ("window_ms", "lta_ms", "stabilization", "low_cut_hz",
+                                  "high_cut_hz", "slope_db_per_octave", "search_start_ms",
+                                  "search_end_ms", "guide_velocity", "guide_t0_ms", "before_ms",
+                                  "after_ms", "snr_window_ms", "low_db", "high_db")]
+
+
+class OrFbPlan(C.Structure):
+    _fields_ = [("p", OrFbParams), ("dt_ms", C.c_double), ("samples", C.c_int64),
+                ("w", C.c_int64), ("lta", C.c_int64), ("snr", C.c_int64), ("before", C.c_int64),
+                ("after", C.c_int64), ("filtered", C.c_int32), ("whole", C.c_int32),
+                ("filter", OrIir)]
+
+
 class OracleError(ValueError):
     pass
 
@@ -105,6 +120,14 @@ def lib():
         L.or_is_fast_len.argtypes = [C.c_int64]
         L.or_mute_time_ms.argtypes = [C.POINTER(OrOp), C.c_double]
         L.or_mute_time_ms.restype = C.c_double
+        L.or_fb_init.argtypes = [C.POINTER(OrFbPlan), C.POINTER(OrFbParams), C.c_double,
+                                 C.c_int64, C.c_char_p, C.c_int]
+        L.or_fb_required_margin.argtypes = [C.POINTER(OrFbPlan)]
+        L.or_fb_required_margin.restype = C.c_int64
+        for sfx in ("f32", "f64"):
+            getattr(L, f"or_fb_pick_{sfx}").argtypes = [
+                C.POINTER(OrFbPlan), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_iir_init.argtypes = [C.POINTER(OrIir), C.POINTER(OrSpec), C.c_double, C.c_int64,
                                   C.c_int32, C.c_int32, C.c_char_p, C.c_int]
         L.or_iir_magnitude_at.argtypes = [C.POINTER(OrIir), C.c_double]
@@ -555,6 +578,53 @@ def correlate(src: np.ndarray, row_len, pilot, dt_ms, record_samples, out_sample
         record_samples, out_samples, mode, eps_percent, 1 if normalize else 0, src, rows, cols,
         np.ascontiguousarray(row_len, np.int32), dst, dst.shape[1], e, 512), e)
     return dst
+
+
+# ----------------------------------------------------------- first breaks --
+
+_FB_FIELDS = (("window_ms", "windowMs"), ("lta_ms", "ltaMs"), ("stabilization", "stabilization"),
+              ("low_cut_hz", "lowCutHz"), ("high_cut_hz", "highCutHz"),
+              ("slope_db_per_octave", "slopeDbPerOctave"), ("search_start_ms", "searchStartMs"),
+              ("search_end_ms", "searchEndMs"), ("guide_velocity", "guideVelocity"),
+              ("guide_t0_ms", "guideT0Ms"), ("before_ms", "beforeMs"), ("after_ms", "afterMs"),
+              ("snr_window_ms", "snrWindowMs"), ("low_db", "lowDb"), ("high_db", "highDb"))
+
+
+def fb_plan(params, dt_ms: float, samples: int) -> OrFbPlan:
+    """initFirstBreakPlan (qc/firstbreak.nim:195-251); params carries the
+    FirstBreakParams fields under the reference's names (pickMethod 0
+    pmStaLta, 1 pmEnergyRatio, 2 pmCoppens)."""
+    fp = OrFbParams()
+    fp.method = int(params.pickMethod)
+    for c_name, name in _FB_FIELDS:
+        setattr(fp, c_name, float(getattr(params, name)))
+    plan = OrFbPlan()
+    e = _err()
+    _check(lib().or_fb_init(C.byref(plan), C.byref(fp), dt_ms, samples, e, 512), e)
+    return plan
+
+
+def fb_required_margin(plan: OrFbPlan) -> int:
+    return int(lib().or_fb_required_margin(C.byref(plan)))
+
+
+def fb_pick(plan: OrFbPlan, x: np.ndarray, row_len, keys=None, first_sample: int = 0):
+    """pickInto (qc/firstbreak.nim:492-541) on the rows of x [rows, cols]:
+    row_len[r] < 0 a row not live; keys [rows, 2] (t0Ms, offset) or None.
+    Returns (time_ms, col, snr_db, confidence) arrays."""
+    x = np.ascontiguousarray(x)
+    rows, cols = x.shape
+    rl = np.ascontiguousarray(row_len, np.int32)
+    k = None if keys is None else np.ascontiguousarray(keys, np.float64)
+    t = np.empty(rows)
+    c = np.empty(rows, np.int64)
+    snr = np.empty(rows)
+    conf = np.empty(rows)
+    getattr(lib(), f"or_fb_pick_{_sfx(x.dtype)}")(
+        C.byref(plan), x.ctypes.data, rows, cols, rl.ctypes.data,
+        None if k is None else k.ctypes.data, int(first_sample), t.ctypes.data, c.ctypes.data,
+        snr.ctypes.data, conf.ctypes.data)
+    return t, c, snr, conf
 
 
 # ------------------------------------------------------------- sweeps --

@@ -1276,3 +1276,202 @@ int SFX(or_correlate)(const double *pilot, int64_t pilot_len, double pilot_dt_ms
   SFX(corr_free)(&c);
   return 0;
 }
+
+/* --------------------------------------------------------- first breaks -- */
+
+/* liveSpan (stack/stack.nim:292-300): the first and last non-zero sample;
+ * *a > *b when all are zero. */
+static void SFX(live_span)(const T *x, int64_t n, int64_t *a, int64_t *b) {
+  int64_t i = 0;
+  while (i < n && x[i] == 0) i++;
+  if (i == n) { *a = 0; *b = -1; return; }
+  int64_t j = n - 1;
+  while (x[j] == 0) j--;
+  *a = i;
+  *b = j;
+}
+
+/* meanPower (qc/snr.nim:151-161) over the columns c0 .. c1 (inclusive) cut
+ * to 0 .. n - 1; NaN when none. */
+static double SFX(mean_power)(const T *x, int64_t n, int64_t c0, int64_t c1) {
+  const int64_t a = c0 > 0 ? c0 : 0, b = c1 < n - 1 ? c1 : n - 1;
+  if (a > b) return NAN;
+  double s = 0;
+  for (int64_t j = a; j <= b; j++) {
+    const double v = (double)x[j];
+    s += v * v;
+  }
+  return s / (double)(b - a + 1);
+}
+
+/* stretchOf (qc/firstbreak.nim:394-430): what the pick of one row reads. */
+#ifndef OR_FB_STRETCH
+#define OR_FB_STRETCH
+typedef struct { int64_t first, len, lo, hi; double origin; } or_fb_stretch;
+#endif
+
+static or_fb_stretch SFX(fb_stretch)(const or_fb_plan *P, const T *src, int64_t n,
+                                     double offset, double t0_ms, double start_col) {
+  or_fb_stretch s = {0, 0, 0, 0, NAN};
+  if (n < 2) return s;
+  const or_fb_params *pr = &P->p;
+  const double dt = P->dt_ms;
+  double t_lo = isnan(pr->search_start_ms) ? -INFINITY : pr->search_start_ms;
+  double t_hi = isnan(pr->search_end_ms) ? INFINITY : pr->search_end_ms;
+  if (!isnan(pr->guide_velocity)) {
+    if (isnan(offset)) return s;
+    const double tl = pr->guide_t0_ms + 1000 * fabs(offset) / pr->guide_velocity;
+    t_lo = fmax(t_lo, tl - pr->before_ms);
+    t_hi = fmin(t_hi, tl + pr->after_ms);
+  }
+  const double t0c = t0_ms + start_col * dt;
+  const double origin = isinf(t0c) ? NAN : t0c;
+  if (isnan(origin) && (t_lo > -INFINITY || t_hi < INFINITY)) return s;
+  const double c_lo = isnan(origin) ? -INFINITY : (t_lo - origin) / dt;
+  const double c_hi = isnan(origin) ? INFINITY : (t_hi - origin) / dt;
+  int64_t la, lb;
+  SFX(live_span)(src, n, &la, &lb);
+  if (lb - la + 1 < 2) return s;
+  const int64_t lo_c = c_lo <= 0 ? 0 : c_lo >= (double)n ? n : (int64_t)ceil(c_lo);
+  const int64_t hi_c = c_hi < 0 ? 0 : c_hi >= (double)(n - 1) ? n : (int64_t)floor(c_hi) + 1;
+  const int64_t lo = la > lo_c ? la : lo_c;
+  const int64_t hi = lb + 1 < hi_c ? lb + 1 : hi_c;
+  if (lo >= hi) return s;
+  const int64_t a = P->whole ? la : (la > lo - P->before ? la : lo - P->before);
+  const int64_t b = P->whole ? lb + 1 : (lb + 1 < hi + P->after ? lb + 1 : hi + P->after);
+  s.first = a;
+  s.len = b - a;
+  s.lo = lo - a;
+  s.hi = hi - a;
+  s.origin = origin;
+  return s;
+}
+
+/* The pick on the m samples x of a stretch (pickStretch, qc/firstbreak.nim:
+ * 432-461, with prefixEnergy and argmaxAttribute :302-376).  p, a: m + 1
+ * doubles of scratch. */
+static void SFX(fb_pick_stretch)(const or_fb_plan *P, const T *x, or_fb_stretch st, double *p,
+                                 double *a, double *time_ms, int64_t *col, double *snr_db,
+                                 double *conf) {
+  *time_ms = NAN;
+  *col = -1;
+  *snr_db = NAN;
+  *conf = 0;
+  const int64_t m = st.len;
+  double s = 0;
+  p[0] = 0;
+  for (int64_t i = 0; i < m; i++) {
+    const double v = (double)x[i];
+    s += v * v;
+    p[i + 1] = s;
+  }
+  const double total = p[m];
+  if (!(total > 0) || total == INFINITY) return;
+  const double mean = total / (double)m;
+  const double eps = 1e-6 * mean;
+  const double beta = P->p.stabilization * mean * (double)P->w;
+  const int64_t w = P->w, lta = P->lta, lo = st.lo, hi = st.hi, n = m;
+  /* argmaxAttribute */
+  int64_t best_i = -1;
+  double best = 0.0;
+  const int64_t sw = w / 2 > 1 ? w / 2 : 1;
+  const int64_t last_read = n - 1 < hi - 2 + sw ? n - 1 : hi - 2 + sw;
+  int64_t i0 = 0;
+  if (P->p.method == 0) {
+    i0 = lo > w + sw ? lo : w + sw;
+    for (int64_t i = i0 - 1; i <= last_read; i++) {
+      const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+      const int64_t m0 = s0 - lta > 0 ? s0 - lta : 0;
+      a[i] = s0 - m0 < sw ? 0.0
+                          : ((p[i + 1] - p[s0]) / (double)(i + 1 - s0)) /
+                                ((p[s0] - p[m0]) / (double)(s0 - m0) + eps);
+    }
+  } else if (P->p.method == 1) {
+    const int64_t q = w / 4 > 1 ? w / 4 : 1;
+    for (int64_t i = lo; i < hi; i++) {
+      int64_t k = i < n - i ? i : n - i;
+      k = w < k ? w : k;
+      if (k >= sw) {
+        const int64_t e = n < i + q ? n : i + q;
+        const double r = (p[i + k] - p[i]) / ((p[i] - p[i - k]) + eps * (double)k) * sqrt(p[e] - p[i]);
+        if (r > best) {
+          best = r;
+          best_i = i;
+        }
+      }
+    }
+  } else {
+    i0 = lo > w ? lo : w;
+    for (int64_t i = i0 - 1; i <= last_read; i++) {
+      const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+      a[i] = (p[i + 1] - p[s0]) / (p[i + 1] + beta + eps);
+    }
+  }
+  if (P->p.method != 1) {
+    /* risePick: the largest mean rise of a[] over sw steps */
+    for (int64_t i = i0; i < hi; i++) {
+      const int64_t j = n - 1 < i - 1 + sw ? n - 1 : i - 1 + sw;
+      if (j > i - 1) {
+        const double r = (a[j] - a[i - 1]) / (double)(j - i + 1);
+        if (r > best) {
+          best = r;
+          best_i = i;
+        }
+      }
+    }
+  }
+  if (best_i < 0) return;
+  const int64_t c = best_i;
+  const int64_t gap_end = c - w / 2 > 0 ? c - w / 2 : 0;
+  *col = st.first + c;
+  *time_ms = st.origin + (double)(st.first + c) * P->dt_ms;
+  const int64_t nz0 = gap_end - P->snr > 0 ? gap_end - P->snr : 0;
+  const double noise = SFX(mean_power)(x, m, nz0, gap_end - 1);
+  if (noise > 0) {
+    const int64_t sig1 = m < c + P->snr ? m : c + P->snr;
+    const double sg = SFX(mean_power)(x, m, c, sig1 - 1);
+    /* powerSnrDb (qc/snr.nim:163-177), seRatio, floor -Inf */
+    const double ss = sg + eps, nn = noise + eps;
+    double snr = NAN;
+    if (!isnan(ss) && nn > 0) {
+      const double r = ss / nn;
+      snr = isnan(r) ? NAN : (r > 0 ? 10 * log10(r) : -INFINITY);
+    }
+    if (!isnan(snr)) {
+      *snr_db = snr;
+      double cf = (snr - P->p.low_db) / (P->p.high_db - P->p.low_db);
+      *conf = cf < 0 ? 0.0 : (cf > 1 ? 1.0 : cf);
+    }
+  }
+}
+
+void SFX(or_fb_pick)(const or_fb_plan *P, const T *src, int32_t rows, int32_t cols,
+                     const int32_t *row_len, const double *keys, int64_t first_sample,
+                     double *time_ms, int64_t *col, double *snr_db, double *conf) {
+  const int64_t cap = P->samples + 1;
+  double *p = (double *)malloc(sizeof(double) * (size_t)cap);
+  double *a = (double *)malloc(sizeof(double) * (size_t)cap);
+  T *copy = (T *)malloc(sizeof(T) * (size_t)(cols > 0 ? cols : 1));
+  for (int32_t r = 0; r < rows; r++) {
+    time_ms[r] = NAN;
+    col[r] = -1;
+    snr_db[r] = NAN;
+    conf[r] = 0;
+    if (row_len[r] < 0) continue;
+    const int64_t n = row_len[r] < cols ? row_len[r] : cols;
+    const T *x = src + (int64_t)r * cols;
+    const double t0 = keys ? keys[2 * r] : 0.0, off = keys ? keys[2 * r + 1] : NAN;
+    or_fb_stretch st = SFX(fb_stretch)(P, x, n, off, t0, (double)first_sample);
+    if (st.len == 0) continue;
+    const T *y = x + st.first;
+    if (P->filtered) {
+      for (int64_t j = 0; j < st.len; j++) copy[j] = y[j];
+      SFX(or_iir_apply)(&P->filter, copy, st.len);
+      y = copy;
+    }
+    SFX(fb_pick_stretch)(P, y, st, p, a, time_ms + r, col + r, snr_db + r, conf + r);
+  }
+  free(p);
+  free(a);
+  free(copy);
+}

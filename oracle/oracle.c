@@ -1010,3 +1010,71 @@ int or_pool_apply_f32(or_pool *pool, const or_op *ops, int32_t n_ops, double dt_
   free(P);
   return 0;
 }
+
+/* ---------------------------------------------------------- first breaks -- */
+
+static int or_fb_positive(double x) { return x > 0 && x < INFINITY; }
+
+/* samplesIn (qc/firstbreak.nim:189-192): a window in samples, 1 .. samples + 1. */
+static int64_t or_fb_samples_in(double ms, double dt, int64_t samples) {
+  double v = round(ms / dt);
+  if (v < 1) v = 1;
+  if (v > (double)(samples + 1)) v = (double)(samples + 1);
+  return (int64_t)v;
+}
+
+int or_fb_init(or_fb_plan *P, const or_fb_params *p, double dt, int64_t samples, char *err,
+               int errlen) {
+  memset(P, 0, sizeof(*P));
+  if (!or_fb_positive(dt)) return or_seterr(err, errlen, "dtMs must be finite and positive");
+  if (samples < 1) return or_seterr(err, errlen, "a first-break plan needs samples >= 1");
+  if (!(or_fb_positive(p->window_ms) && or_fb_positive(p->lta_ms) && or_fb_positive(p->snr_window_ms)))
+    return or_seterr(err, errlen, "first-break windows must be finite and positive");
+  if (p->lta_ms < p->window_ms) return or_seterr(err, errlen, "ltaMs must be >= windowMs");
+  if (!(p->stabilization >= 0 && p->stabilization < INFINITY))
+    return or_seterr(err, errlen, "stabilization must be finite and >= 0");
+  if (!(p->low_db < p->high_db) || p->low_db == -INFINITY || p->high_db == INFINITY)
+    return or_seterr(err, errlen, "need finite lowDb < highDb");
+  if (p->search_end_ms < p->search_start_ms) return or_seterr(err, errlen, "searchEndMs < searchStartMs");
+  if (p->search_start_ms == INFINITY || p->search_end_ms == -INFINITY)
+    return or_seterr(err, errlen, "the search window is empty");
+  if (p->method < 0 || p->method > 2) return or_seterr(err, errlen, "unknown pick method");
+  if (!isnan(p->guide_velocity)) {
+    if (!or_fb_positive(p->guide_velocity))
+      return or_seterr(err, errlen, "guideVelocity must be finite and positive (or NaN)");
+    if (!(p->before_ms >= 0 && p->after_ms >= 0 && p->before_ms < INFINITY && p->after_ms < INFINITY))
+      return or_seterr(err, errlen, "beforeMs and afterMs must be finite and >= 0");
+    if (!isfinite(p->guide_t0_ms)) return or_seterr(err, errlen, "guideT0Ms must be finite");
+  }
+  P->p = *p;
+  P->dt_ms = dt;
+  P->samples = samples;
+  P->w = or_fb_samples_in(p->window_ms, dt, samples);
+  P->lta = or_fb_samples_in(p->lta_ms, dt, samples);
+  P->snr = or_fb_samples_in(p->snr_window_ms, dt, samples);
+  if (!(p->low_cut_hz >= 0 && p->high_cut_hz > p->low_cut_hz))
+    return or_seterr(err, errlen, "need 0 <= lowCutHz < highCutHz");
+  P->filtered = p->low_cut_hz > 0 || p->high_cut_hz < INFINITY;
+  P->whole = p->method == 2 ||
+             (isnan(p->guide_velocity) && isnan(p->search_start_ms) && isnan(p->search_end_ms));
+  if (P->filtered) {
+    or_spec spec;
+    memset(&spec, 0, sizeof spec);
+    spec.kind = 1;
+    spec.f3 = spec.f4 = INFINITY;
+    spec.low_hz = p->low_cut_hz;
+    spec.low_slope = p->slope_db_per_octave;
+    spec.high_hz = p->high_cut_hz;
+    spec.high_slope = p->slope_db_per_octave;
+    if (or_iir_init(&P->filter, &spec, dt, samples, 1, P->whole ? 2 : 0, err, errlen)) return -1;
+  }
+  const int64_t w = P->w, snr = P->snr;
+  const int64_t need = p->method == 0 ? w + P->lta : w;
+  const int64_t b0 = snr + w / 2 > need ? snr + w / 2 : need;
+  P->before = b0 + (P->filtered ? P->filter.pad : 0);
+  P->after = w > snr ? w : snr;
+  return 0;
+}
+
+int64_t or_fb_required_margin(const or_fb_plan *P) { return P->whole ? P->samples : P->before; }
+

@@ -271,6 +271,42 @@ int or_correlate_f64(const double *pilot, int64_t pilot_len, double pilot_dt_ms,
                      const int32_t *row_len, double *dst, int32_t dst_cols,
                      char *err, int errlen);
 
+/* ---------------------------------------------------------- first breaks -- */
+/* FirstBreakParams (qc/firstbreak.nim:146-170).  method: 0 pmStaLta,
+ * 1 pmEnergyRatio, 2 pmCoppens. */
+typedef struct or_fb_params {
+  int32_t method;
+  int32_t reserved;
+  double window_ms, lta_ms, stabilization;
+  double low_cut_hz, high_cut_hz, slope_db_per_octave;
+  double search_start_ms, search_end_ms;
+  double guide_velocity, guide_t0_ms, before_ms, after_ms;
+  double snr_window_ms, low_db, high_db;
+} or_fb_params;
+/* FirstBreakPlan (qc/firstbreak.nim:172-183). */
+typedef struct or_fb_plan {
+  or_fb_params p;
+  double dt_ms;
+  int64_t samples, w, lta, snr, before, after;
+  int32_t filtered, whole;
+  or_iir filter;
+} or_fb_plan;
+/* initFirstBreakPlan (qc/firstbreak.nim:195-251); 0 ok, -1 invalid. */
+int or_fb_init(or_fb_plan *plan, const or_fb_params *p, double dt_ms, int64_t samples, char *err,
+               int errlen);
+/* requiredMargin (qc/firstbreak.nim:265-275). */
+int64_t or_fb_required_margin(const or_fb_plan *plan);
+/* pickInto (qc/firstbreak.nim:492-541) over rows [rows, cols]: row_len[r] the
+ * live samples (< 0: not live, no pick); keys (t0Ms, offset) per row, or
+ * NULL (0, NaN); first_sample the block's first column.  Per row: time_ms,
+ * col (-1: no pick), snr_db, confidence. */
+void or_fb_pick_f32(const or_fb_plan *plan, const float *src, int32_t rows, int32_t cols,
+                    const int32_t *row_len, const double *keys, int64_t first_sample,
+                    double *time_ms, int64_t *col, double *snr_db, double *confidence);
+void or_fb_pick_f64(const or_fb_plan *plan, const double *src, int32_t rows, int32_t cols,
+                    const int32_t *row_len, const double *keys, int64_t first_sample,
+                    double *time_ms, int64_t *col, double *snr_db, double *confidence);
+
 /* ------------------------------------------------------ sweep / pilot -- */
 int64_t or_samples_in_length(double length_ms, double dt_ms);
 /* linearSweep + sweepInto + applyTaper(tpCos2) (vibroseis/sweep.nim:185-342,

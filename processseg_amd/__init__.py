@@ -27,6 +27,9 @@ from .batch import BatchItem, PipelineBatch  # noqa: F401
 from .spectrum import SpectrumPlan, initSpectrumPlan  # noqa: F401
 from .hilbert import HilbertPlan, initHilbertPlan  # noqa: F401
 from .spectrogram import SpectrogramPlan, initSpectrogramPlan  # noqa: F401
+from .firstbreak import (  # noqa: F401
+    FirstBreakParams, FirstBreakPicks, FirstBreakPlan, initFirstBreakPlan, pmCoppens,
+    pmEnergyRatio, pmStaLta)
 from .decode import (  # noqa: F401
     decodeSamples, fmtIbm32, fmtIeee32, fmtInt8, fmtInt16, fmtInt32)
 from .serve import ServeResult, Server  # noqa: F401

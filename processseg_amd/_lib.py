@@ -140,6 +140,8 @@ SYMBOLS = [
     "pseg_spectrum_amplitude", "pseg_spectrogram_average", "pseg_spectrogram_average_workspace_size", "pseg_hilbert_analytic", "pseg_hilbert_compile", "pseg_hilbert_destroy",
     "pseg_hilbert_nfft", "pseg_hilbert_apply", "pseg_spectrogram_compile",
     "pseg_spectrogram_destroy", "pseg_spectrogram_info_get", "pseg_spectrogram_apply",
+    "pseg_firstbreak_compile", "pseg_firstbreak_destroy", "pseg_firstbreak_required_margin",
+    "pseg_firstbreak_workspace_size", "pseg_firstbreak_pick",
 ]
 
 
@@ -190,6 +192,12 @@ def lib():
     L.pseg_spectrogram_destroy.argtypes = [vp]
     L.pseg_spectrogram_info_get.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.pseg_spectrogram_apply.argtypes = [vp, vp, i32, vp, vp]
+    L.pseg_firstbreak_compile.argtypes = [i32, vp, f64, i64, C.POINTER(vp)]
+    L.pseg_firstbreak_destroy.argtypes = [vp]
+    L.pseg_firstbreak_required_margin.argtypes = [vp, C.POINTER(i64)]
+    L.pseg_firstbreak_workspace_size.argtypes = [vp, i32, i32, C.POINTER(C.c_size_t)]
+    L.pseg_firstbreak_pick.argtypes = [vp, vp, i32, i32, vp, vp, i64, vp, C.c_size_t, vp, vp, vp,
+                                       vp, vp]
     L.pseg_plan_apply_raw.argtypes = [vp, i32, vp, C.c_int64, C.c_int64, vp, i32, i32, vp, vp,
                                       C.c_int64, vp]
     L.pseg_corr_compile.argtypes = [i32, vp, i32, f64, f64, f64, i32, i32, i32, f64, i32,

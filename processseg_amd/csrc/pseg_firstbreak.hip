@@ -1,0 +1,337 @@
+// First-break picking (qc/firstbreak.nim) on the GPU, one thread per row.
+// A pick is an integer column chosen by an argmax over float64 attributes;
+// to give the same column as the serial reference, every row runs the
+// reference's serial arithmetic in the same order: the minimum-phase
+// band-pass's recursion (filter/iir.nim, with its trend, pads and flush),
+// the prefix energies, the picker's attribute and its argmax, the
+// confidence.  No multiply-add is contracted in this file, so each operation
+// rounds as the serial code's does.  The scratch arrays are laid out
+// [index][rows]: the lanes of a wave step through neighbouring rows'
+// values together, so their loads and stores coalesce.
+#pragma clang fp contract(off)
+
+#include "pseg_internal.hpp"
+
+#include <math.h>
+
+namespace pseg {
+
+namespace {
+
+struct Col {
+  double *base;
+  int64_t rows, r;
+  __device__ __forceinline__ double &operator[](int64_t i) const { return base[i * rows + r]; }
+};
+template <typename T> struct ColT {
+  T *base;
+  int64_t rows, r;
+  __device__ __forceinline__ T &operator[](int64_t i) const { return base[i * rows + r]; }
+};
+
+// sum64 / sumIndexed64 (core/sums.nim:12-40): eight accumulators, then the
+// rest, in float64.
+template <typename X>
+__device__ double sum64(const X &x, int64_t n) {
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, r = 0;
+  const int64_t n8 = n / 8 * 8;
+  for (int64_t i = 0; i < n8; i += 8)
+    for (int j = 0; j < 8; j++) acc[j] += (double)x[i + j];
+  for (int j = 0; j < 8; j++) r += acc[j];
+  for (int64_t k = n8; k < n; k++) r += (double)x[k];
+  return r;
+}
+template <typename X>
+__device__ double sum_indexed64(const X &x, int64_t n) {
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, idx[8], r = 0;
+  for (int j = 0; j < 8; j++) idx[j] = (double)j;
+  const int64_t n8 = n / 8 * 8;
+  for (int64_t i = 0; i < n8; i += 8)
+    for (int j = 0; j < 8; j++) {
+      acc[j] += idx[j] * (double)x[i + j];
+      idx[j] += 8;
+    }
+  for (int j = 0; j < 8; j++) r += acc[j];
+  for (int64_t k = n8; k < n; k++) r += (double)k * (double)x[k];
+  return r;
+}
+
+// applyFilter on the stretch copy x[0 ..< n] (filter/iir.nim:363-433,
+// minimum phase: one causal pass per biquad): trend removed, the odd
+// reflection before (a steady-state start), the passes in float64 with the
+// states flushed below 1e-250 every 256 samples, the trend's steady-state
+// response added back.
+template <typename T>
+__device__ void iir_row(const DevFirstBreak &P, const ColT<T> &x, int64_t n, const Col &buf) {
+  if (n <= 0) return;
+  double a = 0, b = 0;
+  if (P.trend == 1) {
+    a = sum64(x, n) / (double)n;
+  } else if (P.trend == 2) {
+    const double sx = sum64(x, n), nf = (double)n, mean = sx / nf;
+    if (n == 1) {
+      a = mean;
+    } else {
+      const double c = (nf - 1) / 2;
+      const double slope = (sum_indexed64(x, n) - c * sx) / (nf * (nf * nf - 1) / 12);
+      a = mean - slope * c;
+      b = slope;
+    }
+  }
+  const int64_t L = P.lead;
+  for (int64_t i = 0; i < n; i++) buf[L + i] = (double)x[i] - (a + b * (double)i);
+  if (P.reflect) {
+    const double d0 = buf[L];
+    for (int64_t k = 1; k <= L; k++) buf[L - k] = 2 * d0 - buf[L + (k < n - 1 ? k : n - 1)];
+  }
+  const int64_t m = L + n;
+  for (int s = 0; s < P.n_sections; s++) {
+    const double b0 = P.sec[s][0], b1 = P.sec[s][1], b2 = P.sec[s][2];
+    const double a1 = P.sec[s][3], a2 = P.sec[s][4], g = P.sec[s][5];
+    double s1 = 0, s2 = 0;
+    if (P.reflect && m > 0) {
+      const double u = buf[0];
+      const double y = g * u;
+      s2 = b2 * u - a2 * y;
+      s1 = y - b0 * u;
+    }
+    int64_t done = 0;
+    while (done < m) {
+      const int64_t run = m - done < 256 ? m - done : 256;
+      for (int64_t k = 0; k < run; k++) {
+        const int64_t j = done + k;
+        const double xv = buf[j];
+        const double y = b0 * xv + s1;
+        s1 = b1 * xv - a1 * y + s2;
+        s2 = b2 * xv - a2 * y;
+        buf[j] = y;
+      }
+      s1 = fabs(s1) < 1e-250 ? 0.0 : s1;
+      s2 = fabs(s2) < 1e-250 ? 0.0 : s2;
+      done += run;
+    }
+  }
+  const double ga = a * P.dc_gain - b * P.dc_slope, gb = b * P.dc_gain;
+  for (int64_t i = 0; i < n; i++) x[i] = (T)(buf[L + i] + (ga + gb * (double)i));
+}
+
+// meanPower (qc/snr.nim:151-161) over the columns c0 .. c1 cut to 0 .. n - 1.
+template <typename X>
+__device__ double mean_power(const X &x, int64_t n, int64_t c0, int64_t c1) {
+  const int64_t a = c0 > 0 ? c0 : 0, b = c1 < n - 1 ? c1 : n - 1;
+  if (a > b) return NAN;
+  double s = 0;
+  for (int64_t j = a; j <= b; j++) {
+    const double v = (double)x[j];
+    s += v * v;
+  }
+  return s / (double)(b - a + 1);
+}
+
+// A strided view of a row of src (the lanes' rows are cols apart).
+template <typename T> struct RowT {
+  const T *p;
+  __device__ __forceinline__ T operator[](int64_t i) const { return p[i]; }
+};
+
+template <typename T, typename X>
+__device__ void pick_stretch(const DevFirstBreak &P, const X &x, int64_t first, int64_t m,
+                             int64_t lo, int64_t hi, double origin, const Col &p, const Col &a,
+                             double &time_ms, int64_t &col, double &snr_db, double &conf) {
+  double s = 0;
+  p[0] = 0;
+  for (int64_t i = 0; i < m; i++) {
+    const double v = (double)x[i];
+    s += v * v;
+    p[i + 1] = s;
+  }
+  const double total = p[m];
+  if (!(total > 0) || total == INFINITY) return;
+  const double mean = total / (double)m;
+  const double eps = 1e-6 * mean;
+  const double beta = P.stabilization * mean * (double)P.w;
+  const int64_t w = P.w, lta = P.lta, n = m;
+  int64_t best_i = -1;
+  double best = 0.0;
+  const int64_t sw = w / 2 > 1 ? w / 2 : 1;
+  const int64_t last_read = n - 1 < hi - 2 + sw ? n - 1 : hi - 2 + sw;
+  int64_t i0 = 0;
+  if (P.method == 0) {
+    i0 = lo > w + sw ? lo : w + sw;
+    for (int64_t i = i0 - 1; i <= last_read; i++) {
+      const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+      const int64_t m0 = s0 - lta > 0 ? s0 - lta : 0;
+      a[i] = s0 - m0 < sw ? 0.0
+                          : ((p[i + 1] - p[s0]) / (double)(i + 1 - s0)) /
+                                ((p[s0] - p[m0]) / (double)(s0 - m0) + eps);
+    }
+  } else if (P.method == 1) {
+    const int64_t q = w / 4 > 1 ? w / 4 : 1;
+    for (int64_t i = lo; i < hi; i++) {
+      int64_t k = i < n - i ? i : n - i;
+      k = w < k ? w : k;
+      if (k >= sw) {
+        const int64_t e = n < i + q ? n : i + q;
+        const double r = (p[i + k] - p[i]) / ((p[i] - p[i - k]) + eps * (double)k) *
+                         __dsqrt_rn(p[e] - p[i]);
+        if (r > best) {
+          best = r;
+          best_i = i;
+        }
+      }
+    }
+  } else {
+    i0 = lo > w ? lo : w;
+    for (int64_t i = i0 - 1; i <= last_read; i++) {
+      const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+      a[i] = (p[i + 1] - p[s0]) / (p[i + 1] + beta + eps);
+    }
+  }
+  if (P.method != 1) {
+    for (int64_t i = i0; i < hi; i++) {
+      const int64_t j = n - 1 < i - 1 + sw ? n - 1 : i - 1 + sw;
+      if (j > i - 1) {
+        const double r = (a[j] - a[i - 1]) / (double)(j - i + 1);
+        if (r > best) {
+          best = r;
+          best_i = i;
+        }
+      }
+    }
+  }
+  if (best_i < 0) return;
+  const int64_t c = best_i;
+  const int64_t gap_end = c - w / 2 > 0 ? c - w / 2 : 0;
+  col = first + c;
+  time_ms = origin + (double)(first + c) * P.dt;
+  const int64_t nz0 = gap_end - P.snr > 0 ? gap_end - P.snr : 0;
+  const double noise = mean_power(x, m, nz0, gap_end - 1);
+  if (noise > 0) {
+    const int64_t sig1 = m < c + P.snr ? m : c + P.snr;
+    const double sg = mean_power(x, m, c, sig1 - 1);
+    const double ss = sg + eps, nn = noise + eps;
+    double snr = NAN;
+    if (!isnan(ss) && nn > 0) {
+      const double r = ss / nn;
+      snr = isnan(r) ? (double)NAN : (r > 0 ? 10 * log10(r) : -INFINITY);
+    }
+    if (!isnan(snr)) {
+      snr_db = snr;
+      const double cf = (snr - P.low_db) / (P.high_db - P.low_db);
+      conf = cf < 0 ? 0.0 : (cf > 1 ? 1.0 : cf);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void firstbreak_kernel(const DevFirstBreak *__restrict__ plan,
+                                                        const T *__restrict__ src, int32_t rows,
+                                                        int32_t cols,
+                                                        const int32_t *__restrict__ row_len,
+                                                        const double *__restrict__ keys,
+                                                        double first_sample, double *p_, double *a_,
+                                                        double *buf_, T *copy_,
+                                                        double *__restrict__ time_ms,
+                                                        int64_t *__restrict__ col,
+                                                        double *__restrict__ snr_db,
+                                                        double *__restrict__ conf) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const DevFirstBreak &P = *plan;
+  double t_out = NAN, s_out = NAN, c_out = 0;
+  int64_t col_out = -1;
+  const int live = row_len[r];
+  if (live >= 0) {
+    const int64_t n = live < cols ? live : cols;
+    const RowT<T> x{src + r * (int64_t)cols};
+    const double t0 = keys ? keys[2 * r] : 0.0, offset = keys ? keys[2 * r + 1] : (double)NAN;
+    // stretchOf (qc/firstbreak.nim:394-430)
+    int64_t first = 0, len = 0, lo = 0, hi = 0;
+    double origin = NAN;
+    bool ok = n >= 2;
+    double t_lo = isnan(P.search_start) ? -INFINITY : P.search_start;
+    double t_hi = isnan(P.search_end) ? INFINITY : P.search_end;
+    if (ok && !isnan(P.guide_v)) {
+      if (isnan(offset)) {
+        ok = false;
+      } else {
+        const double tl = P.guide_t0 + 1000 * fabs(offset) / P.guide_v;
+        t_lo = fmax(t_lo, tl - P.before_ms);
+        t_hi = fmin(t_hi, tl + P.after_ms);
+      }
+    }
+    if (ok) {
+      const double t0c = t0 + first_sample * P.dt;
+      origin = isinf(t0c) ? (double)NAN : t0c;
+      if (isnan(origin) && (t_lo > -INFINITY || t_hi < INFINITY)) ok = false;
+    }
+    int64_t la = 0, lb = -1;
+    if (ok) {
+      int64_t i = 0;
+      while (i < n && x[i] == (T)0) i++;
+      if (i < n) {
+        int64_t j = n - 1;
+        while (x[j] == (T)0) j--;
+        la = i;
+        lb = j;
+      }
+      if (lb - la + 1 < 2) ok = false;
+    }
+    if (ok) {
+      const double c_lo = isnan(origin) ? -INFINITY : (t_lo - origin) / P.dt;
+      const double c_hi = isnan(origin) ? INFINITY : (t_hi - origin) / P.dt;
+      const int64_t lo_c = c_lo <= 0 ? 0 : c_lo >= (double)n ? n : (int64_t)ceil(c_lo);
+      const int64_t hi_c = c_hi < 0 ? 0 : c_hi >= (double)(n - 1) ? n : (int64_t)floor(c_hi) + 1;
+      const int64_t wlo = la > lo_c ? la : lo_c;
+      const int64_t whi = lb + 1 < hi_c ? lb + 1 : hi_c;
+      if (wlo >= whi) {
+        ok = false;
+      } else {
+        const int64_t a = P.whole ? la : (la > wlo - P.before ? la : wlo - P.before);
+        const int64_t b = P.whole ? lb + 1 : (lb + 1 < whi + P.after ? lb + 1 : whi + P.after);
+        first = a;
+        len = b - a;
+        lo = wlo - a;
+        hi = whi - a;
+      }
+    }
+    if (ok && len > 0) {
+      const Col p{p_, rows, r}, aa{a_, rows, r};
+      if (P.filtered) {
+        const ColT<T> c{copy_, rows, r};
+        for (int64_t j = 0; j < len; j++) c[j] = x[first + j];
+        iir_row<T>(P, c, len, Col{buf_, rows, r});
+        pick_stretch<T>(P, c, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
+      } else {
+        const RowT<T> y{src + r * (int64_t)cols + first};
+        pick_stretch<T>(P, y, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
+      }
+    }
+  }
+  time_ms[r] = t_out;
+  col[r] = col_out;
+  snr_db[r] = s_out;
+  conf[r] = c_out;
+}
+
+}  // namespace
+
+hipError_t launch_firstbreak(int dtype, const DevFirstBreak *plan, const void *src, int32_t rows,
+                             int32_t cols, const int32_t *row_len, const double *keys,
+                             int64_t first_sample, double *p, double *a, double *buf, void *copy,
+                             double *time_ms, int64_t *col, double *snr_db, double *conf,
+                             hipStream_t stream) {
+  if (rows == 0) return hipSuccess;
+  const int grid = (rows + 63) / 64;
+  if (dtype == 0)
+    hipLaunchKernelGGL(firstbreak_kernel<float>, dim3(grid), dim3(64), 0, stream, plan,
+                       (const float *)src, rows, cols, row_len, keys, (double)first_sample, p, a,
+                       buf, (float *)copy, time_ms, col, snr_db, conf);
+  else
+    hipLaunchKernelGGL(firstbreak_kernel<double>, dim3(grid), dim3(64), 0, stream, plan,
+                       (const double *)src, rows, cols, row_len, keys, (double)first_sample, p, a,
+                       buf, (double *)copy, time_ms, col, snr_db, conf);
+  return hipGetLastError();
+}
+
+}  // namespace pseg

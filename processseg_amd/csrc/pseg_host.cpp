@@ -2287,3 +2287,203 @@ pseg_status pseg_linear_sweep(double f1, double f2, double length_ms, double dt_
 }
 
 }  // extern "C"
+
+// ============================================================ first breaks
+
+struct pseg_firstbreak_plan {
+  int dtype;
+  int64_t samples;
+  DevFirstBreak host;
+  DevFirstBreak *dev;
+};
+
+namespace {
+
+bool fb_positive(double x) { return x > 0 && x < kInf; }
+
+// samplesIn (qc/firstbreak.nim:189-192): a window in samples, rounded as a
+// float first and held to 1 .. samples + 1.
+int64_t fb_samples_in(double ms, double dt, int64_t samples) {
+  return (int64_t)std::min(std::max(std::round(ms / dt), 1.0), (double)(samples + 1));
+}
+
+// initFirstBreakPlan (qc/firstbreak.nim:195-251): the checks and messages,
+// the windows in samples, the band-pass (minimum phase, trend removed only
+// when the whole trace is read), and what a pick reads about its window.
+pseg_status fb_compile(int dtype, const pseg_fb_params &p, double dt, int64_t samples,
+                       pseg_firstbreak_plan **out) {
+  if (!fb_positive(dt)) return fail(PSEG_E_INVALID, "dtMs must be finite and positive");
+  if (samples < 1) return fail(PSEG_E_INVALID, "a first-break plan needs samples >= 1");
+  if (samples > INT32_MAX) return fail(PSEG_E_INVALID, "samples must fit an int32");
+  if (!(fb_positive(p.window_ms) && fb_positive(p.lta_ms) && fb_positive(p.snr_window_ms)))
+    return fail(PSEG_E_INVALID, "first-break windows must be finite and positive");
+  if (p.lta_ms < p.window_ms) return fail(PSEG_E_INVALID, "ltaMs must be >= windowMs");
+  if (!(p.stabilization >= 0 && p.stabilization < kInf))
+    return fail(PSEG_E_INVALID, "stabilization must be finite and >= 0");
+  if (!(p.low_db < p.high_db) || p.low_db == -kInf || p.high_db == kInf)
+    return fail(PSEG_E_INVALID, "need finite lowDb < highDb");
+  if (p.search_end_ms < p.search_start_ms) return fail(PSEG_E_INVALID, "searchEndMs < searchStartMs");
+  if (p.search_start_ms == kInf || p.search_end_ms == -kInf)
+    return fail(PSEG_E_INVALID, "the search window is empty");
+  if (p.method < 0 || p.method > 2) return fail(PSEG_E_INVALID, "unknown pick method %d", p.method);
+  if (!std::isnan(p.guide_velocity)) {
+    if (!fb_positive(p.guide_velocity))
+      return fail(PSEG_E_INVALID, "guideVelocity must be finite and positive (or NaN)");
+    if (!(p.before_ms >= 0 && p.after_ms >= 0 && p.before_ms < kInf && p.after_ms < kInf))
+      return fail(PSEG_E_INVALID, "beforeMs and afterMs must be finite and >= 0");
+    if (!std::isfinite(p.guide_t0_ms)) return fail(PSEG_E_INVALID, "guideT0Ms must be finite");
+  }
+  if (!(p.low_cut_hz >= 0 && p.high_cut_hz > p.low_cut_hz))
+    return fail(PSEG_E_INVALID, "need 0 <= lowCutHz < highCutHz");
+  DevFirstBreak D;
+  memset(&D, 0, sizeof D);
+  D.method = p.method;
+  D.samples = samples;
+  D.dt = dt;
+  D.w = fb_samples_in(p.window_ms, dt, samples);
+  D.lta = fb_samples_in(p.lta_ms, dt, samples);
+  D.snr = fb_samples_in(p.snr_window_ms, dt, samples);
+  D.stabilization = p.stabilization;
+  D.search_start = p.search_start_ms;
+  D.search_end = p.search_end_ms;
+  D.guide_v = p.guide_velocity;
+  D.guide_t0 = p.guide_t0_ms;
+  D.before_ms = p.before_ms;
+  D.after_ms = p.after_ms;
+  D.low_db = p.low_db;
+  D.high_db = p.high_db;
+  D.filtered = p.low_cut_hz > 0 || p.high_cut_hz < kInf;
+  D.whole = p.method == 2 || (std::isnan(p.guide_velocity) && std::isnan(p.search_start_ms) &&
+                              std::isnan(p.search_end_ms));
+  int64_t pad = 0;
+  if (D.filtered) {
+    pseg_filter_spec spec;
+    memset(&spec, 0, sizeof spec);
+    spec.kind = 1;
+    spec.f3 = spec.f4 = kInf;
+    spec.low_hz = p.low_cut_hz;
+    spec.low_slope = p.slope_db_per_octave;
+    spec.high_hz = p.high_cut_hz;
+    spec.high_slope = p.slope_db_per_octave;
+    pseg_status st = guarded([&] {
+      const IirDesign d = design_iir(spec, dt, 1);
+      if ((int)d.secs.size() > kFbMaxSections)
+        return fail(PSEG_E_UNSUPPORTED, "a band-pass of %d sections", (int)d.secs.size());
+      D.n_sections = (int32_t)d.secs.size();
+      for (size_t i = 0; i < d.secs.size(); i++) {
+        const Sos &s = d.secs[i];
+        const double v[6] = {s.b0, s.b1, s.b2, s.a1, s.a2, s.g};
+        for (int k = 0; k < 6; k++) D.sec[i][k] = v[k];
+      }
+      pad = d.pad;
+      D.dc_gain = d.dc_gain;
+      D.dc_slope = d.dc_slope;
+      return PSEG_OK;
+    });
+    if (st != PSEG_OK) return st;
+    D.trend = D.whole ? 2 : 0;
+    D.reflect = std::fabs(D.dc_gain) >= 0.5 || D.trend != 2;
+    D.lead = D.reflect ? pad : 0;
+  }
+  const int64_t need = p.method == 0 ? D.w + D.lta : D.w;
+  D.before = std::max(D.snr + D.w / 2, need) + pad;
+  D.after = std::max(D.w, D.snr);
+  DevFirstBreak *dev = nullptr;
+  hipError_t e = hipMalloc((void **)&dev, sizeof D);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc of a first-break plan");
+  e = hipMemcpy(dev, &D, sizeof D, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(dev);
+    return hip_fail(e, "hipMemcpy of a first-break plan");
+  }
+  pseg_firstbreak_plan *P = new pseg_firstbreak_plan;
+  P->dtype = dtype;
+  P->samples = samples;
+  P->host = D;
+  P->dev = dev;
+  *out = P;
+  return PSEG_OK;
+}
+
+// The scratch of a pick: prefix energies and the attribute (cols + 1
+// doubles a row each), the filter's padded trace (lead + cols + 1 doubles)
+// and the filtered copy (cols samples), each [index][rows].
+struct FbWs {
+  size_t p, a, buf, copy, total;
+};
+
+FbWs fb_ws(const pseg_firstbreak_plan *P, int32_t rows, int32_t cols) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t r = (size_t)rows, c = (size_t)cols + 1;
+  FbWs w;
+  w.p = 0;
+  w.a = up(r * c * 8);
+  w.buf = w.a + up(r * c * 8);
+  const size_t nbuf = P->host.filtered ? r * ((size_t)P->host.lead + c) * 8 : 0;
+  w.copy = w.buf + up(nbuf);
+  const size_t ncopy = P->host.filtered ? r * (size_t)cols * (P->dtype == PSEG_F32 ? 4 : 8) : 0;
+  w.total = w.copy + up(ncopy);
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+pseg_status pseg_firstbreak_compile(int32_t dtype, const pseg_fb_params *params, double dt_ms,
+                                    int64_t samples, pseg_firstbreak_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  if (!params) return fail(PSEG_E_INVALID, "params is NULL");
+  if (dtype != PSEG_F32 && dtype != PSEG_F64) return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
+  return fb_compile(dtype, *params, dt_ms, samples, out);
+}
+
+pseg_status pseg_firstbreak_destroy(pseg_firstbreak_plan *plan) {
+  if (!plan) return PSEG_OK;
+  (void)hipFree(plan->dev);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_firstbreak_required_margin(const pseg_firstbreak_plan *plan, int64_t *out) {
+  if (!plan || !out) return fail(PSEG_E_INVALID, "NULL plan or output");
+  *out = plan->host.whole ? plan->samples : plan->host.before;
+  return PSEG_OK;
+}
+
+pseg_status pseg_firstbreak_workspace_size(const pseg_firstbreak_plan *plan, int32_t rows,
+                                           int32_t cols, size_t *out) {
+  if (!plan || !out) return fail(PSEG_E_INVALID, "NULL plan or output");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  *out = fb_ws(plan, rows, cols).total;
+  return PSEG_OK;
+}
+
+pseg_status pseg_firstbreak_pick(const pseg_firstbreak_plan *plan, const void *src, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *keys,
+                                 int64_t first_sample, void *ws, size_t ws_bytes, double *time_ms,
+                                 int64_t *col, double *snr_db, double *conf, void *stream) {
+  if (!plan) return fail(PSEG_E_INVALID, "a plan must come from pseg_firstbreak_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (cols > plan->samples)
+    return fail(PSEG_E_INVALID, "the block has %d columns, the plan takes %lld", cols,
+                (long long)plan->samples);
+  if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0");
+  if (rows == 0) return PSEG_OK;
+  if (!row_len || !time_ms || !col || !snr_db || !conf || (cols > 0 && !src))
+    return fail(PSEG_E_INVALID, "NULL panel, row_len or pick outputs");
+  const FbWs w = fb_ws(plan, rows, cols);
+  if (!ws || ws_bytes < w.total || ((uintptr_t)ws & 255))
+    return fail(PSEG_E_INVALID, "ws must be 256-byte aligned and hold %zu bytes, got %zu",
+                w.total, ws_bytes);
+  char *b = (char *)ws;
+  hipError_t e = launch_firstbreak(plan->dtype, plan->dev, src, rows, cols, row_len, keys,
+                                   first_sample, (double *)(b + w.p), (double *)(b + w.a),
+                                   (double *)(b + w.buf), b + w.copy, time_ms, col, snr_db, conf,
+                                   (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "first-break launch");
+  return PSEG_OK;
+}
+
+}  // extern "C"

@@ -293,6 +293,25 @@ hipError_t launch_analytic(int dtype, int threads, const DevFft *dev, size_t lds
 hipError_t launch_spectrogram(int dtype, int threads, const DevSpec *dev, size_t lds,
                               const void *x, int32_t len, int32_t hop, int32_t frames,
                               double *amp, hipStream_t s);
+// A first-break plan on the device (qc/firstbreak.nim:172-183): the
+// parameters, the windows in samples, and for a band-pass the minimum-phase
+// biquads (b0, b1, b2, a1, a2, g each) with their pads and trend.
+constexpr int kFbMaxSections = 32;
+struct DevFirstBreak {
+  int32_t method, filtered, whole, n_sections, reflect, trend;
+  int64_t samples, w, lta, snr, before, after, lead;
+  double dt, stabilization, search_start, search_end, guide_v, guide_t0, before_ms, after_ms;
+  double low_db, high_db, dc_gain, dc_slope;
+  double sec[kFbMaxSections][6];
+};
+// pickInto (qc/firstbreak.nim:492-541), one thread per row; the scratch
+// arrays are [index][rows] (the lanes of a wave read neighbouring values).
+hipError_t launch_firstbreak(int dtype, const DevFirstBreak *plan, const void *src, int32_t rows,
+                             int32_t cols, const int32_t *row_len, const double *keys,
+                             int64_t first_sample, double *p, double *a, double *buf, void *copy,
+                             double *time_ms, int64_t *col, double *snr_db, double *conf,
+                             hipStream_t stream);
+
 // n float64 at p set to 0 by a kernel (graph-capturable, unlike a memset).
 hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t stream);
 // averageSpectrogramInto (fft/spectrogram.nim:370-430) over `rows` rows
