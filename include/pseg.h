@@ -324,6 +324,56 @@ pseg_status pseg_firstbreak_pick(const pseg_firstbreak_plan *plan, const void *s
                                  int64_t first_sample, void *ws, size_t ws_bytes, double *time_ms,
                                  int64_t *col, double *snr_db, double *conf, void *stream);
 
+/* ------------------------------------------------------------ fold maps -- */
+
+/* BinningKind (stack/binning.nim:48-53). */
+enum { PSEG_BIN_CDP = 0, PSEG_BIN_LINE = 1, PSEG_BIN_GRID = 2, PSEG_BIN_INLINE_CROSSLINE = 3 };
+
+/* Binning (stack/binning.nim:55-66) as plain data, built and validated by
+ * cdpBinning / lineBinning / gridBinning / inlineCrosslineBinning
+ * (:80-175); 2D binnings have inlines = 1.  Bins are numbered
+ * inline * crosslines + crossline. */
+typedef struct pseg_binning {
+  int32_t kind;
+  int32_t reserved;
+  int64_t inlines, crosslines;
+  double first0, step0, first1, step1;
+  double x0, y0, ux, uy, vx, vy;
+  double size0, size1, max_off;
+} pseg_binning;
+
+/* The trace keys fold counting reads (TraceKeys fields of parseSEG), one
+ * row of PSEG_FOLD_KEYS float64 per trace, NaN where a header is absent. */
+enum { PSEG_KEY_SRC_X, PSEG_KEY_SRC_Y, PSEG_KEY_RCV_X, PSEG_KEY_RCV_Y, PSEG_KEY_CMP_X,
+       PSEG_KEY_CMP_Y, PSEG_KEY_OFFSET, PSEG_KEY_ENSEMBLE, PSEG_KEY_INLINE, PSEG_KEY_CROSSLINE,
+       PSEG_FOLD_KEYS };
+
+/* addFold(m, meta) (qc/fold.nim:583-588, with addTrace :561-581 and
+ * addToBin :429-451): every row of keys ([rows][PSEG_FOLD_KEYS], device)
+ * with row_len[r] >= 0 (pseg_meta_flatten: live) is counted into the map's
+ * device arrays: fold [bins]; offset_fold [n_bounds - 1][bins] when
+ * n_bounds > 1 (bounds: device, ascending); azimuth_fold [sectors][bins]
+ * when sectors > 0; offset_min / offset_max [bins] when offset_range;
+ * counters [4] += binned, outside, no coordinates, not live.  Integer
+ * atomics and compare-and-swap extremes: the map does not depend on the
+ * order the rows are counted in. */
+pseg_status pseg_fold_add(const pseg_binning *binning, const double *bounds, int32_t n_bounds,
+                          int32_t sectors, double sector_start_deg, int32_t offset_range,
+                          const double *keys, const int32_t *row_len, int32_t rows,
+                          int32_t *fold, int32_t *offset_fold, int32_t *azimuth_fold,
+                          float *offset_min, float *offset_max, uint64_t *counters,
+                          void *stream);
+/* clear (qc/fold.nim:416-427): counts and counters to 0, the offset range
+ * to NaN, on the device (a kernel, no memset). */
+pseg_status pseg_fold_clear(int64_t bins, int32_t classes, int32_t sectors, int32_t offset_range,
+                            int32_t *fold, int32_t *offset_fold, int32_t *azimuth_fold,
+                            float *offset_min, float *offset_max, uint64_t *counters,
+                            void *stream);
+/* foldDifference (qc/fold.nim:612-622): out = pre - actual per bin
+ * (wrapping int32), device arrays of bins. */
+pseg_status pseg_fold_difference(const int32_t *pre, const int32_t *actual, int32_t *out,
+                                 int64_t bins, void *stream);
+
 /* --------------------------------------------------------------- errors -- */
 
 /* The last error of this thread (the ValueError text of the reference,

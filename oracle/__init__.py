@@ -128,6 +128,13 @@ def lib():
             getattr(L, f"or_fb_pick_{sfx}").argtypes = [
                 C.POINTER(OrFbPlan), C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                 C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_bin_of.argtypes = [C.POINTER(OrBinning), C.c_void_p]
+        L.or_bin_of.restype = C.c_int64
+        L.or_bin_at.argtypes = [C.POINTER(OrBinning), C.c_double, C.c_double]
+        L.or_bin_at.restype = C.c_int64
+        L.or_fold_add.argtypes = [C.POINTER(OrBinning), C.c_void_p, C.c_int32, C.c_int32,
+                                  C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32] + \
+            [C.c_void_p] * 6
         L.or_iir_init.argtypes = [C.POINTER(OrIir), C.POINTER(OrSpec), C.c_double, C.c_int64,
                                   C.c_int32, C.c_int32, C.c_char_p, C.c_int]
         L.or_iir_magnitude_at.argtypes = [C.POINTER(OrIir), C.c_double]
@@ -625,6 +632,71 @@ def fb_pick(plan: OrFbPlan, x: np.ndarray, row_len, keys=None, first_sample: int
         None if k is None else k.ctypes.data, int(first_sample), t.ctypes.data, c.ctypes.data,
         snr.ctypes.data, conf.ctypes.data)
     return t, c, snr, conf
+
+
+# ---------------------------------------------------------- fold maps --
+
+class OrBinning(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("inlines", C.c_int64),
+                ("crosslines", C.c_int64)] + [
+        (k, C.c_double) for k in ("first0", "step0", "first1", "step1", "x0", "y0", "ux", "uy",
+                                  "vx", "vy", "size0", "size1", "max_off")]
+
+
+def _binning(b) -> OrBinning:
+    o = OrBinning()
+    o.kind, o.inlines, o.crosslines = int(b.kind), int(b.inlines), int(b.crosslines)
+    for f in ("first0", "step0", "first1", "step1", "x0", "y0", "ux", "uy", "vx", "vy", "size0",
+              "size1"):
+        setattr(o, f, float(getattr(b, f)))
+    o.max_off = float(b.maxOff)
+    return o
+
+
+def bin_of(binning, keys: np.ndarray) -> np.ndarray:
+    """binOf (stack/binning.nim:228-237) of each row of a [rows, 10] key table."""
+    k = np.ascontiguousarray(keys, np.float64)
+    b = _binning(binning)
+    L = lib()
+    return np.array([L.or_bin_of(C.byref(b), k[r].ctypes.data) for r in range(k.shape[0])],
+                    np.int64)
+
+
+class OrFold:
+    """A fold map on the host (qc/fold.nim:293-303) counted by the oracle."""
+
+    def __init__(self, binning, opts):
+        n, m = int(binning.inlines), int(binning.crosslines)
+        self.binning = binning
+        self.bounds = np.ascontiguousarray(tuple(opts.offsetBounds), np.float64)
+        self.sectors = int(opts.azimuthSectors)
+        self.start = float(opts.sectorStartDeg)
+        self.offset_range = bool(opts.offsetRange)
+        classes = max(len(self.bounds) - 1, 0)
+        self.fold = np.zeros((n, m), np.int32)
+        self.offsetFold = np.zeros((classes, n, m) if classes else (0,), np.int32)
+        self.azimuthFold = np.zeros((self.sectors, n, m) if self.sectors else (0,), np.int32)
+        self.offsetMin = np.full((n, m) if self.offset_range else (0,), np.nan, np.float32)
+        self.offsetMax = self.offsetMin.copy()
+        self.counters = np.zeros(4, np.int64)
+
+    def add(self, keys: np.ndarray, row_len) -> "OrFold":
+        """addFold (qc/fold.nim:583-588): rows with row_len < 0 are not live."""
+        k = np.ascontiguousarray(keys, np.float64)
+        rl = np.ascontiguousarray(row_len, np.int32)
+        b = _binning(self.binning)
+        ptr = lambda a: a.ctypes.data if a.size else None  # noqa: E731
+        lib().or_fold_add(C.byref(b), ptr(self.bounds), len(self.bounds), self.sectors,
+                          C.c_double(self.start), 1 if self.offset_range else 0, ptr(k),
+                          rl.ctypes.data, k.shape[0], self.fold.ctypes.data,
+                          ptr(self.offsetFold), ptr(self.azimuthFold), ptr(self.offsetMin),
+                          ptr(self.offsetMax), self.counters.ctypes.data)
+        return self
+
+    binned = property(lambda self: int(self.counters[0]))
+    outside = property(lambda self: int(self.counters[1]))
+    noCoordinates = property(lambda self: int(self.counters[2]))
+    notLive = property(lambda self: int(self.counters[3]))
 
 
 # ------------------------------------------------------------- sweeps --

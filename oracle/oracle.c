@@ -1078,3 +1078,110 @@ int or_fb_init(or_fb_plan *P, const or_fb_params *p, double dt, int64_t samples,
 
 int64_t or_fb_required_margin(const or_fb_plan *P) { return P->whole ? P->samples : P->before; }
 
+
+/* ------------------------------------------------------------ fold maps -- */
+
+/* nearest (stack/binning.nim:197-206): bin i holds [i - 0.5, i + 0.5); -1
+ * outside [-0.5, n - 0.5) or NaN; the half-way test on u - floor(u), which
+ * is exact. */
+static int64_t or_nearest(double u, int64_t n) {
+  if (!(u >= -0.5 && u < (double)n - 0.5)) return -1;
+  const double whole = floor(u);
+  const int64_t k = (int64_t)whole + (u - whole >= 0.5 ? 1 : 0);
+  return k < n - 1 ? k : n - 1;
+}
+
+static int64_t or_grid_bin(const or_binning *b, double u, double v) {
+  const int64_t i = or_nearest(v, b->inlines), j = or_nearest(u, b->crosslines);
+  return i < 0 || j < 0 ? -1 : i * b->crosslines + j;
+}
+
+int64_t or_bin_at(const or_binning *b, double x, double y) {
+  const double dx = x - b->x0, dy = y - b->y0;
+  if (b->kind == 1) {
+    if (!(fabs(dx * b->uy - dy * b->ux) <= b->max_off)) return -1;
+    return or_nearest((dx * b->ux + dy * b->uy) / b->size0, b->crosslines);
+  }
+  if (b->kind == 2)
+    return or_grid_bin(b, (dx * b->ux + dy * b->uy) / b->size0, (dx * b->vx + dy * b->vy) / b->size1);
+  return -1;
+}
+
+int64_t or_bin_of(const or_binning *b, const double *k) {
+  if (b->kind == 0) return or_nearest((k[OR_K_ENSEMBLE] - b->first0) / b->step0, b->crosslines);
+  if (b->kind == 3)
+    return or_grid_bin(b, (k[OR_K_CROSSLINE] - b->first1) / b->step1,
+                       (k[OR_K_INLINE] - b->first0) / b->step0);
+  return or_bin_at(b, k[OR_K_CMP_X], k[OR_K_CMP_Y]);
+}
+
+static int or_finite(double x) { return isfinite(x); }
+
+/* floorMod for floats (Nim std/math): fmod, then moved to the sign of y. */
+static double or_floor_mod(double x, double y) {
+  double r = fmod(x, y);
+  if ((r > 0 && y < 0) || (r < 0 && y > 0)) r += y;
+  return r;
+}
+
+/* addToBin (qc/fold.nim:429-451). */
+static void or_add_to_bin(int64_t bin, int64_t size, double offset, double bearing,
+                          const double *bounds, int32_t nb, int32_t sectors, double start,
+                          int32_t offset_range, int32_t *fold, int32_t *ofold, int32_t *afold,
+                          float *omin, float *omax, int64_t *counters) {
+  counters[0]++;
+  fold[bin]++;
+  if (nb > 1 && offset >= bounds[0] && offset < bounds[nb - 1]) {
+    int32_t k = 0; /* upperBound - 1: bounds[k] <= offset < bounds[k + 1] */
+    while (k + 1 < nb && bounds[k + 1] <= offset) k++;
+    ofold[(int64_t)k * size + bin]++;
+  }
+  if (offset_range && !isnan(offset)) {
+    const float f = (float)offset;
+    if (!(omin[bin] <= f)) omin[bin] = f;
+    if (!(omax[bin] >= f)) omax[bin] = f;
+  }
+  if (sectors > 0 && !isnan(bearing)) {
+    int64_t k = (int64_t)(or_floor_mod(bearing - start, 360.0) * (double)sectors / 360.0);
+    if (k > sectors - 1) k = sectors - 1;
+    afold[k * size + bin]++;
+  }
+}
+
+/* radToDeg (Nim std/math): d / (PI / 180). */
+static double or_rad_to_deg(double d) { return d / (OR_PI / 180.0); }
+
+void or_fold_add(const or_binning *b, const double *bounds, int32_t n_bounds, int32_t sectors,
+                 double sector_start_deg, int32_t offset_range, const double *keys,
+                 const int32_t *row_len, int32_t rows, int32_t *fold, int32_t *offset_fold,
+                 int32_t *azimuth_fold, float *offset_min, float *offset_max, int64_t *counters) {
+  const int64_t size = b->inlines * b->crosslines;
+  for (int32_t r = 0; r < rows; r++) {
+    if (row_len[r] < 0) { counters[3]++; continue; }
+    /* addTrace (qc/fold.nim:561-581) */
+    const double *k = keys + (int64_t)r * OR_FOLD_KEYS;
+    const int coords = or_finite(k[OR_K_SRC_X]) && or_finite(k[OR_K_SRC_Y]) &&
+                       or_finite(k[OR_K_RCV_X]) && or_finite(k[OR_K_RCV_Y]);
+    int64_t bin;
+    if (b->kind == 1 || b->kind == 2) {
+      double x = k[OR_K_CMP_X], y = k[OR_K_CMP_Y];
+      if (!(or_finite(x) && or_finite(y))) {
+        if (!coords) { counters[2]++; continue; }
+        x = (k[OR_K_SRC_X] + k[OR_K_RCV_X]) * 0.5;
+        y = (k[OR_K_SRC_Y] + k[OR_K_RCV_Y]) * 0.5;
+      }
+      bin = or_bin_at(b, x, y);
+    } else {
+      bin = or_bin_of(b, k);
+    }
+    if (bin < 0) { counters[1]++; continue; }
+    const double offset = coords ? hypot(k[OR_K_RCV_X] - k[OR_K_SRC_X], k[OR_K_RCV_Y] - k[OR_K_SRC_Y])
+                                 : fabs(k[OR_K_OFFSET]);
+    const double bearing = coords && sectors > 0
+                               ? or_rad_to_deg(atan2(k[OR_K_RCV_X] - k[OR_K_SRC_X],
+                                                     k[OR_K_RCV_Y] - k[OR_K_SRC_Y]))
+                               : NAN;
+    or_add_to_bin(bin, size, offset, bearing, bounds, n_bounds, sectors, sector_start_deg,
+                  offset_range, fold, offset_fold, azimuth_fold, offset_min, offset_max, counters);
+  }
+}

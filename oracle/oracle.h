@@ -307,6 +307,33 @@ void or_fb_pick_f64(const or_fb_plan *plan, const double *src, int32_t rows, int
                     const int32_t *row_len, const double *keys, int64_t first_sample,
                     double *time_ms, int64_t *col, double *snr_db, double *confidence);
 
+/* ------------------------------------------------------------ fold maps -- */
+/* Binning (stack/binning.nim:48-66): kind 0 bnCdp, 1 bnLine, 2 bnGrid,
+ * 3 bnInlineCrossline; 2D binnings have inlines = 1. */
+typedef struct or_binning {
+  int32_t kind;
+  int32_t reserved;
+  int64_t inlines, crosslines;
+  double first0, step0, first1, step1;
+  double x0, y0, ux, uy, vx, vy;
+  double size0, size1, max_off;
+} or_binning;
+/* The trace keys fold counting reads, [rows][OR_FOLD_KEYS] float64. */
+enum { OR_K_SRC_X, OR_K_SRC_Y, OR_K_RCV_X, OR_K_RCV_Y, OR_K_CMP_X, OR_K_CMP_Y, OR_K_OFFSET,
+       OR_K_ENSEMBLE, OR_K_INLINE, OR_K_CROSSLINE, OR_FOLD_KEYS };
+/* binAt / binOf (stack/binning.nim:208-237); -1 for no bin. */
+int64_t or_bin_at(const or_binning *b, double x, double y);
+int64_t or_bin_of(const or_binning *b, const double *keys);
+/* addFold(meta) (qc/fold.nim:429-588): the live rows (row_len >= 0) of
+ * keys counted into fold [bins], offset_fold [classes][bins] (n_bounds
+ * edges, NULL for none), azimuth_fold [sectors][bins], offset_min/max
+ * [bins] (offset_range); counters: binned, outside, no coordinates, not
+ * live. */
+void or_fold_add(const or_binning *b, const double *bounds, int32_t n_bounds, int32_t sectors,
+                 double sector_start_deg, int32_t offset_range, const double *keys,
+                 const int32_t *row_len, int32_t rows, int32_t *fold, int32_t *offset_fold,
+                 int32_t *azimuth_fold, float *offset_min, float *offset_max, int64_t *counters);
+
 /* ------------------------------------------------------ sweep / pilot -- */
 int64_t or_samples_in_length(double length_ms, double dt_ms);
 /* linearSweep + sweepInto + applyTaper(tpCos2) (vibroseis/sweep.nim:185-342,

@@ -2487,3 +2487,68 @@ pseg_status pseg_firstbreak_pick(const pseg_firstbreak_plan *plan, const void *s
 }
 
 }  // extern "C"
+
+// ================================================================ fold maps
+
+extern "C" {
+
+pseg_status pseg_fold_add(const pseg_binning *binning, const double *bounds, int32_t n_bounds,
+                          int32_t sectors, double sector_start_deg, int32_t offset_range,
+                          const double *keys, const int32_t *row_len, int32_t rows,
+                          int32_t *fold, int32_t *offset_fold, int32_t *azimuth_fold,
+                          float *offset_min, float *offset_max, uint64_t *counters,
+                          void *stream) {
+  if (!binning) return fail(PSEG_E_INVALID, "binning is NULL");
+  const pseg_binning &b = *binning;
+  if (b.kind < PSEG_BIN_CDP || b.kind > PSEG_BIN_INLINE_CROSSLINE)
+    return fail(PSEG_E_INVALID, "unknown binning kind %d", b.kind);
+  if (b.inlines < 1 || b.crosslines < 1 || b.inlines > INT64_MAX / b.crosslines)
+    return fail(PSEG_E_INVALID, "a binning needs positive inline and crossline counts");
+  if ((b.kind == PSEG_BIN_INLINE_CROSSLINE || b.kind == PSEG_BIN_CDP) &&
+      !(b.step0 > 0 && b.step0 < kInf && (b.kind == PSEG_BIN_CDP || (b.step1 > 0 && b.step1 < kInf))))
+    return fail(PSEG_E_INVALID, "binning steps must be finite and positive");
+  if ((b.kind == PSEG_BIN_LINE || b.kind == PSEG_BIN_GRID) &&
+      !(b.size0 > 0 && b.size0 < kInf && (b.kind == PSEG_BIN_LINE || (b.size1 > 0 && b.size1 < kInf))))
+    return fail(PSEG_E_INVALID, "bin sizes must be finite and positive");
+  if (b.kind != PSEG_BIN_GRID && b.kind != PSEG_BIN_INLINE_CROSSLINE && b.inlines != 1)
+    return fail(PSEG_E_INVALID, "a 2D binning has one inline");
+  if (rows < 0) return fail(PSEG_E_INVALID, "rows must be >= 0");
+  if (n_bounds < 0 || n_bounds == 1) return fail(PSEG_E_INVALID, "offset bounds need 0 or >= 2 edges");
+  if (sectors < 0) return fail(PSEG_E_INVALID, "azimuthSectors must be >= 0");
+  if (!std::isfinite(sector_start_deg)) return fail(PSEG_E_INVALID, "sectorStartDeg must be finite");
+  if (!fold || !counters || (n_bounds > 1 && (!bounds || !offset_fold)) ||
+      (sectors > 0 && !azimuth_fold) || (offset_range && (!offset_min || !offset_max)))
+    return fail(PSEG_E_INVALID, "NULL fold, counters, or an array its options need");
+  if (rows > 0 && (!keys || !row_len)) return fail(PSEG_E_INVALID, "NULL keys or row_len");
+  hipError_t e = launch_fold_add(b, bounds, n_bounds, sectors, sector_start_deg, offset_range ? 1 : 0,
+                                 keys, row_len, rows, fold, offset_fold, azimuth_fold, offset_min,
+                                 offset_max, (unsigned long long *)counters, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "fold launch");
+  return PSEG_OK;
+}
+
+pseg_status pseg_fold_clear(int64_t bins, int32_t classes, int32_t sectors, int32_t offset_range,
+                            int32_t *fold, int32_t *offset_fold, int32_t *azimuth_fold,
+                            float *offset_min, float *offset_max, uint64_t *counters,
+                            void *stream) {
+  if (bins < 1 || classes < 0 || sectors < 0) return fail(PSEG_E_INVALID, "bins >= 1, classes and sectors >= 0");
+  if (!fold || !counters || (classes > 0 && !offset_fold) || (sectors > 0 && !azimuth_fold) ||
+      (offset_range && (!offset_min || !offset_max)))
+    return fail(PSEG_E_INVALID, "NULL fold, counters, or an array its options need");
+  hipError_t e = launch_fold_clear(bins, classes, sectors, offset_range ? 1 : 0, fold, offset_fold,
+                                   azimuth_fold, offset_min, offset_max,
+                                   (unsigned long long *)counters, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "fold clear launch");
+  return PSEG_OK;
+}
+
+pseg_status pseg_fold_difference(const int32_t *pre, const int32_t *actual, int32_t *out,
+                                 int64_t bins, void *stream) {
+  if (bins < 0) return fail(PSEG_E_INVALID, "bins must be >= 0");
+  if (bins > 0 && (!pre || !actual || !out)) return fail(PSEG_E_INVALID, "NULL fold array");
+  hipError_t e = launch_fold_diff(pre, actual, out, bins, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "fold difference launch");
+  return PSEG_OK;
+}
+
+}  // extern "C"

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "../../include/pseg.h"
 
 namespace pseg {
 
@@ -311,6 +312,18 @@ hipError_t launch_firstbreak(int dtype, const DevFirstBreak *plan, const void *s
                              int64_t first_sample, double *p, double *a, double *buf, void *copy,
                              double *time_ms, int64_t *col, double *snr_db, double *conf,
                              hipStream_t stream);
+
+// addFold / clear / foldDifference (qc/fold.nim) on the device.
+hipError_t launch_fold_add(const pseg_binning &b, const double *bounds, int32_t nb,
+                           int32_t sectors, double start_deg, int32_t offset_range,
+                           const double *keys, const int32_t *row_len, int32_t rows, int32_t *fold,
+                           int32_t *ofold, int32_t *afold, float *omin, float *omax,
+                           unsigned long long *counters, hipStream_t stream);
+hipError_t launch_fold_clear(int64_t bins, int32_t classes, int32_t sectors, int32_t offset_range,
+                             int32_t *fold, int32_t *ofold, int32_t *afold, float *omin,
+                             float *omax, unsigned long long *counters, hipStream_t stream);
+hipError_t launch_fold_diff(const int32_t *pre, const int32_t *act, int32_t *out, int64_t n,
+                            hipStream_t stream);
 
 // n float64 at p set to 0 by a kernel (graph-capturable, unlike a memset).
 hipError_t launch_zero_f64(double *p, int64_t n, hipStream_t stream);

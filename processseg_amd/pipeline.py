@@ -563,11 +563,14 @@ class ChunkMeta:
     """The per-row facts of a block (core/types.nim:69-122): status per row
     (tsLive, tsUnknown, tsDead, tsAuxiliary), trace length numSamples per
     row, failed rows, dt and firstSample, and the trace keys mutes read
-    (t0Ms, offset per row; default 0 and NaN).  ``row_len(cols)`` flattens
+    (t0Ms, offset per row; default 0 and NaN; traceKeys, the source,
+    receiver and midpoint coordinates and header numbers fold maps read).
+    ``row_len(cols)`` flattens
     isLive/liveSamples to one int32 per row (-1: not live)."""
 
     def __init__(self, rows: int, dtMs: float, numSamples=None, status=None,
-                 failed: Iterable[int] = (), firstSample: int = 0, t0Ms=None, offset=None):
+                 failed: Iterable[int] = (), firstSample: int = 0, t0Ms=None, offset=None,
+                 traceKeys=None):
         if not (isinstance(dtMs, (int, float)) and math.isfinite(dtMs) and dtMs > 0):
             raise ValueError(f"dtMs must be finite and positive, got {dtMs}")
         if firstSample < 0:
@@ -587,6 +590,11 @@ class ChunkMeta:
                      else np.ascontiguousarray(t0Ms, np.float64).reshape(rows))
         self.offset = (np.full(rows, np.nan) if offset is None
                        else np.ascontiguousarray(offset, np.float64).reshape(rows))
+        # the keys fold counting reads ([rows, 10] float64, fold.KEY_COLUMNS)
+        self.traceKeys = (None if traceKeys is None
+                          else np.ascontiguousarray(traceKeys, np.float64))
+        if self.traceKeys is not None and self.traceKeys.shape != (rows, 10):
+            raise ValueError(f"traceKeys must be [rows, 10], got {self.traceKeys.shape}")
         self._cache = {}
 
     def __len__(self):
