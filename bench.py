@@ -41,29 +41,62 @@ def make_flow():
     return big_flow()
 
 
-def cpu_baseline_leg(flow, x, live, gpu_out, seconds=12.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _timed_samples(fn, per_sample, warm=3, samples=15):
+    """`warm` untimed samples, then `samples` timed ones, each `per_sample`
+    calls of fn; returns the seconds per call of every timed sample."""
+    for _ in range(warm):
+        for _ in range(per_sample):
+            fn()
+    out = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        for _ in range(per_sample):
+            fn()
+        out.append((time.perf_counter() - t0) / per_sample)
+    return out
+
+
+def cpu_baseline_leg(flow, x, live, gpu_out):
     """The oracle (a C restatement of the reference) on the host: the
-    ProcessPool clone over the whole panel with every core, repeated for about
-    `seconds`, plus a one-core sample; and the accuracy of the GPU output
+    ProcessPool clone over the whole panel on the host cores the process may
+    use (at most 16, one GPU's share of a box), 3 warm-up and 15 timed
+    samples of about half a second each, median and best; then one core the
+    same way (samples of two applies); and the accuracy of the GPU output
     against the float64 oracle on 64 rows."""
     import oracle as O
-    cores = os.cpu_count() or 1
+    nproc = os.cpu_count() or 1
     try:
-        cores = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except Exception:
-        pass
-    cores = min(cores, 16)
+        avail = nproc
+    cores = min(avail, 16)
     row_len = live.astype(np.int32)
-    pool = O.Pool(cores)
     dst = np.empty_like(x)
-    pool.apply_f32(flow.ops, x, dst, row_len, DT)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
-        pool.apply_f32(flow.ops, x, dst, row_len, DT)
-        reps += 1
-    el = time.perf_counter() - t0
-    pool.close()
-    value = reps * x.size / el / 1e6
+    res = {}
+    for name, threads, per in (("all", cores, None), ("one", 1, 2)):
+        pool = O.Pool(threads)
+        fn = lambda: pool.apply_f32(flow.ops, x, dst, row_len, DT)  # noqa: E731
+        if per is None:
+            t0 = time.perf_counter()
+            fn()
+            per = max(1, int(0.5 / max(time.perf_counter() - t0, 1e-4)))
+        secs = _timed_samples(fn, per)
+        pool.close()
+        rates = [x.size / s / 1e6 for s in secs]
+        res[name] = {"median": float(np.median(rates)), "best": float(np.max(rates)),
+                     "applies": per * 18, "threads": threads}
+    a, o = res["all"], res["one"]
     # accuracy vs the float64 oracle on 64 rows spread over the panel
     idx = np.linspace(0, x.shape[0] - 1, 64).astype(int)
     want = O.pipeline_apply(flow.ops, x[idx].astype(np.float64), row_len[idx], DT)
@@ -72,9 +105,14 @@ def cpu_baseline_leg(flow, x, live, gpu_out, seconds=12.0):
     mabs = float(np.max(np.abs(got - want)))
     mirror = dst[idx].astype(np.float64)
     rel_cpu = float(np.linalg.norm(mirror - want) / np.linalg.norm(want))
-    return ({"value": round(value, 2), "unit": "Msamples/s", "cores": cores, "kind": "port",
-             "sample": f"the whole {x.shape[0]}x{x.shape[1]} panel, {reps} applies in "
-                       f"{el:.1f} s on {cores} threads (ProcessPool clone, runs of 256 KiB)"},
+    return ({"value": round(a["median"], 2), "unit": "Msamples/s", "cores": cores, "kind": "port",
+             "best": round(a["best"], 2),
+             "one_core": {"value": round(o["median"], 2), "best": round(o["best"], 2)},
+             "cpu": _cpu_model(), "nproc": nproc, "affinity": avail,
+             "sample": f"the whole {x.shape[0]}x{x.shape[1]} panel: 3 warm-up + 15 timed samples "
+                       f"of {a['applies'] // 18} applies on {cores} threads (ProcessPool clone, "
+                       f"runs of 256 KiB), median (best {a['best']:.0f}); one core: samples of "
+                       f"{o['applies'] // 18} applies"},
             {"vs": "float64 oracle, 64 rows", "rel_l2": rel, "max_abs": mabs,
              "cpu_mirror_rel_l2": rel_cpu})
 

@@ -422,11 +422,189 @@ __device__ __forceinline__ void pass_single(int r, V2<T> *a, CTab<T> t, int m, i
   }
 }
 
+// ------------------------------------------- length-specialised passes
+//
+// For the transform lengths static_forward lists, the passes are compiled with
+// the length's strides as constants: the LDS loads and stores take their
+// offsets as immediates, the butterfly index divides by a constant, the
+// iteration guards vanish where the butterflies fill every thread, and the
+// twiddle offsets fold into the loads.  The butterflies, twiddles and their
+// order are the generic pass's: the results are the same.
+
+// radices_of (the host's plan): 4s, at most one 2, 3s, then 5s.
+struct Radices {
+  int n;
+  int r[kMaxRadices];
+};
+__host__ __device__ constexpr Radices radices_const(int m) {
+  Radices R{0, {}};
+  while (m % 4 == 0) { R.r[R.n++] = 4; m /= 4; }
+  if (m % 2 == 0) { R.r[R.n++] = 2; m /= 2; }
+  while (m % 3 == 0) { R.r[R.n++] = 3; m /= 3; }
+  while (m % 5 == 0) { R.r[R.n++] = 5; m /= 5; }
+  return R;
+}
+
+// first_pass_pad of the host: the first pass's butterfly size when a second
+// pass follows.
+__host__ __device__ constexpr int first_pad_const(int m, int fuse) {
+  const Radices R = radices_const(m);
+  if (R.n < 2) return 0;
+  const bool pair = fusable(R.r[0], R.r[1], fuse);
+  const int used = pair ? 2 : 1;
+  return R.n > used ? (pair ? R.r[0] * R.r[1] : R.r[0]) : 0;
+}
+
+template <typename T, int R1, int R2, int NTH, bool LAST, int M, int S, int SUB1, int PIN,
+          int POUT>
+__device__ __forceinline__ void pass_static(V2<T> *__restrict__ x, CTab<T> tw1, CTab<T> tw2) {
+  constexpr int R = R1 * R2;
+  constexpr int SUB2 = SUB1 / R2;
+  constexpr int NB = M / R;
+  constexpr int IT = (NB + NTH - 1) / NTH;
+  constexpr bool twiddle1 = !LAST || R2 > 1;
+  constexpr bool twiddle2 = R2 > 1 && !LAST;
+  constexpr int IN1 = S * SUB1 + PIN * SUB1, IN2 = S * SUB2 + PIN * SUB2;
+  constexpr int OUT2 = S * R1;
+  constexpr int INP = S + PIN, OUTP = S * R + POUT;
+  constexpr int T1STEP = SUB2 * (R1 - 1);
+  V2<T> v[IT][R2][R1];
+  {
+    const int tid_ = opaque_tid();
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int b = tid_ + NTH * it;
+      if (NB % NTH == 0 || it + 1 < IT || b < NB) {
+        const int p2 = b / S;
+        const int q = b - p2 * S;
+        const V2<T> *xi = x + (q + INP * p2);
+#pragma unroll
+        for (int j2 = 0; j2 < R2; j2++)
+#pragma unroll
+          for (int j = 0; j < R1; j++) v[it][j2][j] = xi[j2 * IN2 + j * IN1];
+      }
+    }
+  }
+  __syncthreads();
+  const int tid_ = opaque_tid();
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int b = tid_ + NTH * it;
+    if (NB % NTH == 0 || it + 1 < IT || b < NB) {
+      const int p2 = b / S;
+      const int q = b - p2 * S;
+      V2<T> *yo = x + (q + OUTP * p2);
+      CTab<T> t1 = tw1 + (p2 * (R1 - 1) - 1);
+#pragma unroll
+      for (int j2 = 0; j2 < R2; j2++) {
+        Bfly<T, R1>::run(v[it][j2]);
+        if constexpr (twiddle1) {
+#pragma unroll
+          for (int k = 1; k < R1; k++)
+            v[it][j2][k] = cmul<T>(v[it][j2][k], t1[j2 * T1STEP + k]);
+        }
+      }
+      if constexpr (R2 == 1) {
+#pragma unroll
+        for (int k = 0; k < R1; k++) yo[k * S] = v[it][0][k];
+      } else {
+        CTab<T> t2 = tw2 + (p2 * (R2 - 1) - 1);
+#pragma unroll
+        for (int k = 0; k < R1; k++) {
+          V2<T> u[R2];
+#pragma unroll
+          for (int j2 = 0; j2 < R2; j2++) u[j2] = v[it][j2][k];
+          Bfly<T, R2>::run(u);
+          if constexpr (twiddle2) {
+#pragma unroll
+            for (int k2 = 1; k2 < R2; k2++) u[k2] = cmul<T>(u[k2], t2[k2]);
+          }
+#pragma unroll
+          for (int k2 = 0; k2 < R2; k2++) yo[k * S + k2 * OUT2] = u[k2];
+        }
+      }
+    }
+    PSEG_ONE_AT_A_TIME();
+  }
+}
+
+// The passes of the M-point transform from stage I (S points done).
+template <typename T, int NTH, int FUSE, int M, int I, int S>
+__device__ __forceinline__ void static_passes(const DevFft &f, V2<T> *a, CTab<T> tw) {
+  constexpr Radices P = radices_const(M);
+  if constexpr (I < P.n) {
+    constexpr int r = P.r[I];
+    constexpr int r2 = I + 1 < P.n ? P.r[I + 1] : 1;
+    constexpr bool pair = I + 1 < P.n && fusable(r, r2, FUSE);
+    constexpr int pad = first_pad_const(M, FUSE);
+    constexpr int po = pad != 0 && I == 0, pi = pad != 0 && S != 1 && S == pad;
+    constexpr int sub = M / (S * r);
+    CTab<T> t = tw + uni(f.tw_off[I]);
+    if constexpr (pair) {
+      CTab<T> t2 = tw + uni(f.tw_off[I + 1]);
+      pass_static<T, r, r2, NTH, I + 2 >= P.n, M, S, sub, pi, po>(a, t, t2);
+      __syncthreads();
+      static_passes<T, NTH, FUSE, M, I + 2, S * r * r2>(f, a, tw);
+    } else {
+      pass_static<T, r, 1, NTH, I + 1 >= P.n, M, S, sub, pi, po>(a, t, t);
+      __syncthreads();
+      static_passes<T, NTH, FUSE, M, I + 1, S * r>(f, a, tw);
+    }
+  }
+}
+
+// Whether every pass of the M-point transform fits NTH threads within the
+// register budget (the host's fft_fits for the same plan).
+template <int NTH, int FUSE, int BUDGET>
+__host__ __device__ constexpr bool static_fits(int m) {
+  const Radices P = radices_const(m);
+  for (int i = 0; i < P.n;) {
+    const bool pair = i + 1 < P.n && fusable(P.r[i], P.r[i + 1], FUSE);
+    const int r = pair ? P.r[i] * P.r[i + 1] : P.r[i];
+    if ((m / r + NTH - 1) / NTH > pass_iters_max(r, BUDGET)) return false;
+    i += pair ? 2 : 1;
+  }
+  return true;
+}
+
+template <typename T, int NTH, int FUSE, int BUDGET, int M>
+__device__ __forceinline__ bool try_static(const DevFft &f, V2<T> *a, int m) {
+  if constexpr (static_fits<NTH, FUSE, BUDGET>(M)) {
+    if (m == M) {
+      static_passes<T, NTH, FUSE, M, 0, 1>(f, a, ctab<T>(uni_ptr(f.tw)));
+      return true;
+    }
+  }
+  return false;
+}
+
+// The complex lengths compiled with constant strides, in float32 kernels
+// (each adds its passes to every such kernel): the row kernel's 1125, 1536
+// and 1728 (real transforms of 2250, 3072 and 3456 points: band-passes of
+// 2001- and 3001-sample traces at 2 ms) and the correlation's 8100 (16200
+// points: 10001-sample records with a 6000-sample sweep).  Every other
+// length runs the generic passes.
+template <typename T, int NTH, int FUSE, int BUDGET>
+__device__ __forceinline__ bool static_forward(const DevFft &f, V2<T> *a) {
+  if constexpr (sizeof(T) == 4) {
+    const int m = uni(f.m);
+    if constexpr (FUSE == kFusePipeline && BUDGET == kBudgetPipeline) {
+      if (try_static<T, NTH, FUSE, BUDGET, 1728>(f, a, m)) return true;
+      if (try_static<T, NTH, FUSE, BUDGET, 1536>(f, a, m)) return true;
+      if (try_static<T, NTH, FUSE, BUDGET, 1125>(f, a, m)) return true;
+    } else {
+      if (try_static<T, NTH, FUSE, BUDGET, 8100>(f, a, m)) return true;
+    }
+  }
+  return false;
+}
+
 // The complex forward FFT of the m points at a, in place (a holds f.stride
 // complex values): the plan's stages in passes, stage pairs of at most FUSE
 // points fused (the host built f's padded layout with the same FUSE).
 template <typename T, int NTH, int FUSE, int BUDGET = kBudgetCorr>
 __device__ __forceinline__ void complex_forward(const DevFft &f, V2<T> *a) {
+  if (static_forward<T, NTH, FUSE, BUDGET>(f, a)) return;
   CTab<T> tw = ctab<T>(uni_ptr(f.tw));
   int s = 1;
   const int nrad = uni(f.nrad), m = uni(f.m);
