@@ -379,6 +379,31 @@ def decode_leg(torch, P, dev, steps, warmup):
                          "note": "wall clock over back-to-back launches, not kernel events"}}
 
 
+def host_staged_leg(torch, P, dev, steps, warmup):
+    """The headline flow on a panel held in host memory (Plan.applyHost:
+    pseg_plan_apply_host copies the 2000 x 3001 block to the device, runs
+    the plan, copies it back): the rate a caller of the reference's
+    Tensor-in-host-memory apply sees, PCIe copies included (pageable host
+    arrays)."""
+    from processseg_amd.workloads import big_flow, big_panel
+    x, live = big_panel()
+    rows, cols = x.shape
+    meta = P.ChunkMeta(rows, DT, numSamples=live)
+    plan = big_flow().compile(np.float32, DT, cols)
+    dst = np.empty_like(x)
+    for _ in range(warmup):
+        plan.applyHost(x, dst, meta)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        plan.applyHost(x, dst, meta)
+    t = (time.perf_counter() - t1) / steps
+    return {"workload": "the headline flow on a 2000x3001 f32 panel in host memory (copies to "
+                        "and from the device included)",
+            "value": round(rows * cols / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 4),
+            "host_bytes_per_step": rows * cols * 8}
+
+
 def raw_flow_leg(torch, P, dev, steps, warmup):
     """The headline flow from raw SEG-Y IBM traces of the 2000 x 3001 panel:
     Plan.applyRaw (each row decoded as the kernel loads it) against
@@ -633,7 +658,7 @@ def main():
                     help="only the headline panel (no correlation leg)")
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
-                         "gain_flow, iir_flow, decode, raw_flow, spectrum, hilbert, spectrogram, "
+                         "gain_flow, iir_flow, decode, host_staged, raw_flow, spectrum, hilbert, spectrogram, "
                          "f64_panel, correlation, uneven_batch, serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
@@ -778,6 +803,10 @@ def main():
         if want("decode"):
             log("decode leg")
             sec["decode"] = decode_leg(torch, P, dev, max(20, args.steps // 2), 5)
+            torch.cuda.empty_cache()
+        if want("host_staged"):
+            log("host-staged leg")
+            sec["host_staged"] = host_staged_leg(torch, P, dev, max(10, args.steps // 10), 3)
             torch.cuda.empty_cache()
         if want("raw_flow"):
             log("raw flow leg")

@@ -470,6 +470,22 @@ pseg_status pseg_decode_samples(int32_t format, const void *raw, int64_t pitch, 
                                 int32_t rows, int32_t cols, int32_t dtype, void *dst,
                                 void *stream);
 
+/* apply(plan, src: Tensor[T], dst: var Tensor[T], meta, ws) and the
+ * in-place apply(plan, samples: var Tensor[T], meta, ws) (pipeline.nim:
+ * 758-772) for a caller whose panels live in host memory: src, dst, row_len
+ * (pseg_meta_flatten's) and row_keys (NULL or [rows, 2] float64) are HOST
+ * arrays; src == dst runs in place.  The block is copied to device memory
+ * the library keeps per thread and device (grown on demand; the plan's
+ * workspace is carved from it too), the plan runs there on `stream`, and
+ * the result is copied back: the call returns when dst holds it.  The same
+ * checks and errors as pseg_plan_apply_keys, before any copy. */
+pseg_status pseg_plan_apply_host(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *row_keys,
+                                 int64_t first_sample, void *stream);
+/* Frees this thread's staging memory of pseg_plan_apply_host (on every
+ * device it used); the next call allocates it again. */
+pseg_status pseg_host_staging_release(void);
+
 /* pseg_plan_apply_keys with the source as raw SEG-Y traces (as
  * pseg_decode_samples reads them) instead of a panel: each row is decoded
  * as it is loaded and the plan runs on it, so the traces cross HBM once,

@@ -142,7 +142,7 @@ SYMBOLS = [
     "pseg_spectrogram_destroy", "pseg_spectrogram_info_get", "pseg_spectrogram_apply",
     "pseg_firstbreak_compile", "pseg_firstbreak_destroy", "pseg_firstbreak_required_margin",
     "pseg_firstbreak_workspace_size", "pseg_firstbreak_pick", "pseg_fold_add", "pseg_fold_clear",
-    "pseg_fold_difference",
+    "pseg_fold_difference", "pseg_plan_apply_host", "pseg_host_staging_release",
 ]
 
 
@@ -197,6 +197,8 @@ def lib():
     L.pseg_firstbreak_destroy.argtypes = [vp]
     L.pseg_firstbreak_required_margin.argtypes = [vp, C.POINTER(i64)]
     L.pseg_firstbreak_workspace_size.argtypes = [vp, i32, i32, C.POINTER(C.c_size_t)]
+    L.pseg_plan_apply_host.argtypes = [vp, vp, vp, i32, i32, vp, vp, i64, vp]
+    L.pseg_host_staging_release.argtypes = []
     L.pseg_fold_add.argtypes = [vp, vp, i32, i32, f64, i32, vp, vp, i32, vp, vp, vp, vp, vp, vp,
                                 vp]
     L.pseg_fold_clear.argtypes = [i64, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]

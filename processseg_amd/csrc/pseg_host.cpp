@@ -2552,3 +2552,100 @@ pseg_status pseg_fold_difference(const int32_t *pre, const int32_t *actual, int3
 }
 
 }  // extern "C"
+
+// =========================================================== host staging
+
+namespace {
+
+// Device memory pseg_plan_apply_host stages a host block through: one
+// region per device, per thread, grown to the largest block seen.
+struct Staging {
+  int device;
+  char *base;
+  size_t bytes;
+};
+thread_local std::vector<Staging> t_staging;
+
+char *staging_for(size_t bytes, hipError_t &e) {
+  int dev = 0;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return nullptr;
+  for (Staging &st : t_staging) {
+    if (st.device != dev) continue;
+    if (st.bytes >= bytes) return st.base;
+    (void)hipFree(st.base);
+    st.base = nullptr;
+    st.bytes = 0;
+    e = hipMalloc((void **)&st.base, bytes);
+    if (e != hipSuccess) return nullptr;
+    st.bytes = bytes;
+    return st.base;
+  }
+  Staging st{dev, nullptr, 0};
+  e = hipMalloc((void **)&st.base, bytes);
+  if (e != hipSuccess) return nullptr;
+  st.bytes = bytes;
+  t_staging.push_back(st);
+  return st.base;
+}
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+pseg_status pseg_plan_apply_host(const pseg_plan *P, const void *src, void *dst, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *row_keys,
+                                 int64_t first_sample, void *stream) {
+  // checkApply (pipeline.nim:619-641) before any copy
+  if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
+  if (rows == 0) return PSEG_OK;
+  if (!src || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL panel or row_len");
+  if (cols > P->samples)
+    return fail(PSEG_E_INVALID, "the block has %d columns, the plan takes %d samples", cols, P->samples);
+  const size_t esz = P->dtype == PSEG_F32 ? 4 : 8;
+  const size_t panel = (size_t)rows * (size_t)cols * esz;
+  if (overlap(src, dst, panel))
+    return fail(PSEG_E_INVALID, "src and dst overlap without being the same block");
+  size_t need = 0;
+  pseg_plan_workspace_size(P, rows, cols, &need);
+  const size_t o_len = up256(panel), o_keys = o_len + up256((size_t)rows * 4);
+  const size_t o_ws = o_keys + (row_keys ? up256((size_t)rows * 16) : 0);
+  const size_t total = o_ws + up256(std::max<size_t>(need, 1));
+  hipError_t e;
+  char *base = staging_for(total, e);
+  if (!base) return hip_fail(e, "hipMalloc of the host staging block");
+  hipStream_t s = (hipStream_t)stream;
+  if (panel > 0 && (e = hipMemcpyAsync(base, src, panel, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "copy of the block to the device");
+  if ((e = hipMemcpyAsync(base + o_len, row_len, (size_t)rows * 4, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "copy of row_len to the device");
+  if (row_keys &&
+      (e = hipMemcpyAsync(base + o_keys, row_keys, (size_t)rows * 16, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return hip_fail(e, "copy of the row keys to the device");
+  pseg_status st = pseg_plan_apply_keys(P, base, base, rows, cols, (const int32_t *)(base + o_len),
+                                        row_keys ? (const double *)(base + o_keys) : nullptr,
+                                        first_sample, need ? base + o_ws : nullptr, need, stream);
+  if (st != PSEG_OK) return st;
+  if (panel > 0 && (e = hipMemcpyAsync(dst, base, panel, hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return hip_fail(e, "copy of the block to the host");
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "host staging");
+  return PSEG_OK;
+}
+
+pseg_status pseg_host_staging_release(void) {
+  for (Staging &st : t_staging) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(st.device);
+    (void)hipFree(st.base);
+    (void)hipSetDevice(cur);
+  }
+  t_staging.clear();
+  return PSEG_OK;
+}
+
+}  // extern "C"

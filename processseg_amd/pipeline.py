@@ -847,6 +847,42 @@ class PipelinePlan:
             ws.data_ptr() if ws is not None else None, need, _stream_handle(stream)))
         return dst
 
+    def applyHost(self, src: np.ndarray, dst: Optional[np.ndarray] = None,
+                  meta: Optional[ChunkMeta] = None, stream=None) -> np.ndarray:
+        """apply(plan, src: Tensor[T], dst: var Tensor[T], meta, ws) and the
+        in-place apply(plan, samples, meta, ws) (pipeline.nim:758-772) on
+        host arrays: the block is staged through device memory the library
+        keeps, the plan runs there, and the result is in dst when this
+        returns.  src, dst: C-contiguous [rows, cols] numpy arrays of the
+        plan's type (dst None: in place)."""
+        if isinstance(dst, ChunkMeta) and meta is None:
+            meta, dst = dst, None
+        if dst is None:
+            dst = src
+        for name, a in (("src", src), ("dst", dst)):
+            if not isinstance(a, np.ndarray) or a.ndim != 2 or a.dtype != self.T \
+                    or not a.flags.c_contiguous:
+                raise ValueError(f"{name} must be a C-contiguous [rows, cols] {self.T} array")
+        if not dst.flags.writeable:
+            raise ValueError("dst must be writeable")
+        if src.shape != dst.shape:
+            raise ValueError(f"src is {src.shape[0]} × {src.shape[1]}, dst {dst.shape[0]} × "
+                             f"{dst.shape[1]}")
+        if meta is None:
+            raise ValueError("applyHost needs a ChunkMeta")
+        rows, cols = src.shape
+        if len(meta) != rows:
+            raise ValueError(f"samples have {rows} rows, the meta {len(meta)}")
+        if not sameInterval(meta.dtMs, self.dtMs):
+            raise ValueError(f"the plan is for dtMs = {self.dtMs}, the rows have {meta.dtMs}")
+        rl = meta.row_len(cols)
+        keys = np.ascontiguousarray(meta.keys()) if self.readsKeys else None
+        _lib.check(_lib.lib().pseg_plan_apply_host(
+            self._h, src.ctypes.data, dst.ctypes.data, rows, cols, rl.ctypes.data,
+            keys.ctypes.data if keys is not None else None, int(meta.firstSample),
+            _stream_handle(stream)))
+        return dst
+
     def applyRaw(self, raw, fmt: int, dst, meta: Optional[ChunkMeta] = None, *, row_len=None,
                  keys=None, first_sample: int = 0, offset: int = 240,
                  pitch: Optional[int] = None, stream=None):
