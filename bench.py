@@ -36,9 +36,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_flow():
+def make_flow(n_ops=None):
+    """The headline flow, or its first `n_ops` ops (the per-step profile
+    runs the flow cut after each op)."""
     from processseg_amd.workloads import big_flow
-    return big_flow()
+    flow = big_flow()
+    if n_ops is not None:
+        import processseg_amd as P
+        flow = P.initPipeline(flow.ops[:n_ops])
+    return flow
+
+
+FLOW_STEPS = ["demean", "band-pass 5-10-70-90 Hz + 50 Hz notch",
+              "Butterworth high cut 80 Hz 24 dB/oct", "AGC 250 ms"]
+
+
+def flow_name(n_ops=None):
+    return " > ".join(FLOW_STEPS[:n_ops])
 
 
 def _cpu_model():
@@ -663,6 +677,9 @@ def main():
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="length of the Poisson request stream of the serving leg")
+    ap.add_argument("--flow-ops", type=int, default=None,
+                    help="run only the first N ops of the headline flow (a per-step profile; "
+                         "the line then names the cut flow)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device copy of the panel instead (the calibration "
                          "kernel for the HBM byte counters)")
@@ -683,7 +700,7 @@ def main():
     from processseg_amd.workloads import big_panel
     log("generating the 2000 x 3001 panel")
     x, live = big_panel(ROWS, COLS, np.float32)
-    flow = make_flow()
+    flow = make_flow(args.flow_ops)
     plan = flow.compile(np.float32, DT, COLS)
     row_len = torch.from_numpy(live.astype(np.int32)).to(dev)
     srcs = [torch.from_numpy(x).to(dev) for _ in range(args.pairs)]
@@ -726,7 +743,7 @@ def main():
     traffic = None
     valu = None
     tpath = os.path.join(ROOT, "profiles", "traffic_bandpass_agc_2000x3001_f32.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.flow_ops is None:
         with open(tpath) as f:
             prof = json.load(f)
         traffic = prof.get("bytes_per_launch")
@@ -757,8 +774,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Ricker + hyperbolic events + Gaussian noise)",
-        "config": {"workload": "demean > band-pass 5-10-70-90 Hz + 50 Hz notch > Butterworth "
-                               "high cut 80 Hz 24 dB/oct > AGC 250 ms, 2000x3001 f32, dt 2 ms, "
+        "config": {"workload": flow_name(args.flow_ops) + ", 2000x3001 f32, dt 2 ms, "
                                "live lengths U[2400, 3001]",
                    "rows": ROWS, "cols": COLS, "dt_ms": DT,
                    "nfft": [s["nfft"] for s in plan.steps if s["nfft"]],

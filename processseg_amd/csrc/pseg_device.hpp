@@ -673,6 +673,82 @@ template <typename T> struct CplxMul {
   }
 };
 
+// Four T per bin pair in global memory (16- or 32-byte aligned rows).
+template <typename T> struct Quad {
+  T x, y, z, w;
+};
+template <typename T> struct QTab {
+  gptr<T> p;
+  __device__ __forceinline__ Quad<T> operator[](int i) const {
+    gptr<T> q = (gptr<T>)__builtin_assume_aligned((const T *)(p + 4 * i), 4 * sizeof(T));
+    return Quad<T>{q[0], q[1], q[2], q[3]};
+  }
+};
+
+// The split, a per-bin multiplier and the unsplit of the inverse, folded into
+// one linear map per bin pair (k, m - k): the three steps of `combine` below
+// with GainMul or CplxMul are, in exact arithmetic,
+//   W[k] = swap(alpha_k Z[k] + beta_k conj(Z[m - k]))
+// with w = exp(-2 pi i k / n), s = Im w, c = Re w and the multipliers M:
+//   alpha_k = M_k (2 + 2s) + conj(M_{m-k}) (2 - 2s),
+//   beta_k  = 2 i c (M_k - conj(M_{m-k})).
+// For a real gain alpha is real and beta imaginary, and beta_{m-k} =
+// beta_k, so a bin pair costs 8 multiply-adds instead of about 40 (the
+// split's and unsplit's twiddles and the multiplies).  The host computes the
+// coefficients in float64 and rounds them once; bin 0 and the Nyquist bin
+// keep the split's form (both real).
+//
+// Real gains: coef[0] = (M_0, M_m, 0, 0); coef[k] = (alpha_k, alpha_{m-k},
+// Im beta_k, 0) for k = 1 .. m/2.
+template <typename T, int NTH>
+__device__ __forceinline__ void combine_gain(const DevFft &f, V2<T> *A, QTab<T> coef) {
+  const int tid_ = opaque_tid();
+  const int m = uni(f.m);
+  for (int k = tid_; k <= m / 2; k += NTH) {
+    const Quad<T> c = coef[k];
+    if (k == 0) {
+      const V2<T> z = A[0];
+      const T x0 = (z.x + z.y) * c.x, xm = (z.x - z.y) * c.y;
+      A[0] = mk<T>((x0 - xm) * (T)2, (x0 + xm) * (T)2);
+    } else {
+      const int kk = m - k;
+      const V2<T> a = A[k], b = A[kk];
+      // alpha a + i beta conj(b), swapped
+      A[k] = mk<T>(c.x * a.y + c.z * b.x, c.x * a.x + c.z * b.y);
+      if (kk != k) A[kk] = mk<T>(c.y * b.y + c.z * a.x, c.y * b.x + c.z * a.y);
+    }
+  }
+}
+
+// Complex multipliers: coef[0] = (Re M_0, Re M_m, 0, 0), coef[1] unused;
+// coef[2k] = (alpha_k, beta_k), coef[2k + 1] = (alpha_{m-k}, beta_{m-k}) as
+// (re, im, re, im) for k = 1 .. m/2.
+template <typename T, int NTH>
+__device__ __forceinline__ void combine_cplx(const DevFft &f, V2<T> *A, QTab<T> coef) {
+  const int tid_ = opaque_tid();
+  const int m = uni(f.m);
+  for (int k = tid_; k <= m / 2; k += NTH) {
+    if (k == 0) {
+      const Quad<T> c = coef[0];
+      const V2<T> z = A[0];
+      const T x0 = (z.x + z.y) * c.x, xm = (z.x - z.y) * c.y;
+      A[0] = mk<T>((x0 - xm) * (T)2, (x0 + xm) * (T)2);
+    } else {
+      const int kk = m - k;
+      const V2<T> a = A[k], b = A[kk];
+      const Quad<T> c = coef[2 * k];
+      // alpha a + beta conj(b), swapped
+      A[k] = mk<T>(c.x * a.y + c.y * a.x + c.w * b.x - c.z * b.y,
+                   c.x * a.x - c.y * a.y + c.z * b.x + c.w * b.y);
+      if (kk != k) {
+        const Quad<T> e = coef[2 * k + 1];
+        A[kk] = mk<T>(e.x * b.y + e.y * b.x + e.w * a.x - e.z * a.y,
+                      e.x * b.x - e.y * b.y + e.z * a.x + e.w * a.y);
+      }
+    }
+  }
+}
+
 // Z (the forward complex FFT of the packed samples) -> W, the swapped input
 // of the inverse: split into bins, multiply, unsplit.  W may be Z: each
 // thread reads bins k and m - k before it writes them.  The split table is
@@ -797,7 +873,7 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   for (int dir = 0; dir < 2; dir++) {
     complex_forward<T, NTH, kFusePipeline, kBudgetPipeline>(f, A);
     if (dir == 0) {
-      combine<T, NTH>(f, A, A, GainMul<T>{as_global<T>(uni_ptr(F.gain))});
+      combine_gain<T, NTH>(f, A, QTab<T>{as_global<T>(uni_ptr(F.coef))});
       __syncthreads();
     }
   }

@@ -426,6 +426,51 @@ bool fft_fits(int64_t m, int threads, int fuse, int budget) {
   return true;
 }
 
+// The coefficients of combine_gain / combine_cplx (the split, the per-bin
+// multiplier and the inverse's unsplit as one linear map per bin pair) from
+// the multipliers in float64, rounded once to T.  n even, m = n / 2; mul holds
+// m + 1 bins (the device's 1/2 and 1/n already folded in).
+template <typename T>
+std::vector<T> lin_coef_gain(const std::vector<double> &g, int64_t n) {
+  const int64_t m = n / 2;
+  std::vector<T> c(4 * (m / 2 + 1), (T)0);
+  c[0] = (T)g[0];
+  c[1] = (T)g[m];
+  for (int64_t k = 1; k <= m / 2; k++) {
+    const double th = 2 * kPi * (double)k / (double)n;
+    const double sn = -std::sin(th), cs = std::cos(th);
+    const double gk = g[k], gkk = g[m - k];
+    c[4 * k] = (T)(gk * (2 + 2 * sn) + gkk * (2 - 2 * sn));
+    c[4 * k + 1] = (T)(gkk * (2 + 2 * sn) + gk * (2 - 2 * sn));
+    c[4 * k + 2] = (T)(2 * cs * (gk - gkk));
+  }
+  return c;
+}
+
+template <typename T>
+std::vector<T> lin_coef_cplx(const std::vector<cd> &mul, int64_t n) {
+  const int64_t m = n / 2;
+  std::vector<T> c(8 * (m / 2 + 1), (T)0);
+  c[0] = (T)mul[0].real();
+  c[1] = (T)mul[m].real();
+  const cd I(0, 1);
+  for (int64_t k = 1; k <= m / 2; k++) {
+    const double th = 2 * kPi * (double)k / (double)n;
+    const double sn = -std::sin(th), cs = std::cos(th);
+    const cd a = mul[k], b = mul[m - k];
+    const cd al = a * (2 + 2 * sn) + std::conj(b) * (2 - 2 * sn);
+    const cd be = 2 * cs * I * (a - std::conj(b));
+    const cd al2 = b * (2 + 2 * sn) + std::conj(a) * (2 - 2 * sn);
+    const cd be2 = -2 * cs * I * (b - std::conj(a));
+    const cd v[4] = {al, be, al2, be2};
+    for (int i = 0; i < 4; i++) {
+      c[8 * k + 2 * i] = (T)v[i].real();
+      c[8 * k + 2 * i + 1] = (T)v[i].imag();
+    }
+  }
+  return c;
+}
+
 template <typename T>
 void build_fft(int64_t n, DevFft &f, Blob &blob, int fuse) {
   memset(&f, 0, sizeof f);
@@ -692,19 +737,19 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       if (n > (1 << 20)) invalid("an FFT of %lld points is beyond this GPU path", (long long)n);
       DevFilter &F = st.filter;
       build_fft<T>(n, F.fft, blob, kFusePipeline);
-      std::vector<T> gain(n / 2 + 1);
+      std::vector<double> gain(n / 2 + 1);
       double df = 1000 / ((double)n * dt);
       // the device folds the split's 1/2 and the inverse's 1/n into the gain
       const double fold = 0.5 / (double)n;
       for (int64_t k = 0; k <= n / 2; k++)
-        gain[k] = (T)(response(spec, (double)k * df, dt) * fold);
-      if (spec.zero_dc) gain[0] = (T)0;
+        gain[k] = response(spec, (double)k * df, dt) * fold;
+      if (spec.zero_dc) gain[0] = 0;
       std::vector<T> fall(std::max<int64_t>(pad, 1));
       for (int64_t k = 1; k <= pad; k++) {
         double c = std::cos(kPi / 2 * (double)k / (double)(pad + 1));
         fall[k - 1] = (T)(c * c);
       }
-      set_off(F.gain, blob.add_vec(gain));
+      set_off(F.coef, blob.add_vec(lin_coef_gain<T>(gain, n)));
       set_off(F.fall, blob.add_vec(fall));
       F.pad = (int32_t)pad;
       F.reflect = reflect;
@@ -912,7 +957,7 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
 void patch_step(DevStep &st, char *base) {
   if (st.kind == kFilter) {
     patch_fft(st.filter.fft, base);
-    patch(st.filter.gain, base);
+    patch(st.filter.coef, base);
     patch(st.filter.fall, base);
   } else if (st.kind == kAgc) {
     patch(st.agc.inv, base);
@@ -2147,7 +2192,7 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
     return fail(PSEG_E_INVALID, "%s", e.msg.c_str());
   }
   // conj(P) w scale exp(+2 pi i k frac / n)
-  std::vector<double> mul(2 * bins);
+  std::vector<cd> mul(bins);
   // the device folds the split's 1/2 and the inverse's 1/n into the multiplier
   const double fold = 0.5 / (double)n;
   for (int64_t k = 0; k < bins; k++) {
@@ -2155,16 +2200,11 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
     double ar = P[k].real() * ws, ai = -P[k].imag() * ws;
     double th = 2 * kPi * (double)k * frac / (double)n;
     double er = std::cos(th), ei = std::sin(th);
-    mul[2 * k] = ar * er - ai * ei;
-    mul[2 * k + 1] = ar * ei + ai * er;
+    mul[k] = cd(ar * er - ai * ei, ar * ei + ai * er);
   }
-  if (n % 2 == 0) mul[2 * (bins - 1) + 1] = 0;  // the inverse takes Nyquist as real
-  if (dtype == PSEG_F32) {
-    std::vector<float> mf(mul.begin(), mul.end());
-    set_off(C.mul, blob.add_vec(mf));
-  } else {
-    set_off(C.mul, blob.add_vec(mul));
-  }
+  // the inverse takes Nyquist as real (only its real part enters the coefficients)
+  if (dtype == PSEG_F32) set_off(C.coef, blob.add_vec(lin_coef_cplx<float>(mul, n)));
+  else set_off(C.coef, blob.add_vec(lin_coef_cplx<double>(mul, n)));
   C.shift = (int32_t)whole;
   C.pilot = plen;
   C.record = record;
@@ -2173,7 +2213,7 @@ pseg_status pseg_corr_compile(int32_t dtype, const double *pilot, int32_t plen, 
   hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
   if (e != hipSuccess) return hip_fail(e, "hipMalloc of a correlation plan");
   patch_fft(C.fft, base);
-  patch(C.mul, base);
+  patch(C.coef, base);
   memcpy(blob.bytes.data(), &C, sizeof C);
   e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {

@@ -90,7 +90,7 @@ struct DevFilter {
   int32_t zero_dc;
   int32_t trend;      // 0 none, 1 mean, 2 linear
   double dc_gain;
-  const void *gain;   // T, m + 1 bins
+  const void *coef;   // T4, m / 2 + 1: the split, gain and unsplit per bin pair
   const void *fall;   // T, pad values
 };
 
@@ -210,7 +210,7 @@ struct JobLaunch {
 
 struct DevCorr {
   DevFft fft;
-  const void *mul;    // T2 multiplier per bin, m + 1 bins
+  const void *coef;   // T4, m + 2: the split, multiplier and unsplit per bin pair
   int32_t shift;
   int32_t pilot;
   int32_t record;

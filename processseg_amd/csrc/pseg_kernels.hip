@@ -299,7 +299,7 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
   for (int j = threadIdx.x; j < n; j += NTC) buf[j] = j < live ? s[j] : (T)0;
   __syncthreads();
   complex_forward<T, NTC, kFuseCorr>(f, A);
-  combine<T, NTC>(f, A, A, CplxMul<T>{ctab<T>(c.mul)});
+  combine_cplx<T, NTC>(f, A, QTab<T>{as_global<T>(c.coef)});
   __syncthreads();
   complex_forward<T, NTC, kFuseCorr>(f, A);
   const T *out = (const T *)A;
