@@ -16,6 +16,8 @@
 
 #include <math.h>
 
+#include <type_traits>
+
 namespace pseg {
 
 namespace {
@@ -28,6 +30,14 @@ using gptr = const X __attribute__((address_space(1))) *;
 template <typename X>
 __device__ __forceinline__ gptr<X> as_global(const void *p) {
   return (gptr<X>)(const X *)p;
+}
+// A row of a panel in global memory, written: stores through it are global
+// (not flat) stores, with the row's base in SGPRs and a 32-bit lane offset.
+template <typename X>
+using gmut = X __attribute__((address_space(1))) *;
+template <typename X>
+__device__ __forceinline__ gmut<X> as_global_mut(void *p) {
+  return (gmut<X>)(X *)p;
 }
 
 template <typename T> struct V2T;
@@ -802,18 +812,19 @@ __device__ __forceinline__ void trend_of(const T (&v)[PER], int n, int trend, Re
     b = 0;
     return;
   }
-  double sx = 0, sxi = 0;  // slots past the live end hold +0
+  double sx = 0, skx = 0;  // slots past the live end hold +0
 #pragma unroll
   for (int k = 0; k < PER; k++) {
-    int j = tid_ + NTH * k;
     sx += (double)v[k];
-    sxi += (double)j * (double)v[k];
+    skx += (double)k * (double)v[k];
   }
   if (trend == 1) {
     a = block_sum<NTH>(sx, red.next()) / (double)n;
     b = 0;
     return;
   }
+  // sum of j x_j over this thread's slots j = tid + NTH k
+  double sxi = (double)tid_ * sx + (double)NTH * skx;
   block_sum2<NTH>(sx, sxi, red.next());
   double nf = (double)n, mean = sx / nf;
   if (n == 1) {
@@ -839,12 +850,17 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   trend_of<T, NTH, PER>(v, n, uni(F.trend), red, a, b);
   const int p = uni(F.pad), nf = fn;
   const int reflect = uni(F.reflect);
-  // pack (packInto, filter/apply.nim:194-216): detrended data ...
+  // pack (packInto, filter/apply.nim:194-216): detrended data ...  The line
+  // a + b j at slot k is (a + b tid) + (b NTH) k.
+  {
+    const double a0 = a + b * (double)tid_, bs = b * (double)NTH;
+    T *bp = buf + p + tid_;
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    int j = tid_ + NTH * k;
-    if (j < n) buf[p + j] = (T)((double)v[k] - (a + b * (double)j));
-    PSEG_ONE_AT_A_TIME();
+    for (int k = 0; k < PER; k++) {
+      int j = tid_ + NTH * k;
+      if (j < n) bp[NTH * k] = (T)((double)v[k] - (a0 + bs * (double)k));
+      PSEG_ONE_AT_A_TIME();
+    }
   }
   if (!reflect) {
     for (int j = tid_; j < p; j += NTH) buf[j] = (T)0;
@@ -889,21 +905,27 @@ __device__ __forceinline__ void filter_step(const DevFilter &F, T (&v)[PER], int
   // transforms.  The indices come from a fresh thread id: reusing the pack's
   // would keep them live across both transforms.
   const int tid2 = opaque_tid();
-  const int last_slot = 2 * uni(f.stride) - 2;  // an even index inside the buffer
-  // (slots past the live end read any sample of the transform buffer)
+  // Element (p + j) ^ 1 for j = tid + NTH k is ((p + tid) ^ 1) + NTH k (NTH
+  // is even): one base, immediate offsets, and every read of the row in
+  // flight at once.  Slots past the live end read whatever lies there (a
+  // filter step's LDS covers p plus the widest row of NTH threads, plus 2).
+  {
+    const T *ob = out + ((p + tid2) ^ 1);
+#pragma unroll
+    for (int k = 0; k < PER; k++) v[k] = ob[NTH * k];
+  }
   if (plain) {  // no trend to add back (a 0 Hz gain of 0, or no trend)
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       int j = tid2 + NTH * k;
-      T y = out[min(p + j, last_slot) ^ 1];
-      v[k] = j < n ? y : (T)0;
+      v[k] = j < n ? v[k] : (T)0;
     }
   } else {
+    const double g0 = ga + gb * (double)tid2, gs = gb * (double)NTH;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       int j = tid2 + NTH * k;
-      T y = out[min(p + j, last_slot) ^ 1];
-      y = (T)((double)y + (ga + gb * (double)j));
+      T y = (T)((double)v[k] + (g0 + gs * (double)k));
       v[k] = j < n ? y : (T)0;
       PSEG_ONE_AT_A_TIME();
     }
@@ -1016,8 +1038,13 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   const int headCut = min(max(n - h, 0), first);
   const int lastBlock = (n - 1) / w * w;
   const int tailCut = min(max(lastBlock + h, last), n);
-  const double unit = mode == 0 ? 1 / (rms * rms) : 1 / rms;
-  const double unitW = unit / (double)w;
+  // The window sums run on x / rms (so x^2 * unit, unit = 1 / rms^2 in rms
+  // mode, |x| * unit, unit = 1 / rms, in mean |x| mode): every term is at
+  // most n, so the sums stored as T neither overflow nor fall into
+  // subnormals whatever the trace's amplitude, and the means are the
+  // reference's scaled means (gain/gain.nim:430-461) with no further scale.
+  const double ir = 1 / rms;
+  const T invW = (T)(1.0 / (double)w);
   // LDS, n + first + 1 values: s[0, n) holds x, then the suffix sums S in
   // place, then over s[0, n - w] the interior means (S[e] + P[e + w]) / w in
   // T; hp[0, headCut) holds the prefix sums P[h + 1 + i] the head range
@@ -1050,23 +1077,24 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   if (c <= w) {
     // A run holds at most one block start, at ks: the suffix part is the
     // terms before it, the prefix part the terms from it (the same sums, in
-    // the same order, as the general loop below).
+    // the same order, as the general loop below).  The run's samples are
+    // read at once (one wait), and the mode picks one of two loops.
     const int len = c1 - c0;
     const int ks = r0 == 0 ? 0 : w - r0;
     double lo = 0, hi = 0;
+    auto terms = [&](auto rms_mode) {
 #pragma unroll
-    for (int k = 0; k < RUN; k++) {
-      const int e = c0 + k;
-      double t = 0;
-      if (e < c1) {
-        const double x = (double)s[e];
-        t = mode == 0 ? x * x : fabs(x);
+      for (int k = 0; k < RUN; k++) {
+        const double x = (double)(c0 + k < c1 ? s[c0 + k] : (T)0) * ir;  // 0 past the run
+        const double t = decltype(rms_mode)::value ? x * x : fabs(x);
+        tv[k] = t;
+        if (k < ks) lo += t;
+        else hi += t;
+        PSEG_ONE_AT_A_TIME();
       }
-      tv[k] = t;
-      if (k < ks) lo += t;
-      else hi += t;
-      PSEG_ONE_AT_A_TIME();
-    }
+    };
+    if (mode == 0) terms(std::true_type{});
+    else terms(std::false_type{});
     const bool has = ks < len;
     pre.v = has ? hi : lo;
     pre.f = has;
@@ -1079,7 +1107,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
       const int e = c0 + k;
       double t = 0;
       if (e < c1) {
-        const double x = (double)s[e];
+        const double x = (double)s[e] * ir;
         t = mode == 0 ? x * x : fabs(x);
         const bool start = r == 0;
         // prefix: the sum since the last block start in the run
@@ -1152,7 +1180,7 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
         pj = r == 0 ? (T)0 : (T)run;
       }
       const int q = c0 + k + 1 - w;  // j - w
-      if (q >= 0) s[q] = (T)(((double)s[q] + (double)pj) * unitW);
+      if (q >= 0) s[q] = (s[q] + pj) * invW;
       if ((unsigned)(jh + k) < (unsigned)headCut) hp[jh + k] = pj;
       plast = pj;
     }
@@ -1166,16 +1194,21 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
   // 1 / count for a window of `count` samples: inv[c] = 1/c; the tail's
   // rev[m] = 1/(2h+1-m) is inv[2h+1-m], the same float64 value.
   gptr<double> inv = as_global<double>(uni_ptr(G.inv));
-  const double headWhole = headCut < first ? pn * unit * inv[n] : 0.0;
+  const double headWhole = headCut < first ? pn * inv[n] : 0.0;
   const int tid3 = opaque_tid();  // fresh: nothing per slot stays live across the scans
   const int wave0 = __builtin_amdgcn_readfirstlane(tid3);  // the wave's first thread
+  // s[i - h] of every slot, read at once: the interior mean, or the suffix
+  // sum an edge range adds (clamped into the row for slots that use neither)
+  T sm[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) sm[k] = s[min(max(tid3 + NTH * k - h, 0), n - 1)];
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int i = tid3 + NTH * k;
     const int lo = wave0 + NTH * k;  // the wave's samples: lo .. lo + 63
     if (lo >= first && lo + 63 < last) {
       // the whole wave in the interior range (most of the row): full windows
-      const T mean = s[i - h];
+      const T mean = sm[k];
       const T lvl = mode == 0 ? agc_sqrt(mean) : mean;
       const T gi = agc_div(kk, lvl + eps);
       v[k] = (T)(v[k] * gi);
@@ -1187,12 +1220,12 @@ __device__ __forceinline__ void agc_step(const DevAgc &G, T (&v)[PER], int n, ch
       const bool whole = i >= headCut && i < first, tail1 = i >= last && i < tailCut;
       T mean;
       if (mid) {
-        mean = s[i - h];
+        mean = sm[k];
       } else {
-        const double sv = i >= first ? (double)s[max(i - h, 0)] : 0.0;
+        const double sv = i >= first ? (double)sm[k] : 0.0;
         const double pv = head ? (double)hp[i] : (tail1 ? pn : 0.0);
         const int c = head ? i + h + 1 : n - i + h;
-        const double M = unit * inv[min(max(c, 1), w)];
+        const double M = inv[min(max(c, 1), w)];
         mean = (T)(whole ? headWhole : (sv + pv) * M);
       }
       const T lvl = mode == 0 ? agc_sqrt(mean) : mean;

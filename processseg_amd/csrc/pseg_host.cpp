@@ -1033,6 +1033,21 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
     return fail(PSEG_E_UNSUPPORTED, "the plan's FFTs are too long for the in-place transform "
                 "of one workgroup (at most 512 threads)");
   }
+  // a filter step's unpack reads element (pad + j) ^ 1 for every slot j of
+  // the launch, up to the widest row of the plan's threads per row (a batch
+  // or a served launch may hold wider rows of other plans)
+  for (int i = 0; i < n_ops; i++) {
+    if (H.steps[i].kind != kFilter) continue;
+    const size_t reach = (size_t)(P->step_pad[i] + pipeline_max_cols(P->threads) + 2) *
+                         (P->dtype == PSEG_F32 ? sizeof(float) : sizeof(double));
+    P->step_lds[i] = std::max(P->step_lds[i], reach);
+    if (P->step_lds[i] > kMaxRowLds) {
+      const size_t need = P->step_lds[i];
+      delete P;
+      return fail(PSEG_E_UNSUPPORTED, "op %d (%s): %zu bytes of LDS per row do not fit one "
+                  "workgroup (at most %zu)", i, op_name(ops[i].kind), need, kMaxRowLds);
+    }
+  }
   // an IIR step's scan: one affine map (a 2 x 2 matrix and a vector, six
   // float64) per thread after the padded row
   for (int i = 0; i < n_ops; i++) {

@@ -38,9 +38,12 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   const int tid_ = opaque_tid();
   const DevPlan *plan = J.plan;
   const int cols = J.cols;
-  const T *s = (const T *)J.src + r * J.src_pitch;
-  T *d = (T *)J.dst + r * J.dst_pitch;
-  const int num = J.num[r];
+  // the row's samples through global (not flat) pointers: the loads and
+  // stores take the row base in SGPRs and a 32-bit lane offset, and do not
+  // count against the LDS wait counter
+  gptr<T> s = as_global<T>((const T *)J.src + r * J.src_pitch);
+  gmut<T> d = as_global_mut<T>((T *)J.dst + r * J.dst_pitch);
+  const int num = as_global<int32_t>(J.num)[r];
   const int n_row = num < 0 ? -1 : min(max(num - J.first, 0), cols);
   const int n = n_row;
   T v[PER];
@@ -74,7 +77,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   const int c0 = J.crop0, c1 = J.crop1;
   if (J.src != J.dst || c0 != 0) {
     const int live = max(n, 0);
-    T *dd = d - c0;
+    gmut<T> dd = d - c0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       int j = tid_ + NTH * k;
@@ -201,7 +204,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   // the live samples
   {
     const int tid_ = opaque_tid();
-    T *dd = d - c0;
+    gmut<T> dd = d - c0;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       int j = tid_ + NTH * k;
