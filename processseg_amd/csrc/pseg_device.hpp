@@ -465,8 +465,18 @@ __host__ __device__ constexpr int first_pad_const(int m, int fuse) {
   return R.n > used ? (pair ? R.r[0] * R.r[1] : R.r[0]) : 0;
 }
 
+// A twiddle of iteration `it`: from the registers loaded early (EARLY, the
+// first iteration) or from the table.
+template <bool EARLY, typename T, int N>
+__device__ __forceinline__ V2<T> twiddle(int it, const V2<T> (&w)[N], int i, CTab<T> t, int j) {
+  if constexpr (EARLY) {
+    if (it == 0) return w[i];
+  }
+  return t[j];
+}
+
 template <typename T, int R1, int R2, int NTH, bool LAST, int M, int S, int SUB1, int PIN,
-          int POUT>
+          int POUT, bool EARLY>
 __device__ __forceinline__ void pass_static(V2<T> *__restrict__ x, CTab<T> tw1, CTab<T> tw2) {
   constexpr int R = R1 * R2;
   constexpr int SUB2 = SUB1 / R2;
@@ -479,6 +489,12 @@ __device__ __forceinline__ void pass_static(V2<T> *__restrict__ x, CTab<T> tw1, 
   constexpr int INP = S + PIN, OUTP = S * R + POUT;
   constexpr int T1STEP = SUB2 * (R1 - 1);
   V2<T> v[IT][R2][R1];
+  // EARLY: the first iteration's twiddles are loaded with the inputs, before
+  // the block waits, so their latency (an L2 read) passes during the wait
+  // instead of after it; the later iterations' as they run (registers).
+  constexpr int NT1 = EARLY && twiddle1 ? R2 * (R1 - 1) : 0;
+  constexpr int NT2 = EARLY && twiddle2 ? R2 - 1 : 0;
+  V2<T> w1[NT1 > 0 ? NT1 : 1], w2[NT2 > 0 ? NT2 : 1];
   {
     const int tid_ = opaque_tid();
 #pragma unroll
@@ -492,6 +508,20 @@ __device__ __forceinline__ void pass_static(V2<T> *__restrict__ x, CTab<T> tw1, 
         for (int j2 = 0; j2 < R2; j2++)
 #pragma unroll
           for (int j = 0; j < R1; j++) v[it][j2][j] = xi[j2 * IN2 + j * IN1];
+        if constexpr (EARLY) {
+          if (it == 0) {
+            CTab<T> t1 = tw1 + (p2 * (R1 - 1) - 1);
+#pragma unroll
+            for (int j2 = 0; j2 < R2; j2++)
+#pragma unroll
+              for (int k = 1; k < R1; k++)
+                if constexpr (twiddle1) w1[j2 * (R1 - 1) + k - 1] = t1[j2 * T1STEP + k];
+            CTab<T> t2 = tw2 + (p2 * (R2 - 1) - 1);
+#pragma unroll
+            for (int k2 = 1; k2 < R2; k2++)
+              if constexpr (twiddle2) w2[k2 - 1] = t2[k2];
+          }
+        }
       }
     }
   }
@@ -511,7 +541,8 @@ __device__ __forceinline__ void pass_static(V2<T> *__restrict__ x, CTab<T> tw1, 
         if constexpr (twiddle1) {
 #pragma unroll
           for (int k = 1; k < R1; k++)
-            v[it][j2][k] = cmul<T>(v[it][j2][k], t1[j2 * T1STEP + k]);
+            v[it][j2][k] = cmul<T>(v[it][j2][k], twiddle<EARLY>(it, w1, j2 * (R1 - 1) + k - 1,
+                                                                t1, j2 * T1STEP + k));
         }
       }
       if constexpr (R2 == 1) {
@@ -527,7 +558,8 @@ __device__ __forceinline__ void pass_static(V2<T> *__restrict__ x, CTab<T> tw1, 
           Bfly<T, R2>::run(u);
           if constexpr (twiddle2) {
 #pragma unroll
-            for (int k2 = 1; k2 < R2; k2++) u[k2] = cmul<T>(u[k2], t2[k2]);
+            for (int k2 = 1; k2 < R2; k2++)
+              u[k2] = cmul<T>(u[k2], twiddle<EARLY>(it, w2, k2 - 1, t2, k2));
           }
 #pragma unroll
           for (int k2 = 0; k2 < R2; k2++) yo[k * S + k2 * OUT2] = u[k2];
@@ -552,11 +584,11 @@ __device__ __forceinline__ void static_passes(const DevFft &f, V2<T> *a, CTab<T>
     CTab<T> t = tw + uni(f.tw_off[I]);
     if constexpr (pair) {
       CTab<T> t2 = tw + uni(f.tw_off[I + 1]);
-      pass_static<T, r, r2, NTH, I + 2 >= P.n, M, S, sub, pi, po>(a, t, t2);
+      pass_static<T, r, r2, NTH, I + 2 >= P.n, M, S, sub, pi, po, FUSE == kFusePipeline>(a, t, t2);
       __syncthreads();
       static_passes<T, NTH, FUSE, M, I + 2, S * r * r2>(f, a, tw);
     } else {
-      pass_static<T, r, 1, NTH, I + 1 >= P.n, M, S, sub, pi, po>(a, t, t);
+      pass_static<T, r, 1, NTH, I + 1 >= P.n, M, S, sub, pi, po, FUSE == kFusePipeline>(a, t, t);
       __syncthreads();
       static_passes<T, NTH, FUSE, M, I + 1, S * r>(f, a, tw);
     }
