@@ -119,6 +119,17 @@ typedef struct pseg_op {
    * ipMinimum (one causal pass, multiples of 6). */
   int32_t iir_phase;
   int32_t reserved2;
+  /* calibrateOp / uncalibrateOp (pipeline.nim:324-334, 444-447): the
+   * Calibration's rules (calib_n_rules of them, read during
+   * pseg_plan_compile only), its target (pseg_calib_target), how the file's
+   * own factors rank (pseg_file_factors) and whether auxiliary rows are
+   * converted to volts.  The rules' errors read "op i (opCalibrate): rule
+   * j: ..." as the reference's compile raises them. */
+  const struct pseg_calib_rule *calib_rules;
+  int32_t calib_n_rules;
+  int32_t calib_target;
+  int32_t calib_file_factors;
+  int32_t calib_scale_aux;
 } pseg_op;
 
 typedef struct pseg_plan pseg_plan;
@@ -142,6 +153,9 @@ typedef struct pseg_plan_info {
   size_t lds_bytes;        /* LDS one row of the fused kernel holds */
   int32_t threads_per_row; /* workgroup size of the fused kernel for this plan
                               (128, 256 or 512: rows up to 3072, 6144, 12288) */
+  int32_t amplitude_units; /* amplitudeUnits (pipeline.nim:571-577): 0 paRecorded,
+                              1 paCalibrated, 2 paRelative */
+  int32_t n_calib;         /* calibration steps (calibrateOp, uncalibrateOp) */
   int32_t reserved;
 } pseg_plan_info;
 
@@ -385,6 +399,105 @@ size_t pseg_last_error(char *buf, size_t n);
 /* Number of visible devices; 0 without a GPU.  Never fails. */
 int32_t pseg_device_count(void);
 
+/* ----------------------------------------------------------- calibration -- */
+
+/* AmplitudeUnit (core/units.nim:57-64), CalibrationTarget and FileFactors
+ * (core/calibrate.nim:104-115), CalibrationStatus and FactorSource
+ * (core/calibrate.nim:155-165), in the reference's order. */
+typedef enum pseg_amplitude_unit {
+  PSEG_AU_RECORDED = 0, PSEG_AU_VOLT = 1, PSEG_AU_VELOCITY = 2, PSEG_AU_ACCELERATION = 3,
+  PSEG_AU_PRESSURE = 4, PSEG_AU_FORCE = 5
+} pseg_amplitude_unit;
+typedef enum pseg_calib_target { PSEG_CAL_PHYSICAL = 0, PSEG_CAL_VOLTS = 1 } pseg_calib_target;
+typedef enum pseg_file_factors {
+  PSEG_FF_FALLBACK = 0, PSEG_FF_FIRST = 1, PSEG_FF_IGNORE = 2
+} pseg_file_factors;
+typedef enum pseg_calib_status {
+  PSEG_CS_CONVERTED = 0, PSEG_CS_UNRESOLVED = 1, PSEG_CS_KEPT = 2
+} pseg_calib_status;
+typedef enum pseg_factor_source {
+  PSEG_FS_NONE = 0, PSEG_FS_RULE = 1, PSEG_FS_FILE = 2
+} pseg_factor_source;
+
+/* CalibrationRule (core/calibrate.nim:133-153) with RuleMatch's criteria
+ * (-1 / NaN / has_station 0: any) and its factors in SI, as Descale.
+ * voltsPerCount and Sensitivity.voltsPerUnit give them (core/units.nim:
+ * 265-268, 347-349): NaN where the rule gives none. */
+typedef struct pseg_calib_rule {
+  int32_t sensor_type;      /* SEG-D sensor type 0 .. 255, -1: any */
+  int32_t component;        /* 0 .. 255, -1: any */
+  int32_t channel_set;      /* >= 0, -1: any */
+  int32_t has_station;      /* 0: any station */
+  double channel;           /* NaN: any */
+  double station_line, station_point;
+  int32_t station_index;    /* 0: any index */
+  int32_t sensitivity_unit; /* the quantity of the sensitivity (pseg_amplitude_unit) */
+  double descale;           /* volts per count */
+  double sensitivity;       /* volts per m/s, m/s², Pa or N */
+  double gain_db;           /* a recorder gain the descale leaves out */
+} pseg_calib_rule;
+
+/* What resolveRow reads of one row (core/calibrate.nim:430-497): its status,
+ * whether it failed, TraceKeys' sensorType, component, channel and receiver
+ * station, TraceRows.channelSetIndex, and the row's FileScale (core/units.nim:
+ * 105-117; NaN where the headers say nothing). */
+typedef struct pseg_calib_row {
+  uint8_t status;           /* 0 tsLive, 1 tsUnknown, 2 tsDead, 3 tsAuxiliary */
+  uint8_t failed;
+  uint8_t sensor_type;
+  uint8_t file_physical;    /* FileScale.physical */
+  int32_t component;
+  int32_t channel_set;      /* -1: none */
+  int32_t rcv_index;        /* 0: none */
+  double channel, rcv_line, rcv_point;
+  double file_descale, file_sensitivity;
+  int32_t file_unit;        /* pseg_amplitude_unit */
+  int32_t reserved;
+} pseg_calib_row;
+
+/* RowCalibration (core/calibrate.nim:167-182). */
+typedef struct pseg_row_calibration {
+  double factor;            /* 1 for a kept row, NaN for an unresolved one */
+  int32_t unit, status;     /* pseg_amplitude_unit, pseg_calib_status */
+  int32_t descale_from, sensitivity_from;     /* pseg_factor_source */
+  int32_t descale_rule, sensitivity_rule, gain_rule;  /* -1: none */
+  int32_t conflict, mismatch;
+} pseg_row_calibration;
+
+/* CalibrationSummary (core/calibrate.nim:184-193). */
+typedef struct pseg_calib_summary {
+  int32_t units[6];         /* rows by unit after calibration */
+  int32_t converted, unresolved, kept;
+  int32_t no_descale, no_sensitivity;
+  int32_t conflicts, mismatches;
+  int32_t reserved;
+} pseg_calib_summary;
+
+typedef struct pseg_calib pseg_calib;
+/* initCalibrationPlan (core/calibrate.nim:316-360): validates the rules
+ * (PSEG_E_INVALID "rule i: ..." with the reference's text) and indexes them. */
+pseg_status pseg_calib_compile(const pseg_calib_rule *rules, int32_t n_rules, int32_t target,
+                               int32_t file_factors, int32_t scale_auxiliary, pseg_calib **out);
+pseg_status pseg_calib_destroy(pseg_calib *cal);
+/* resolveInto (core/calibrate.nim:514-527): each row's RowCalibration into
+ * out (n_rows of them) and the counts into summary (may be NULL);
+ * has_file_scales: the meta carries a FileScale column (ChunkMeta.
+ * hasFileScales).  Host memory; allocates nothing. */
+pseg_status pseg_calib_resolve(const pseg_calib *cal, const pseg_calib_row *rows, int32_t n_rows,
+                               int32_t has_file_scales, pseg_row_calibration *out,
+                               pseg_calib_summary *summary);
+
+/* The multipliers of a plan's calibration steps for the rows of a block
+ * (calibrateImpl's k, core/calibrate.nim:615-624, from each step's
+ * resolveInto): factors[s * n_rows + r] for the s-th calibration step in
+ * the plan's order: the row's factor for calibrateOp, its inverse for
+ * uncalibrateOp, NaN for an unresolved row, 1 for a kept one.  Host arrays;
+ * factors may be NULL to ask for n_calib only; summaries NULL or one per
+ * calibration step. */
+pseg_status pseg_plan_calib_factors(const pseg_plan *plan, const pseg_calib_row *rows,
+                                    int32_t n_rows, int32_t has_file_scales, double *factors,
+                                    pseg_calib_summary *summaries, int32_t *n_calib);
+
 /* ------------------------------------------------------------------ meta -- */
 
 /* Replaces ChunkMeta's isLive / liveSamples / isFailed (core/types.nim:
@@ -392,8 +505,9 @@ int32_t pseg_device_count(void);
  * one host array: row_len[r] = clamp(num_samples[r] - first_sample, 0, cols)
  * for a row the op uses, else -1.  status: 0 tsLive, 1 tsUnknown, 2 tsDead,
  * 3 tsAuxiliary.  failed: ascending row indices (may be NULL when n_failed is
- * 0).  for_correlation != 0: rows not dead and not failed (auxiliary rows
- * included); else live or unknown rows that did not fail. */
+ * 0).  for_correlation == 1: rows not dead and not failed (auxiliary rows
+ * included); 2: every row (the live length a calibration step scales); 0:
+ * live or unknown rows that did not fail. */
 pseg_status pseg_meta_flatten(int32_t rows, int32_t cols, int64_t first_sample,
                               const int64_t *num_samples, const uint8_t *status,
                               const int64_t *failed, int32_t n_failed,
@@ -416,8 +530,9 @@ double pseg_response(const pseg_filter_spec *spec, double f_hz, double dt_ms);
 /* Pipeline.compile (pipeline.nim:466-523): validates every op for blocks of up
  * to `samples` samples at dt_ms and builds the device tables (twiddles, gains,
  * pad tapers, AGC tables) on the current device.  record_samples: 0, or the
- * trace length a window of `samples` is cut from.  Kinds outside the GPU path
- * (mutes, IIR, gain curves, calibration) give PSEG_E_UNSUPPORTED. */
+ * trace length a window of `samples` is cut from.  Every op kind runs on the
+ * GPU; PSEG_E_UNSUPPORTED is left for transforms beyond one workgroup's LDS
+ * (the caller keeps the CPU plan for such a flow). */
 pseg_status pseg_plan_compile(const pseg_op *ops, int32_t n_ops, int32_t dtype, double dt_ms,
                               int32_t samples, int32_t record_samples, pseg_plan **out);
 pseg_status pseg_plan_destroy(pseg_plan *plan);
@@ -458,6 +573,22 @@ pseg_status pseg_plan_apply_keys(const pseg_plan *plan, const void *src, void *d
                                  int32_t cols, const int32_t *row_len, const double *row_keys,
                                  int64_t first_sample, void *ws, size_t ws_bytes,
                                  void *stream);
+
+/* pseg_plan_apply_keys for a plan with calibration steps (calibrateOp,
+ * uncalibrateOp; pipeline.nim:686-688): calib_len holds every row's live
+ * length whatever its status (liveSamples, core/types.nim:226-230;
+ * pseg_meta_flatten with for_correlation = 2) and calib_factors the
+ * multipliers of pseg_plan_calib_factors, [n_calib][rows] float64, both in
+ * device memory.  A calibration step scales a row's live samples by its
+ * multiplier in T, dead and auxiliary rows included (pipeline.nim:44-45);
+ * a multiplier of 1 leaves the row as it is.  Every other step leaves rows
+ * with row_len < 0 alone.  The other apply entry points refuse a plan with
+ * calibration steps (PSEG_E_INVALID). */
+pseg_status pseg_plan_apply_calib(const pseg_plan *plan, const void *src, void *dst, int32_t rows,
+                                  int32_t cols, const int32_t *row_len, const double *row_keys,
+                                  int64_t first_sample, const int32_t *calib_len,
+                                  const double *calib_factors, void *ws, size_t ws_bytes,
+                                  void *stream);
 
 /* Decodes `rows` traces of big-endian samples on the device into a [rows,
  * cols] panel of `dtype` (C-contiguous): trace r's sample j is the word at

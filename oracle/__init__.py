@@ -991,3 +991,238 @@ def spectrogram_average(x: np.ndarray, row_len, window_len: int, hop: int, dt_ms
         n = nfft if nfft else 1 << (window_len - 1).bit_length()
         return np.zeros((frames, n // 2 + 1)), 0
     return np.sqrt(acc / used), used
+
+
+# ------------------------------------------------------------ calibration
+#
+# A restatement of core/calibrate.nim's plan and per-row resolution in plain
+# Python: rules as dicts with the keys sensorType, component, channelSet
+# (-1: any), channel (NaN: any), station (None or (line, point, index)),
+# descale (V/count, NaN: none), sensitivity (V per SI unit, NaN: none),
+# unit (the sensitivity's quantity: 2 m/s, 3 m/s², 4 Pa, 5 N), gainDb (NaN:
+# none).  Rows as dicts: status (0 live, 1 unknown, 2 dead, 3 aux), failed,
+# sensorType, component, channelSet (-1: none), channel, rcvLine, rcvPoint,
+# rcvIndex, and the FileScale fields fsDescale, fsSensitivity, fsUnit,
+# fsPhysical.
+
+_CAL_BITS = {"sensorType": 1, "component": 2, "channelSet": 4, "channel": 8, "station": 16}
+_F32_MAX = float(np.finfo(np.float32).max)
+
+
+def _hundredths(x):
+    # round half away from zero (Nim's round), core/calibrate.nim:225
+    y = x * 100
+    r = math.floor(abs(y) + 0.5) * (1 if y >= 0 else -1)
+    return r / 100 + 0.0
+
+
+def _rule_key(r):
+    """keyOf (core/calibrate.nim:256-264): (mask, the masked values)."""
+    mask, vals = 0, []
+    if r["sensorType"] >= 0:
+        mask |= 1
+        vals.append(("st", r["sensorType"]))
+    if r["component"] >= 0:
+        mask |= 2
+        vals.append(("co", r["component"]))
+    if r["channelSet"] >= 0:
+        mask |= 4
+        vals.append(("cs", r["channelSet"]))
+    if not math.isnan(r["channel"]):
+        mask |= 8
+        vals.append(("ch", _hundredths(r["channel"])))
+    if r["station"] is not None:
+        mask |= 16
+        line, point, index = r["station"]
+        vals.append(("sta", _hundredths(line), _hundredths(point), int(index)))
+    return mask, tuple(vals)
+
+
+def calib_plan(rules, target=0, file_factors=0, scale_aux=True):
+    """initCalibrationPlan (core/calibrate.nim:316-360); ValueError("rule i:
+    ...") as the reference raises it."""
+    p = {"target": target, "ff": file_factors, "aux": scale_aux, "catch": -1, "index": {},
+         "d": [], "s": [], "g": [], "u": []}
+    masks = set()
+    for i, r in enumerate(rules):
+        has_d, has_s, has_g = (not math.isnan(r["descale"]), not math.isnan(r["sensitivity"]),
+                               not math.isnan(r["gainDb"]))
+        if not (has_d or has_s or has_g):
+            raise ValueError(f"rule {i}: gives no factor (set a descale, a sensitivity or gainDb)")
+        lin = math.nan
+        if has_g:
+            if not math.isfinite(r["gainDb"]):
+                raise ValueError(f"rule {i}: gainDb must be finite, got {r['gainDb']}")
+            try:
+                lin = 10.0 ** (r["gainDb"] / 20)
+            except OverflowError:
+                lin = math.inf
+            if not (math.isfinite(lin) and lin > 0):
+                raise ValueError(f"rule {i}: gainDb = {r['gainDb']} is out of range")
+        if not -1 <= r["sensorType"] <= 255:
+            raise ValueError(f"rule {i}: sensorType must be -1 (any) or 0 .. 255")
+        if not -1 <= r["component"] <= 255:
+            raise ValueError(f"rule {i}: component must be -1 (any) or 0 .. 255")
+        if r["channelSet"] < -1:
+            raise ValueError(f"rule {i}: channelSet must be -1 (any) or ≥ 0")
+        if not math.isnan(r["channel"]) and not math.isfinite(r["channel"]):
+            raise ValueError(f"rule {i}: channel must be finite (NaN: any)")
+        if r["station"] is not None and not all(math.isfinite(v) for v in r["station"][:2]):
+            raise ValueError(f"rule {i}: the station's line and point must be finite")
+        key = _rule_key(r)
+        if key[0] == 0:
+            if p["catch"] >= 0:
+                raise ValueError(f"rule {i}: has no criteria, like rule {p['catch']}: one "
+                                 "catch-all only")
+            p["catch"] = i
+        else:
+            if key in p["index"]:
+                raise ValueError(f"rule {i}: matches the same rows as rule {p['index'][key]}")
+            p["index"][key] = i
+            masks.add(key[0])
+        p["d"].append(r["descale"] if has_d else math.nan)
+        p["s"].append(r["sensitivity"] if has_s else math.nan)
+        p["u"].append(r.get("unit", 0) if has_s else 0)
+        p["g"].append(lin)
+    p["classes"] = sorted(masks, reverse=True)
+    return p
+
+
+def _row_key(mask, row, index=None):
+    vals = []
+    if mask & 1:
+        vals.append(("st", int(row["sensorType"])))
+    if mask & 2:
+        vals.append(("co", int(row["component"])))
+    if mask & 4:
+        vals.append(("cs", int(row["channelSet"])))
+    if mask & 8:
+        vals.append(("ch", _hundredths(row["channel"])))
+    if mask & 16:
+        vals.append(("sta", _hundredths(row["rcvLine"]), _hundredths(row["rcvPoint"]), index))
+    return mask, tuple(vals)
+
+
+def _specific(p, row):
+    """specificRules (core/calibrate.nim:383-410)."""
+    f = {"d": -1, "s": -1, "g": -1}
+
+    def take(r):
+        for k in "dsg":
+            if f[k] < 0 and not math.isnan(p[k][r]):
+                f[k] = r
+    for mask in p["classes"]:
+        if mask & 4 and row["channelSet"] < 0:
+            continue
+        if mask & 8 and math.isnan(row["channel"]):
+            continue
+        if mask & 16:
+            if math.isnan(row["rcvLine"]) or math.isnan(row["rcvPoint"]):
+                continue
+            idx = int(row["rcvIndex"])
+            for t in ((idx, 0) if idx != 0 else (0, 1)):
+                r = p["index"].get(_row_key(mask, row, t), -1)
+                if r >= 0:
+                    take(r)
+        else:
+            r = p["index"].get(_row_key(mask, row), -1)
+            if r >= 0:
+                take(r)
+        if min(f.values()) >= 0:
+            break
+    return f
+
+
+def calib_resolve_row(p, row, has_file_scales=False):
+    """resolveRow (core/calibrate.nim:430-497): a dict with factor, unit,
+    status (0 converted, 1 unresolved, 2 kept), descaleFrom /
+    sensitivityFrom (0 none, 1 rule, 2 file), descaleRule, sensitivityRule,
+    gainRule, conflict, mismatch."""
+    res = {"factor": 1.0, "unit": 0, "status": 2, "descaleFrom": 0, "sensitivityFrom": 0,
+           "descaleRule": -1, "sensitivityRule": -1, "gainRule": -1, "conflict": False,
+           "mismatch": False}
+    if row.get("failed", False):
+        return res
+    aux = row["status"] == 3
+    live = row["status"] in (0, 1)
+    if aux and not p["aux"]:
+        return res
+    use_file = p["ff"] != 2 and has_file_scales
+    fd = row.get("fsDescale", math.nan) if use_file else math.nan
+    fsen = row.get("fsSensitivity", math.nan) if use_file else math.nan
+    fu = row.get("fsUnit", 0) if use_file else 0
+    fphys = bool(row.get("fsPhysical", False)) and use_file
+    spec = _specific(p, row)
+    c = p["catch"]
+
+    def rule_for(k):
+        if spec[k] >= 0:
+            return spec[k]
+        return c if c >= 0 and not math.isnan(p[k][c]) else -1
+    rd, rs, rg = rule_for("d"), rule_for("s"), rule_for("g")
+    unit, factor = 0, math.nan
+    phys_units = (2, 3, 4, 5)
+    if fphys:
+        res["conflict"] = rd >= 0 or rs >= 0 or rg >= 0
+        if p["target"] == 0 and not aux and math.isfinite(fd) and fd > 0:
+            factor, unit = fd, fu
+            res["descaleFrom"] = res["sensitivityFrom"] = 2
+    else:
+        file_d = math.isfinite(fd) and fd > 0
+        file_s = math.isfinite(fsen) and fsen > 0 and fu in phys_units
+
+        def pick(file_has, file_v, si, ri, k):
+            if p["ff"] == 1 and file_has:
+                return file_v, 2, -1
+            if si >= 0:
+                return p[k][si], 1, si
+            if file_has:
+                return file_v, 2, -1
+            if ri >= 0:
+                return p[k][ri], 1, ri
+            return math.nan, 0, -1
+        d, res["descaleFrom"], res["descaleRule"] = pick(file_d, fd, spec["d"], rd, "d")
+        want_s = not aux and p["target"] == 0
+        s = math.nan
+        s_unit = 0
+        if want_s:
+            s, res["sensitivityFrom"], res["sensitivityRule"] = pick(file_s, fsen, spec["s"], rs, "s")
+            s_unit = (fu if res["sensitivityFrom"] == 2 else
+                      p["u"][res["sensitivityRule"]] if res["sensitivityRule"] >= 0 else 0)
+
+        def differs(a, b):
+            return abs(a - b) > 1e-3 * max(abs(a), abs(b))
+        if file_d and rd >= 0 and differs(fd, p["d"][rd]):
+            res["conflict"] = True
+        if want_s and file_s and rs >= 0 and (fu != p["u"][rs] or differs(fsen, p["s"][rs])):
+            res["conflict"] = True
+        g = p["g"][rg] if rg >= 0 else 1.0
+        res["gainRule"] = rg
+        if want_s and not math.isnan(s):
+            st = int(row["sensorType"])
+            want = 4 if st == 1 else 2 if 2 <= st <= 5 else 3 if 6 <= st <= 9 else 0
+            res["mismatch"] = (want != 0 and want != s_unit) or (fu in phys_units and fu != s_unit)
+        if not math.isnan(d):
+            if aux or p["target"] == 1:
+                factor, unit = d / g, 1
+            elif not math.isnan(s):
+                factor, unit = d / (g * s), s_unit
+    if factor > 0 and factor <= _F32_MAX and 1 / factor <= _F32_MAX:
+        res.update(factor=factor, unit=unit, status=0)
+    elif live:
+        res.update(factor=math.nan, status=1)
+    return res
+
+
+def calibrate_rows(x: np.ndarray, k, live_len) -> np.ndarray:
+    """calibrateImpl's per-row scale (core/calibrate.nim:615-624, scaleRow
+    :562-574): row r's first live_len[r] samples times T(k[r]); k 1 leaves
+    the row as it is (a kept row is copied whole).  Returns a new array."""
+    out = np.array(x, copy=True)
+    for r in range(x.shape[0]):
+        if k[r] == 1.0:
+            continue
+        n = int(live_len[r])
+        with np.errstate(invalid="ignore", over="ignore"):
+            out[r, :n] = out[r, :n] * x.dtype.type(k[r])
+    return out

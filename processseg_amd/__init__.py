@@ -2,7 +2,7 @@
 
 The panel path (batched mixed-radix real FFT, zero-phase FFT filters with the
 reference's edge handling, demean, AGC, normalization and vibroseis
-correlation, the mutes and time-variant gain curves) runs in hand-written gfx950 kernels behind the C ABI of
+correlation, the mutes, time-variant gain curves and calibration) runs in hand-written gfx950 kernels behind the C ABI of
 include/pseg.h (libpseg.so).  This package mirrors the reference's Nim API
 over that ABI: initPipeline / compile / PipelinePlan.apply and
 initCorrelationPlan / correlateInto, plus PipelineBatch (many records with
@@ -19,7 +19,16 @@ from .pipeline import (  # noqa: F401
     decodeWindow, demeanOp, detrendOp, filterOp, float32, float64, gainDbOp, highCut, iirOp,
     initPipeline, lowCut, nextFastLen, normalizeOp, normMax, normPercentile, normRms, perPanel,
     perTrace, polarityOp, sameInterval, scaleOp, trLinear, trMean, trNone, tsAuxiliary, tsDead,
-    tsLive, tsUnknown)
+    tsLive, tsUnknown, calibrateOp, uncalibrateOp)
+from .calibrate import (  # noqa: F401
+    Calibration, CalibrationPlan, CalibrationRule, CalibrationSummary, Descale, FileScale,
+    RowCalibration, RuleMatch, Sensitivity, StationKey, auAcceleration, auForce, auPressure,
+    auRecorded, auVelocity, auVolt, csConverted, csKept, csUnresolved, ctPhysical, ctVolts,
+    descale, duMicroVPerCount, duMilliVPerCount, duNanoVPerCount, duVPerCount, ffFallback,
+    ffFirst, ffIgnore, fsFile, fsNone, fsRule, initCalibration, initCalibrationPlan,
+    initFileScale, openCircuitSensitivity, quantityOf, ratioDbInto, scaleRows, sensitivity,
+    stationKey, suDbReVPerMicroPa, suMicroVPerPa, suVPerBar, suVPerG, suVPerInPerS, suVPerKN,
+    suVPerLbf, suVPerMPerS, suVPerMPerS2, suVPerN, suVPerPa)
 from .correlate import (  # noqa: F401
     CorrelationPlan, Pilot, SweepSpec, SweepTaper, cmCorrelate, cmDeconvolve, generatePilot,
     initCorrelationPlan, initSweepTaper, linearSweep, samplesInLength, sweepSamples)

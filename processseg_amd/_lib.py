@@ -28,6 +28,15 @@ class FilterSpecC(C.Structure):
                 ("notch_hz", C.c_double * MAX_NOTCH), ("notch_width", C.c_double * MAX_NOTCH)]
 
 
+class CalibRuleC(C.Structure):
+    _fields_ = [("sensor_type", C.c_int32), ("component", C.c_int32),
+                ("channel_set", C.c_int32), ("has_station", C.c_int32),
+                ("channel", C.c_double), ("station_line", C.c_double),
+                ("station_point", C.c_double), ("station_index", C.c_int32),
+                ("sensitivity_unit", C.c_int32), ("descale", C.c_double),
+                ("sensitivity", C.c_double), ("gain_db", C.c_double)]
+
+
 class OpC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("trend", C.c_int32), ("mode", C.c_int32),
                 ("stat", C.c_int32), ("scope", C.c_int32), ("reserved", C.c_int32),
@@ -43,7 +52,32 @@ class OpC(C.Structure):
                 ("gain_power", C.c_double), ("gain_ref_ms", C.c_double),
                 ("gain_min_ms", C.c_double), ("gain_db_per_s", C.c_double),
                 ("gain_tmin_ms", C.c_double), ("gain_tmax_ms", C.c_double),
-                ("iir_phase", C.c_int32), ("reserved2", C.c_int32)]
+                ("iir_phase", C.c_int32), ("reserved2", C.c_int32),
+                ("calib_rules", C.POINTER(CalibRuleC)), ("calib_n_rules", C.c_int32),
+                ("calib_target", C.c_int32), ("calib_file_factors", C.c_int32),
+                ("calib_scale_aux", C.c_int32)]
+
+
+class CalibRowC(C.Structure):
+    _fields_ = [("status", C.c_uint8), ("failed", C.c_uint8), ("sensor_type", C.c_uint8),
+                ("file_physical", C.c_uint8), ("component", C.c_int32),
+                ("channel_set", C.c_int32), ("rcv_index", C.c_int32),
+                ("channel", C.c_double), ("rcv_line", C.c_double), ("rcv_point", C.c_double),
+                ("file_descale", C.c_double), ("file_sensitivity", C.c_double),
+                ("file_unit", C.c_int32), ("reserved", C.c_int32)]
+
+
+class RowCalibrationC(C.Structure):
+    _fields_ = [("factor", C.c_double), ("unit", C.c_int32), ("status", C.c_int32),
+                ("descale_from", C.c_int32), ("sensitivity_from", C.c_int32),
+                ("descale_rule", C.c_int32), ("sensitivity_rule", C.c_int32),
+                ("gain_rule", C.c_int32), ("conflict", C.c_int32), ("mismatch", C.c_int32)]
+
+
+class CalibSummaryC(C.Structure):
+    _fields_ = [("units", C.c_int32 * 6), ("converted", C.c_int32), ("unresolved", C.c_int32),
+                ("kept", C.c_int32), ("no_descale", C.c_int32), ("no_sensitivity", C.c_int32),
+                ("conflicts", C.c_int32), ("mismatches", C.c_int32), ("reserved", C.c_int32)]
 
 
 class PlanInfoC(C.Structure):
@@ -53,7 +87,8 @@ class PlanInfoC(C.Structure):
                 ("step_kind", C.c_int32 * MAX_OPS), ("step_nfft", C.c_int32 * MAX_OPS),
                 ("step_pad", C.c_int32 * MAX_OPS), ("step_margin", C.c_int32 * MAX_OPS),
                 ("step_reflect", C.c_int32 * MAX_OPS), ("lds_bytes", C.c_size_t),
-                ("threads_per_row", C.c_int32), ("reserved", C.c_int32)]
+                ("threads_per_row", C.c_int32), ("amplitude_units", C.c_int32),
+                ("n_calib", C.c_int32), ("reserved", C.c_int32)]
 
 
 class CorrInfoC(C.Structure):
@@ -174,6 +209,12 @@ def lib():
     L.pseg_plan_workspace_size.argtypes = [vp, i32, i32, C.POINTER(C.c_size_t)]
     L.pseg_plan_apply.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_size_t, vp]
     L.pseg_plan_apply_keys.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, C.c_size_t, vp]
+    L.pseg_plan_apply_calib.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, vp, vp,
+                                        C.c_size_t, vp]
+    L.pseg_calib_compile.argtypes = [vp, i32, i32, i32, i32, C.POINTER(vp)]
+    L.pseg_calib_destroy.argtypes = [vp]
+    L.pseg_calib_resolve.argtypes = [vp, vp, i32, i32, vp, vp]
+    L.pseg_plan_calib_factors.argtypes = [vp, vp, i32, i32, vp, vp, C.POINTER(i32)]
     L.pseg_decode_samples.argtypes = [i32, vp, C.c_int64, C.c_int64, i32, i32, i32, vp, vp]
     L.pseg_spectrum_compile.argtypes = [i32, f64, i32, f64, i32, i32, C.POINTER(vp)]
     L.pseg_spectrum_destroy.argtypes = [vp]

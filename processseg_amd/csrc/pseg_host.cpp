@@ -946,8 +946,22 @@ void compile_step(const pseg_op &op, double dt, int32_t samples, DevStep &st, Bl
       break;
     }
     case PSEG_OP_CALIBRATE:
-    case PSEG_OP_UNCALIBRATE:
-      throw Invalid{"UNSUPPORTED"};
+    case PSEG_OP_UNCALIBRATE: {
+      // initCalibrationPlan(op.calibration) (pipeline.nim:444-447): the
+      // rules are checked and indexed here; each apply resolves the rows'
+      // factors (pseg_plan_calib_factors) and the step multiplies by them
+      if (P.n_calib >= 2) invalid("at most one calibration and one uncalibration per pipeline");
+      auto cp = calib_new();
+      const std::string e = calib_compile(op.calib_rules, op.calib_n_rules, op.calib_target,
+                                          op.calib_file_factors, op.calib_scale_aux, *cp);
+      if (!e.empty()) throw Invalid{e};
+      st.kind = kCalibrate;
+      st.stat = P.n_calib;
+      P.calib[P.n_calib] = cp;
+      P.calib_inverse[P.n_calib] = op.kind == PSEG_OP_UNCALIBRATE;
+      P.n_calib++;
+      break;
+    }
     default:
       invalid("unknown op kind %d", op.kind);
   }
@@ -988,8 +1002,7 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   if (pipeline_threads_for(samples) == 0)
     return fail(PSEG_E_UNSUPPORTED, "rows of %d samples are beyond this GPU path (at most %d)",
                 samples, pipeline_max_cols(512));
-  pseg_plan *P = new pseg_plan;
-  memset(P, 0, sizeof *P);
+  pseg_plan *P = new pseg_plan();  // value-initialized: every plain field 0
   P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
   P->dt_ms = dt;
   P->samples = samples;
@@ -998,7 +1011,32 @@ pseg_status compile_plan(const pseg_op *ops, int32_t n_ops, double dt, int32_t s
   blob.bytes.resize(sizeof(DevPlan));
   DevPlan &H = P->host;
   memset(&H, 0, sizeof H);
+  // the calibration's place (pipeline.nim:495-511): before every op that
+  // changes amplitudes, one of each kind; and the output's amplitude units
+  int changed_at = -1, seen_cal[2] = {-1, -1};
   for (int i = 0; i < n_ops; i++) {
+    const int kind = ops[i].kind;
+    if (kind == PSEG_OP_CALIBRATE || kind == PSEG_OP_UNCALIBRATE) {
+      if (changed_at >= 0) {
+        delete P;
+        return fail(PSEG_E_INVALID, "op %d (%s): a calibration must come before every op that "
+                    "changes amplitudes; op %d (%s) does", i, op_name(kind), changed_at,
+                    op_name(ops[changed_at].kind));
+      }
+      int &seen = seen_cal[kind - PSEG_OP_CALIBRATE];
+      if (seen >= 0) {
+        const int first = seen;
+        delete P;
+        return fail(PSEG_E_INVALID, "op %d (%s): one per pipeline: op %d is one too", i,
+                    op_name(kind), first);
+      }
+      seen = i;
+      P->amplitude_units = kind == PSEG_OP_CALIBRATE ? 1 : 0;
+    } else if (kind == PSEG_OP_SCALE || kind == PSEG_OP_GAIN_DB || kind == PSEG_OP_CLIP ||
+               kind == PSEG_OP_AGC || kind == PSEG_OP_NORMALIZE || kind == PSEG_OP_GAIN_CURVE) {
+      if (changed_at < 0) changed_at = i;
+      if (P->amplitude_units == 1) P->amplitude_units = 2;
+    }
     try {
       compile_step<T>(ops[i], dt, samples, H.steps[i], blob, *P, i);
       if (P->step_lds[i] > kMaxRowLds)
@@ -1229,7 +1267,9 @@ pseg_status pseg_meta_flatten(int32_t rows, int32_t cols, int64_t first_sample,
     while (f < n_failed && failed[f] < r) f++;
     bool is_failed = f < n_failed && failed[f] == r;
     uint8_t s = status[r];
-    bool ok = for_correlation ? (!is_failed && s != 2) : (!is_failed && (s == 0 || s == 1));
+    bool ok = for_correlation == 2   ? true
+              : for_correlation != 0 ? (!is_failed && s != 2)
+                                     : (!is_failed && (s == 0 || s == 1));
     int64_t live = std::min<int64_t>(std::max<int64_t>(num_samples[r] - first_sample, 0), cols);
     row_len[r] = ok ? (int32_t)live : -1;
   }
@@ -1290,6 +1330,8 @@ pseg_status pseg_plan_info_get(const pseg_plan *P, pseg_plan_info *o) {
   }
   o->lds_bytes = lds;
   o->threads_per_row = P->threads;
+  o->amplitude_units = P->amplitude_units;
+  o->n_calib = P->n_calib;
   return PSEG_OK;
 }
 
@@ -1344,12 +1386,19 @@ pseg_status pseg_plan_apply(const pseg_plan *P, const void *src, void *dst, int3
   return pseg_plan_apply_keys(P, src, dst, rows, cols, row_len, nullptr, 0, ws, ws_bytes, stream);
 }
 
-pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst, int32_t rows,
-                                 int32_t cols, const int32_t *row_len, const double *row_keys,
-                                 int64_t first_sample, void *ws, size_t ws_bytes,
-                                 void *stream) {
+namespace {
+
+const char *kNeedsCalib = "the plan has a calibration step: apply it with the rows' factors "
+                          "(pseg_plan_apply_calib)";
+
+pseg_status apply_plan(const pseg_plan *P, const void *src, void *dst, int32_t rows, int32_t cols,
+                       const int32_t *row_len, const double *row_keys, int64_t first_sample,
+                       const int32_t *cal_len, const double *cal_k, void *ws, size_t ws_bytes,
+                       void *stream) {
   // checkApply (pipeline.nim:619-641): every check before any sample changes
   if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
+  if (P->n_calib > 0 && (!cal_len || !cal_k) && rows > 0)
+    return fail(PSEG_E_INVALID, "%s", kNeedsCalib);
   if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
   if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
   if (rows == 0) return PSEG_OK;
@@ -1381,6 +1430,10 @@ pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst,
     J.dst = dst;
     J.num = row_len;
     J.keys = row_keys;
+    if (P->n_calib > 0) {
+      J.cal_len = cal_len;
+      J.cal_k = cal_k;
+    }
     J.col0 = first_sample;
     J.src_pitch = J.dst_pitch = cols;
     J.rows = rows;
@@ -1414,6 +1467,27 @@ pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst,
   return PSEG_OK;
 }
 
+}  // namespace
+
+pseg_status pseg_plan_apply_keys(const pseg_plan *P, const void *src, void *dst, int32_t rows,
+                                 int32_t cols, const int32_t *row_len, const double *row_keys,
+                                 int64_t first_sample, void *ws, size_t ws_bytes,
+                                 void *stream) {
+  return apply_plan(P, src, dst, rows, cols, row_len, row_keys, first_sample, nullptr, nullptr, ws,
+                    ws_bytes, stream);
+}
+
+pseg_status pseg_plan_apply_calib(const pseg_plan *P, const void *src, void *dst, int32_t rows,
+                                  int32_t cols, const int32_t *row_len, const double *row_keys,
+                                  int64_t first_sample, const int32_t *calib_len,
+                                  const double *calib_factors, void *ws, size_t ws_bytes,
+                                  void *stream) {
+  if (P && P->n_calib > 0 && rows > 0 && (!calib_len || !calib_factors))
+    return fail(PSEG_E_INVALID, "NULL calib_len or calib_factors");
+  return apply_plan(P, src, dst, rows, cols, row_len, row_keys, first_sample, calib_len,
+                    calib_factors, ws, ws_bytes, stream);
+}
+
 pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *raw,
                                 int64_t pitch, int64_t offset, void *dst, int32_t rows,
                                 int32_t cols, const int32_t *row_len, const double *row_keys,
@@ -1428,6 +1502,9 @@ pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *
     if (P->host.steps[i].kind == kNormPanel)
       return fail(PSEG_E_UNSUPPORTED, "a plan with a per-panel normalization reads its panel "
                   "statistics from samples: decode first (pseg_decode_samples), then apply");
+  if (P->n_calib > 0)
+    return fail(PSEG_E_UNSUPPORTED, "a plan with a calibration step: decode first "
+                "(pseg_decode_samples), then pseg_plan_apply_calib");
   if (rows == 0 || cols == 0) return PSEG_OK;
   if (!raw || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL traces, panel or row_len");
   if (cols > P->samples)
@@ -1983,6 +2060,9 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
     if (P->n_parts != 1)
       return fail(PSEG_E_UNSUPPORTED, "item %d: a plan with a per-panel normalization runs on its "
                   "own (pseg_plan_apply)", i);
+    if (P->n_calib > 0)
+      return fail(PSEG_E_UNSUPPORTED, "item %d: a plan with a calibration step runs on its own "
+                  "(pseg_plan_apply_calib)", i);
   }
   // no item writes where another reads or writes
   for (int32_t i = 0; i < n; i++) {
@@ -2655,6 +2735,7 @@ pseg_status pseg_plan_apply_host(const pseg_plan *P, const void *src, void *dst,
                                  int64_t first_sample, void *stream) {
   // checkApply (pipeline.nim:619-641) before any copy
   if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
+  if (P->n_calib > 0 && rows > 0) return fail(PSEG_E_INVALID, "%s", kNeedsCalib);
   if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
   if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
   if (rows == 0) return PSEG_OK;

@@ -27,6 +27,7 @@ enum StepKind : int32_t {
   kMuteBottom = 10,
   kGainCurve = 11,
   kIir = 12,
+  kCalibrate = 13,    // calibrateOp and uncalibrateOp: a per-row multiplier
 };
 
 // Divide by a runtime constant with a multiply and a shift (Granlund-
@@ -149,7 +150,7 @@ struct DevIir {
 
 struct DevStep {
   int32_t kind;
-  int32_t stat;       // normalize
+  int32_t stat;       // normalize; calibration: which row of the job's multipliers
   double factor;      // scale
   double lo, hi;      // clip
   double target;      // normalize
@@ -191,6 +192,11 @@ struct Job {
   int32_t step_first, step_last, has_factor;
   int32_t src_fmt;    // 0: src holds T; else raw SEG-Y traces of this format,
                       // src_pitch bytes apart (the kernel form with every step)
+  // Calibration steps: every row's live length whatever its status (the
+  // same convention as num) and the multipliers, [calibration step][rows]
+  // (NULL when the plan has none).
+  const int32_t *cal_len;
+  const double *cal_k;
 };
 
 // One launch: a single job by value (n_jobs == 0) or n_jobs jobs in device

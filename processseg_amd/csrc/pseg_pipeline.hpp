@@ -45,7 +45,17 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
   gmut<T> d = as_global_mut<T>((T *)J.dst + r * J.dst_pitch);
   const int num = as_global<int32_t>(J.num)[r];
   const int n_row = num < 0 ? -1 : min(max(num - J.first, 0), cols);
-  const int n = n_row;
+  // A calibration step scales every row's live part, dead and auxiliary
+  // rows included (pipeline.nim:44-45): n_all, the samples any step may
+  // change, is that length on a plan with one, else the live length.
+  int n_all = n_row;
+  if constexpr (FORM >= 1) {
+    if (J.cal_len != nullptr) {
+      const int cl = as_global<int32_t>(J.cal_len)[r];
+      n_all = max(n_row, min(max(cl - J.first, 0), cols));
+    }
+  }
+  const int n = n_all;
   T v[PER];
   bool raw = false;
   if constexpr (FORM >= 1) raw = uni(J.src_fmt) != 0;
@@ -90,7 +100,7 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
     v[k] = j < n ? v[k] : (T)0;
   }
   if (n > 0) {
-    if (J.has_factor) {
+    if (J.has_factor && n_row > 0) {
       // a per-panel factor at the start of the part (pipeline.nim:717-718)
       const double f = *J.factor;
       if (!isnan(f)) {
@@ -104,6 +114,25 @@ __device__ __forceinline__ void run_row(const Job &J, int64_t r, char *lds, RedS
     }
     for (int i = J.step_first; i < J.step_last; i++) {
       const DevStep &st = plan->steps[i];
+      if constexpr (FORM >= 1) {
+        if (uni(st.kind) == kCalibrate) {
+          // calibrate / uncalibrate (core/calibrate.nim:582-593, 615-624):
+          // the row's live part times its multiplier in T; 1 keeps the row
+          const double kd = uni(as_global<double>(J.cal_k)[(int64_t)uni(st.stat) * J.rows + r]);
+          if (kd != 1.0) {
+            const int tid_ = opaque_tid();
+            const int nc = opaque_s(uni(n_all));
+            const T kk = (T)kd;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+              int j = tid_ + NTH * k;
+              if (j < nc) v[k] = v[k] * kk;
+            }
+          }
+          continue;
+        }
+        if (n_row <= 0) continue;  // every other step: live rows only
+      }
       const int n = opaque_s(uni(n_row));
       switch (uni(st.kind)) {
         case kDemean: {
@@ -275,6 +304,8 @@ __global__ __launch_bounds__(NTH, sizeof(T) == 4 ? PSEG_F32_WAVES : 1) void pipe
       J.step_first = uni(G.step_first);
       J.step_last = uni(G.step_last);
       J.has_factor = uni(G.has_factor);
+      J.cal_len = uni_ptr(G.cal_len);
+      J.cal_k = uni_ptr(G.cal_k);
       row = item - J.item0;
     }
     (void)jp;

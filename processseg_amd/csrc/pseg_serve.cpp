@@ -464,11 +464,16 @@ pseg_status pseg_server_add_flow(pseg_server *S, const pseg_op *ops, int32_t n_o
   if (!S || !flow) return set_error(PSEG_E_INVALID, "NULL argument");
   if (n_ops < 0 || n_ops > PSEG_MAX_OPS || (n_ops > 0 && !ops))
     return set_error(PSEG_E_INVALID, fmt("a flow has 0 .. %d ops", PSEG_MAX_OPS));
-  for (int i = 0; i < n_ops; i++)
+  for (int i = 0; i < n_ops; i++) {
     if (ops[i].kind == PSEG_OP_NORMALIZE && ops[i].scope == 1)
       return set_error(PSEG_E_UNSUPPORTED,
                        fmt("op %d (normalize): a per-panel normalization is served by "
                            "pseg_plan_apply, not by the batching server", i));
+    if (ops[i].kind == PSEG_OP_CALIBRATE || ops[i].kind == PSEG_OP_UNCALIBRATE)
+      return set_error(PSEG_E_UNSUPPORTED,
+                       fmt("op %d: a calibration is applied by pseg_plan_apply_calib, not by "
+                           "the batching server", i));
+  }
   int32_t m = 0;
   pseg_status st = pseg_required_margin(ops, n_ops, S->cfg.dt_ms, &m);
   if (st != PSEG_OK) return st;

@@ -427,6 +427,7 @@ class Op:
     curve: Optional[GainCurve] = None  # gainCurve, tabulated over t_min_ms .. t_max_ms
     t_min_ms: float = 0.0
     t_max_ms: float = math.nan
+    calibration: Optional[object] = None  # calibrate, uncalibrate (calibrate.Calibration)
 
     def to_c(self) -> _lib.OpC:
         o = _lib.OpC()
@@ -449,6 +450,19 @@ class Op:
             mute_fields(o, self.mute, self.taper_ms)
         if self.curve is not None:
             gain_fields(o, self.curve, self.t_min_ms, self.t_max_ms)
+        if self.calibration is not None:
+            cal = self.calibration
+            n = len(cal.rules)
+            # the struct keeps the rule array it points to alive
+            if n:
+                arr = (_lib.CalibRuleC * n)()
+                for i, r in enumerate(cal.rules):
+                    arr[i] = r.to_c()
+                o.calib_rules = arr
+            o.calib_n_rules = n
+            o.calib_target = int(cal.target)
+            o.calib_file_factors = int(cal.fileFactors)
+            o.calib_scale_aux = int(bool(cal.scaleAuxiliary))
         return o
 
 
@@ -550,6 +564,19 @@ def iirOp(spec: FilterSpec, phase: int = 0, trend: int = trLinear) -> Op:
     return Op(opIir, spec=spec, trend=trend, phase=int(phase))
 
 
+def calibrateOp(cal) -> Op:
+    """calibrateOp (pipeline.nim:324-329): recorded values to volts or
+    physical units per row (a calibrate.Calibration); it must come before
+    every op that changes amplitudes."""
+    return Op(opCalibrate, calibration=cal)
+
+
+def uncalibrateOp(cal) -> Op:
+    """uncalibrateOp (pipeline.nim:331-334): rows in `cal`'s units back to
+    recorded values."""
+    return Op(opUncalibrate, calibration=cal)
+
+
 def _ops_c(ops: Sequence[Op]):
     arr = (_lib.OpC * max(1, len(ops)))()
     for i, op in enumerate(ops):
@@ -570,7 +597,8 @@ class ChunkMeta:
 
     def __init__(self, rows: int, dtMs: float, numSamples=None, status=None,
                  failed: Iterable[int] = (), firstSample: int = 0, t0Ms=None, offset=None,
-                 traceKeys=None):
+                 traceKeys=None, sensorType=None, component=None, channel=None, rcvLine=None,
+                 rcvPoint=None, rcvIndex=None, channelSet=None, scales=None):
         if not (isinstance(dtMs, (int, float)) and math.isfinite(dtMs) and dtMs > 0):
             raise ValueError(f"dtMs must be finite and positive, got {dtMs}")
         if firstSample < 0:
@@ -595,19 +623,69 @@ class ChunkMeta:
                           else np.ascontiguousarray(traceKeys, np.float64))
         if self.traceKeys is not None and self.traceKeys.shape != (rows, 10):
             raise ValueError(f"traceKeys must be [rows, 10], got {self.traceKeys.shape}")
+        # what a calibration reads (TraceKeys.sensorType, component, channel,
+        # rcvLine / rcvPoint / rcvIndex; TraceRows.channelSetIndex; the
+        # optional FileScale column, core/calibrate.nim:430-497)
+
+        def col(v, fill, dt):
+            a = np.full(rows, fill, dt) if v is None else np.ascontiguousarray(v, dt).reshape(rows)
+            return a
+        self.sensorType = col(sensorType, 0, np.uint8)
+        self.component = col(component, 0, np.int32)
+        self.channel = col(channel, np.nan, np.float64)
+        self.rcvLine = col(rcvLine, np.nan, np.float64)
+        self.rcvPoint = col(rcvPoint, np.nan, np.float64)
+        self.rcvIndex = col(rcvIndex, 0, np.int32)
+        self.channelSet = col(channelSet, -1, np.int32)
+        self.scales = None if scales is None else list(scales)
+        if self.scales is not None and len(self.scales) != rows:
+            raise ValueError(f"scales needs one FileScale per row: {rows} rows, "
+                             f"{len(self.scales)} given")
         self._cache = {}
 
     def __len__(self):
         return self.rows
 
-    def row_len(self, cols: int, for_correlation: bool = False) -> np.ndarray:
+    def row_len(self, cols: int, for_correlation=False) -> np.ndarray:
+        """for_correlation: False / 0 live rows, True / 1 rows a correlation
+        uses, 2 every row (the length a calibration step scales)."""
         out = np.zeros(self.rows, np.int32)
         fl = self.failed if self.failed.size else np.zeros(1, np.int64)
         _lib.check(_lib.lib().pseg_meta_flatten(
             self.rows, cols, self.firstSample, self.numSamples.ctypes.data,
             self.status.ctypes.data, fl.ctypes.data, int(self.failed.size),
-            1 if for_correlation else 0, out.ctypes.data))
+            int(for_correlation), out.ctypes.data))
         return out
+
+    @property
+    def hasFileScales(self) -> bool:
+        return self.scales is not None
+
+    def isFailed(self, row: int) -> bool:
+        return bool(np.any(self.failed == row))
+
+    def calib_rows(self):
+        """The rows as resolveRow reads them (pseg_calib_row per row)."""
+        n = self.rows
+        arr = (_lib.CalibRowC * max(1, n))()
+        failed = set(int(f) for f in self.failed)
+        for r in range(n):
+            c = arr[r]
+            c.status = int(self.status[r])
+            c.failed = 1 if r in failed else 0
+            c.sensor_type = int(self.sensorType[r])
+            c.component = int(self.component[r])
+            c.channel_set = int(self.channelSet[r])
+            c.rcv_index = int(self.rcvIndex[r])
+            c.channel = float(self.channel[r])
+            c.rcv_line = float(self.rcvLine[r])
+            c.rcv_point = float(self.rcvPoint[r])
+            fs = self.scales[r] if self.scales is not None else None
+            c.file_descale = fs.descale if fs is not None else math.nan
+            c.file_sensitivity = fs.sensitivity if fs is not None else math.nan
+            c.file_unit = fs.unit if fs is not None else 0
+            c.file_physical = 1 if (fs is not None and fs.physical) else 0
+        return arr
 
     def keys(self) -> np.ndarray:
         """(t0Ms, offset) per row, [rows, 2] float64 (meta.keys(row))."""
@@ -622,9 +700,9 @@ class ChunkMeta:
             self._cache[key] = t
         return t
 
-    def device_row_len(self, cols: int, device, for_correlation: bool = False):
+    def device_row_len(self, cols: int, device, for_correlation=False):
         import torch
-        key = (cols, str(device), for_correlation)
+        key = (cols, str(device), int(for_correlation))
         t = self._cache.get(key)
         if t is None:
             t = torch.from_numpy(self.row_len(cols, for_correlation)).to(device)
@@ -780,6 +858,9 @@ class PipelinePlan:
                            reflect=bool(info.step_reflect[i])) for i in range(n)]
         # mutes read the rows' trace keys (t0Ms, offset) from the meta
         self.readsKeys = any(op.kind in (opMuteTop, opMuteBottom, opGainCurve) for op in flow.ops)
+        # amplitudeUnits (pipeline.nim:571-577): paRecorded, paCalibrated, paRelative
+        self.amplitudeUnits = int(info.amplitude_units)
+        self.nCalib = int(info.n_calib)
         self._ws = {}
 
     def __len__(self):
@@ -802,13 +883,26 @@ class PipelinePlan:
             self._ws[key] = ws
         return ws, need
 
+    def calibFactors(self, meta: ChunkMeta) -> np.ndarray:
+        """The multipliers of the plan's calibration steps for the rows of
+        `meta`, [nCalib, rows] float64 (pseg_plan_calib_factors)."""
+        n = len(meta)
+        out = np.zeros((max(self.nCalib, 1), max(n, 1)), np.float64)
+        cnt = C.c_int32()
+        _lib.check(_lib.lib().pseg_plan_calib_factors(
+            self._h, meta.calib_rows(), n, int(meta.hasFileScales), out.ctypes.data, None,
+            C.byref(cnt)))
+        return out[:self.nCalib, :n]
+
     def apply(self, src, dst=None, meta: Optional[ChunkMeta] = None, *, row_len=None,
-              ws=None, stream=None, keys=None, first_sample: int = 0):
+              ws=None, stream=None, keys=None, first_sample: int = 0, calib_factors=None):
         """apply(src, dst, meta) (src -> dst) or apply(samples, meta) (in
         place), pipeline.nim:736-772.  `row_len` (a device int32 tensor) may
         replace `meta`; then `keys` ([rows, 2] float64 device tensor of
         t0Ms, offset) and `first_sample` give what mutes and gain curves read
-        from a meta.
+        from a meta.  A plan with calibration steps needs the meta (each
+        row's factors resolve from its keys; ``calib_factors``, [nCalib,
+        rows], replaces them).
         Asynchronous on `stream` (default: torch's current)."""
         import torch
         if isinstance(dst, ChunkMeta) and meta is None:
@@ -841,6 +935,23 @@ class PipelinePlan:
             ws, need = self._workspace(rows, cols, src.device)
         else:
             need = ws.numel()
+        if self.nCalib > 0:
+            if meta is None:
+                raise ValueError("a plan with a calibration step needs the ChunkMeta (the rows' "
+                                 "keys give their factors)")
+            k = (self.calibFactors(meta) if calib_factors is None
+                 else np.ascontiguousarray(calib_factors, np.float64))
+            if k.shape != (self.nCalib, rows):
+                raise ValueError(f"calibration factors must be [{self.nCalib}, {rows}]")
+            kd = torch.from_numpy(k.copy()).to(src.device)
+            cl = meta.device_row_len(cols, src.device, 2)
+            self._hold = (kd, cl)  # alive until the next apply on this plan
+            _lib.check(_lib.lib().pseg_plan_apply_calib(
+                self._h, src.data_ptr(), dst.data_ptr(), rows, cols, row_len.data_ptr(),
+                keys.data_ptr() if keys is not None else None, int(first_sample),
+                cl.data_ptr(), kd.data_ptr(), ws.data_ptr() if ws is not None else None, need,
+                _stream_handle(stream)))
+            return dst
         _lib.check(_lib.lib().pseg_plan_apply_keys(
             self._h, src.data_ptr(), dst.data_ptr(), rows, cols, row_len.data_ptr(),
             keys.data_ptr() if keys is not None else None, int(first_sample),

@@ -268,5 +268,8 @@ def test_compile_errors_name_the_op(oracle):
         assert str(e.value).startswith(prefix), str(e.value)
     with pytest.raises(ValueError):
         oracle.plan_info([P.agcOp(100)], np.float32, 2.0, 0)
-    with pytest.raises(P.UnsupportedOp):
-        P.initPipeline([P.demeanOp(), P.Op(P.pipeline.opCalibrate)]).compile(np.float32, 2.0, 1000)
+    # a calibration after an op that changes amplitudes (pipeline.nim:495-511)
+    with pytest.raises(ValueError) as e:
+        P.initPipeline([P.agcOp(100), P.calibrateOp(P.initCalibration())]).compile(
+            np.float32, 2.0, 1000)
+    assert str(e.value).startswith("op 1 (opCalibrate): a calibration must come before"), str(e.value)
