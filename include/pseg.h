@@ -296,6 +296,38 @@ pseg_status pseg_hilbert_analytic(const pseg_hilbert_plan *plan, const void *src
                                   double floor_db, double *env, double *phase, double *hz,
                                   void *stream);
 
+/* ------------------------------------------------------------------- f-k -- */
+
+/* An FkPlan (fft/fk.nim:60-106): the f-k amplitude of `traces` live rows by
+ * windows of `samples` columns at dt_ms, tapered over taper_fraction (0 ..
+ * 0.5) of each axis; nTime = nextFastLen(samples), nK = nextFastLen(2·traces).
+ * PSEG_E_INVALID for traces < 4, samples < 8 and the reference's other
+ * refusals; PSEG_E_UNSUPPORTED when a transform exceeds one workgroup. */
+typedef struct pseg_fk_plan pseg_fk_plan;
+pseg_status pseg_fk_compile(int32_t dtype, double dt_ms, int32_t traces, int32_t samples,
+                            double taper_fraction, pseg_fk_plan **plan);
+pseg_status pseg_fk_destroy(pseg_fk_plan *plan);
+/* FkAxes (fft/fk.nim:50-58): frequency bins, wavenumber bins, Hz per bin,
+ * cycles per trace per wavenumber bin. */
+pseg_status pseg_fk_axes(const pseg_fk_plan *plan, int32_t *n_freq, int32_t *n_k, double *df_hz,
+                         double *dk_per_trace);
+/* The device workspace pseg_fk_amplitude needs (the half spectra of the
+ * traces between the two passes, and a flag). */
+pseg_status pseg_fk_workspace_size(const pseg_fk_plan *plan, size_t *bytes);
+/* fkAmplitudeInto (fft/fk.nim:438-504): the unscaled f-k modulus of the
+ * window first_col ..< first_col + samples of src ([rows, cols]) into amp
+ * (device, n_k·n_freq float64, frequency fastest, zero wavenumber at n_k/2,
+ * later-on-later events at positive k).  live_rows: device int32, the row
+ * index of each of the plan's `traces` rows that enter (fkRows, fft/fk.nim:
+ * 204-216: live rows with a sample at first_col or later, in order);
+ * row_len: device, pseg_meta_flatten's.  The modulus is sqrt(re² + im²) in
+ * float64, or hypot when a float64 sample of the window leaves [1e-100,
+ * 1e140] (fft/fk.nim:164-202).  Asynchronous on `stream`. */
+pseg_status pseg_fk_amplitude(const pseg_fk_plan *plan, const void *src, int32_t rows,
+                              int32_t cols, const int32_t *row_len, const int32_t *live_rows,
+                              int32_t first_col, double *amp, void *ws, size_t ws_bytes,
+                              void *stream);
+
 /* --------------------------------------------------------- first breaks -- */
 
 /* FirstBreakParams (qc/firstbreak.nim:146-170): method 0 pmStaLta,

@@ -1226,3 +1226,93 @@ def calibrate_rows(x: np.ndarray, k, live_len) -> np.ndarray:
         with np.errstate(invalid="ignore", over="ignore"):
             out[r, :n] = out[r, :n] * x.dtype.type(k[r])
     return out
+
+
+# -------------------------------------------------------------------- f-k
+
+def _tukey_fk(n: int, fraction: float, dtype):
+    """tukey (fft/fk.nim:70-80): 1 in the middle, 0.5 - 0.5 cos(pi (d + 0.5)
+    / ramp) over ramp = round(fraction n) samples at each end, as T."""
+    ramp = 0 if fraction <= 0 else _round_half_away(fraction * n)
+    w = np.ones(n)
+    for i in range(n):
+        d = min(i, n - 1 - i)
+        if ramp > 0 and d < ramp:
+            w[i] = 0.5 - 0.5 * math.cos(math.pi * (d + 0.5) / ramp)
+    return w.astype(dtype)
+
+
+def fk_plan(traces: int, samples: int, dt_ms: float, taper_fraction: float = 0.1,
+            dtype=np.float64) -> dict:
+    """initFkPlan (fft/fk.nim:82-106): the time and wavenumber lengths, the
+    axes and the tapers; ValueError as the reference raises it."""
+    if traces < 4:
+        raise ValueError(f"an f-k gather needs ≥ 4 live rows, got {traces}")
+    if samples < 8:
+        raise ValueError(f"an f-k window needs ≥ 8 samples, got {samples}")
+    if not (dt_ms > 0 and math.isfinite(dt_ms)):
+        raise ValueError(f"dtMs must be finite and > 0, got {dt_ms}")
+    if not (0 <= taper_fraction <= 0.5):
+        raise ValueError(f"taperFraction must be in 0 .. 0.5, got {taper_fraction}")
+    n_time = next_fast_len(samples)
+    n_k = next_fast_len(2 * traces)
+    return dict(traces=traces, samples=samples, dt_ms=dt_ms, dtype=np.dtype(dtype),
+                nTime=n_time, nK=n_k, nFreq=n_time // 2 + 1, dfHz=1000 / (n_time * dt_ms),
+                dkPerTrace=1 / n_k, timeW=_tukey_fk(samples, taper_fraction, dtype),
+                traceW=_tukey_fk(traces, taper_fraction, dtype))
+
+
+def _leaves_f64_fk(x: np.ndarray) -> bool:
+    """leavesFloat64Range (fft/fk.nim:164-181): NaN, infinite, subnormal,
+    below 1e-100 or above 1e140 (zero allowed)."""
+    a = np.abs(x.astype(np.float64))
+    return bool(np.any(~np.isfinite(a) | (a > 1e140) | ((a < 1e-100) & (a != 0))))
+
+
+def fk_amplitude(p: dict, x: np.ndarray, row_len, first_col: int) -> tuple:
+    """fkAmplitudeInto (fft/fk.nim:438-504): returns (amp [nK * nFreq]
+    float64, rows used).  row_len: the live length of each live row, -1 for
+    the others; a live row enters when its live length passes first_col.
+    The time pass is the oracle's real FFT of each tapered trace in T; the
+    spatial pass two real FFTs (the real and imaginary parts across traces)
+    combined with the sign flip, the modulus in float64 (hypot when a
+    float64 sample leaves range)."""
+    T = p["dtype"].type
+    x = np.asarray(x)
+    live = [r for r in range(x.shape[0]) if row_len[r] >= 0 and row_len[r] > first_col]
+    if len(live) != p["traces"]:
+        raise ValueError(f"the window has {len(live)} live rows, the plan was built for "
+                         f"{p['traces']}")
+    n_time, n_k, n_freq, S = p["nTime"], p["nK"], p["nFreq"], p["samples"]
+    spec_re = np.zeros((len(live), n_freq), p["dtype"])
+    spec_im = np.zeros((len(live), n_freq), p["dtype"])
+    extreme = False
+    for ti, r in enumerate(live):
+        last = min(int(row_len[r]), first_col + S)
+        seg = x[r, first_col:last].astype(p["dtype"])
+        if p["dtype"] == np.float64 and _leaves_f64_fk(seg):
+            extreme = True
+        lane = np.zeros(n_time, p["dtype"])
+        with np.errstate(invalid="ignore", over="ignore"):
+            lane[:last - first_col] = (seg * p["timeW"][:last - first_col]) * p["traceW"][ti]
+        spec_re[ti], spec_im[ti] = rfft(lane)
+    amp = np.zeros(n_k * n_freq)
+    half = n_k // 2
+    for jf in range(n_freq):
+        a = np.zeros(n_k, p["dtype"])
+        b = np.zeros(n_k, p["dtype"])
+        a[:len(live)] = spec_re[:, jf]
+        b[:len(live)] = spec_im[:, jf]
+        ar, ai = (v.astype(np.float64) for v in rfft(a))
+        br, bi = (v.astype(np.float64) for v in rfft(b))
+        for ik in range(n_k):
+            raw = half - ik
+            sk = raw if raw >= 0 else raw + n_k
+            if sk <= half:
+                re, im = float(ar[sk] - bi[sk]), float(ai[sk] + br[sk])
+            else:
+                k = n_k - sk
+                re, im = float(ar[k] + bi[k]), float(br[k] - ai[k])
+            amp[ik * n_freq + jf] = (math.hypot(re, im) if extreme
+                                     else math.sqrt(re * re + im * im))
+    return amp, len(live)

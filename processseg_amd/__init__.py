@@ -36,6 +36,7 @@ from .batch import BatchItem, PipelineBatch  # noqa: F401
 from .spectrum import SpectrumPlan, initSpectrumPlan  # noqa: F401
 from .hilbert import HilbertPlan, initHilbertPlan  # noqa: F401
 from .spectrogram import SpectrogramPlan, initSpectrogramPlan  # noqa: F401
+from .fk import FkAxes, FkPlan, fkRows, initFkPlan  # noqa: F401
 from .fold import (  # noqa: F401
     Binning, FoldMap, FoldOptions, bnCdp, bnGrid, bnInlineCrossline, bnLine, cdpBinning,
     foldDifference, gridBinning, initFoldMap, inlineCrosslineBinning, lineBinning, traceKeyTable)

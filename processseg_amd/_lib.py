@@ -211,6 +211,12 @@ def lib():
     L.pseg_plan_apply_keys.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, C.c_size_t, vp]
     L.pseg_plan_apply_calib.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.c_int64, vp, vp, vp,
                                         C.c_size_t, vp]
+    L.pseg_fk_compile.argtypes = [i32, f64, i32, i32, f64, C.POINTER(vp)]
+    L.pseg_fk_destroy.argtypes = [vp]
+    L.pseg_fk_axes.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(f64),
+                               C.POINTER(f64)]
+    L.pseg_fk_workspace_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+    L.pseg_fk_amplitude.argtypes = [vp, vp, i32, i32, vp, vp, i32, vp, vp, C.c_size_t, vp]
     L.pseg_calib_compile.argtypes = [vp, i32, i32, i32, i32, C.POINTER(vp)]
     L.pseg_calib_destroy.argtypes = [vp]
     L.pseg_calib_resolve.argtypes = [vp, vp, i32, i32, vp, vp]

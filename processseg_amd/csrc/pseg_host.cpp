@@ -2785,3 +2785,160 @@ pseg_status pseg_host_staging_release(void) {
 }
 
 }  // extern "C"
+
+// -------------------------------------------------------------------- f-k
+
+struct pseg_fk_plan {
+  int dtype;
+  int32_t traces, samples, n_time, n_k, n_freq, threads;
+  double dt_ms;
+  size_t lds_time, lds_space;
+  DevFk *dev;
+  char *base;
+};
+
+namespace {
+
+// tukey (fft/fk.nim:70-80): 1 in the middle, 0.5 - 0.5 cos(pi (d + 0.5) /
+// ramp) over ramp = round(fraction n) samples at each end.
+template <typename T>
+std::vector<T> fk_tukey(int32_t n, double fraction) {
+  std::vector<T> w(n);
+  const int64_t ramp = fraction <= 0 ? 0 : (int64_t)std::floor(fraction * (double)n + 0.5);
+  for (int32_t i = 0; i < n; i++) {
+    const int64_t d = std::min<int64_t>(i, n - 1 - i);
+    const double v = (ramp <= 0 || d >= ramp) ? 1.0
+                     : 0.5 - 0.5 * std::cos(kPi * ((double)d + 0.5) / (double)ramp);
+    w[i] = (T)v;
+  }
+  return w;
+}
+
+template <typename T>
+pseg_status fk_compile(double dt, int32_t traces, int32_t samples, double taper,
+                       pseg_fk_plan **out) {
+  // initFkPlan (fft/fk.nim:82-106)
+  if (traces < 4) return fail(PSEG_E_INVALID, "an f-k gather needs ≥ 4 live rows, got %d", traces);
+  if (samples < 8) return fail(PSEG_E_INVALID, "an f-k window needs ≥ 8 samples, got %d", samples);
+  if (!(dt > 0 && dt < kInf)) return fail(PSEG_E_INVALID, "dtMs must be finite and > 0, got %g", dt);
+  if (!(taper >= 0 && taper <= 0.5))
+    return fail(PSEG_E_INVALID, "taperFraction must be in 0 .. 0.5, got %g", taper);
+  const int64_t nt = next_fast(samples, true), nk = next_fast(2 * (int64_t)traces, true);
+  int threads = 0;
+  for (int t = 256; t <= 1024 && threads == 0; t *= 2)
+    if (fft_fits(nt / 2, t, kFuseCorr, kBudgetCorr) && fft_fits(nk / 2, t, kFuseCorr, kBudgetCorr))
+      threads = t;
+  const size_t lds_time = std::max(fft_lds_bytes(nt, sizeof(T), kFuseCorr), (size_t)nt * sizeof(T));
+  const size_t space = (std::max(fft_lds_bytes(nk, sizeof(T), kFuseCorr), (size_t)nk * sizeof(T)) +
+                        15) / 16 * 16;
+  const size_t lds_space = space + (size_t)(nk / 2 + 1) * 2 * sizeof(T);
+  if (threads == 0 || lds_time > kMaxRowLds || lds_space > kMaxRowLds)
+    return fail(PSEG_E_UNSUPPORTED, "an f-k transform of %lld x %lld points does not fit one "
+                "workgroup", (long long)nt, (long long)nk);
+  Blob blob;
+  blob.bytes.resize(sizeof(DevFk));
+  DevFk H;
+  memset(&H, 0, sizeof H);
+  build_fft<T>(nt, H.time, blob, kFuseCorr);
+  build_fft<T>(nk, H.space, blob, kFuseCorr);
+  set_off(H.time_w, blob.add_vec(fk_tukey<T>(samples, taper)));
+  set_off(H.trace_w, blob.add_vec(fk_tukey<T>(traces, taper)));
+  H.traces = traces;
+  H.samples = samples;
+  H.n_freq = (int32_t)(nt / 2 + 1);
+  H.n_k = (int32_t)nk;
+  H.space_lds = (int64_t)space;
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc of an f-k plan");
+  patch_fft(H.time, base);
+  patch_fft(H.space, base);
+  patch(H.time_w, base);
+  patch(H.trace_w, base);
+  memcpy(blob.bytes.data(), &H, sizeof H);
+  e = hipMemcpy(base, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    return hip_fail(e, "hipMemcpy of an f-k plan");
+  }
+  pseg_fk_plan *P = new pseg_fk_plan;
+  P->dtype = sizeof(T) == 4 ? PSEG_F32 : PSEG_F64;
+  P->traces = traces;
+  P->samples = samples;
+  P->n_time = (int32_t)nt;
+  P->n_k = (int32_t)nk;
+  P->n_freq = H.n_freq;
+  P->threads = threads;
+  P->dt_ms = dt;
+  P->lds_time = lds_time;
+  P->lds_space = lds_space;
+  P->dev = (DevFk *)base;
+  P->base = base;
+  *out = P;
+  return PSEG_OK;
+}
+
+size_t fk_spec_bytes(const pseg_fk_plan *P) {
+  const size_t esz = P->dtype == PSEG_F32 ? 4 : 8;
+  return ((size_t)P->traces * P->n_freq * 2 * esz + 255) / 256 * 256;
+}
+
+}  // namespace
+
+extern "C" {
+
+pseg_status pseg_fk_compile(int32_t dtype, double dt_ms, int32_t traces, int32_t samples,
+                            double taper_fraction, pseg_fk_plan **out) {
+  if (!out) return fail(PSEG_E_INVALID, "out is NULL");
+  *out = nullptr;
+  return guarded([&] {
+    if (dtype == PSEG_F32) return fk_compile<float>(dt_ms, traces, samples, taper_fraction, out);
+    if (dtype == PSEG_F64) return fk_compile<double>(dt_ms, traces, samples, taper_fraction, out);
+    return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
+  });
+}
+
+pseg_status pseg_fk_destroy(pseg_fk_plan *plan) {
+  if (!plan) return PSEG_OK;
+  (void)hipFree(plan->base);
+  delete plan;
+  return PSEG_OK;
+}
+
+pseg_status pseg_fk_axes(const pseg_fk_plan *P, int32_t *n_freq, int32_t *n_k, double *df_hz,
+                         double *dk_per_trace) {
+  if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_fk_compile");
+  if (n_freq) *n_freq = P->n_freq;
+  if (n_k) *n_k = P->n_k;
+  if (df_hz) *df_hz = 1000 / ((double)P->n_time * P->dt_ms);
+  if (dk_per_trace) *dk_per_trace = 1 / (double)P->n_k;
+  return PSEG_OK;
+}
+
+pseg_status pseg_fk_workspace_size(const pseg_fk_plan *P, size_t *bytes) {
+  if (!P || !bytes) return fail(PSEG_E_INVALID, "NULL argument");
+  *bytes = fk_spec_bytes(P) + 256;
+  return PSEG_OK;
+}
+
+pseg_status pseg_fk_amplitude(const pseg_fk_plan *P, const void *src, int32_t rows, int32_t cols,
+                              const int32_t *row_len, const int32_t *live_rows, int32_t first_col,
+                              double *amp, void *ws, size_t ws_bytes, void *stream) {
+  if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_fk_compile");
+  if (first_col < 0) return fail(PSEG_E_INVALID, "firstCol must be ≥ 0, got %d", first_col);
+  if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
+  if (!src || !row_len || !live_rows || !amp) return fail(PSEG_E_INVALID, "NULL argument");
+  size_t need = 0;
+  pseg_fk_workspace_size(P, &need);
+  if (!ws || ws_bytes < need)
+    return fail(PSEG_E_WORKSPACE, "the plan needs a workspace of %zu bytes, got %zu", need,
+                ws ? ws_bytes : (size_t)0);
+  char *w = (char *)ws;
+  hipError_t e = launch_fk(P->dtype, P->threads, P->dev, P->lds_time, P->lds_space, P->traces,
+                           P->n_freq, src, cols, row_len, live_rows, first_col, w,
+                           (uint32_t *)(w + fk_spec_bytes(P)), amp, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "f-k launch");
+  return PSEG_OK;
+}
+
+}  // extern "C"

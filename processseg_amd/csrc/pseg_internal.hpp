@@ -258,6 +258,25 @@ hipError_t launch_panel_factor(int dtype, const void *x, int32_t rows, int32_t c
 // The correlation kernel with `threads` (512 or 1024) threads per record.
 // A SpectrumPlan (fft/spectrum.nim:61-101): the real FFT of nfft points, the
 // window's Tukey taper (T, `window` values) and its float64 sum.
+// An f-k plan (fft/fk.nim:60-68): the time transform (nTime real points),
+// the spatial one (nK real points, run twice per frequency), the tapers
+// (T) and where the spatial pass keeps the first half spectrum in LDS.
+struct DevFk {
+  DevFft time;
+  DevFft space;
+  const void *time_w;   // T, samples
+  const void *trace_w;  // T, traces
+  int32_t traces, samples, n_freq, n_k;
+  int64_t space_lds;    // bytes of the spatial transform's buffer
+};
+// fkAmplitudeInto (fft/fk.nim:438-504): spec (workspace, traces x n_freq T
+// pairs) and flag (one uint32, cleared first) between the two passes; amp
+// (n_k x n_freq float64, frequency fastest).
+hipError_t launch_fk(int dtype, int threads, const DevFk *dev, size_t lds_time, size_t lds_space,
+                     int traces, int n_freq, const void *src, int32_t cols,
+                     const int32_t *row_len, const int32_t *live_rows, int32_t first_col,
+                     void *spec, uint32_t *flag, double *amp, hipStream_t s);
+
 struct DevSpec {
   DevFft fft;
   const void *taper;

@@ -1249,3 +1249,174 @@ hipError_t launch_spectrum_noalloc(int dtype, int threads, const DevSpec *dev, s
 }
 
 }  // namespace pseg
+
+// ------------------------------------------------------------------- f-k
+//
+// fkAmplitudeInto (fft/fk.nim:438-504) in two launches.  The time pass: one
+// workgroup per trace that enters, its window tapered on both axes (the
+// sample times the time weight, then the trace weight, in T), zero-padded to
+// nTime and through the real FFT, the half spectrum X[k] (T pairs) into the
+// workspace, trace-major.  The spatial pass: one workgroup per frequency
+// bin, as the reference: the real parts across traces and the imaginary
+// parts, each zero-padded to nK, through the real FFT; the complex
+// transform of (re + i im) read off the two half spectra with the sign flip
+// that puts later-on-later events at positive k (fft/fk.nim:400-410), the
+// modulus in float64, sqrt(re^2 + im^2), or hypot when a float64 sample of
+// the window leaves [1e-100, 1e140] (flag set by the time pass).
+
+namespace pseg {
+namespace {
+
+// The half spectrum of the time pass: X[k] of trace ti (the split gives
+// 2 X[k] between 0 Hz and Nyquist).
+template <typename T> struct FkSpecOut {
+  V2<T> *spec;
+  int m;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    spec[k] = (k != 0 && k != m) ? mk<T>(x.x * (T)0.5, x.y * (T)0.5) : x;
+    return x;
+  }
+};
+
+// The first spatial transform (the real parts): its half spectrum into LDS.
+template <typename T> struct FkKeepA {
+  V2<T> *a;
+  int m;
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    a[k] = (k != 0 && k != m) ? mk<T>(x.x * (T)0.5, x.y * (T)0.5) : x;
+    return x;
+  }
+};
+
+// The second (the imaginary parts): bin k of both half spectra gives the
+// outputs whose spatial bin is k (ik = nK/2 - k) and nK - k (ik = nK/2 + k,
+// fft/fk.nim:400-410).
+template <typename T> struct FkOut {
+  const V2<T> *a;
+  double *amp;
+  int m, n_freq, jf;
+  bool extreme;
+  __device__ double modulus(double re, double im) const {
+    return extreme ? hypot(re, im) : sqrt(re * re + im * im);
+  }
+  __device__ V2<T> operator()(V2<T> x, int k) const {
+    const V2<T> b = (k != 0 && k != m) ? mk<T>(x.x * (T)0.5, x.y * (T)0.5) : x;
+    const V2<T> av = a[k];
+    const double ar = (double)av.x, ai = (double)av.y, br = (double)b.x, bi = (double)b.y;
+    const int half = m;  // nK / 2
+    // sk = k <= half: re = ar - bi, im = ai + br
+    amp[(int64_t)(half - k) * n_freq + jf] = modulus(ar - bi, ai + br);
+    if (k != 0 && k != half) {
+      // sk = nK - k > half: re = ar + bi, im = br - ai
+      amp[(int64_t)(half + k) * n_freq + jf] = modulus(ar + bi, br - ai);
+    }
+    return x;
+  }
+};
+
+// leavesFloat64Range (fft/fk.nim:164-181) of one sample.
+__device__ __forceinline__ bool fk_leaves(double x) {
+  const double a = fabs(x);
+  return !(a <= 1e140) || (a < 1e-100 && a != 0);
+}
+
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void fk_time_kernel(const DevFk *__restrict__ plan,
+                                                      const T *__restrict__ src, int32_t cols,
+                                                      const int32_t *__restrict__ row_len,
+                                                      const int32_t *__restrict__ live_rows,
+                                                      int32_t first_col, V2<T> *__restrict__ spec,
+                                                      uint32_t *__restrict__ flag) {
+  extern __shared__ __align__(16) char lds[];
+  const DevFk &P = *plan;
+  const DevFft &f = P.time;
+  const int ti = blockIdx.x;
+  const int64_t r = live_rows[ti];
+  const int live = min(row_len[r], cols);
+  const int n = max(min(live, first_col + P.samples) - first_col, 0);
+  const T *s = src + r * (int64_t)cols + first_col;
+  gptr<T> tw = as_global<T>(P.time_w);
+  const T trw = as_global<T>(P.trace_w)[ti];
+  T *buf = (T *)lds;
+  bool bad = false;
+  for (int j = threadIdx.x; j < f.n; j += NTC) {
+    T y = (T)0;
+    if (j < n) {
+      const T x = s[j];
+      if (sizeof(T) == 8) bad |= fk_leaves((double)x);
+      y = (x * tw[j]) * trw;
+    }
+    buf[j] = y;
+  }
+  if (sizeof(T) == 8 && bad) atomicOr(flag, 1u);
+  __syncthreads();
+  real_forward<T, NTC>(f, (V2<T> *)lds, FkSpecOut<T>{spec + (int64_t)ti * P.n_freq, f.m});
+}
+
+template <typename T, int NTC>
+__global__ __launch_bounds__(NTC) void fk_space_kernel(const DevFk *__restrict__ plan,
+                                                       const V2<T> *__restrict__ spec,
+                                                       const uint32_t *__restrict__ flag,
+                                                       double *__restrict__ amp) {
+  extern __shared__ __align__(16) char lds[];
+  const DevFk &P = *plan;
+  const DevFft &f = P.space;
+  const int jf = blockIdx.x;
+  const int traces = P.traces, nf = P.n_freq;
+  V2<T> *A = (V2<T> *)lds;
+  T *buf = (T *)lds;
+  V2<T> *keep = (V2<T> *)(lds + P.space_lds);
+  const bool extreme = sizeof(T) == 8 && *flag != 0;
+  // the real parts across traces, zero-padded to nK
+  for (int t = threadIdx.x; t < f.n; t += NTC) buf[t] = t < traces ? spec[(int64_t)t * nf + jf].x : (T)0;
+  __syncthreads();
+  real_forward<T, NTC>(f, A, FkKeepA<T>{keep, f.m});
+  __syncthreads();
+  // the imaginary parts
+  for (int t = threadIdx.x; t < f.n; t += NTC) buf[t] = t < traces ? spec[(int64_t)t * nf + jf].y : (T)0;
+  __syncthreads();
+  real_forward<T, NTC>(f, A, FkOut<T>{keep, amp, f.m, nf, jf, extreme});
+}
+
+template <typename T, int NTC>
+hipError_t fk_typed(const DevFk *dev, size_t lds_time, size_t lds_space, int traces, int n_freq,
+                    const T *src, int32_t cols, const int32_t *row_len, const int32_t *live_rows,
+                    int32_t first_col, V2<T> *spec, uint32_t *flag, double *amp,
+                    hipStream_t stream) {
+  static const hipError_t a1 = hipFuncSetAttribute(
+      (const void *)fk_time_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  static const hipError_t a2 = hipFuncSetAttribute(
+      (const void *)fk_space_kernel<T, NTC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)kMaxRowLds);
+  if (a1 != hipSuccess) return a1;
+  if (a2 != hipSuccess) return a2;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, flag, 1);
+  hipLaunchKernelGGL((fk_time_kernel<T, NTC>), dim3(traces), dim3(NTC), lds_time, stream, dev, src,
+                     cols, row_len, live_rows, first_col, spec, flag);
+  hipLaunchKernelGGL((fk_space_kernel<T, NTC>), dim3(n_freq), dim3(NTC), lds_space, stream, dev,
+                     (const V2<T> *)spec, (const uint32_t *)flag, amp);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fk(int dtype, int threads, const DevFk *dev, size_t lds_time, size_t lds_space,
+                     int traces, int n_freq, const void *src, int32_t cols,
+                     const int32_t *row_len, const int32_t *live_rows, int32_t first_col,
+                     void *spec, uint32_t *flag, double *amp, hipStream_t s) {
+  if (dtype == 0) {
+    const float *x = (const float *)src;
+    float2 *sp = (float2 *)spec;
+    return threads == 256 ? fk_typed<float, 256>(dev, lds_time, lds_space, traces, n_freq, x, cols, row_len, live_rows, first_col, sp, flag, amp, s)
+         : threads == 512 ? fk_typed<float, 512>(dev, lds_time, lds_space, traces, n_freq, x, cols, row_len, live_rows, first_col, sp, flag, amp, s)
+                          : fk_typed<float, 1024>(dev, lds_time, lds_space, traces, n_freq, x, cols, row_len, live_rows, first_col, sp, flag, amp, s);
+  }
+  const double *x = (const double *)src;
+  double2 *sp = (double2 *)spec;
+  return threads == 256 ? fk_typed<double, 256>(dev, lds_time, lds_space, traces, n_freq, x, cols, row_len, live_rows, first_col, sp, flag, amp, s)
+       : threads == 512 ? fk_typed<double, 512>(dev, lds_time, lds_space, traces, n_freq, x, cols, row_len, live_rows, first_col, sp, flag, amp, s)
+                        : fk_typed<double, 1024>(dev, lds_time, lds_space, traces, n_freq, x, cols, row_len, live_rows, first_col, sp, flag, amp, s);
+}
+
+}  // namespace pseg
