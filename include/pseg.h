@@ -161,15 +161,23 @@ typedef struct pseg_plan_info {
 
 /* ---------------------------------------------------------- sample decode -- */
 
-/* SEG-Y data sample formats (SEG-Y rev 2, binary header bytes 3225-3226),
- * all big-endian: the decode parseSEG runs before the prefetcher's transform
- * hands a chunk to the plan (batch.nim:267-309, docs/integration.md:294-298). */
+/* SEG-Y data sample formats (SEG-Y rev 2, binary header bytes 3225-3226)
+ * and SEG-D demultiplexed formats, all big-endian: the decode parseSEG runs
+ * before the prefetcher's transform hands a chunk to the plan
+ * (batch.nim:267-309, docs/integration.md:294-298). */
 typedef enum pseg_sample_format {
   PSEG_FMT_IBM32 = 1,   /* IBM System/360 float: (-1)^s * f/2^24 * 16^(e-64) */
   PSEG_FMT_INT32 = 2,
   PSEG_FMT_INT16 = 3,
   PSEG_FMT_IEEE32 = 5,
-  PSEG_FMT_INT8 = 8
+  PSEG_FMT_INT8 = 8,
+  /* SEG-D demultiplexed data (general header 1 format code, bytes 3-4):
+   * the formats of the reference's SEG-D test files (tests/utils/
+   * calibfiles.nim: 8058, 8036) and their companions.  Big-endian. */
+  PSEG_FMT_SEGD_INT24 = 8036,   /* 24-bit two's complement integer */
+  PSEG_FMT_SEGD_INT32 = 8038,   /* 32-bit two's complement integer */
+  PSEG_FMT_SEGD_IEEE32 = 8058,  /* IEEE float32 */
+  PSEG_FMT_SEGD_IEEE64 = 8080   /* IEEE float64 */
 } pseg_sample_format;
 
 /* --------------------------------------------------------------- spectra -- */
@@ -625,8 +633,10 @@ pseg_status pseg_plan_apply_calib(const pseg_plan *plan, const void *src, void *
 /* Decodes `rows` traces of big-endian samples on the device into a [rows,
  * cols] panel of `dtype` (C-contiguous): trace r's sample j is the word at
  * raw + r*pitch + offset + j*bytes (a SEG-Y file's traces: offset 240, pitch
- * 240 + bytes * samples).  IBM, int16, int8 and IEEE words decode exactly;
- * int32 to float32 rounds to nearest.  For 4-byte formats raw, pitch and
+ * 240 + bytes * samples; a SEG-D record's demultiplexed traces: offset the
+ * trace header and its extensions).  IBM, int8, int16, int24 and IEEE words
+ * decode exactly (IEEE float64 to float32 rounds to nearest), int32 to
+ * float32 rounds to nearest.  For 4- and 8-byte formats raw, pitch and
  * offset must be multiples of 4.  Asynchronous on `stream`.  No counterpart
  * in processSEG itself: parseSEG decodes on the host (its decode workers). */
 pseg_status pseg_decode_samples(int32_t format, const void *raw, int64_t pitch, int64_t offset,
@@ -689,8 +699,18 @@ typedef struct pseg_batch_info {
   int64_t rows;            /* rows of all items */
   int64_t samples;         /* rows x cols of all items */
   int32_t threads[12];     /* threads per row of each launch */
-  int32_t grid[12];        /* workgroups of each launch */
+  int32_t grid[12];        /* workgroups of each persistent launch; 0: one
+                              workgroup per row (pseg_batch_tune chose it) */
 } pseg_batch_info;
+
+/* What pseg_batch_tune measured and chose per launch class. */
+typedef struct pseg_batch_tuning {
+  int32_t launches;
+  int32_t grid[12];            /* the workgroups chosen */
+  int32_t direct[12];          /* 1: one workgroup per row, no work counter */
+  float best_ms[12];           /* the chosen shape's median launch time */
+  float candidate_ms[12][4];   /* 1/2, 3/4, all of the resident workgroups, direct (0: skipped) */
+} pseg_batch_tuning;
 
 /* Validates every item as pseg_plan_apply would (before any sample changes;
  * the first bad item is named: "item i: ...") and keeps the work table on the
@@ -701,6 +721,16 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n_items, pse
 pseg_status pseg_batch_launch(const pseg_batch *batch, void *stream);
 pseg_status pseg_batch_info_get(const pseg_batch *batch, pseg_batch_info *out);
 pseg_status pseg_batch_destroy(pseg_batch *batch);
+/* No counterpart in the reference (its CPU pool has no launch shape): times
+ * each launch class of the batch `repeats` times (after one warm-up run)
+ * with a persistent grid of 1/2, 3/4 and all of the workgroups the device
+ * holds at once, and with one workgroup per row, and keeps the fastest
+ * shape in the batch.  A row's arithmetic does not depend on the launch
+ * shape, so later launches give the same bits, faster.  The batch runs
+ * 4·(repeats + 1) times per class: items must not run in place
+ * (PSEG_E_INVALID).  Synchronizes `stream`. */
+pseg_status pseg_batch_tune(pseg_batch *batch, int32_t repeats, void *stream,
+                            pseg_batch_tuning *out);
 
 /* -------------------------------------------------------------- serving -- */
 

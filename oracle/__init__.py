@@ -730,8 +730,10 @@ def decode_samples(raw: np.ndarray, fmt: int, samples: int, rows: int, offset: i
     """SEG-Y sample decode (SEG-Y rev 2 data sample format codes, big-endian
     words), restated from the standard's definitions in numpy: IBM float32
     (-1)^s · f/2^24 · 16^(e−64), exact in float64 and rounded once to
-    float32; int32, int16, int8 two's complement; IEEE float32 as is."""
-    nb = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1}[fmt]
+    float32; int32, int16, int8 two's complement; IEEE float32 as is.  SEG-D
+    demultiplexed codes: 8036 24-bit two's complement, 8038 int32, 8058 IEEE
+    float32, 8080 IEEE float64 (rounded once to float32)."""
+    nb = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1, 8036: 3, 8038: 4, 8058: 4, 8080: 8}[fmt]
     if pitch is None:
         pitch = offset + nb * samples
     raw = np.ascontiguousarray(raw, np.uint8)
@@ -745,8 +747,18 @@ def decode_samples(raw: np.ndarray, fmt: int, samples: int, rows: int, offset: i
             mag = np.ldexp(frac, (4 * expo - 280).astype(np.int32))
             with np.errstate(over="ignore"):
                 out[r] = np.where((w >> 31) == 1, -mag, mag).astype(dtype)
-        elif fmt == 2:
+        elif fmt in (2, 8038):
             out[r] = b.view(">i4").astype(dtype)
+        elif fmt == 8036:
+            t = b.reshape(-1, 3).astype(np.int64)
+            u = (t[:, 0] << 16) | (t[:, 1] << 8) | t[:, 2]
+            out[r] = np.where(u >= 1 << 23, u - (1 << 24), u).astype(dtype)
+        elif fmt == 8058:
+            with np.errstate(invalid="ignore"):
+                out[r] = b.view(">f4").astype(dtype)
+        elif fmt == 8080:
+            with np.errstate(invalid="ignore", over="ignore"):
+                out[r] = b.view(">f8").astype(dtype)
         elif fmt == 3:
             out[r] = b.view(">i2").astype(dtype)
         elif fmt == 5:

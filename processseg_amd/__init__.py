@@ -44,5 +44,6 @@ from .firstbreak import (  # noqa: F401
     FirstBreakParams, FirstBreakPicks, FirstBreakPlan, initFirstBreakPlan, pmCoppens,
     pmEnergyRatio, pmStaLta)
 from .decode import (  # noqa: F401
-    decodeSamples, fmtIbm32, fmtIeee32, fmtInt8, fmtInt16, fmtInt32)
+    decodeSamples, fmtIbm32, fmtIeee32, fmtInt8, fmtInt16, fmtInt32, fmtSegd8036, fmtSegd8038,
+    fmtSegd8058, fmtSegd8080)
 from .serve import ServeResult, Server  # noqa: F401

@@ -110,6 +110,11 @@ class BatchInfoC(C.Structure):
                 ("samples", C.c_int64), ("threads", C.c_int32 * 12), ("grid", C.c_int32 * 12)]
 
 
+class BatchTuningC(C.Structure):
+    _fields_ = [("launches", C.c_int32), ("grid", C.c_int32 * 12), ("direct", C.c_int32 * 12),
+                ("best_ms", C.c_float * 12), ("candidate_ms", (C.c_float * 4) * 12)]
+
+
 class ServerConfigC(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("traces", C.c_int32), ("samples", C.c_int32),
                 ("max_batch_requests", C.c_int32), ("pitch", C.c_int64), ("survey", C.c_void_p),
@@ -178,6 +183,9 @@ SYMBOLS = [
     "pseg_firstbreak_compile", "pseg_firstbreak_destroy", "pseg_firstbreak_required_margin",
     "pseg_firstbreak_workspace_size", "pseg_firstbreak_pick", "pseg_fold_add", "pseg_fold_clear",
     "pseg_fold_difference", "pseg_plan_apply_host", "pseg_host_staging_release",
+    "pseg_plan_apply_calib", "pseg_calib_compile", "pseg_calib_destroy", "pseg_calib_resolve",
+    "pseg_plan_calib_factors", "pseg_fk_compile", "pseg_fk_destroy", "pseg_fk_axes",
+    "pseg_fk_workspace_size", "pseg_fk_amplitude", "pseg_batch_tune",
 ]
 
 
@@ -264,6 +272,7 @@ def lib():
     L.pseg_batch_launch.argtypes = [vp, vp]
     L.pseg_batch_info_get.argtypes = [vp, C.POINTER(BatchInfoC)]
     L.pseg_batch_destroy.argtypes = [vp]
+    L.pseg_batch_tune.argtypes = [vp, i32, vp, C.POINTER(BatchTuningC)]
     L.pseg_server_create.argtypes = [C.POINTER(ServerConfigC), C.POINTER(vp)]
     L.pseg_server_add_flow.argtypes = [vp, C.POINTER(OpC), i32, C.POINTER(i32)]
     L.pseg_server_submit.argtypes = [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_uint64)]

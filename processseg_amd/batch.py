@@ -88,6 +88,22 @@ class PipelineBatch:
         self.threads = [int(info.threads[g]) for g in range(self.launches)]
         self.grid = [int(info.grid[g]) for g in range(self.launches)]
 
+    def tune(self, repeats: int = 5, stream=None) -> dict:
+        """Times each launch class with persistent grids of 1/2, 3/4 and all
+        of the resident workgroups and with one workgroup per row, and keeps
+        the fastest (pseg_batch_tune): the same bits, from a launch shape
+        measured on this device.  Items must not run in place."""
+        t = _lib.BatchTuningC()
+        _lib.check(_lib.lib().pseg_batch_tune(self._h, int(repeats), _stream_handle(stream),
+                                              C.byref(t)))
+        info = _lib.BatchInfoC()
+        _lib.check(_lib.lib().pseg_batch_info_get(self._h, C.byref(info)))
+        self.grid = [int(info.grid[g]) for g in range(self.launches)]
+        return {"grid": [int(t.grid[g]) for g in range(t.launches)],
+                "direct": [bool(t.direct[g]) for g in range(t.launches)],
+                "best_ms": [float(t.best_ms[g]) for g in range(t.launches)],
+                "candidate_ms": [[float(x) for x in t.candidate_ms[g]] for g in range(t.launches)]}
+
     def launch(self, stream=None):
         """Run every item on `stream` (default: torch's current stream)."""
         _lib.check(_lib.lib().pseg_batch_launch(self._h, _stream_handle(stream)))

@@ -1,9 +1,11 @@
-// SEG-Y sample decode on the device: rows of big-endian samples as a file
-// holds them (each trace a header then its samples, `pitch` bytes apart) to
-// a [rows, cols] float32 or float64 panel, one pass over HBM.  Formats
-// (SEG-Y rev 2, data sample format code): 1 IBM float32, 2 int32, 3 int16,
-// 5 IEEE float32, 8 int8 — what parseSEG's decode produces before the
-// prefetcher's transform runs the plan (batch.nim:267-309).
+// SEG-Y and SEG-D sample decode on the device: rows of big-endian samples as
+// a file holds them (each trace a header then its samples, `pitch` bytes
+// apart) to a [rows, cols] float32 or float64 panel, one pass over HBM.
+// Formats: SEG-Y rev 2 data sample format codes 1 IBM float32, 2 int32,
+// 3 int16, 5 IEEE float32, 8 int8; SEG-D demultiplexed codes 8036 24-bit
+// integer, 8038 int32, 8058 IEEE float32, 8080 IEEE float64 — what
+// parseSEG's decode produces before the prefetcher's transform runs the plan
+// (batch.nim:267-309).
 #include "pseg_device.hpp"
 
 namespace pseg {
@@ -18,12 +20,12 @@ template <int FMT, typename T>
 __global__ __launch_bounds__(256) void decode_kernel(const unsigned char *__restrict__ raw,
                                                      int64_t pitch, int64_t offset, int32_t cols,
                                                      T *__restrict__ dst) {
-  constexpr int B = FMT == 3 ? 2 : FMT == 8 ? 1 : 4;
+  constexpr int B = sample_bytes(FMT);
   const int64_t r = blockIdx.x;
   const unsigned char *row = raw + r * pitch + offset;
   T *out = dst + r * (int64_t)cols;
   for (int j0 = (int)threadIdx.x; j0 < cols; j0 += K * 256) {
-    uint32_t w[K];
+    uint64_t w[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
       const int j = j0 + 256 * k;
@@ -48,6 +50,10 @@ hipError_t launch_fmt(int format, const void *raw, int64_t pitch, int64_t offset
     case 3: hipLaunchKernelGGL((decode_kernel<3, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
     case 5: hipLaunchKernelGGL((decode_kernel<5, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
     case 8: hipLaunchKernelGGL((decode_kernel<8, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
+    case 8036: hipLaunchKernelGGL((decode_kernel<8036, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
+    case 8038: hipLaunchKernelGGL((decode_kernel<8038, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
+    case 8058: hipLaunchKernelGGL((decode_kernel<8058, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
+    case 8080: hipLaunchKernelGGL((decode_kernel<8080, T>), dim3(rows), dim3(256), 0, s, p, pitch, offset, cols, d); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

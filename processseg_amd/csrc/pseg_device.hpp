@@ -1696,7 +1696,7 @@ __device__ __forceinline__ void iir_step(const DevIir &I, T (&v)[PER], int n, ch
   __syncthreads();  // before the next step writes LDS
 }
 
-// ------------------------------------------------- SEG-Y sample words
+// ------------------------------------------ SEG-Y and SEG-D sample words
 
 // IBM System/360 single precision: sign, 7-bit base-16 exponent in excess
 // 64, 24-bit fraction below the radix point: (-1)^s · f/2^24 · 16^(e-64).
@@ -1707,28 +1707,42 @@ __device__ __forceinline__ double ibm_to_double(uint32_t w) {
   return (w >> 31) ? -mag : mag;
 }
 
-// The raw word of sample j (big-endian, as a file holds it), widened.
+// Bytes per sample of a format: SEG-Y codes 1 (IBM), 2 (int32), 3 (int16),
+// 5 (IEEE float32), 8 (int8); SEG-D demultiplexed codes 8036 (24-bit
+// integer), 8038 (int32), 8058 (IEEE float32), 8080 (IEEE float64).
+constexpr int sample_bytes(int fmt) {
+  return fmt == 3 ? 2 : fmt == 8 ? 1 : fmt == 8036 ? 3 : fmt == 8080 ? 8 : 4;
+}
+
+// The raw word of sample j (big-endian, as a file holds it), widened; 8-byte
+// words from two aligned 4-byte loads.
 template <int FMT>
-__device__ __forceinline__ uint32_t load_word(const unsigned char *p) {
-  if constexpr (FMT == 3) return (uint32_t)((p[0] << 8) | p[1]);
-  else if constexpr (FMT == 8) return (uint32_t)p[0];
+__device__ __forceinline__ uint64_t load_word(const unsigned char *p) {
+  if constexpr (FMT == 3) return (uint64_t)((p[0] << 8) | p[1]);
+  else if constexpr (FMT == 8) return (uint64_t)p[0];
+  else if constexpr (FMT == 8036) return (uint64_t)((p[0] << 16) | (p[1] << 8) | p[2]);
+  else if constexpr (FMT == 8080)
+    return ((uint64_t)__builtin_bswap32(((const uint32_t *)p)[0]) << 32) |
+           __builtin_bswap32(((const uint32_t *)p)[1]);
   else return __builtin_bswap32(*(const uint32_t *)p);
 }
 
-// The word's value as T: IBM through its exact float64 value; int32 straight
-// to T (one rounding to nearest for float32); IEEE float32 as it is (a NaN
-// keeps its payload in float32).
+// The word's value as T: IBM through its exact float64 value; integers
+// straight to T (exact but for int32 to float32, one rounding to nearest);
+// IEEE words as they are (a NaN keeps its payload in its own width).
 template <int FMT, typename T>
-__device__ __forceinline__ T word_value(uint32_t w) {
-  if constexpr (FMT == 1) return (T)ibm_to_double(w);
-  else if constexpr (FMT == 2) return (T)(int32_t)w;
+__device__ __forceinline__ T word_value(uint64_t w) {
+  if constexpr (FMT == 1) return (T)ibm_to_double((uint32_t)w);
+  else if constexpr (FMT == 2 || FMT == 8038) return (T)(int32_t)(uint32_t)w;
   else if constexpr (FMT == 3) return (T)(int16_t)(uint16_t)w;
-  else if constexpr (FMT == 5) return (T)__uint_as_float(w);
+  else if constexpr (FMT == 5 || FMT == 8058) return (T)__uint_as_float((uint32_t)w);
+  else if constexpr (FMT == 8036) return (T)(((int32_t)((uint32_t)w << 8)) >> 8);
+  else if constexpr (FMT == 8080) return (T)__longlong_as_double((long long)w);
   else return (T)(int8_t)(uint8_t)w;
 }
 
-// A raw word of format `fmt` (1, 2, 3, 5 or 8) at p as T, the format chosen
-// at run time (one copy of each conversion, for loops that are not unrolled).
+// A raw word of format `fmt` at p as T, the format chosen at run time (one
+// copy of each conversion, for loops that are not unrolled).
 template <typename T>
 __device__ __forceinline__ T raw_sample(int fmt, const unsigned char *p) {
   switch (fmt) {
@@ -1736,11 +1750,15 @@ __device__ __forceinline__ T raw_sample(int fmt, const unsigned char *p) {
     case 2: return word_value<2, T>(load_word<2>(p));
     case 3: return word_value<3, T>(load_word<3>(p));
     case 5: return word_value<5, T>(load_word<5>(p));
+    case 8036: return word_value<8036, T>(load_word<8036>(p));
+    case 8038: return word_value<8038, T>(load_word<8038>(p));
+    case 8058: return word_value<8058, T>(load_word<8058>(p));
+    case 8080: return word_value<8080, T>(load_word<8080>(p));
     default: return word_value<8, T>(load_word<8>(p));
   }
 }
 
-__device__ __forceinline__ int raw_sample_bytes(int fmt) { return fmt == 3 ? 2 : fmt == 8 ? 1 : 4; }
+__device__ __forceinline__ int raw_sample_bytes(int fmt) { return sample_bytes(fmt); }
 
 // Per-trace normalization (gain/gain.nim:198-219).
 template <typename T, int NTH, int PER>

@@ -1171,6 +1171,8 @@ struct pseg_batch {
   // one launch per (dtype, threads per row, kernel form)
   int dtype[kMaxBatchClasses], threads[kMaxBatchClasses], full[kMaxBatchClasses];
   JobLaunch launch[kMaxBatchClasses];
+  int resident[kMaxBatchClasses];  // workgroups of the class resident at once
+  bool in_place;                   // an item runs in place (src == dst)
   char *dev;  // the job tables, then one 256-byte counter per launch
 };
 
@@ -1221,22 +1223,35 @@ int32_t pseg_device_count(void) {
   return c;
 }
 
+// Bytes per sample of a format the decode knows, else 0.
+static int64_t format_bytes(int32_t format) {
+  switch (format) {
+    case PSEG_FMT_IBM32: case PSEG_FMT_INT32: case PSEG_FMT_IEEE32:
+    case PSEG_FMT_SEGD_INT32: case PSEG_FMT_SEGD_IEEE32: return 4;
+    case PSEG_FMT_INT16: return 2;
+    case PSEG_FMT_INT8: return 1;
+    case PSEG_FMT_SEGD_INT24: return 3;
+    case PSEG_FMT_SEGD_IEEE64: return 8;
+    default: return 0;
+  }
+}
+
 pseg_status pseg_decode_samples(int32_t format, const void *raw, int64_t pitch, int64_t offset,
                                 int32_t rows, int32_t cols, int32_t dtype, void *dst,
                                 void *stream) {
-  if (format != PSEG_FMT_IBM32 && format != PSEG_FMT_INT32 && format != PSEG_FMT_INT16 &&
-      format != PSEG_FMT_IEEE32 && format != PSEG_FMT_INT8)
-    return fail(PSEG_E_INVALID, "unknown sample format %d (SEG-Y codes 1, 2, 3, 5 and 8)", format);
+  if (format_bytes(format) == 0)
+    return fail(PSEG_E_INVALID, "unknown sample format %d (SEG-Y codes 1, 2, 3, 5, 8; SEG-D "
+                "8036, 8038, 8058, 8080)", format);
   if (dtype != PSEG_F32 && dtype != PSEG_F64) return fail(PSEG_E_INVALID, "unknown dtype %d", dtype);
   if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
   if (rows == 0 || cols == 0) return PSEG_OK;
   if (!raw || !dst) return fail(PSEG_E_INVALID, "raw or dst is NULL");
-  const int64_t bytes = format == PSEG_FMT_INT16 ? 2 : format == PSEG_FMT_INT8 ? 1 : 4;
+  const int64_t bytes = format_bytes(format);
   if (offset < 0 || pitch < offset + (int64_t)cols * bytes)
     return fail(PSEG_E_INVALID, "%d samples of %lld bytes from byte %lld do not fit a trace of %lld "
                 "bytes", cols, (long long)bytes, (long long)offset, (long long)pitch);
-  if (bytes == 4 && ((((uintptr_t)raw) | (uintptr_t)pitch | (uintptr_t)offset) & 3))
-    return fail(PSEG_E_INVALID, "4-byte samples need raw, pitch and offset at multiples of 4");
+  if (bytes >= 4 && ((((uintptr_t)raw) | (uintptr_t)pitch | (uintptr_t)offset) & 3))
+    return fail(PSEG_E_INVALID, "4- and 8-byte samples need raw, pitch and offset at multiples of 4");
   const size_t in_bytes = (size_t)(rows - 1) * (size_t)pitch + (size_t)offset + (size_t)cols * bytes;
   const size_t out_bytes = (size_t)rows * (size_t)cols * (dtype == PSEG_F32 ? 4 : 8);
   const uintptr_t a = (uintptr_t)raw, b = (uintptr_t)dst;
@@ -1493,9 +1508,9 @@ pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *
                                 int32_t cols, const int32_t *row_len, const double *row_keys,
                                 int64_t first_sample, void *stream) {
   if (!P) return fail(PSEG_E_INVALID, "a plan must come from pseg_plan_compile");
-  if (format != PSEG_FMT_IBM32 && format != PSEG_FMT_INT32 && format != PSEG_FMT_INT16 &&
-      format != PSEG_FMT_IEEE32 && format != PSEG_FMT_INT8)
-    return fail(PSEG_E_INVALID, "unknown sample format %d (SEG-Y codes 1, 2, 3, 5 and 8)", format);
+  if (format_bytes(format) == 0)
+    return fail(PSEG_E_INVALID, "unknown sample format %d (SEG-Y codes 1, 2, 3, 5, 8; SEG-D "
+                "8036, 8038, 8058, 8080)", format);
   if (rows < 0 || cols < 0) return fail(PSEG_E_INVALID, "rows and cols must be >= 0");
   if (first_sample < 0) return fail(PSEG_E_INVALID, "firstSample must be >= 0, got %lld", (long long)first_sample);
   for (int i = 0; i < P->n_steps; i++)
@@ -1509,12 +1524,12 @@ pseg_status pseg_plan_apply_raw(const pseg_plan *P, int32_t format, const void *
   if (!raw || !dst || !row_len) return fail(PSEG_E_INVALID, "NULL traces, panel or row_len");
   if (cols > P->samples)
     return fail(PSEG_E_INVALID, "the block has %d columns, the plan takes %d samples", cols, P->samples);
-  const int64_t bytes = format == PSEG_FMT_INT16 ? 2 : format == PSEG_FMT_INT8 ? 1 : 4;
+  const int64_t bytes = format_bytes(format);
   if (offset < 0 || pitch < offset + (int64_t)cols * bytes)
     return fail(PSEG_E_INVALID, "%d samples of %lld bytes from byte %lld do not fit a trace of %lld "
                 "bytes", cols, (long long)bytes, (long long)offset, (long long)pitch);
-  if (bytes == 4 && ((((uintptr_t)raw) | (uintptr_t)pitch | (uintptr_t)offset) & 3))
-    return fail(PSEG_E_INVALID, "4-byte samples need raw, pitch and offset at multiples of 4");
+  if (bytes >= 4 && ((((uintptr_t)raw) | (uintptr_t)pitch | (uintptr_t)offset) & 3))
+    return fail(PSEG_E_INVALID, "4- and 8-byte samples need raw, pitch and offset at multiples of 4");
   const size_t esz = P->dtype == PSEG_F32 ? 4 : 8;
   const size_t in_bytes = (size_t)(rows - 1) * (size_t)pitch + (size_t)offset + (size_t)cols * bytes;
   const uintptr_t a = (uintptr_t)raw, b = (uintptr_t)dst;
@@ -2153,6 +2168,7 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
       return hip_fail(e, "occupancy of the pipeline kernel");
     }
     L.grid = std::max(1, std::min(L.total, resident));
+    B->resident[g] = std::max(1, resident);
     bytes += (tables[g].size() * sizeof(Job) + 255) / 256 * 256;
   }
   bytes += 256 * (size_t)std::max(B->n_launches, 1);
@@ -2174,8 +2190,80 @@ pseg_status pseg_batch_create(const pseg_batch_item *items, int32_t n, pseg_batc
     off += (tb + 255) / 256 * 256;
   }
   for (int g = 0; g < B->n_launches; g++) B->launch[g].counter = (uint32_t *)(B->dev + off + 256 * g);
+  B->in_place = false;
+  for (int32_t i = 0; i < n; i++) B->in_place |= items[i].src == items[i].dst;
   *out = B;
   return PSEG_OK;
+}
+
+// The launch shape of each class by timing (no counterpart in the
+// reference: its CPU pool has no launch shape).  Every candidate gives each
+// row the same arithmetic (a row's bits depend on its plan and samples only),
+// so the choice changes the time of a batch, never its results.
+pseg_status pseg_batch_tune(pseg_batch *B, int32_t repeats, void *stream, pseg_batch_tuning *out) {
+  if (!B) return fail(PSEG_E_INVALID, "a batch must come from pseg_batch_create");
+  if (repeats < 1) return fail(PSEG_E_INVALID, "repeats must be >= 1, got %d", repeats);
+  if (B->in_place)
+    return fail(PSEG_E_INVALID, "tuning runs the batch many times: an item in place would be "
+                "processed again each time (tune a copy src -> dst)");
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+  pseg_batch_tuning t;
+  memset(&t, 0, sizeof t);
+  t.launches = B->n_launches;
+  pseg_status st = PSEG_OK;
+  for (int g = 0; g < B->n_launches && st == PSEG_OK; g++) {
+    JobLaunch &L = B->launch[g];
+    uint32_t *counter = L.counter;
+    const int res = B->resident[g];
+    // persistent grids of 1/2, 3/4 and all of the resident workgroups, and
+    // the direct launch (one workgroup per row, the dispatcher's order)
+    const int grids[4] = {std::max(1, res / 2), std::max(1, 3 * res / 4), res, L.total};
+    float best = 0;
+    int best_i = -1;
+    for (int c = 0; c < 4 && st == PSEG_OK; c++) {
+      const bool direct = c == 3;
+      if (!direct && grids[c] >= L.total && c < 2) continue;
+      L.grid = std::max(1, std::min(grids[c], L.total));
+      L.counter = direct ? nullptr : counter;
+      std::vector<float> ms;
+      for (int k = 0; k <= repeats && st == PSEG_OK; k++) {  // k = 0 warms up
+        hipError_t le = hipEventRecord(e0, s);
+        if (le == hipSuccess) le = launch_pipeline(B->dtype[g], B->threads[g], L, s);
+        if (le == hipSuccess) le = hipEventRecord(e1, s);
+        if (le == hipSuccess) le = hipEventSynchronize(e1);
+        float m = 0;
+        if (le == hipSuccess) le = hipEventElapsedTime(&m, e0, e1);
+        if (le != hipSuccess) st = hip_fail(le, "batch tuning launch");
+        else if (k > 0) ms.push_back(m);
+      }
+      if (st != PSEG_OK) break;
+      std::sort(ms.begin(), ms.end());
+      const float med = ms[ms.size() / 2];
+      t.candidate_ms[g][c] = med;
+      if (best_i < 0 || med < best) {
+        best = med;
+        best_i = c;
+      }
+    }
+    if (st != PSEG_OK) {
+      L.grid = std::max(1, std::min(L.total, res));
+      L.counter = counter;
+      break;
+    }
+    L.grid = std::max(1, std::min(grids[best_i], L.total));
+    L.counter = best_i == 3 ? nullptr : counter;
+    t.grid[g] = L.grid;
+    t.direct[g] = best_i == 3;
+    t.best_ms[g] = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st == PSEG_OK && out) *out = t;
+  return st;
 }
 
 pseg_status pseg_batch_launch(const pseg_batch *B, void *stream) {
@@ -2196,7 +2284,7 @@ pseg_status pseg_batch_info_get(const pseg_batch *B, pseg_batch_info *o) {
   o->samples = B->samples;
   for (int g = 0; g < B->n_launches; g++) {
     o->threads[g] = B->threads[g];
-    o->grid[g] = B->launch[g].grid;
+    o->grid[g] = B->launch[g].counter ? B->launch[g].grid : 0;
   }
   return PSEG_OK;
 }

@@ -1003,9 +1003,11 @@ class PipelinePlan:
         loads it, so the traces cross HBM once into `dst` ([rows, cols] of the
         plan's dtype).  As decodeSamples then apply, in one pass."""
         import torch
-        nb = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1}.get(fmt)
+        from .decode import SAMPLE_BYTES
+        nb = SAMPLE_BYTES.get(fmt)
         if nb is None:
-            raise ValueError(f"unknown sample format {fmt} (SEG-Y codes 1, 2, 3, 5 and 8)")
+            raise ValueError(f"unknown sample format {fmt} (SEG-Y codes 1, 2, 3, 5, 8; SEG-D "
+                             "8036, 8038, 8058, 8080)")
         if not isinstance(raw, torch.Tensor) or not raw.is_cuda or raw.dtype != torch.uint8 \
                 or not raw.is_contiguous():
             raise ValueError("raw must be a contiguous uint8 GPU tensor")

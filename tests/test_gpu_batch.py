@@ -160,3 +160,48 @@ def test_batch_of_flows_reading_trace_keys(gpu):
     torch.cuda.synchronize()
     for i, (it, ref) in enumerate(zip(items, want)):
         assert torch.equal(it.dst, ref), f"record {i} differs from its own apply"
+
+
+def test_tuned_launch_shape_keeps_the_bits(gpu):
+    """pseg_batch_tune picks each class's launch shape by timing (persistent
+    grids of 1/2, 3/4 and all resident workgroups, or one workgroup per
+    row); the batch then gives the same bits as before tuning, in eager
+    launches and in a captured graph; an in-place batch is refused."""
+    import torch
+    recs = make_records(32, seed=21)
+    flow = W.mixed_flow()
+    plans = plans_for(recs, flow)
+    srcs = [torch.from_numpy(x).cuda() for x, _, _ in recs]
+    lens = [torch.from_numpy(l).cuda() for _, l, _ in recs]
+    before = [torch.full_like(s, 7.0) for s in srcs]
+    P.PipelineBatch([P.BatchItem(p, s, d, row_len=l)
+                     for p, s, d, l in zip(plans, srcs, before, lens)]).launch()
+    dsts = [torch.full_like(s, 5.0) for s in srcs]
+    batch = P.PipelineBatch([P.BatchItem(p, s, d, row_len=l)
+                             for p, s, d, l in zip(plans, srcs, dsts, lens)])
+    t = batch.tune(repeats=2)
+    assert len(t["grid"]) == batch.launches
+    for g in range(batch.launches):
+        assert t["best_ms"][g] > 0 and t["best_ms"][g] == min(x for x in t["candidate_ms"][g] if x > 0)
+        assert (batch.grid[g] == 0) == t["direct"][g]
+    for d in dsts:
+        d.fill_(3.0)
+    batch.launch()
+    torch.cuda.synchronize()
+    for a, b in zip(before, dsts):
+        assert torch.equal(a, b)
+    for d in dsts:
+        d.fill_(1.0)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        batch.launch(stream=s)
+    for _ in range(2):
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(before, dsts):
+            assert torch.equal(a, b)
+    inplace = P.PipelineBatch([P.BatchItem(plans[0], srcs[0].clone(), None, row_len=lens[0])])
+    with pytest.raises(ValueError, match="in place"):
+        inplace.tune()

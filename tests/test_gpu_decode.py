@@ -1,5 +1,6 @@
-"""SEG-Y sample decode on the GPU against the oracle, bit for bit: every
-format (IBM float32, int32, int16, IEEE float32, int8) to float32 and
+"""SEG-Y and SEG-D sample decode on the GPU against the oracle, bit for bit:
+every format (IBM float32, int32, int16, IEEE float32, int8; SEG-D 24-bit
+and 32-bit integers, IEEE float32 and float64) to float32 and
 float64, random words of every bit pattern (IEEE NaNs keep their payload in
 float32), traces with 240-byte headers and odd sample counts, empty shapes,
 the refusals, a decoded IEEE panel equal to the panel it was written from at
@@ -13,8 +14,9 @@ from processseg_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-FMTS = [P.fmtIbm32, P.fmtInt32, P.fmtInt16, P.fmtIeee32, P.fmtInt8]
-NB = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1}
+FMTS = [P.fmtIbm32, P.fmtInt32, P.fmtInt16, P.fmtIeee32, P.fmtInt8, P.fmtSegd8036,
+        P.fmtSegd8038, P.fmtSegd8058, P.fmtSegd8080]
+NB = {1: 4, 2: 4, 3: 2, 5: 4, 8: 1, 8036: 3, 8038: 4, 8058: 4, 8080: 8}
 
 
 def traces(fmt, rows, ns, seed):
@@ -115,7 +117,8 @@ def ibm_words(x):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-@pytest.mark.parametrize("fmt", [P.fmtIbm32, P.fmtInt16, P.fmtIeee32])
+@pytest.mark.parametrize("fmt", [P.fmtIbm32, P.fmtInt16, P.fmtIeee32, P.fmtSegd8036,
+                                 P.fmtSegd8080])
 def test_apply_raw_equals_decode_then_apply(gpu, oracle, dtype, fmt):
     """Plan.applyRaw on raw traces equals decodeSamples then Plan.apply:
     bit for bit, for a core flow and for a flow with a mute and a gain curve
@@ -129,11 +132,20 @@ def test_apply_raw_equals_decode_then_apply(gpu, oracle, dtype, fmt):
         w, b = ibm_words(xd), ">u4"
     elif fmt == P.fmtInt16:
         w, b = np.clip(np.round(xd), -32768, 32767).astype(np.int64), ">i2"
+    elif fmt == P.fmtSegd8036:
+        w, b = np.clip(np.round(xd * 100), -2 ** 23, 2 ** 23 - 1).astype(np.int64), "i24"
+    elif fmt == P.fmtSegd8080:
+        w, b = xd, ">f8"
     else:
         w, b = xd.astype(np.float32), ">f4"
-    nb = np.dtype(b).itemsize
+    nb = 3 if b == "i24" else np.dtype(b).itemsize
     blob = np.zeros((rows, 240 + nb * ns), np.uint8)
-    blob[:, 240:] = np.asarray(w).astype(b).view(np.uint8).reshape(rows, nb * ns)
+    if b == "i24":
+        u = np.asarray(w) & 0xFFFFFF
+        b3 = np.stack([(u >> 16) & 0xFF, (u >> 8) & 0xFF, u & 0xFF], axis=-1).astype(np.uint8)
+        blob[:, 240:] = b3.reshape(rows, 3 * ns)
+    else:
+        blob[:, 240:] = np.asarray(w).astype(b).view(np.uint8).reshape(rows, nb * ns)
     raw = torch.from_numpy(blob.reshape(-1)).cuda()
     meta = P.ChunkMeta(rows, 2.0, numSamples=live + first, firstSample=first,
                        t0Ms=np.linspace(0, 30, rows), offset=np.linspace(100, 2500, rows))

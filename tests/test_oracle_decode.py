@@ -74,3 +74,35 @@ def test_trace_layout():
     blob = b"".join(bytes(range(240)) + struct.pack(f">{ns}h", *r.tolist()) for r in vals)
     got = O.decode_samples(np.frombuffer(blob, np.uint8), 3, ns, rows)
     assert np.array_equal(got, vals.astype(np.float32))
+
+
+def test_segd_formats():
+    """SEG-D demultiplexed formats against Python's big-endian unpacking:
+    8036 24-bit two's complement (the rails +8388607 and -8388608, -1,
+    random words), 8038 int32, 8058 IEEE float32, 8080 IEEE float64 (NaN
+    payloads and infinities kept in float64, rounded once to float32)."""
+    rng = np.random.default_rng(11)
+    i24 = np.concatenate([[8388607, -8388608, -1, 0, 1],
+                          rng.integers(-2 ** 23, 2 ** 23, 495, dtype=np.int64)])
+    b24 = b"".join(int(v).to_bytes(3, "big", signed=True) for v in i24)
+    got = O.decode_samples(np.frombuffer(b24, np.uint8), 8036, 500, 1, offset=0,
+                           dtype=np.float64)[0]
+    assert np.array_equal(got, i24.astype(np.float64))
+    got32 = O.decode_samples(np.frombuffer(b24, np.uint8), 8036, 500, 1, offset=0)[0]
+    assert np.array_equal(got32.astype(np.float64), i24.astype(np.float64))  # exact in float32
+    i32 = rng.integers(-2 ** 31, 2 ** 31, 500, dtype=np.int64)
+    got = O.decode_samples(np.frombuffer(struct.pack(">500i", *i32.tolist()), np.uint8), 8038,
+                           500, 1, offset=0, dtype=np.float64)[0]
+    assert np.array_equal(got, i32.astype(np.float64))
+    f32 = rng.standard_normal(500).astype(np.float32)
+    got = O.decode_samples(np.frombuffer(struct.pack(">500f", *f32.tolist()), np.uint8), 8058,
+                           500, 1, offset=0)[0]
+    assert np.array_equal(got, f32)
+    f64 = np.concatenate([[math.inf, -math.inf, 1e300, -1e-300, 0.0],
+                          rng.standard_normal(495) * 1e3])
+    raw64 = np.frombuffer(struct.pack(">500d", *f64.tolist()), np.uint8)
+    got = O.decode_samples(raw64, 8080, 500, 1, offset=0, dtype=np.float64)[0]
+    assert np.array_equal(got, f64)
+    with np.errstate(over="ignore"):
+        assert np.array_equal(O.decode_samples(raw64, 8080, 500, 1, offset=0)[0],
+                              f64.astype(np.float32))
