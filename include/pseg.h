@@ -578,6 +578,20 @@ pseg_status pseg_plan_compile(const pseg_op *ops, int32_t n_ops, int32_t dtype, 
 pseg_status pseg_plan_destroy(pseg_plan *plan);
 pseg_status pseg_plan_info_get(const pseg_plan *plan, pseg_plan_info *out);
 
+/* responseModuli (pipeline.nim:774-838): gain[i] = |Σ_j x[j] exp(-2πi
+ * hz[i] j dt_s)| for the n host samples x at m arbitrary frequencies, one
+ * sine and cosine per frequency.  Host arrays. */
+pseg_status pseg_response_moduli(const double *x, int32_t n, const double *hz, int32_t m,
+                                 double dt_s, double *gain);
+/* appliedResponseInto (pipeline.nim:840-880): the response of the flow as
+ * applied — the flow with every FFT and IIR filter's trend removal off,
+ * compiled for `samples` samples at dt_ms, applied on the current device
+ * to a unit impulse at sample samples/2 of one live row (t0Ms 0, offset 0,
+ * no file scales), and pseg_response_moduli of that output at hz (host,
+ * m values) into gain (host).  PSEG_E_INVALID as pseg_plan_compile. */
+pseg_status pseg_applied_response(const pseg_op *ops, int32_t n_ops, int32_t dtype, double dt_ms,
+                                  int32_t samples, const double *hz, int32_t m, double *gain);
+
 /* Pipeline.requiredMargin(dtMs) without compiling (pipeline.nim:525-548). */
 pseg_status pseg_required_margin(const pseg_op *ops, int32_t n_ops, double dt_ms,
                                  int32_t *margin);

@@ -185,7 +185,8 @@ SYMBOLS = [
     "pseg_fold_difference", "pseg_plan_apply_host", "pseg_host_staging_release",
     "pseg_plan_apply_calib", "pseg_calib_compile", "pseg_calib_destroy", "pseg_calib_resolve",
     "pseg_plan_calib_factors", "pseg_fk_compile", "pseg_fk_destroy", "pseg_fk_axes",
-    "pseg_fk_workspace_size", "pseg_fk_amplitude", "pseg_batch_tune",
+    "pseg_fk_workspace_size", "pseg_fk_amplitude", "pseg_batch_tune", "pseg_response_moduli",
+    "pseg_applied_response",
 ]
 
 
@@ -273,6 +274,8 @@ def lib():
     L.pseg_batch_info_get.argtypes = [vp, C.POINTER(BatchInfoC)]
     L.pseg_batch_destroy.argtypes = [vp]
     L.pseg_batch_tune.argtypes = [vp, i32, vp, C.POINTER(BatchTuningC)]
+    L.pseg_response_moduli.argtypes = [vp, i32, vp, i32, f64, vp]
+    L.pseg_applied_response.argtypes = [C.POINTER(OpC), i32, i32, f64, i32, vp, i32, vp]
     L.pseg_server_create.argtypes = [C.POINTER(ServerConfigC), C.POINTER(vp)]
     L.pseg_server_add_flow.argtypes = [vp, C.POINTER(OpC), i32, C.POINTER(i32)]
     L.pseg_server_submit.argtypes = [vp, i32, i32, i32, i32, i32, C.POINTER(C.c_uint64)]

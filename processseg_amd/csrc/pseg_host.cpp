@@ -3030,3 +3030,107 @@ pseg_status pseg_fk_amplitude(const pseg_fk_plan *P, const void *src, int32_t ro
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- applied response
+
+extern "C" {
+
+// |Σ x[j] exp(-2πi f j dt)| at each frequency (responseModuli,
+// pipeline.nim:774-838): a unit phasor advanced by the rotation exp(-2πi f
+// dt) each sample, then turned onto the exact angle -2π f (j+1) dt by the
+// first-order correction for the rounding of that step, so no drift builds up
+// over long traces; one sine and cosine per frequency.
+pseg_status pseg_response_moduli(const double *x, int32_t n, const double *hz, int32_t m,
+                                 double dt_s, double *gain) {
+  if (n < 0 || m < 0) return fail(PSEG_E_INVALID, "n and m must be >= 0");
+  if ((n > 0 && !x) || (m > 0 && (!hz || !gain))) return fail(PSEG_E_INVALID, "NULL argument");
+  for (int32_t i = 0; i < m; i++) {
+    const double a = -2.0 * kPi * hz[i];
+    const double w = a * dt_s, cw = std::cos(w), sw = std::sin(w);
+    double c = 1, s = 0, at = 0, re = 0, im = 0;
+    for (int32_t j = 0; j < n; j++) {
+      re += x[j] * c;
+      im += x[j] * s;
+      const double next = (a * (double)(j + 1)) * dt_s;  // the exact angle of sample j + 1
+      const double step = next - at;                      // this step, as rounded
+      const double e = step - w;                          // its difference from w
+      const double rc = cw - sw * e, rs = sw + cw * e;    // rotation by w + e
+      const double nc = c * rc - s * rs, ns = c * rs + s * rc;
+      const double d = next - (at + step);                // what the step left over
+      c = nc - ns * d;
+      s = ns + nc * d;
+      at = next;
+    }
+    gain[i] = std::hypot(re, im);
+  }
+  return PSEG_OK;
+}
+
+// appliedResponseInto (pipeline.nim:840-880): the flow compiled with every
+// FFT and IIR filter's trend removal off, applied on the current device to a
+// unit impulse at sample samples/2 of one live row (t0Ms 0, offset 0, no file
+// scales), and the moduli of that output's DFT at hz.
+pseg_status pseg_applied_response(const pseg_op *ops, int32_t n_ops, int32_t dtype, double dt_ms,
+                                  int32_t samples, const double *hz, int32_t m, double *gain) {
+  if (n_ops < 0 || n_ops > PSEG_MAX_OPS || (n_ops > 0 && !ops))
+    return fail(PSEG_E_INVALID, "a flow has 0 .. %d ops", PSEG_MAX_OPS);
+  if (m < 0 || (m > 0 && (!hz || !gain))) return fail(PSEG_E_INVALID, "NULL hz or gain");
+  std::vector<pseg_op> copy(ops, ops + n_ops);
+  for (pseg_op &o : copy)
+    if (o.kind == PSEG_OP_FILTER || o.kind == PSEG_OP_IIR) o.trend = 0;
+  pseg_plan *P = nullptr;
+  pseg_status st = pseg_plan_compile(copy.data(), n_ops, dtype, dt_ms, samples, 0, &P);
+  if (st != PSEG_OK) return st;
+  const size_t esz = dtype == PSEG_F32 ? 4 : 8;
+  std::vector<char> host((size_t)samples * esz, 0);
+  if (dtype == PSEG_F32) ((float *)host.data())[samples / 2] = 1;
+  else ((double *)host.data())[samples / 2] = 1;
+  size_t ws_bytes = 0;
+  pseg_plan_workspace_size(P, 1, samples, &ws_bytes);
+  // the row, row_len, every row's length, the keys, one factor per
+  // calibration step, the workspace
+  const size_t off_len = ((size_t)samples * esz + 255) / 256 * 256;
+  const size_t off_keys = off_len + 256, off_k = off_keys + 256, off_ws = off_k + 256;
+  char *dev = nullptr;
+  hipError_t e = hipMalloc((void **)&dev, off_ws + ws_bytes + 256);
+  if (e != hipSuccess) {
+    pseg_plan_destroy(P);
+    return hip_fail(e, "hipMalloc of the impulse row");
+  }
+  auto done = [&](pseg_status s) {
+    (void)hipFree(dev);
+    pseg_plan_destroy(P);
+    return s;
+  };
+  const int32_t len = samples;
+  const double keys[2] = {0.0, 0.0};
+  double k[2] = {1.0, 1.0};
+  if (P->n_calib > 0) {
+    // the impulse row's keys: live, no sensor, channel or station, no file scales
+    pseg_calib_row row;
+    memset(&row, 0, sizeof row);
+    row.channel_set = -1;
+    row.channel = row.rcv_line = row.rcv_point = NAN;
+    row.file_descale = row.file_sensitivity = NAN;
+    st = pseg_plan_calib_factors(P, &row, 1, 0, k, nullptr, nullptr);
+    if (st != PSEG_OK) return done(st);
+  }
+  e = hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dev + off_len, &len, sizeof len, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dev + off_keys, keys, sizeof keys, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dev + off_k, k, sizeof k, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return done(hip_fail(e, "copy of the impulse row"));
+  st = pseg_plan_apply_calib(P, dev, dev, 1, samples, (const int32_t *)(dev + off_len),
+                             (const double *)(dev + off_keys), 0, (const int32_t *)(dev + off_len),
+                             (const double *)(dev + off_k), ws_bytes ? dev + off_ws : nullptr,
+                             ws_bytes, nullptr);
+  if (st != PSEG_OK) return done(st);
+  e = hipMemcpy(host.data(), dev, host.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return done(hip_fail(e, "copy of the impulse response"));
+  std::vector<double> x(samples);
+  for (int32_t j = 0; j < samples; j++)
+    x[j] = dtype == PSEG_F32 ? (double)((float *)host.data())[j] : ((double *)host.data())[j];
+  return done(pseg_response_moduli(x.data(), samples, hz, m, dt_ms / 1000, gain));
+}
+
+}  // extern "C"

@@ -752,6 +752,24 @@ class Pipeline:
         """compile (pipeline.nim:466-523) into a plan on the current GPU."""
         return PipelinePlan(self, T, dtMs, samples, recordSamples)
 
+    def appliedResponseInto(self, T, dtMs: float, samples: int, hz, gain=None) -> np.ndarray:
+        """appliedResponseInto (pipeline.nim:840-880): the response of the
+        flow as applied (every FFT and IIR filter's trend removal off) to a
+        unit impulse at sample samples // 2, |DFT| at the frequencies `hz`
+        (Hz); the impulse runs through the plan on the GPU.  Fills and
+        returns `gain` (float64, len(hz))."""
+        h = np.ascontiguousarray(hz, np.float64)
+        out = np.zeros(h.size) if gain is None else gain
+        if len(out) != h.size:
+            raise ValueError("hz and gain need the same length")
+        g = np.zeros(h.size)
+        arr = _ops_c(self.ops)
+        _lib.check(_lib.lib().pseg_applied_response(arr, len(self.ops), _dtype_code(T),
+                                                    float(dtMs), int(samples), h.ctypes.data,
+                                                    h.size, g.ctypes.data))
+        out[:] = g
+        return out
+
     def requiredMargin(self, dtMs: float) -> int:
         """requiredMargin (pipeline.nim:525-548) without compiling."""
         m = C.c_int32()
