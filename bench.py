@@ -591,14 +591,21 @@ def batch_leg(torch, dev, steps, warmup):
             "achieved_GBps": round(samples * 8 / t_graph / 1e9, 1)}
 
 
+SERVE_COLD_S = 5.0                       # the cold-cache stream (plans compiled on use)
+SERVE_SWEEP = (1000.0, 4000.0, 8000.0, 16000.0)   # offered loads above 200/s, req/s
+SERVE_SWEEP_S = 1.5
+
+
 def serve_leg(torch, dev, seconds, rate_hz):
     """Zoom windows of a resident 20000 x 4001 f32 survey (320 MB; a seeded
     2000 x 4001 gather tiled 10 times) under a Poisson stream of `rate_hz`
-    requests/s for `seconds` (100-2000 traces x 200-4001 shown samples, three
-    flows, plans compiled per window width on first use).  The continuously
-    batched server against one launch per request in arrival order; then the
-    same stream again with every plan cached, and the whole stream submitted
-    at once (throughput)."""
+    requests/s (100-2000 traces x 200-4001 shown samples, three flows).  The
+    continuously batched server against one launch per request in arrival
+    order, each with: a short stream while plans compile per window width on
+    first use (cold), the `seconds`-long stream with every plan cached (warm:
+    precompiled plans), the same stream at higher offered loads (a load sweep,
+    where the one-launch path queues), and the whole stream submitted at once
+    (throughput)."""
     import queue
     import threading
     import processseg_amd as P
@@ -608,9 +615,13 @@ def serve_leg(torch, dev, seconds, rate_hz):
     survey = torch.from_numpy(tile).to(dev).repeat(10, 1).contiguous()
     trace_len = torch.full((survey.shape[0],), survey.shape[1], dtype=torch.int32, device=dev)
     reqs = serve_requests(int(seconds * rate_hz), survey.shape[0], survey.shape[1], rate_hz)
+    cold = reqs[:int(min(seconds, SERVE_COLD_S) * rate_hz)]
     flows = serve_flows()
 
-    def run(srv, ids, paced):
+    def pct(x, q):
+        return round(float(np.percentile(x, q)), 3)
+
+    def run(srv, ids, paced, reqs):
         q = queue.Queue()
         res = []
 
@@ -630,32 +641,51 @@ def serve_leg(torch, dev, seconds, rate_hz):
                 while time.perf_counter() - t0 < at:
                     time.sleep(min(0.0005, max(0.0, at - (time.perf_counter() - t0))))
             q.put(srv.submit(ids[f], tr0, ntr, a, b))
+        t_sub = time.perf_counter() - t0
         q.put(None)
         th.join()
         wall = time.perf_counter() - t0
         lat = np.array([r.latency_ms for r in res])
+        queued = np.array([(r.start_ns - r.submit_ns) / 1e6 for r in res])
         samples = sum(r.rows * r.window for r in res)
-        return {"p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        return {"requests": len(res), "submit_s": round(t_sub, 3), "wall_s": round(wall, 3),
+                "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                 "mean_ms": round(float(lat.mean()), 3),
+                "queued_p50_ms": pct(queued, 50), "queued_p99_ms": pct(queued, 99),
                 "compiled_frac": round(float(np.mean([r.plan_compiled for r in res])), 3),
                 "mean_batch": round(float(np.mean([r.batch_requests for r in res])), 2),
                 "requests_per_s": round(len(res) / wall, 1),
                 "Msamples_per_s": round(samples / wall / 1e6, 1)}
 
-    out = {"workload": f"{len(reqs)} zoom-window requests, Poisson {rate_hz:g}/s, survey "
-                       f"{survey.shape[0]}x{survey.shape[1]} f32 resident, 3 flows",
-           "unit": "ms (submit to result on the device)"}
+    out = {"workload": f"{len(reqs)} zoom-window requests, Poisson {rate_hz:g}/s for "
+                       f"{seconds:g} s, survey {survey.shape[0]}x{survey.shape[1]} f32 resident, "
+                       "3 flows",
+           "stream_s": seconds, "cold_stream_s": min(seconds, SERVE_COLD_S),
+           "unit": "ms (submit to result on the device; queued: submit to launch)"}
     for name, mb, inflight in (("batched", 256, 2), ("one_launch_per_request", 1, 1)):
         srv = P.Server(survey, trace_len, 2.0, arena_bytes=4 << 30, max_batch_requests=mb,
                        max_inflight=inflight)
         ids = [srv.add_flow(f) for f in flows]
-        log(f"serving: {name}, cold plan cache")
-        out[name] = {"cold": run(srv, ids, True)}
-        log(f"serving: {name}, warm plan cache")
-        out[name]["warm"] = run(srv, ids, True)
+        log(f"serving: {name}, cold plan cache ({len(cold)} requests)")
+        out[name] = {"cold": run(srv, ids, True, cold)}
+        # precompiled plans: one single-trace request per (flow, window) of the stream
+        uniq = sorted({(f, a, b) for _, f, _, _, a, b in reqs})
+        t0 = time.perf_counter()
+        run(srv, ids, False, [(0.0, f, 0, 1, a, b) for f, a, b in uniq])
+        out[name]["precompile"] = {"windows": len(uniq),
+                                   "s": round(time.perf_counter() - t0, 2)}
+        log(f"serving: {name}, warm plan cache ({len(reqs)} requests)")
+        out[name]["warm"] = run(srv, ids, True, reqs)
+        sweep = {}
+        for hz in SERVE_SWEEP:
+            # the warm stream's requests at a higher offered load
+            sr = [(at * rate_hz / hz,) + tuple(rest)
+                  for at, *rest in reqs[:int(SERVE_SWEEP_S * hz)]]
+            log(f"serving: {name}, {hz:g} req/s offered ({len(sr)} requests)")
+            sweep[f"{hz:g}"] = run(srv, ids, True, sr)
+        out[name]["load_sweep"] = sweep
         log(f"serving: {name}, burst")
-        out[name]["burst"] = run(srv, ids, False)
+        out[name]["burst"] = run(srv, ids, False, reqs[:1600])
         out[name]["server"] = srv.info()
         srv.close()
     return out
@@ -675,8 +705,9 @@ def main():
                          "gain_flow, iir_flow, decode, host_staged, raw_flow, spectrum, hilbert, spectrogram, "
                          "f64_panel, correlation, uneven_batch, serving), "
                          "or all")
-    ap.add_argument("--serve-seconds", type=float, default=8.0,
-                    help="length of the Poisson request stream of the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the serving leg's Poisson stream with precompiled plans "
+                         "(the cold-cache stream is its first 5 s)")
     ap.add_argument("--flow-ops", type=int, default=None,
                     help="run only the first N ops of the headline flow (a per-step profile; "
                          "the line then names the cut flow)")

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -169,6 +170,9 @@ struct pseg_server {
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_ticket = 1;
   bool stop = false;
+  // bumped (under mu) by every submit and release: the worker's unlocked
+  // poll of its launches also watches for work it could admit
+  std::atomic<uint64_t> wake_seq{0};
   pseg_server_info info{};
   std::thread worker;
 
@@ -401,9 +405,21 @@ void pseg_server::run() {
     }
     if (launched) continue;
     if (busy > 0) {
-      // a launch is running: poll its event soon
+      // A launch is running: poll its event without the lock (slots are the
+      // worker's alone) until one finishes, or new work arrives while a slot
+      // is free.  A timed sleep would add the kernel's timer slack (about
+      // 50 us) to every result; the spin lasts only while a launch is on the
+      // device, and submitters and waiters never wait on it for the lock.
+      const uint64_t seen = wake_seq.load(std::memory_order_acquire);
       lk.unlock();
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      for (;;) {
+        bool finished = false;
+        for (int i = 0; i < inflight && !finished; i++)
+          finished = slots[i].busy && hipEventQuery(slots[i].done) != hipErrorNotReady;
+        if (finished) break;
+        if (busy < inflight && wake_seq.load(std::memory_order_acquire) != seen) break;
+        std::this_thread::yield();
+      }
       lk.lock();
     } else {
       work_cv.wait_for(lk, std::chrono::milliseconds(50));
@@ -531,6 +547,7 @@ pseg_status pseg_server_submit(pseg_server *S, int32_t flow, int32_t trace0, int
   S->reqs.emplace(t, std::move(r));
   S->queue.push_back(t);
   S->info.requests++;
+  S->wake_seq.fetch_add(1, std::memory_order_release);
   S->work_cv.notify_one();
   *ticket = t;
   return PSEG_OK;
@@ -592,6 +609,7 @@ pseg_status pseg_server_release(pseg_server *S, uint64_t ticket) {
   if (it->second.state != kDone) return set_error(PSEG_E_BUSY, "the request is not done");
   if (it->second.out) S->arena.release(it->second.out);
   S->reqs.erase(it);
+  S->wake_seq.fetch_add(1, std::memory_order_release);
   S->work_cv.notify_one();  // room for queued requests
   return PSEG_OK;
 }
