@@ -532,6 +532,177 @@ def spectrogram_leg(torch, P, dev, steps, warmup):
             "ms_per_step": round(t * 1000, 5)}
 
 
+def _timed(torch, fn, steps, warmup):
+    for i in range(warmup):
+        fn(i)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(steps):
+        fn(i)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t1) / steps
+
+
+def calibrated_flow_leg(torch, P, dev, steps, warmup):
+    """calibrateOp (two sensor rules over file descales) > the headline flow
+    on the 2000 x 3001 panel: Plan.apply with the meta, so each call
+    resolves the rows' factors on the host (pseg_plan_calib_factors) and
+    copies them with the row lengths; and the same apply with the factors
+    resolved once (the calibration step of the row kernel alone)."""
+    from processseg_amd import calibrate as K
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    sensor = np.where(np.arange(ROWS) % 4 == 0, 1, 2).astype(np.uint8)
+    scales = [K.FileScale(1e-6 * (1 + r % 5), float("nan"), K.auRecorded) for r in range(ROWS)]
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live, sensorType=sensor, scales=scales)
+    cal = K.initCalibration([
+        K.CalibrationRule(K.RuleMatch(sensorType=1), sensitivity=K.sensitivity(20, K.suVPerBar)),
+        K.CalibrationRule(K.RuleMatch(sensorType=2),
+                          sensitivity=K.sensitivity(22.5, K.suVPerMPerS))])
+    flow = make_flow()
+    plan = P.initPipeline([P.calibrateOp(cal)] + flow.ops).compile(np.float32, DT, COLS)
+    src = torch.from_numpy(x).to(dev)
+    outs = [torch.empty_like(src) for _ in range(4)]
+    t_meta = _timed(torch, lambda i: plan.apply(src, outs[i % 4], meta), steps, warmup)
+    k = plan.calibFactors(meta)
+    t_k = _timed(torch, lambda i: plan.apply(src, outs[i % 4], meta, calib_factors=k), steps,
+                 warmup)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.calibFactors(meta)
+    t_res = (time.perf_counter() - t0) / steps
+    return {"workload": "calibrate (hydrophone / geophone rules over file descales) > "
+                        + flow_name() + ", 2000x3001 f32",
+            "value": round(ROWS * COLS / t_meta / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t_meta * 1000, 5),
+            "ms_factors_given": round(t_k * 1000, 5),
+            "ms_host_resolution": round(t_res * 1000, 5)}
+
+
+def segd_decode_leg(torch, P, dev, steps, warmup):
+    """SEG-D 8036 (24-bit two's-complement integers, big-endian) traces of
+    the 2000 x 3001 panel's size (no headers) to float32, 8 rotating buffer
+    pairs.  Algorithmic bytes: 3 read + 4 written per sample."""
+    rows, ns = ROWS, COLS
+    v = np.random.default_rng(3).integers(-(1 << 23), 1 << 23, (rows, ns)).astype(np.int64)
+    u = (v & 0xFFFFFF).astype(np.uint32)
+    blob = np.stack([(u >> 16) & 0xFF, (u >> 8) & 0xFF, u & 0xFF], -1).astype(np.uint8)
+    raws = [torch.from_numpy(blob.reshape(-1).copy()).to(dev) for _ in range(8)]
+    outs = [torch.empty(rows, ns, device=dev) for _ in range(8)]
+    t = _timed(torch, lambda i: P.decodeSamples(raws[i % 8], P.fmtSegd8036, ns, rows=rows,
+                                                offset=0, out=outs[i % 8]), steps, warmup)
+    algo = rows * ns * 7
+    return {"workload": "SEG-D 8036 24-bit integer traces, 2000 x 3001, to float32 on the device",
+            "value": round(rows * ns / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5),
+            "achieved_GBps": round(algo / t / 1e9, 1)}
+
+
+def analytic_leg(torch, P, dev, steps, warmup):
+    """analyticInto of every live row of the 2000 x 3001 panel (nfft 6144):
+    envelope, instantaneous phase and frequency as float64 (24 B written
+    per sample)."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    hp = P.initHilbertPlan(np.float32, COLS)
+    src = torch.from_numpy(x).to(dev)
+    rl = meta.device_row_len(COLS, dev)
+    outs = [tuple(torch.empty(ROWS, COLS, dtype=torch.float64, device=dev) for _ in range(3))
+            for _ in range(2)]
+    t = _timed(torch, lambda i: hp.analyticInto(src, DT, *outs[i % 2], row_len=rl), steps, warmup)
+    return {"workload": "analytic signal (envelope, phase, frequency; nfft 6144) of the "
+                        "2000x3001 f32 panel",
+            "value": round(ROWS * COLS / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5)}
+
+
+def spectrogram_average_leg(torch, P, dev, steps, warmup):
+    """averageSpectrogramInto over the 2000 x 3001 panel: 257-sample Hann
+    frames every 10 samples (301 frames per row, nfft 512), one launch over
+    rows x frames, partials summed in a fixed order."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    sg = P.initSpectrogramPlan(np.float32, 257, 10, DT, COLS)
+    src = torch.from_numpy(x).to(dev)
+    rl = meta.device_row_len(COLS, dev)
+    amp = torch.empty((sg.frames, sg.bins), dtype=torch.float64, device=dev)
+    ws = torch.empty(max(sg.averageWorkspace(ROWS), 1), dtype=torch.uint8, device=dev)
+    t = _timed(torch, lambda i: sg.averageSpectrogramInto(src, amp=amp, row_len=rl, ws=ws,
+                                                          sync=False), steps, warmup)
+    return {"workload": "average spectrogram of the 2000x3001 f32 panel (257-sample Hann frames "
+                        "every 10 samples, nfft 512)",
+            "value": round(ROWS * sg.frames / t / 1e6, 2), "unit": "Mframes/s",
+            "ms_per_step": round(t * 1000, 5), "frames_per_row": sg.frames}
+
+
+def fk_leg(torch, P, dev, steps, warmup):
+    """The f-k amplitude of the 2000 x 3001 panel (10 % Tukey tapers, float32
+    transforms, float64 output): a time pass per trace and a spatial pass per
+    frequency."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    traces = P.fkRows(meta, COLS, 0, COLS)
+    plan = P.initFkPlan(np.float32, DT, traces, COLS, 0.1)
+    src = torch.from_numpy(x).to(dev)
+    amp = torch.empty(plan.nK * plan.nFreq, dtype=torch.float64, device=dev)
+    t = _timed(torch, lambda i: plan.fkAmplitudeInto(src, meta, 0, amp), steps, warmup)
+    return {"workload": f"f-k amplitude of the 2000x3001 f32 panel ({plan.nK} wavenumbers x "
+                        f"{plan.nFreq} frequencies)",
+            "value": round(ROWS * COLS / t / 1e6, 1), "unit": "Msamples/s",
+            "ms_per_step": round(t * 1000, 5)}
+
+
+def firstbreak_leg(torch, P, dev, steps, warmup):
+    """First-break picks (energy ratio, the reference's defaults, a band-pass
+    8-70 Hz) of every live row of the 2000 x 3001 panel, one thread per row
+    in the reference's serial float64 order."""
+    from processseg_amd.firstbreak import FirstBreakParams
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(ROWS, DT, numSamples=live)
+    gp = P.initFirstBreakPlan(np.float32, FirstBreakParams(lowCutHz=8.0, highCutHz=70.0), DT,
+                              COLS)
+    src = torch.from_numpy(x).to(dev)
+    rl = meta.device_row_len(COLS, dev)
+    t = _timed(torch, lambda i: gp.pickInto(src, row_len=rl), steps, warmup)
+    return {"workload": "first-break picks (energy ratio, 8-70 Hz band-pass) of the 2000x3001 "
+                        "f32 panel",
+            "value": round(ROWS / t / 1e6, 4), "unit": "Mtraces/s",
+            "ms_per_step": round(t * 1000, 5)}
+
+
+def fold_leg(torch, P, dev, steps, warmup):
+    """Fold counting of two million traces over a 500 x 500 grid at azimuth 37
+    with 5 offset classes, 8 azimuth sectors and offset ranges (integer
+    atomics), the map cleared between calls."""
+    from processseg_amd.fold import FoldOptions, gridBinning
+    n = 2_000_000
+    rng = np.random.default_rng(11)
+    b = gridBinning(600_000.0, 4_100_000.0, 37.0, 12.5, 25.0, inlines=500, crosslines=500)
+    srcxy = rng.uniform(-2000, 14000, (n, 2)) + (600_000.0, 4_100_000.0)
+    rcv = srcxy + rng.normal(0, 1500, (n, 2))
+    keys = P.traceKeyTable(n, srcX=srcxy[:, 0], srcY=srcxy[:, 1], rcvX=rcv[:, 0], rcvY=rcv[:, 1],
+                           cmpX=(srcxy[:, 0] + rcv[:, 0]) / 2, cmpY=(srcxy[:, 1] + rcv[:, 1]) / 2,
+                           offset=rng.normal(0, 3000, n))
+    opts = FoldOptions(offsetBounds=(0, 500, 1000, 2000, 4000, 8000.0), azimuthSectors=8,
+                       sectorStartDeg=-10, offsetRange=True)
+    m = P.initFoldMap(b, opts)
+    kd = torch.from_numpy(np.ascontiguousarray(keys)).to(dev)
+    rl = torch.zeros(n, dtype=torch.int32, device=dev)
+
+    def step(i):
+        m.clear()
+        m.addFold(keys=kd, row_len=rl)
+    t = _timed(torch, step, steps, warmup)
+    return {"workload": "fold of 2M traces over a 500x500 grid (5 offset classes, 8 sectors, "
+                        "offset ranges)",
+            "value": round(n / t / 1e6, 1), "unit": "Mtraces/s",
+            "ms_per_step": round(t * 1000, 5)}
+
+
 def batch_leg(torch, dev, steps, warmup):
     """64 records of an uneven survey (rows U[240, 2400], samples U[1000,
     6001], dt 1, 2 or 4 ms; band-pass 5-10-70-90 Hz + AGC 500 ms, one plan per
@@ -703,6 +874,7 @@ def main():
     ap.add_argument("--legs", default="all",
                     help="comma-separated secondary legs to run (panels_as_batch, muted_flow, "
                          "gain_flow, iir_flow, decode, host_staged, raw_flow, spectrum, hilbert, spectrogram, "
+                         "calibrated_flow, segd_decode, analytic, spectrogram_average, fk, firstbreak, fold, "
                          "f64_panel, correlation, uneven_batch, serving), "
                          "or all")
     ap.add_argument("--serve-seconds", type=float, default=60.0,
@@ -870,6 +1042,18 @@ def main():
         if want("spectrogram"):
             log("spectrogram leg")
             sec["spectrogram"] = spectrogram_leg(torch, P, dev, max(20, args.steps), 5)
+        for name, fn, n_steps in (
+                ("calibrated_flow", calibrated_flow_leg, max(10, args.steps // 4)),
+                ("segd_decode", segd_decode_leg, max(20, args.steps // 2)),
+                ("analytic", analytic_leg, max(10, args.steps // 10)),
+                ("spectrogram_average", spectrogram_average_leg, max(10, args.steps // 10)),
+                ("fk", fk_leg, max(10, args.steps // 10)),
+                ("firstbreak", firstbreak_leg, max(5, args.steps // 20)),
+                ("fold", fold_leg, max(5, args.steps // 20))):
+            if want(name):
+                log(f"{name} leg")
+                sec[name] = fn(torch, P, dev, n_steps, 2)
+                torch.cuda.empty_cache()
         if want("f64_panel"):
             log("float64 panel leg")
             sec["f64_panel"] = f64_panel_leg(torch, P, flow, dev, max(10, args.steps // 4), 3)

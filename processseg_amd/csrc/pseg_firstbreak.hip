@@ -7,7 +7,12 @@
 // confidence.  No multiply-add is contracted in this file, so each operation
 // rounds as the serial code's does.  The scratch arrays are laid out
 // [index][rows]: the lanes of a wave step through neighbouring rows'
-// values together, so their loads and stores coalesce.
+// values together, so their loads and stores coalesce.  Every per-sample
+// loop reads its inputs 8 or 16 samples at a time before it computes on them: a
+// load that follows a store to a scratch array the compiler cannot tell
+// apart would otherwise wait out the memory latency once per sample, and
+// with one lane per row those waits are the kernel's time.  The arithmetic
+// and its order are the serial code's.
 #pragma clang fp contract(off)
 
 #include "pseg_internal.hpp"
@@ -17,6 +22,9 @@
 namespace pseg {
 
 namespace {
+
+constexpr int CH = 8;    // samples loaded at once per lane, loops over several arrays
+constexpr int CH1 = 16;  // the same for loops over one array
 
 struct Col {
   double *base;
@@ -79,7 +87,17 @@ __device__ void iir_row(const DevFirstBreak &P, const ColT<T> &x, int64_t n, con
     }
   }
   const int64_t L = P.lead;
-  for (int64_t i = 0; i < n; i++) buf[L + i] = (double)x[i] - (a + b * (double)i);
+  for (int64_t i0 = 0; i0 < n; i0 += CH1) {
+    T xv[CH1];
+#pragma unroll
+    for (int q = 0; q < CH1; q++)
+      if (i0 + q < n) xv[q] = x[i0 + q];
+#pragma unroll
+    for (int q = 0; q < CH1; q++) {
+      const int64_t i = i0 + q;
+      if (i < n) buf[L + i] = (double)xv[q] - (a + b * (double)i);
+    }
+  }
   if (P.reflect) {
     const double d0 = buf[L];
     for (int64_t k = 1; k <= L; k++) buf[L - k] = 2 * d0 - buf[L + (k < n - 1 ? k : n - 1)];
@@ -98,13 +116,22 @@ __device__ void iir_row(const DevFirstBreak &P, const ColT<T> &x, int64_t n, con
     int64_t done = 0;
     while (done < m) {
       const int64_t run = m - done < 256 ? m - done : 256;
-      for (int64_t k = 0; k < run; k++) {
-        const int64_t j = done + k;
-        const double xv = buf[j];
-        const double y = b0 * xv + s1;
-        s1 = b1 * xv - a1 * y + s2;
-        s2 = b2 * xv - a2 * y;
-        buf[j] = y;
+      for (int64_t k0 = 0; k0 < run; k0 += CH1) {
+        // each index is read and written once per pass, so the chunk's
+        // inputs can be read before its outputs are stored
+        double xv[CH1];
+#pragma unroll
+        for (int q = 0; q < CH1; q++)
+          if (k0 + q < run) xv[q] = buf[done + k0 + q];
+#pragma unroll
+        for (int q = 0; q < CH1; q++) {
+          if (k0 + q < run) {
+            const double y = b0 * xv[q] + s1;
+            s1 = b1 * xv[q] - a1 * y + s2;
+            s2 = b2 * xv[q] - a2 * y;
+            buf[done + k0 + q] = y;
+          }
+        }
       }
       s1 = fabs(s1) < 1e-250 ? 0.0 : s1;
       s2 = fabs(s2) < 1e-250 ? 0.0 : s2;
@@ -112,7 +139,17 @@ __device__ void iir_row(const DevFirstBreak &P, const ColT<T> &x, int64_t n, con
     }
   }
   const double ga = a * P.dc_gain - b * P.dc_slope, gb = b * P.dc_gain;
-  for (int64_t i = 0; i < n; i++) x[i] = (T)(buf[L + i] + (ga + gb * (double)i));
+  for (int64_t i0 = 0; i0 < n; i0 += CH1) {
+    double yv[CH1];
+#pragma unroll
+    for (int q = 0; q < CH1; q++)
+      if (i0 + q < n) yv[q] = buf[L + i0 + q];
+#pragma unroll
+    for (int q = 0; q < CH1; q++) {
+      const int64_t i = i0 + q;
+      if (i < n) x[i] = (T)(yv[q] + (ga + gb * (double)i));
+    }
+  }
 }
 
 // meanPower (qc/snr.nim:151-161) over the columns c0 .. c1 cut to 0 .. n - 1.
@@ -121,9 +158,14 @@ __device__ double mean_power(const X &x, int64_t n, int64_t c0, int64_t c1) {
   const int64_t a = c0 > 0 ? c0 : 0, b = c1 < n - 1 ? c1 : n - 1;
   if (a > b) return NAN;
   double s = 0;
-  for (int64_t j = a; j <= b; j++) {
-    const double v = (double)x[j];
-    s += v * v;
+  for (int64_t j0 = a; j0 <= b; j0 += CH1) {
+    double v[CH1];
+#pragma unroll
+    for (int q = 0; q < CH1; q++)
+      if (j0 + q <= b) v[q] = (double)x[j0 + q];
+#pragma unroll
+    for (int q = 0; q < CH1; q++)
+      if (j0 + q <= b) s += v[q] * v[q];
   }
   return s / (double)(b - a + 1);
 }
@@ -140,10 +182,18 @@ __device__ void pick_stretch(const DevFirstBreak &P, const X &x, int64_t first, 
                              double &time_ms, int64_t &col, double &snr_db, double &conf) {
   double s = 0;
   p[0] = 0;
-  for (int64_t i = 0; i < m; i++) {
-    const double v = (double)x[i];
-    s += v * v;
-    p[i + 1] = s;
+  for (int64_t i0 = 0; i0 < m; i0 += CH1) {
+    double v[CH1];
+#pragma unroll
+    for (int q = 0; q < CH1; q++)
+      if (i0 + q < m) v[q] = (double)x[i0 + q];
+#pragma unroll
+    for (int q = 0; q < CH1; q++) {
+      if (i0 + q < m) {
+        s += v[q] * v[q];
+        p[i0 + q + 1] = s;
+      }
+    }
   }
   const double total = p[m];
   if (!(total > 0) || total == INFINITY) return;
@@ -158,43 +208,105 @@ __device__ void pick_stretch(const DevFirstBreak &P, const X &x, int64_t first, 
   int64_t i0 = 0;
   if (P.method == 0) {
     i0 = lo > w + sw ? lo : w + sw;
-    for (int64_t i = i0 - 1; i <= last_read; i++) {
-      const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
-      const int64_t m0 = s0 - lta > 0 ? s0 - lta : 0;
-      a[i] = s0 - m0 < sw ? 0.0
-                          : ((p[i + 1] - p[s0]) / (double)(i + 1 - s0)) /
-                                ((p[s0] - p[m0]) / (double)(s0 - m0) + eps);
+    for (int64_t b0 = i0 - 1; b0 <= last_read; b0 += CH) {
+      double pe[CH], ps[CH], pm[CH];
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int64_t i = b0 + q;
+        if (i <= last_read) {
+          const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+          const int64_t m0 = s0 - lta > 0 ? s0 - lta : 0;
+          pe[q] = p[i + 1];
+          ps[q] = p[s0];
+          pm[q] = p[m0];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int64_t i = b0 + q;
+        if (i <= last_read) {
+          const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+          const int64_t m0 = s0 - lta > 0 ? s0 - lta : 0;
+          a[i] = s0 - m0 < sw ? 0.0
+                              : ((pe[q] - ps[q]) / (double)(i + 1 - s0)) /
+                                    ((ps[q] - pm[q]) / (double)(s0 - m0) + eps);
+        }
+      }
     }
   } else if (P.method == 1) {
     const int64_t q = w / 4 > 1 ? w / 4 : 1;
-    for (int64_t i = lo; i < hi; i++) {
-      int64_t k = i < n - i ? i : n - i;
-      k = w < k ? w : k;
-      if (k >= sw) {
-        const int64_t e = n < i + q ? n : i + q;
-        const double r = (p[i + k] - p[i]) / ((p[i] - p[i - k]) + eps * (double)k) *
-                         __dsqrt_rn(p[e] - p[i]);
-        if (r > best) {
-          best = r;
-          best_i = i;
+    for (int64_t b0 = lo; b0 < hi; b0 += CH) {
+      double pa[CH], pi[CH], pb[CH], pe[CH];
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int64_t i = b0 + u;
+        int64_t k = i < n - i ? i : n - i;
+        k = w < k ? w : k;
+        if (i < hi && k >= sw) {
+          const int64_t e = n < i + q ? n : i + q;
+          pa[u] = p[i + k];
+          pi[u] = p[i];
+          pb[u] = p[i - k];
+          pe[u] = p[e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CH; u++) {
+        const int64_t i = b0 + u;
+        int64_t k = i < n - i ? i : n - i;
+        k = w < k ? w : k;
+        if (i < hi && k >= sw) {
+          const double r = (pa[u] - pi[u]) / ((pi[u] - pb[u]) + eps * (double)k) *
+                           __dsqrt_rn(pe[u] - pi[u]);
+          if (r > best) {
+            best = r;
+            best_i = i;
+          }
         }
       }
     }
   } else {
     i0 = lo > w ? lo : w;
-    for (int64_t i = i0 - 1; i <= last_read; i++) {
-      const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
-      a[i] = (p[i + 1] - p[s0]) / (p[i + 1] + beta + eps);
+    for (int64_t b0 = i0 - 1; b0 <= last_read; b0 += CH) {
+      double pe[CH], ps[CH];
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int64_t i = b0 + q;
+        if (i <= last_read) {
+          const int64_t s0 = i + 1 - w > 0 ? i + 1 - w : 0;
+          pe[q] = p[i + 1];
+          ps[q] = p[s0];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int64_t i = b0 + q;
+        if (i <= last_read) a[i] = (pe[q] - ps[q]) / (pe[q] + beta + eps);
+      }
     }
   }
   if (P.method != 1) {
-    for (int64_t i = i0; i < hi; i++) {
-      const int64_t j = n - 1 < i - 1 + sw ? n - 1 : i - 1 + sw;
-      if (j > i - 1) {
-        const double r = (a[j] - a[i - 1]) / (double)(j - i + 1);
-        if (r > best) {
-          best = r;
-          best_i = i;
+    for (int64_t b0 = i0; b0 < hi; b0 += CH) {
+      double aj[CH], ai[CH];
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int64_t i = b0 + q;
+        const int64_t j = n - 1 < i - 1 + sw ? n - 1 : i - 1 + sw;
+        if (i < hi && j > i - 1) {
+          aj[q] = a[j];
+          ai[q] = a[i - 1];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < CH; q++) {
+        const int64_t i = b0 + q;
+        const int64_t j = n - 1 < i - 1 + sw ? n - 1 : i - 1 + sw;
+        if (i < hi && j > i - 1) {
+          const double r = (aj[q] - ai[q]) / (double)(j - i + 1);
+          if (r > best) {
+            best = r;
+            best_i = i;
+          }
         }
       }
     }
@@ -299,7 +411,15 @@ __global__ __launch_bounds__(64) void firstbreak_kernel(const DevFirstBreak *__r
       const Col p{p_, rows, r}, aa{a_, rows, r};
       if (P.filtered) {
         const ColT<T> c{copy_, rows, r};
-        for (int64_t j = 0; j < len; j++) c[j] = x[first + j];
+        for (int64_t j0 = 0; j0 < len; j0 += CH1) {
+          T v[CH1];
+#pragma unroll
+          for (int q = 0; q < CH1; q++)
+            if (j0 + q < len) v[q] = x[first + j0 + q];
+#pragma unroll
+          for (int q = 0; q < CH1; q++)
+            if (j0 + q < len) c[j0 + q] = v[q];
+        }
         iir_row<T>(P, c, len, Col{buf_, rows, r});
         pick_stretch<T>(P, c, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
       } else {

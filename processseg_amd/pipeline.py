@@ -586,6 +586,17 @@ def _ops_c(ops: Sequence[Op]):
 
 # ------------------------------------------------------------------- meta
 
+# pseg_calib_row (include/pseg.h) as a numpy record, field for field
+_CALIB_ROW = np.dtype({
+    "names": ["status", "failed", "sensor_type", "file_physical", "component", "channel_set",
+              "rcv_index", "channel", "rcv_line", "rcv_point", "file_descale",
+              "file_sensitivity", "file_unit", "reserved"],
+    "formats": ["u1", "u1", "u1", "u1", "<i4", "<i4", "<i4", "<f8", "<f8", "<f8", "<f8", "<f8",
+                "<i4", "<i4"],
+    "offsets": [0, 1, 2, 3, 4, 8, 12, 16, 24, 32, 40, 48, 56, 60],
+    "itemsize": 64})
+
+
 class ChunkMeta:
     """The per-row facts of a block (core/types.nim:69-122): status per row
     (tsLive, tsUnknown, tsDead, tsAuxiliary), trace length numSamples per
@@ -665,26 +676,33 @@ class ChunkMeta:
         return bool(np.any(self.failed == row))
 
     def calib_rows(self):
-        """The rows as resolveRow reads them (pseg_calib_row per row)."""
+        """The rows as resolveRow reads them (a pseg_calib_row per row), built
+        once per meta like its device copies."""
+        arr = self._cache.get("calib_rows")
+        if arr is not None:
+            return arr
         n = self.rows
-        arr = (_lib.CalibRowC * max(1, n))()
-        failed = set(int(f) for f in self.failed)
-        for r in range(n):
-            c = arr[r]
-            c.status = int(self.status[r])
-            c.failed = 1 if r in failed else 0
-            c.sensor_type = int(self.sensorType[r])
-            c.component = int(self.component[r])
-            c.channel_set = int(self.channelSet[r])
-            c.rcv_index = int(self.rcvIndex[r])
-            c.channel = float(self.channel[r])
-            c.rcv_line = float(self.rcvLine[r])
-            c.rcv_point = float(self.rcvPoint[r])
-            fs = self.scales[r] if self.scales is not None else None
-            c.file_descale = fs.descale if fs is not None else math.nan
-            c.file_sensitivity = fs.sensitivity if fs is not None else math.nan
-            c.file_unit = fs.unit if fs is not None else 0
-            c.file_physical = 1 if (fs is not None and fs.physical) else 0
+        a = np.zeros(max(1, n), _CALIB_ROW)
+        a["status"][:n] = self.status
+        if self.failed.size:
+            a["failed"][self.failed[(self.failed >= 0) & (self.failed < n)]] = 1
+        a["sensor_type"][:n] = self.sensorType
+        a["component"][:n] = self.component
+        a["channel_set"][:n] = self.channelSet
+        a["rcv_index"][:n] = self.rcvIndex
+        a["channel"][:n] = self.channel
+        a["rcv_line"][:n] = self.rcvLine
+        a["rcv_point"][:n] = self.rcvPoint
+        if self.scales is not None:
+            a["file_descale"][:n] = [fs.descale for fs in self.scales]
+            a["file_sensitivity"][:n] = [fs.sensitivity for fs in self.scales]
+            a["file_unit"][:n] = [fs.unit for fs in self.scales]
+            a["file_physical"][:n] = [1 if fs.physical else 0 for fs in self.scales]
+        else:
+            a["file_descale"] = math.nan
+            a["file_sensitivity"] = math.nan
+        arr = (_lib.CalibRowC * max(1, n)).from_buffer(a)
+        self._cache["calib_rows"] = arr
         return arr
 
     def keys(self) -> np.ndarray:
