@@ -1739,6 +1739,7 @@ pseg_status pseg_spectrum_average_noalloc(const pseg_spectrum_plan *plan, const 
 struct pseg_spectrogram_plan {
   int dtype;
   int32_t window, hop, samples, frames, nfft, bins, threads;
+  int32_t avg_threads;  // the averaged form: one wave per frame when its passes fit
   size_t lds;
   DevSpec *dev;
   char *base;
@@ -1802,6 +1803,10 @@ pseg_status spectrogram_compile(int32_t window, int32_t hop, double dt, int32_t 
   P->nfft = (int32_t)n;
   P->bins = (int32_t)(n / 2 + 1);
   P->threads = threads;
+  // The averaged form runs many frames per CU at once: one wave per frame
+  // when every pass fits 64 threads (a 512-point transform's fused passes
+  // hold 16 butterflies, so 240 of 256 threads would idle in them).
+  P->avg_threads = fft_fits(n % 2 ? n : n / 2, 64, kFuseCorr, kBudgetCorr) ? 64 : threads;
   P->lds = lds;
   P->dev = (DevSpec *)base;
   P->base = base;
@@ -1849,7 +1854,7 @@ struct SgramWs {
 };
 SgramWs sgram_ws(const pseg_spectrogram_plan *plan, int32_t rows) {
   SgramWs w;
-  w.groups = sgram_groups(rows, plan->frames);
+  w.groups = sgram_groups(rows, plan->frames, plan->avg_threads);
   w.status_off = 0;
   w.parts_off = ((size_t)rows * sizeof(int32_t) + 255) / 256 * 256;
   w.used_off = w.parts_off + (size_t)w.groups * (size_t)plan->frames * (size_t)plan->bins * sizeof(double);
@@ -1893,7 +1898,7 @@ pseg_status pseg_spectrogram_average(const pseg_spectrogram_plan *plan, const vo
                 ws ? ws_bytes : (size_t)0, rows, w.bytes);
   char *base = (char *)ws;
   hipError_t e = launch_spectrogram_average(
-      plan->dtype, plan->threads, plan->dev, plan->lds, plan->bins, src, rows, cols, row_len,
+      plan->dtype, plan->avg_threads, plan->dev, plan->lds, plan->bins, src, rows, cols, row_len,
       plan->hop, plan->frames, w.groups, (int32_t *)(base + w.status_off),
       (double *)(base + w.parts_off), (double *)(base + w.used_off), amp, used, s);
   if (e != hipSuccess) return hip_fail(e, "spectrogram launch");

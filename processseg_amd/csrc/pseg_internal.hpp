@@ -361,11 +361,13 @@ hipError_t launch_spectrogram_average(int dtype, int threads, const DevSpec *dev
                                       int32_t groups, int32_t *status, double *parts,
                                       double *used, double *amp, double *used_out,
                                       hipStream_t stream);
-// Workgroup rows of an averaged spectrogram: about 2048 workgroups over the
-// frames, at least one row each; a function of the counts only (the sum's
-// order is fixed by it).
-inline int32_t sgram_groups(int32_t rows, int32_t frames) {
-  int64_t g = (2048 + (int64_t)frames - 1) / (frames > 0 ? frames : 1);
+// Workgroup rows of an averaged spectrogram: about 8 waves per SIMD of
+// workgroups over the frames (2048 of 256 threads, 8192 of 64), at least one
+// row each; a function of the counts and the plan only (the sum's order is
+// fixed by it).
+inline int32_t sgram_groups(int32_t rows, int32_t frames, int threads) {
+  const int64_t target = (int64_t)2048 * 256 / (threads > 0 ? threads : 256);
+  int64_t g = (target + (int64_t)frames - 1) / (frames > 0 ? frames : 1);
   if (g > rows) g = rows;
   if (g > 65535) g = 65535;
   return (int32_t)(g < 1 ? 1 : g);

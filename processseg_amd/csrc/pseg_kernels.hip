@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const T *__restrict__ x,
   __shared__ typename Bits<T>::S smax[4];
   using S = typename Bits<T>::S;
   const int64_t row = blockIdx.x;
-  const int n = row_len[row];
+  const int n = min(row_len[row], cols);  // a length past the row reads the row only
   if (n <= 0) {
     if (threadIdx.x == 0) {
       row_sq[row] = n < 0 ? -1.0 : 0.0;
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void hist_kernel(const T *__restrict__ x, int3
                                                    const uint32_t *__restrict__ row_max,
                                                    uint32_t *__restrict__ hist) {
   const int64_t row = blockIdx.x;
-  const int n = row_len[row];
+  const int n = min(row_len[row], cols);
   if (n <= 0 || !row_usable<T>(row_sq, row_max, row)) return;
   const T *r = x + row * (int64_t)cols;
   for (int j = threadIdx.x; j < n; j += 256) {
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(NTC) void corr_kernel(const DevCorr *__restrict__ p
   extern __shared__ __align__(16) char lds[];
   const DevCorr &c = *plan;
   const int64_t row = blockIdx.x;
-  const int live = row_len[row];
+  const int live = min(row_len[row], src_cols);
   T *d = dst + row * (int64_t)c.out;
   if (live < 0) {
     for (int j = threadIdx.x; j < c.out; j += NTC) d[j] = (T)0;
@@ -1166,12 +1166,14 @@ hipError_t launch_spectrogram_average(int dtype, int threads, const DevSpec *dev
                                       hipStream_t s) {
   if (dtype == 0) {
     const float *x = (const float *)src;
-    return threads == 256 ? sgram_avg_typed<float, 256>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+    return threads == 64 ? sgram_avg_typed<float, 64>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+         : threads == 256 ? sgram_avg_typed<float, 256>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
          : threads == 512 ? sgram_avg_typed<float, 512>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
                           : sgram_avg_typed<float, 1024>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s);
   }
   const double *x = (const double *)src;
-  return threads == 256 ? sgram_avg_typed<double, 256>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+  return threads == 64 ? sgram_avg_typed<double, 64>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
+       : threads == 256 ? sgram_avg_typed<double, 256>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
        : threads == 512 ? sgram_avg_typed<double, 512>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s)
                         : sgram_avg_typed<double, 1024>(dev, lds, bins, x, rows, cols, row_len, hop, frames, groups, status, parts, used, amp, used_out, s);
 }
