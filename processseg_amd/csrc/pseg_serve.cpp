@@ -138,13 +138,16 @@ struct Request {
 
 constexpr int kSlots = 2;        // launches in flight at most
 constexpr int kMaxJobs = 1024;   // requests per launch at most
+// launches per batch at most: one per (threads per row: 128, 256, 512) x
+// (kernel form: core, every step but IIR, every step)
+constexpr int kMaxLaunches = 9;
 
 struct Slot {
   bool busy = false;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   Job *host_jobs = nullptr;      // pinned staging
-  char *dev = nullptr;           // job table + one counter per launch (at most 6)
+  char *dev = nullptr;           // job table + one counter per launch (kMaxLaunches)
   std::vector<uint64_t> tickets;
 };
 
@@ -294,9 +297,13 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     size_t lds = 1024;
     int32_t item0 = 0;
     const size_t first_job = n_jobs;
+    // A class of one request runs as a Plan.apply does: its job by value,
+    // one workgroup per row, with no job table to copy and no work counter
+    // to clear (what a lightly loaded server launches for most requests).
+    const bool by_value = v.size() == 1;
     for (const Item &it : v) {
       Request &r = *it.r;
-      Job &J = S.host_jobs[n_jobs++];
+      Job &J = by_value ? L.single : S.host_jobs[n_jobs++];
       memset(&J, 0, sizeof J);
       J.plan = it.plan->dev;
       J.src = (const char *)cfg.survey + ((size_t)r.trace0 * cfg.pitch + r.first) * esz;
@@ -318,8 +325,8 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
       for (int s = 0; s < it.plan->n_steps; s++) lds = std::max(lds, it.plan->step_lds[s]);
       L.max_cols = std::max(L.max_cols, r.window);
     }
-    L.jobs = (const Job *)(S.dev + first_job * sizeof(Job));
-    L.n_jobs = (int32_t)v.size();
+    L.jobs = by_value ? nullptr : (const Job *)(S.dev + first_job * sizeof(Job));
+    L.n_jobs = by_value ? 0 : (int32_t)v.size();
     L.total = item0;
     L.lds_bytes = lds;
     const int threads = kv.first.first;
@@ -341,12 +348,14 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     launch_threads.push_back(threads);
   }
   hipError_t e = hipSuccess;
-  if (n_jobs > 0) {
+  if (n_jobs > 0)
     e = hipMemcpyAsync(S.dev, S.host_jobs, n_jobs * sizeof(Job), hipMemcpyHostToDevice, S.stream);
-    for (size_t i = 0; i < launches.size() && e == hipSuccess; i++) {
+  if (launches.size() > (size_t)kMaxLaunches) e = hipErrorInvalidValue;  // not reached: 3 x 3 classes
+  for (size_t i = 0; i < launches.size() && e == hipSuccess; i++) {
+    // a table launch is persistent (rows from the counter); a by-value one direct
+    if (launches[i].n_jobs > 0)
       launches[i].counter = (uint32_t *)(S.dev + kMaxJobs * sizeof(Job) + 256 * i);
-      e = launch_pipeline(cfg.dtype, launch_threads[i], launches[i], S.stream);
-    }
+    e = launch_pipeline(cfg.dtype, launch_threads[i], launches[i], S.stream);
   }
   if (e == hipSuccess) e = hipEventRecord(S.done, S.stream);
   lk.lock();
@@ -455,7 +464,7 @@ pseg_status pseg_server_create(const pseg_server_config *cfg, pseg_server **out)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess)
       e = hipHostMalloc((void **)&sl.host_jobs, kMaxJobs * sizeof(Job), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc((void **)&sl.dev, kMaxJobs * sizeof(Job) + 256 * 8);
+    if (e == hipSuccess) e = hipMalloc((void **)&sl.dev, kMaxJobs * sizeof(Job) + 256 * kMaxLaunches);
   }
   if (e != hipSuccess) {
     pseg_status st = hip_error(e, "server set-up");
