@@ -204,3 +204,46 @@ def test_served_flows_read_trace_keys(gpu, survey):
                    keys=kd[t0:t0 + n].contiguous(), first_sample=first)
         want = dst.cpu().numpy()[:, a - first:b - first + 1]
         assert np.array_equal(got, want), (t0, n, a, b)
+
+
+def test_every_launch_class_in_one_batch(gpu):
+    """Nine requests that need the nine launch classes (windows of 128, 256
+    and 512 threads per row x the core, mute and IIR kernel forms), queued
+    behind a long request so they share one batch: each equals Plan.apply on
+    its decoded window, bit for bit."""
+    import torch
+    traces, samples = 2000, 7000
+    x, _ = synth.gather(traces, samples, DT, 33)
+    tl = np.full(traces, samples, np.int32)
+    xs, tls = torch.from_numpy(x).cuda(), torch.from_numpy(tl).cuda()
+    F = [P.initPipeline([P.bandPassOp(P.Corners(4, 8, 60, 80)), P.agcOp(300)]),
+         P.initPipeline([P.muteTopOp(P.timeMute(100.0), taperMs=20), P.agcOp(300)]),
+         P.initPipeline([P.iirOp(P.butterBandPass(5, 70, 24)), P.agcOp(300)])]
+    srv = P.Server(xs, tls, DT, arena_bytes=1 << 30, max_inflight=1)
+    ids = [srv.add_flow(f) for f in F]
+    blocker = srv.submit(ids[2], 0, traces, 0, samples - 1)
+    reqs = [(f, 10 + 30 * i, 25, a, a + width - 1)
+            for i, (f, (a, width)) in enumerate(
+                (f, w) for f in range(3) for w in ((100, 1500), (500, 4500), (0, 6900)))]
+    tickets = [srv.submit(ids[f], t0, n, a, b) for f, t0, n, a, b in reqs]
+    srv.wait(blocker)
+    srv.release(blocker)
+    shared = 0
+    for (f, t0, n, a, b), t in zip(reqs, tickets):
+        res = srv.wait(t)
+        shared = max(shared, res.batch_requests)
+        got = srv.read(t).cpu().numpy()
+        srv.release(t)
+        m = F[f].requiredMargin(DT)
+        w = P.decodeWindow(range(a, b + 1), m, samples)
+        first, cols = w.start, len(w)
+        win = torch.from_numpy(np.ascontiguousarray(x[t0:t0 + n, first:first + cols])).cuda()
+        plan = F[f].compile(np.float32, DT, cols, recordSamples=samples)
+        dst = torch.empty_like(win)
+        # the window's column 0 is trace sample `first` (the mute's time axis)
+        plan.apply(win, dst, row_len=torch.full((n,), cols, dtype=torch.int32, device="cuda"),
+                   first_sample=first)
+        want = dst.cpu().numpy()[:, a - first:b - first + 1]
+        assert np.array_equal(got, want), (f, a, b)
+    print("largest batch", shared, srv.info())
+    srv.close()
