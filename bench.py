@@ -777,6 +777,7 @@ def serve_leg(torch, dev, seconds, rate_hz):
     precompiled plans), the same stream at higher offered loads (a load sweep,
     where the one-launch path queues), and the whole stream submitted at once
     (throughput)."""
+    import gc
     import queue
     import threading
     import processseg_amd as P
@@ -804,6 +805,13 @@ def serve_leg(torch, dev, seconds, rate_hz):
                 r = srv.wait(t)
                 res.append(r)
                 srv.release(t)
+        # the load generator is Python: no garbage-collector pauses and a
+        # short GIL switch interval while it paces the stream, so requests
+        # leave on time (a late submitter turns the Poisson stream into
+        # bursts the server then has to absorb)
+        gc.disable()
+        switch = sys.getswitchinterval()
+        sys.setswitchinterval(1e-4)
         th = threading.Thread(target=collect)
         th.start()
         t0 = time.perf_counter()
@@ -815,6 +823,8 @@ def serve_leg(torch, dev, seconds, rate_hz):
         t_sub = time.perf_counter() - t0
         q.put(None)
         th.join()
+        sys.setswitchinterval(switch)
+        gc.enable()
         wall = time.perf_counter() - t0
         lat = np.array([r.latency_ms for r in res])
         queued = np.array([(r.start_ns - r.submit_ns) / 1e6 for r in res])
