@@ -310,3 +310,29 @@ def test_wide_rows_and_mixed_flow():
     live = (meta.row_len(cols) >= 0) & ~np.any(np.isnan(want), axis=1)
     assert O.rel_l2(got[live].astype(np.float64), want[live]) <= 1e-5
     same_bits(got[~(meta.row_len(cols) >= 0)], cal_x[~(meta.row_len(cols) >= 0)])
+
+
+def test_full_panel_calibrated_flow():
+    """The bench's workload: calibrateOp (a hydrophone rule and a geophone
+    rule over file descales) then demean > band-pass + notch > high cut >
+    AGC on the 2000 x 3001 f32 panel; every row computed, 64 rows held to the
+    float64 oracle (its factors, its scale, its pipeline) within 1e-5
+    relative L2."""
+    from processseg_amd.workloads import big_flow, big_panel
+    x, live = big_panel()
+    rows, cols = x.shape
+    sensor = np.where(np.arange(rows) % 4 == 0, 1, 2).astype(np.uint8)
+    scales = [K.FileScale(1e-6 * (1 + r % 5), NaN, K.auRecorded) for r in range(rows)]
+    meta = P.ChunkMeta(rows, DT, numSamples=live, sensorType=sensor, scales=scales)
+    flow = P.initPipeline([P.calibrateOp(CAL)] + big_flow().ops)
+    got, plan = run(flow, x, meta)
+    assert plan.nCalib == 1 and np.all(np.isfinite(got))
+    pick = np.arange(0, rows, rows // 64)[:64]
+    sub = P.ChunkMeta(len(pick), DT, numSamples=live[pick], sensorType=sensor[pick],
+                      scales=[scales[r] for r in pick])
+    k = oracle_k(CAL, sub)
+    xs = x[pick].astype(np.float64)
+    rl = sub.row_len(cols)
+    want = O.pipeline_apply(big_flow().ops, O.calibrate_rows(xs, k, sub.row_len(cols, 2)), rl, DT)
+    e = float(np.linalg.norm(got[pick] - want) / np.linalg.norm(want))
+    assert e <= 1e-5, e

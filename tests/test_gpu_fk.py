@@ -106,3 +106,18 @@ def test_refusals():
                              0, amp)
     with pytest.raises(ValueError):
         plan.fkAmplitudeInto(src, meta, 0, amp[:-1])
+
+
+def test_full_panel():
+    """The bench's workload: the f-k amplitude of the 2000 x 3001 f32 panel
+    (live lengths 2400 .. 3001, 4000 wavenumbers x 1537 frequencies), within
+    2e-6 of the oracle's peak."""
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    meta = P.ChunkMeta(2000, 2.0, numSamples=live)
+    amp, used, plan = gpu_fk(x, meta, 0, 3001, T=np.float32)
+    assert used == 2000 and (plan.nK, plan.nFreq) == (4000, 1537)
+    want, used2 = O.fk_amplitude(O.fk_plan(2000, 3001, 2.0, 0.1, np.float32), x,
+                                 meta.row_len(3001), 0)
+    assert used2 == 2000
+    assert np.max(np.abs(amp - want)) <= 2e-6 * want.max()

@@ -176,3 +176,24 @@ def test_average_spectrogram_reference_cases(gpu, oracle):
         [40], dtype=torch.int32, device="cuda"))
     want, uw = oracle.spectrogram_average(g32, [40], 2, 10, 1.0, 40, 4, dtype=np.float32)
     assert used == uw == 1 and rel_l2(amp.cpu().numpy(), want) <= 1e-5
+
+
+def test_average_spectrogram_full_panel(gpu, oracle):
+    """The bench's workload: the averaged spectrogram of the 2000 x 3001 f32
+    panel (257-sample Hann frames every 10 samples, nfft 512, one wave per
+    frame), within 1e-5 relative L2 of the float64 oracle, the same bits on a
+    second call."""
+    import torch
+    from processseg_amd.workloads import big_panel
+    x, live = big_panel()
+    rl = np.minimum(live, 3001).astype(np.int32)
+    sg = P.initSpectrogramPlan(np.float32, 257, 10, 2.0, 3001)
+    src = torch.from_numpy(x).cuda()
+    rlt = torch.from_numpy(rl).cuda()
+    amp, used = sg.averageSpectrogramInto(src, row_len=rlt)
+    amp2, _ = sg.averageSpectrogramInto(src, row_len=rlt)
+    got = amp.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), amp2.cpu().numpy().view(np.uint64))
+    want, uw = oracle.spectrogram_average(x, rl, 257, 10, 2.0, 3001, 0, dtype=np.float32)
+    assert used == uw == 2000
+    assert rel_l2(got, want) <= 1e-5, rel_l2(got, want)
