@@ -152,6 +152,103 @@ __device__ void iir_row(const DevFirstBreak &P, const ColT<T> &x, int64_t n, con
   }
 }
 
+// iir_row for cascades of at most kFusedSections biquads, in ONE pass over
+// the row and no padded copy: CH1 padded samples at a time (the odd
+// reflection before the row, then the trend-removed row, formed as the
+// serial code forms them) run through every biquad in turn, the biquads'
+// states kept per lane in LDS between chunks, and the last biquad's output
+// goes back to x with the trend's response.  Each biquad sees the same
+// inputs in the same order and does the same operations as in the
+// section-by-section passes (its steady-state start from its own first
+// input, its states flushed at the same 256-sample boundaries, which are
+// chunk ends), so every value is the same; the row crosses memory once
+// instead of once per biquad.
+constexpr int kFusedSections = 8;
+template <typename T>
+__device__ void iir_row_fused(const DevFirstBreak &P, const ColT<T> &x, int64_t n,
+                              double (*st)[2][64]) {
+  if (n <= 0) return;
+  double a = 0, b = 0;
+  if (P.trend == 1) {
+    a = sum64(x, n) / (double)n;
+  } else if (P.trend == 2) {
+    const double sx = sum64(x, n), nf = (double)n, mean = sx / nf;
+    if (n == 1) {
+      a = mean;
+    } else {
+      const double c = (nf - 1) / 2;
+      const double slope = (sum_indexed64(x, n) - c * sx) / (nf * (nf * nf - 1) / 12);
+      a = mean - slope * c;
+      b = slope;
+    }
+  }
+  const int64_t L = P.lead, m = L + n;
+  const int S = P.n_sections;
+  const int lane = threadIdx.x;
+  // buf[L] of the serial code: the first trend-removed sample
+  const double d0 = (double)x[0] - (a + b * (double)0);
+  const double ga = a * P.dc_gain - b * P.dc_slope, gb = b * P.dc_gain;
+  for (int s = 0; s < S; s++) st[s][0][lane] = st[s][1][lane] = 0;
+  static_assert(256 % CH1 == 0, "flush boundaries must be chunk ends");
+  for (int64_t j0 = 0; j0 < m; j0 += CH1) {
+    // the chunk's inputs: padded sample j reads x at min(L - j, n - 1)
+    // (before the row, the reflection) or j - L
+    T xv[CH1];
+#pragma unroll
+    for (int q = 0; q < CH1; q++) {
+      const int64_t j = j0 + q;
+      if (j < m) {
+        const int64_t k = L - j;
+        xv[q] = x[j < L ? (k < n - 1 ? k : n - 1) : j - L];
+      }
+    }
+    double u[CH1];
+#pragma unroll
+    for (int q = 0; q < CH1; q++) {
+      const int64_t j = j0 + q;
+      if (j < L) {
+        const int64_t k = L - j, i = k < n - 1 ? k : n - 1;
+        u[q] = 2 * d0 - ((double)xv[q] - (a + b * (double)i));
+      } else {
+        u[q] = (double)xv[q] - (a + b * (double)(j - L));
+      }
+    }
+    const int cnt = m - j0 < CH1 ? (int)(m - j0) : CH1;
+    const bool flush = (j0 + CH1) % 256 == 0 || j0 + cnt == m;
+    for (int s = 0; s < S; s++) {
+      const double b0 = P.sec[s][0], b1 = P.sec[s][1], b2 = P.sec[s][2];
+      const double a1 = P.sec[s][3], a2 = P.sec[s][4];
+      double s1 = st[s][0][lane], s2 = st[s][1][lane];
+      if (j0 == 0 && P.reflect) {
+        // the steady-state start on this biquad's first input
+        const double y = P.sec[s][5] * u[0];
+        s2 = b2 * u[0] - a2 * y;
+        s1 = y - b0 * u[0];
+      }
+#pragma unroll
+      for (int q = 0; q < CH1; q++) {
+        if (q < cnt) {
+          const double y = b0 * u[q] + s1;
+          s1 = b1 * u[q] - a1 * y + s2;
+          s2 = b2 * u[q] - a2 * y;
+          u[q] = y;
+        }
+      }
+      if (flush) {
+        s1 = fabs(s1) < 1e-250 ? 0.0 : s1;
+        s2 = fabs(s2) < 1e-250 ? 0.0 : s2;
+      }
+      st[s][0][lane] = s1;
+      st[s][1][lane] = s2;
+    }
+#pragma unroll
+    for (int q = 0; q < CH1; q++) {
+      const int64_t j = j0 + q;
+      if (q < cnt && j >= L) x[j - L] = (T)(u[q] + (ga + gb * (double)(j - L)));
+    }
+  }
+}
+
 // meanPower (qc/snr.nim:151-161) over the columns c0 .. c1 cut to 0 .. n - 1.
 template <typename X>
 __device__ double mean_power(const X &x, int64_t n, int64_t c0, int64_t c1) {
@@ -347,6 +444,7 @@ __global__ __launch_bounds__(64) void firstbreak_kernel(const DevFirstBreak *__r
                                                         int64_t *__restrict__ col,
                                                         double *__restrict__ snr_db,
                                                         double *__restrict__ conf) {
+  __shared__ double iir_st[kFusedSections][2][64];  // the biquads' states, per lane
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   const DevFirstBreak &P = *plan;
@@ -420,7 +518,8 @@ __global__ __launch_bounds__(64) void firstbreak_kernel(const DevFirstBreak *__r
           for (int q = 0; q < CH1; q++)
             if (j0 + q < len) c[j0 + q] = v[q];
         }
-        iir_row<T>(P, c, len, Col{buf_, rows, r});
+        if (P.n_sections <= kFusedSections) iir_row_fused<T>(P, c, len, iir_st);
+        else iir_row<T>(P, c, len, Col{buf_, rows, r});
         pick_stretch<T>(P, c, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
       } else {
         const RowT<T> y{src + r * (int64_t)cols + first};
