@@ -66,6 +66,21 @@ def test_methods_and_bands(gpu, oracle, dtype, band, method):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("slope", [6.0, 48.0, 54.0, 96.0])
+def test_band_pass_slopes(gpu, oracle, dtype, slope):
+    """Band-passes of 1 to 16 biquads per edge: up to eight biquads the
+    cascade runs in one pass over the row, beyond in one pass per biquad
+    through the padded copy; both give the oracle's picks exactly."""
+    x, off = shot(0.1, 7)
+    rl = np.full(x.shape[0], COLS, np.int32)
+    for method in METHODS:
+        p = FirstBreakParams(pickMethod=method, lowCutHz=8.0, highCutHz=70.0,
+                             slopeDbPerOctave=slope, **GUIDED)
+        want, got = both(oracle, x, rl, keys_of(off), p, dtype)
+        check(want, got, (slope, method))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_no_picks_and_block_windows(gpu, oracle, dtype):
     x, off = shot(0.05, 4)
     off = off.copy()
