@@ -164,24 +164,38 @@ __device__ void iir_row(const DevFirstBreak &P, const ColT<T> &x, int64_t n, con
 // chunk ends), so every value is the same; the row crosses memory once
 // instead of once per biquad.
 constexpr int kFusedSections = 8;
-template <typename T>
-__device__ void iir_row_fused(const DevFirstBreak &P, const ColT<T> &x, int64_t n,
-                              double (*st)[2][64]) {
-  if (n <= 0) return;
-  double a = 0, b = 0;
-  if (P.trend == 1) {
-    a = sum64(x, n) / (double)n;
-  } else if (P.trend == 2) {
-    const double sx = sum64(x, n), nf = (double)n, mean = sx / nf;
+// The trend iir_row removes (filter/edges.nim through iir_row): the mean,
+// or the least-squares line, from sum64 / sum_indexed64 of x.
+__device__ __forceinline__ void trend_of(int trend, int64_t n, double sx, double sxi, double &a,
+                                         double &b) {
+  a = 0;
+  b = 0;
+  if (trend == 1) {
+    a = sx / (double)n;
+  } else if (trend == 2) {
+    const double nf = (double)n, mean = sx / nf;
     if (n == 1) {
       a = mean;
     } else {
       const double c = (nf - 1) / 2;
-      const double slope = (sum_indexed64(x, n) - c * sx) / (nf * (nf * nf - 1) / 12);
+      const double slope = (sxi - c * sx) / (nf * (nf * nf - 1) / 12);
       a = mean - slope * c;
       b = slope;
     }
   }
+}
+
+// sx, sxi: sum64 and sum_indexed64 of x (taken as x was copied).  p (when
+// not NULL) receives the prefix energies of the output as pick_stretch
+// makes them (p[0] = 0, p[i + 1] = p[i] + y[i]²), their total returned.
+template <typename T>
+__device__ double iir_row_fused(const DevFirstBreak &P, const ColT<T> &x, int64_t n,
+                                double sx, double sxi, double (*st)[2][64], const Col *p) {
+  if (n <= 0) return 0;
+  double a, b;
+  trend_of(P.trend, n, sx, sxi, a, b);
+  double energy = 0;
+  if (p) (*p)[0] = 0;
   const int64_t L = P.lead, m = L + n;
   const int S = P.n_sections;
   const int lane = threadIdx.x;
@@ -244,9 +258,18 @@ __device__ void iir_row_fused(const DevFirstBreak &P, const ColT<T> &x, int64_t 
 #pragma unroll
     for (int q = 0; q < CH1; q++) {
       const int64_t j = j0 + q;
-      if (q < cnt && j >= L) x[j - L] = (T)(u[q] + (ga + gb * (double)(j - L)));
+      if (q < cnt && j >= L) {
+        const T y = (T)(u[q] + (ga + gb * (double)(j - L)));
+        x[j - L] = y;
+        if (p) {
+          const double v = (double)y;
+          energy += v * v;
+          (*p)[j - L + 1] = energy;
+        }
+      }
     }
   }
+  return energy;
 }
 
 // meanPower (qc/snr.nim:151-161) over the columns c0 .. c1 cut to 0 .. n - 1.
@@ -276,10 +299,11 @@ template <typename T> struct RowT {
 template <typename T, typename X>
 __device__ void pick_stretch(const DevFirstBreak &P, const X &x, int64_t first, int64_t m,
                              int64_t lo, int64_t hi, double origin, const Col &p, const Col &a,
-                             double &time_ms, int64_t &col, double &snr_db, double &conf) {
-  double s = 0;
-  p[0] = 0;
-  for (int64_t i0 = 0; i0 < m; i0 += CH1) {
+                             double &time_ms, int64_t &col, double &snr_db, double &conf,
+                             bool have_p = false, double p_total = 0) {
+  double s = have_p ? p_total : 0;
+  if (!have_p) p[0] = 0;
+  for (int64_t i0 = 0; i0 < (have_p ? 0 : m); i0 += CH1) {
     double v[CH1];
 #pragma unroll
     for (int q = 0; q < CH1; q++)
@@ -292,7 +316,7 @@ __device__ void pick_stretch(const DevFirstBreak &P, const X &x, int64_t first, 
       }
     }
   }
-  const double total = p[m];
+  const double total = s;  // p[m]
   if (!(total > 0) || total == INFINITY) return;
   const double mean = total / (double)m;
   const double eps = 1e-6 * mean;
@@ -509,18 +533,43 @@ __global__ __launch_bounds__(64) void firstbreak_kernel(const DevFirstBreak *__r
       const Col p{p_, rows, r}, aa{a_, rows, r};
       if (P.filtered) {
         const ColT<T> c{copy_, rows, r};
+        // the copy, with sum64 and sum_indexed64 of it in their order: eight
+        // accumulators over the first len / 8 * 8 samples (sample i into
+        // accumulator i mod 8, the chunks being multiples of 8), the rest
+        // added after them (read back below)
+        const int64_t n8 = len / 8 * 8;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0}, acci[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int64_t j0 = 0; j0 < len; j0 += CH1) {
           T v[CH1];
 #pragma unroll
           for (int q = 0; q < CH1; q++)
             if (j0 + q < len) v[q] = x[first + j0 + q];
 #pragma unroll
-          for (int q = 0; q < CH1; q++)
-            if (j0 + q < len) c[j0 + q] = v[q];
+          for (int q = 0; q < CH1; q++) {
+            const int64_t j = j0 + q;
+            if (j < len) c[j] = v[q];
+            if (j < n8) {
+              acc[q % 8] += (double)v[q];
+              acci[q % 8] += (double)j * (double)v[q];
+            }
+          }
         }
-        if (P.n_sections <= kFusedSections) iir_row_fused<T>(P, c, len, iir_st);
-        else iir_row<T>(P, c, len, Col{buf_, rows, r});
-        pick_stretch<T>(P, c, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
+        if (P.n_sections <= kFusedSections) {
+          double sx = 0, sxi = 0;
+          for (int k = 0; k < 8; k++) sx += acc[k];
+          for (int k = 0; k < 8; k++) sxi += acci[k];
+          for (int64_t k = n8; k < len; k++) {
+            const double v = (double)c[k];
+            sx += v;
+            sxi += (double)k * v;
+          }
+          const double total = iir_row_fused<T>(P, c, len, sx, sxi, iir_st, &p);
+          pick_stretch<T>(P, c, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out,
+                          true, total);
+        } else {
+          iir_row<T>(P, c, len, Col{buf_, rows, r});
+          pick_stretch<T>(P, c, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
+        }
       } else {
         const RowT<T> y{src + r * (int64_t)cols + first};
         pick_stretch<T>(P, y, first, len, lo, hi, origin, p, aa, t_out, col_out, s_out, c_out);
