@@ -2924,7 +2924,12 @@ pseg_status fk_compile(double dt, int32_t traces, int32_t samples, double taper,
   const size_t lds_time = std::max(fft_lds_bytes(nt, sizeof(T), kFuseCorr), (size_t)nt * sizeof(T));
   const size_t space = (std::max(fft_lds_bytes(nk, sizeof(T), kFuseCorr), (size_t)nk * sizeof(T)) +
                         15) / 16 * 16;
-  const size_t lds_space = space + (size_t)(nk / 2 + 1) * 2 * sizeof(T);
+  const size_t keep = ((size_t)(nk / 2 + 1) * 2 * sizeof(T) + 15) / 16 * 16;
+  // the column's imaginary parts beside the transform, so the spatial pass
+  // reads each (strided) spectrum value once, when that fits
+  const size_t im = (size_t)traces * sizeof(T);
+  const bool keep_im = space + keep + im <= kMaxRowLds;
+  const size_t lds_space = space + keep + (keep_im ? im : 0);
   if (threads == 0 || lds_time > kMaxRowLds || lds_space > kMaxRowLds)
     return fail(PSEG_E_UNSUPPORTED, "an f-k transform of %lld x %lld points does not fit one "
                 "workgroup", (long long)nt, (long long)nk);
@@ -2941,6 +2946,7 @@ pseg_status fk_compile(double dt, int32_t traces, int32_t samples, double taper,
   H.n_freq = (int32_t)(nt / 2 + 1);
   H.n_k = (int32_t)nk;
   H.space_lds = (int64_t)space;
+  H.im_lds = keep_im ? (int64_t)(space + keep) : 0;
   char *base = nullptr;
   hipError_t e = hipMalloc((void **)&base, blob.bytes.size());
   if (e != hipSuccess) return hip_fail(e, "hipMalloc of an f-k plan");

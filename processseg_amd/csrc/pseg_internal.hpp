@@ -268,6 +268,7 @@ struct DevFk {
   const void *trace_w;  // T, traces
   int32_t traces, samples, n_freq, n_k;
   int64_t space_lds;    // bytes of the spatial transform's buffer
+  int64_t im_lds;       // where the spatial pass keeps a column's imaginary parts (0: re-read)
 };
 // fkAmplitudeInto (fft/fk.nim:438-504): spec (workspace, traces x n_freq T
 // pairs) and flag (one uint32, cleared first) between the two passes; amp

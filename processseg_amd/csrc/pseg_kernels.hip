@@ -1363,19 +1363,37 @@ __global__ __launch_bounds__(NTC) void fk_space_kernel(const DevFk *__restrict__
   extern __shared__ __align__(16) char lds[];
   const DevFk &P = *plan;
   const DevFft &f = P.space;
-  const int jf = blockIdx.x;
   const int traces = P.traces, nf = P.n_freq;
+  // Workgroups go round robin over the 8 XCDs (workgroup b on XCD b mod 8):
+  // each XCD takes a contiguous range of frequencies, so the 128-byte lines
+  // of the time pass's spectra (consecutive frequencies of one trace) that
+  // its workgroups read are shared in its own L2.  The grid is 8 x ceil(nf /
+  // 8); workgroups past the last frequency leave at once.
+  const int per = (nf + 7) / 8;
+  const int jf = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (jf >= nf) return;
   V2<T> *A = (V2<T> *)lds;
   T *buf = (T *)lds;
   V2<T> *keep = (V2<T> *)(lds + P.space_lds);
+  T *im = P.im_lds ? (T *)(lds + P.im_lds) : nullptr;
   const bool extreme = sizeof(T) == 8 && *flag != 0;
-  // the real parts across traces, zero-padded to nK
-  for (int t = threadIdx.x; t < f.n; t += NTC) buf[t] = t < traces ? spec[(int64_t)t * nf + jf].x : (T)0;
+  // the real parts across traces, zero-padded to nK (the imaginary parts
+  // kept beside them when there is room)
+  for (int t = threadIdx.x; t < f.n; t += NTC) {
+    T re = (T)0;
+    if (t < traces) {
+      const V2<T> v = spec[(int64_t)t * nf + jf];
+      re = v.x;
+      if (im) im[t] = v.y;
+    }
+    buf[t] = re;
+  }
   __syncthreads();
   real_forward<T, NTC>(f, A, FkKeepA<T>{keep, f.m});
   __syncthreads();
   // the imaginary parts
-  for (int t = threadIdx.x; t < f.n; t += NTC) buf[t] = t < traces ? spec[(int64_t)t * nf + jf].y : (T)0;
+  for (int t = threadIdx.x; t < f.n; t += NTC)
+    buf[t] = t < traces ? (im ? im[t] : spec[(int64_t)t * nf + jf].y) : (T)0;
   __syncthreads();
   real_forward<T, NTC>(f, A, FkOut<T>{keep, amp, f.m, nf, jf, extreme});
 }
@@ -1396,7 +1414,7 @@ hipError_t fk_typed(const DevFk *dev, size_t lds_time, size_t lds_space, int tra
   hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, stream, flag, 1);
   hipLaunchKernelGGL((fk_time_kernel<T, NTC>), dim3(traces), dim3(NTC), lds_time, stream, dev, src,
                      cols, row_len, live_rows, first_col, spec, flag);
-  hipLaunchKernelGGL((fk_space_kernel<T, NTC>), dim3(n_freq), dim3(NTC), lds_space, stream, dev,
+  hipLaunchKernelGGL((fk_space_kernel<T, NTC>), dim3(8 * ((n_freq + 7) / 8)), dim3(NTC), lds_space, stream, dev,
                      (const V2<T> *)spec, (const uint32_t *)flag, amp);
   return hipGetLastError();
 }
