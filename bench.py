@@ -229,18 +229,25 @@ def panels_batch_leg(torch, P, plan, srcs, dsts, row_len, steps, warmup):
     headline, which applies one panel per step as Plan.apply does."""
     batch = P.PipelineBatch([P.BatchItem(plan, s, d, row_len=row_len)
                              for s, d in zip(srcs, dsts)])
-    for _ in range(warmup):
-        batch.launch()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        batch.launch()
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / steps
+
+    def timed():
+        for _ in range(warmup):
+            batch.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.launch()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+    t_rule = timed()
+    grid_rule = list(batch.grid)
+    batch.tune()          # the launch shape by timing (the same bits)
+    t = timed()
     samples = batch.samples
     return {"workload": f"{len(srcs)} panels of 2000x3001 f32, the headline flow, one batch",
             "value": round(samples / t / 1e6, 1), "unit": "Msamples/s",
             "ms_per_batch": round(t * 1000, 4), "grid": batch.grid,
+            "ms_before_tuning": round(t_rule * 1000, 4), "grid_before_tuning": grid_rule,
             "achieved_GBps": round(samples * 8 / t / 1e9, 1)}
 
 
