@@ -12,9 +12,11 @@
 // One worker thread owns the GPU side.  Each turn it retires finished
 // launches (an event per launch), then admits queued requests in arrival
 // order while their results fit in the arena and the batch limits allow,
-// and packs them into one job table per (threads per row) class: one
-// persistent launch per class, the table copied from pinned memory on the
-// launch's own stream.  With two launches in flight the next batch is built
+// and packs them into one job table per (threads per row, kernel form)
+// class: one launch per class, one workgroup per row in heaviest-first
+// order, the table copied from pinned memory on the launch's own stream (a
+// class of one request passes its job by value).  With two launches in
+// flight the next batch is built
 // and queued while the previous one runs.  A row's arithmetic depends only
 // on its plan and its samples, so a request's bits do not depend on what it
 // shares a launch with.
@@ -138,16 +140,14 @@ struct Request {
 
 constexpr int kSlots = 2;        // launches in flight at most
 constexpr int kMaxJobs = 1024;   // requests per launch at most
-// launches per batch at most: one per (threads per row: 128, 256, 512) x
-// (kernel form: core, every step but IIR, every step)
-constexpr int kMaxLaunches = 9;
+
 
 struct Slot {
   bool busy = false;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   Job *host_jobs = nullptr;      // pinned staging
-  char *dev = nullptr;           // job table + one counter per launch (kMaxLaunches)
+  char *dev = nullptr;           // the job table
   std::vector<uint64_t> tickets;
 };
 
@@ -164,8 +164,6 @@ struct pseg_server {
   std::mutex plan_mu;
   std::vector<Flow> flows;
   std::map<std::tuple<int, int, int>, pseg_plan *> plans;
-  // (threads, per, dtype + 2 x form, lds) -> resident workgroups
-  std::map<std::tuple<int, int, int, size_t>, int> resident;
   Slot slots[kSlots];
   std::mutex mu;
   std::condition_variable work_cv, done_cv;
@@ -281,7 +279,7 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     compiled_now += compiled;
     ok.push_back({r, p});
   }
-  // one persistent launch per threads-per-row class; heavier rows first
+  // one launch per (threads per row, kernel form) class; heavier rows first
   std::map<std::pair<int, int>, std::vector<Item>> classes;  // (threads, kernel form)
   for (const Item &it : ok) classes[{it.plan->threads, it.plan->full}].push_back(it);
   size_t n_jobs = 0;
@@ -298,8 +296,8 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     int32_t item0 = 0;
     const size_t first_job = n_jobs;
     // A class of one request runs as a Plan.apply does: its job by value,
-    // one workgroup per row, with no job table to copy and no work counter
-    // to clear (what a lightly loaded server launches for most requests).
+    // with no job table to copy (what a lightly loaded server launches for
+    // most requests).
     const bool by_value = v.size() == 1;
     for (const Item &it : v) {
       Request &r = *it.r;
@@ -331,30 +329,17 @@ bool pseg_server::launch_batch(std::unique_lock<std::mutex> &lk, int slot_index)
     L.lds_bytes = lds;
     const int threads = kv.first.first;
     L.full = kv.first.second;
-    const int per = (L.max_cols + threads - 1) / threads;
-    auto key = std::make_tuple(threads, per, (int)cfg.dtype + 2 * L.full, lds);
-    auto rit = resident.find(key);
-    int blocks = 0;
-    if (rit == resident.end()) {
-      if (pipeline_resident_blocks(cfg.dtype, threads, L.max_cols, lds, L.full, &blocks) !=
-          hipSuccess)
-        blocks = 256;
-      resident[key] = blocks;
-    } else {
-      blocks = rit->second;
-    }
-    L.grid = std::max(1, std::min(L.total, blocks));
+    L.grid = L.total;  // one workgroup per row (no counter)
     launches.push_back(L);
     launch_threads.push_back(threads);
   }
   hipError_t e = hipSuccess;
   if (n_jobs > 0)
     e = hipMemcpyAsync(S.dev, S.host_jobs, n_jobs * sizeof(Job), hipMemcpyHostToDevice, S.stream);
-  if (launches.size() > (size_t)kMaxLaunches) e = hipErrorInvalidValue;  // not reached: 3 x 3 classes
   for (size_t i = 0; i < launches.size() && e == hipSuccess; i++) {
-    // a table launch is persistent (rows from the counter); a by-value one direct
-    if (launches[i].n_jobs > 0)
-      launches[i].counter = (uint32_t *)(S.dev + kMaxJobs * sizeof(Job) + 256 * i);
+    // one workgroup per row (the dispatcher hands rows out as workgroups
+    // finish, heaviest first): measured faster than a persistent grid that
+    // takes rows from a counter, as the batch tuner finds for batches too
     e = launch_pipeline(cfg.dtype, launch_threads[i], launches[i], S.stream);
   }
   if (e == hipSuccess) e = hipEventRecord(S.done, S.stream);
@@ -464,7 +449,7 @@ pseg_status pseg_server_create(const pseg_server_config *cfg, pseg_server **out)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess)
       e = hipHostMalloc((void **)&sl.host_jobs, kMaxJobs * sizeof(Job), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc((void **)&sl.dev, kMaxJobs * sizeof(Job) + 256 * kMaxLaunches);
+    if (e == hipSuccess) e = hipMalloc((void **)&sl.dev, kMaxJobs * sizeof(Job));
   }
   if (e != hipSuccess) {
     pseg_status st = hip_error(e, "server set-up");
