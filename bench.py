@@ -990,7 +990,9 @@ def main():
             peak = VALU_PEAK_GWINST
             valu = {"achieved": round(rate, 1), "peak": peak, "unit": "G wave-instructions/s",
                     "frac": round(rate / peak, 4),
-                    "insts_per_launch": round(winst)}
+                    "insts_per_launch": round(winst),
+                    "insts_source": "SQ_INSTS_VALU of a rocprofv3 --pmc pass of this command, "
+                                    "profiles/" + os.path.basename(tpath)}
 
     out = dsts[(args.steps - 1) % args.pairs].cpu().numpy()
     line = {
@@ -1013,7 +1015,11 @@ def main():
                    "buffer_pairs": args.pairs},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": kernel_name(COLS, plan.threadsPerRow),
+                     "traffic": traffic,
+                     "traffic_source": ("FETCH_SIZE and WRITE_SIZE of rocprofv3 --pmc passes of this "
+                                        "command, scaled by a plain copy's: profiles/"
+                                        + os.path.basename(tpath)) if traffic else None,
+                     "kernel": kernel_name(COLS, plan.threadsPerRow),
                      "avg_launch_ms": round(avg_ms, 5),
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "valu_issue": valu},
