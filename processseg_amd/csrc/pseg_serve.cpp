@@ -141,7 +141,6 @@ struct Request {
 constexpr int kSlots = 2;        // launches in flight at most
 constexpr int kMaxJobs = 1024;   // requests per launch at most
 
-
 struct Slot {
   bool busy = false;
   hipStream_t stream = nullptr;
