@@ -831,43 +831,48 @@ __global__ void sgram_finish_kernel(const double *__restrict__ parts,
 }
 
 // The partials of `groups` workgroups, parts[g] = (bins power sums, rows
-// used), summed in a fixed order: wave w of a block takes the groups w,
-// w + 16, w + 32, ... for the block's 64 bins (one per lane), then the 16
-// waves' sums are added in wave order.  amp[k] = sqrt(Σ / used) · sides(k)
-// (1 at 0 Hz and Nyquist), 0 when no row was used; *used_out (when not
-// NULL) = the rows used.
+// used), summed in a fixed order.  A block of 1024 threads takes kFinishBins
+// bins and has kFinishLanes = 1024 / kFinishBins lanes of groups per bin:
+// lane l adds the groups l, l + kFinishLanes, ... of its bin (a wave reads
+// 64 / kFinishBins groups' runs of kFinishBins contiguous bins), then the
+// lane sums are added in lane order.  Every block adds the rows used the
+// same way.  amp[k] = sqrt(Σ / used) · sides(k) (1 at 0 Hz and Nyquist), 0
+// when no row was used; *used_out (when not NULL) = the rows used.
+constexpr int kFinishBins = 8;
+constexpr int kFinishLanes = 1024 / kFinishBins;
+static_assert(1024 % kFinishBins == 0 && kFinishBins <= 64, "bins per finish block");
 __global__ __launch_bounds__(1024) void spectrum_finish_kernel(const double *__restrict__ parts,
                                                                int32_t groups, int32_t bins,
                                                                int32_t nfft,
                                                                double *__restrict__ amp,
                                                                double *__restrict__ used_out) {
-  __shared__ double s_sum[16][64];
-  __shared__ double s_used[16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = blockIdx.x * 64 + lane;
+  __shared__ double s_sum[kFinishLanes][kFinishBins + 1];
+  __shared__ double s_used[kFinishLanes];
+  const int b = threadIdx.x % kFinishBins, gl = threadIdx.x / kFinishBins;
+  const int k = blockIdx.x * kFinishBins + b;
   const size_t pitch = (size_t)bins + 1;
   const int kk = k < bins ? k : bins;  // lanes past the last bin read the used column
   double s = 0, u = 0;
-#pragma unroll 8
-  for (int g = wave; g < groups; g += 16) {
+#pragma unroll 4
+  for (int g = gl; g < groups; g += kFinishLanes) {
     const double *p = parts + (size_t)g * pitch;
     s += p[kk];
-    u += p[bins];
+    if (b == 0) u += p[bins];
   }
-  s_sum[wave][lane] = s;
-  if (lane == 0) s_used[wave] = u;
+  s_sum[gl][b] = s;
+  if (b == 0) s_used[gl] = u;
   __syncthreads();
-  if (wave == 0) {
+  if (threadIdx.x < kFinishBins) {
     double t = 0, used = 0;
-    for (int w = 0; w < 16; w++) {
-      t += s_sum[w][lane];
+    for (int w = 0; w < kFinishLanes; w++) {
+      t += s_sum[w][b];
       used += s_used[w];
     }
     if (k < bins) {
       const double sides = (k == 0 || 2 * k == nfft) ? 1.0 : 2.0;
       amp[k] = used > 0 ? sqrt(t / used) * sides : 0.0;
     }
-    if (blockIdx.x == 0 && lane == 0 && used_out != nullptr) *used_out = used;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && used_out != nullptr) *used_out = used;
   }
 }
 
@@ -1055,7 +1060,8 @@ hipError_t launch_spectrum(int dtype, int threads, const DevSpec *dev, size_t ld
     }
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(spectrum_finish_kernel, dim3((bins + 63) / 64), dim3(1024), 0, s, parts,
+  hipLaunchKernelGGL(spectrum_finish_kernel, dim3((bins + kFinishBins - 1) / kFinishBins),
+                     dim3(1024), 0, s, parts,
                      groups, bins, nfft, amp, used);
   return hipGetLastError();
 }
