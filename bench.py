@@ -481,12 +481,15 @@ def spectrum_leg(torch, P, dev, steps, warmup):
     rl = meta.device_row_len(COLS, dev)
     amp = torch.empty(sp.bins, dtype=torch.float64, device=dev)
     ws = torch.empty(sp.workspaceSize(ROWS), dtype=torch.uint8, device=dev)
+    used = torch.empty(1, dtype=torch.float64, device=dev)
     for _ in range(warmup):
-        sp.averageAmplitudeInto(src, firstCol=500, amp=amp, row_len=rl, ws=ws, sync=False)
+        sp.averageAmplitudeInto(src, firstCol=500, amp=amp, row_len=rl, ws=ws, used=used,
+                                sync=False)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(steps):
-        sp.averageAmplitudeInto(src, firstCol=500, amp=amp, row_len=rl, ws=ws, sync=False)
+        sp.averageAmplitudeInto(src, firstCol=500, amp=amp, row_len=rl, ws=ws, used=used,
+                                sync=False)
     torch.cuda.synchronize()
     t = (time.perf_counter() - t1) / steps
     return {"workload": "RMS amplitude spectrum of 2000 windows of 2001 samples (nfft 2048) of "

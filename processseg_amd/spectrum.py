@@ -46,14 +46,16 @@ class SpectrumPlan:
         return int(b.value)
 
     def averageAmplitudeInto(self, samples, meta: ChunkMeta | None = None, firstCol: int = 0,
-                             amp=None, *, row_len=None, ws=None, stream=None, sync: bool = True):
+                             amp=None, *, row_len=None, ws=None, used=None, stream=None,
+                             sync: bool = True):
         """averageAmplitudeInto (fft/spectrum.nim:410-433) on a [rows, cols]
         GPU tensor: returns (amp, used) — amp a float64 [bins] GPU tensor,
         used the rows averaged (read back after `stream` has finished when
         `sync`; else a one-value float64 GPU tensor written on `stream`).
         `ws`: a uint8 GPU tensor of at least workspaceSize(rows) bytes, or
-        None for one allocated for this call.  The same input gives the same
-        bits on every call."""
+        None for one allocated for this call; `used`: a one-value float64 GPU
+        tensor to write the rows used into, or None for one allocated for
+        this call.  The same input gives the same bits on every call."""
         import torch
         _check_panel(samples, "samples", self.T)
         rows, cols = samples.shape
@@ -64,18 +66,27 @@ class SpectrumPlan:
         dev = samples.device
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         need = self.workspaceSize(rows)
-        with torch.cuda.stream(s):
-            if ws is None:
-                ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
-            elif (not isinstance(ws, torch.Tensor) or not ws.is_cuda or ws.device != dev
-                  or not ws.is_contiguous() or ws.numel() * ws.element_size() < need):
-                raise ValueError(f"ws must be a contiguous GPU tensor of at least {need} bytes")
-            if amp is None:
-                amp = torch.empty(self.bins, dtype=torch.float64, device=dev)
-            elif (not isinstance(amp, torch.Tensor) or amp.dtype != torch.float64
-                  or amp.numel() != self.bins or not amp.is_contiguous() or amp.device != dev):
-                raise ValueError(f"amp must be {self.bins} contiguous float64 values on {dev}")
-            used = torch.empty(1, dtype=torch.float64, device=dev)
+        if ws is not None and (not isinstance(ws, torch.Tensor) or not ws.is_cuda
+                               or ws.device != dev or not ws.is_contiguous()
+                               or ws.numel() * ws.element_size() < need):
+            raise ValueError(f"ws must be a contiguous GPU tensor of at least {need} bytes")
+        if amp is not None and (not isinstance(amp, torch.Tensor) or amp.dtype != torch.float64
+                                or amp.numel() != self.bins or not amp.is_contiguous()
+                                or amp.device != dev):
+            raise ValueError(f"amp must be {self.bins} contiguous float64 values on {dev}")
+        if used is not None and (not isinstance(used, torch.Tensor)
+                                 or used.dtype != torch.float64 or used.numel() != 1
+                                 or used.device != dev):
+            raise ValueError(f"used must be one float64 value on {dev}")
+        if ws is None or amp is None or used is None:
+            # allocations belong to the stream that writes them
+            with torch.cuda.stream(s):
+                if ws is None:
+                    ws = torch.empty(max(need, 1), dtype=torch.uint8, device=dev)
+                if amp is None:
+                    amp = torch.empty(self.bins, dtype=torch.float64, device=dev)
+                if used is None:
+                    used = torch.empty(1, dtype=torch.float64, device=dev)
         _lib.check(_lib.lib().pseg_spectrum_average(
             self._h, samples.data_ptr(), rows, cols, row_len.data_ptr(), int(firstCol),
             ws.data_ptr(), ws.numel() * ws.element_size(), amp.data_ptr(), used.data_ptr(),

@@ -125,10 +125,15 @@ def test_average_is_bit_reproducible(gpu, oracle):
     b, ub = sp.averageAmplitudeInto(src, firstCol=500, row_len=rl)
     ws = torch.empty(sp.workspaceSize(2000), dtype=torch.uint8, device="cuda")
     c, uc = sp.averageAmplitudeInto(src, firstCol=500, row_len=rl, ws=ws)
+    used = torch.full((1,), -1.0, dtype=torch.float64, device="cuda")
+    d, ud = sp.averageAmplitudeInto(src, firstCol=500, row_len=rl, ws=ws, used=used, sync=False)
+    torch.cuda.synchronize()
+    assert ud is used and used.item() == 2000
     assert ua == ub == uc == 2000
-    a, b, c = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy()
+    a, b, c, d = a.cpu().numpy(), b.cpu().numpy(), c.cpu().numpy(), d.cpu().numpy()
     assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
     assert np.array_equal(a.view(np.uint64), c.view(np.uint64))
+    assert np.array_equal(a.view(np.uint64), d.view(np.uint64))
     p = oracle.spectrum_plan(2.0, 2001, 0.1, 0)
     rows = np.arange(0, 2000, 50)
     want, _ = oracle.spectrum_average(p, x[rows].astype(np.float64), live[rows].astype(np.int32),
@@ -193,7 +198,11 @@ def test_wrappers_refuse_before_launch(gpu):
                                      row_len=good),
               lambda: hp.envelopeInto(x, torch.zeros(rows, cols), row_len=good),
               lambda: sp.amplitudeInto(x, row_len=good, amp=torch.zeros(rows, sp.bins)),
-              lambda: sp.averageAmplitudeInto(x, row_len=good, firstCol=-1)]
+              lambda: sp.averageAmplitudeInto(x, row_len=good, firstCol=-1),
+              lambda: sp.averageAmplitudeInto(x, row_len=good,
+                                              used=torch.zeros(1, dtype=torch.float64)),
+              lambda: sp.averageAmplitudeInto(x, row_len=good,
+                                              used=torch.zeros(1, device="cuda"))]
     for i, c in enumerate(calls):
         with pytest.raises(ValueError):
             c()
